@@ -1,0 +1,208 @@
+#!/usr/bin/env python3
+"""Throughput of the MI355X QG time step in layer-cell-steps/s.
+
+Workload (default): the 3-layer QG double gyre at 1024 x 1024 interior cells
+(3 x 1026 x 1026 float32 state), biharmonic viscosity 2e11*(1/8)^4 m^4/s,
+bottom drag 3.6e-8 /s, tau0 = 0.08 N/m^2, dt = 1200 s, seeded k^-3 random
+initial PV, advanced by the SSP-RK3 step with the mixed-precision MFMA
+inversion (BF16 hi/lo 3-product GEMMs + one refinement pass).  The plain
+all-FP32 step on the same state is timed in the same run as the comparator.
+
+A layer-cell-step is one interior cell of one layer advanced by one full
+SSP-RK3 step.  Inputs are resident in HBM before the timed region; the timed
+region is exactly --steps steps bracketed by torch.cuda.synchronize() and HIP
+events on the stream the kernels run on.
+
+Prints one JSON line.  ``roofline`` is taken from a second, per-launch
+profiled run (HIP events around every launch, somaxhip_qg_step_profile);
+``cpu_baseline`` times the FP64 NumPy oracle on the host for a bounded
+sample of the same workload.
+"""
+
+from __future__ import annotations
+
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+PEAK = {"hbm_GBs": 8000.0, "bf16_TFLOPs": 2500.0, "fp32_TFLOPs": 157.3}
+
+
+def log(msg: str) -> None:
+    print(msg, file=sys.stderr, flush=True)
+
+
+def build_model(n: int, precision: str, advection: str):
+    from somax_amd import testcases as tc
+
+    return tc.config_baroclinic(n, precision=precision, advection=advection)
+
+
+def time_steps(model, q0, dt: float, steps: int, warmup: int):
+    import torch
+
+    q = q0.clone()
+    if warmup:
+        model._step_inplace(q, dt, warmup)
+    torch.cuda.synchronize()
+    st = torch.cuda.current_stream()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    e0.record(st)
+    model._step_inplace(q, dt, steps)
+    e1.record(st)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ms = e0.elapsed_time(e1)
+    if not torch.isfinite(q).all():
+        raise RuntimeError("non-finite state after the timed steps")
+    return ms / steps, wall
+
+
+def kernel_profile(model, q0, dt: float, steps: int):
+    import torch
+
+    from somax_amd import _lib
+
+    q = q0.clone()
+    plan = model._ensure_plan()
+    stats = (_lib.KernelStat * 16)()
+    nst = ctypes.c_int(0)
+    prm = model._c_params()
+    torch.cuda.synchronize()
+    _lib.check(
+        _lib.load().somaxhip_qg_step_profile(plan, q.data_ptr(), steps, dt, ctypes.byref(prm), model._ws.data_ptr(),
+                                            ctypes.c_void_p(torch.cuda.current_stream().cuda_stream), stats, 16,
+                                            ctypes.byref(nst)),
+        "somaxhip_qg_step_profile",
+    )
+    out = {}
+    for k in range(nst.value):
+        s = stats[k]
+        out[s.name.decode()] = dict(launches=s.launches, total_ms=s.total_ms, flops=s.flops, bytes=s.bytes)
+    return out
+
+
+def roofline_entry(name: str, st: dict) -> dict:
+    avg_ms = st["total_ms"] / st["launches"]
+    if name.startswith("dst_gemm"):
+        peak = PEAK["fp32_TFLOPs"] if name.endswith("fp32") else PEAK["bf16_TFLOPs"]
+        per_launch = st["flops"] / st["launches"]
+        achieved = per_launch / (avg_ms * 1e-3) / 1e12
+        issued = achieved * (3.0 if name.endswith("bf16x3") else 1.0)
+        return {"kernel": name, "bound": "mfma", "achieved": round(achieved, 2), "peak": peak, "unit": "TFLOP/s",
+                "frac": round(achieved / peak, 4), "issued_frac": round(issued / peak, 4),
+                "avg_launch_us": round(avg_ms * 1e3, 2), "alg_flops_per_launch": per_launch, "traffic": None}
+    per_launch = st["bytes"] / st["launches"]
+    achieved = per_launch / (avg_ms * 1e-3) / 1e9
+    return {"kernel": name, "bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK["hbm_GBs"], "unit": "GB/s",
+            "frac": round(achieved / PEAK["hbm_GBs"], 4), "avg_launch_us": round(avg_ms * 1e3, 2),
+            "alg_bytes_per_launch": per_launch, "traffic": None}
+
+
+def cpu_baseline(n: int, q64, dt: float, steps: int) -> dict:
+    from oracle import qg as oqg
+    from somax_amd import testcases as tc
+
+    setup = oqg.baroclinic_setup(nx=n, ny=n, Lx=tc.BASIN, Ly=tc.BASIN, f0=tc.F0, beta=tc.BETA, bottom_drag=tc.DRAG,
+                                 wind_amplitude=tc.wind_amplitude(), biharmonic_viscosity=tc.NU4_128 * (128 / n) ** 4)
+    q = q64.copy()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        q = oqg.ssprk3_step(setup, q, dt)
+    el = time.perf_counter() - t0
+    nl = q.shape[0]
+    return {"value": round(nl * n * n * steps / el, 1), "unit": "layer-cell-steps/s", "cores": 1, "kind": "port",
+            "sample": f"{steps} SSP-RK3 steps of the same {nl}x{n}x{n} workload, FP64 NumPy/SciPy oracle "
+                      f"(FFT DST-I), single thread, {el:.1f} s"}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--n", type=int, default=1024, help="interior cells per side")
+    ap.add_argument("--dt", type=float, default=1200.0)
+    ap.add_argument("--advection", default="arakawa", choices=["arakawa", "weno"])
+    ap.add_argument("--profile-steps", type=int, default=2)
+    ap.add_argument("--cpu-steps", type=int, default=2, help="oracle steps timed for cpu_baseline (0 = skip)")
+    args = ap.parse_args()
+
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print(f"bench.py measures one GPU only: this build has no multi-GPU path "
+              f"(--gpus {args.gpus}, WORLD_SIZE {world}); members and rows of a single model are not sharded.",
+              file=sys.stderr)
+        return 2
+
+    import numpy as np
+    import torch
+
+    from somax_amd import testcases as tc
+
+    n, dt = args.n, args.dt
+    log(f"[bench] building models n={n}")
+    mixed = build_model(n, "mixed", args.advection)
+    plain = build_model(n, "fp32", args.advection)
+    q64 = tc.random_pv(mixed, seed=2024)
+    q0 = torch.from_numpy(q64.astype(np.float32)).cuda()
+    nl = q0.shape[0]
+    cells = nl * n * n
+
+    log("[bench] timing mixed")
+    ms_mixed, wall_mixed = time_steps(mixed, q0, dt, args.steps, args.warmup)
+    log(f"[bench] mixed {ms_mixed:.3f} ms/step")
+    log("[bench] timing plain fp32")
+    ms_plain, _ = time_steps(plain, q0, dt, args.steps, args.warmup)
+    log(f"[bench] plain {ms_plain:.3f} ms/step")
+
+    log("[bench] per-launch profile")
+    prof_mixed = kernel_profile(mixed, q0, dt, args.profile_steps)
+    prof_plain = kernel_profile(plain, q0, dt, args.profile_steps)
+    dom = max(prof_mixed, key=lambda k: prof_mixed[k]["total_ms"])
+    roof = roofline_entry(dom, prof_mixed[dom])
+    roof_stencil = roofline_entry("qg_stage", prof_mixed["qg_stage"])
+    dom_p = max(prof_plain, key=lambda k: prof_plain[k]["total_ms"])
+    roof_plain = roofline_entry(dom_p, prof_plain[dom_p])
+
+    cpu = None
+    if args.cpu_steps > 0:
+        log("[bench] cpu baseline (oracle)")
+        cpu = cpu_baseline(n, q64, dt, args.cpu_steps)
+
+    value = cells / (ms_mixed * 1e-3)
+    plain_value = cells / (ms_plain * 1e-3)
+    line = {
+        "metric": "layer-cell-steps/s: 3-layer QG 1024² and 256-member ensemble vs plain GPU",
+        "value": round(value, 1),
+        "unit": "layer-cell-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_mixed, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 stencils; bf16x3 MFMA + f32 refinement inversion",
+        "data": "synthetic (seeded k^-3 random PV)",
+        "config": {"workload": f"qg{nl}x{n}x{n}_ssprk3_mixed", "n": n, "layers": nl, "dt": dt,
+                   "advection": args.advection, "inversion": "mixed"},
+        "plain_fp32": {"value": round(plain_value, 1), "ms_per_step": round(ms_plain, 4),
+                       "speedup_mixed_vs_plain": round(ms_plain / ms_mixed, 3), "roofline": roof_plain},
+        "roofline": roof,
+        "roofline_stencil": roof_stencil,
+        "kernel_profile_mixed": {k: {"launches": v["launches"], "avg_us": round(1e3 * v["total_ms"] / v["launches"], 2)}
+                                 for k, v in prof_mixed.items()},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,173 @@
+/*
+ * somaxhip -- C ABI of the MI355X-native somax QG / shallow-water time step.
+ *
+ * Plain C: pointers, sizes and scalars only.  Every entry point returns an
+ * int status (SOMAXHIP_OK == 0); the text of the last error on the calling
+ * thread is available from somaxhip_last_error().  No C++ exception crosses
+ * this boundary.  All device buffers are caller-owned and must stay valid
+ * until the work queued on `stream` has finished; nothing is retained past
+ * return except inside a plan.  Work is queued on the caller's hipStream_t
+ * (pass NULL for the default stream) and nothing here synchronises the host.
+ *
+ * Layouts are those of the reference state pytrees: a field of a model with
+ * nx x ny interior cells and nl layers is float32[nl][ny+2][nx+2], row-major,
+ * x fastest, ghost ring included (somax/_src/models/qg/baroclinic.py:34,
+ * swm/multilayer.py:38-40).
+ *
+ * Each entry point names the reference interface it stands in for.  The
+ * reference is Python/JAX (jejjohnson/somax v0.0.6); its arithmetic lives in
+ * finitevolX v0.0.39 and diffrax, called from the lines cited.
+ */
+#ifndef SOMAXHIP_H
+#define SOMAXHIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* hipStream_t without pulling in the HIP headers. */
+typedef struct ihipStream_t* somaxhip_stream_t;
+
+enum {
+  SOMAXHIP_OK = 0,
+  SOMAXHIP_ERR_ARG = 1,     /* invalid argument (shape, pointer, flag) */
+  SOMAXHIP_ERR_HIP = 2,     /* a HIP runtime call failed */
+  SOMAXHIP_ERR_ALLOC = 3,   /* device allocation failed */
+  SOMAXHIP_ERR_DEVICE = 4   /* no gfx950 device / wrong architecture */
+};
+
+/* Advection of total PV. */
+enum {
+  SOMAXHIP_ADV_ARAKAWA = 0, /* reference: finitevolx.arakawa_jacobian, baroclinic.py:175 */
+  SOMAXHIP_ADV_WENO = 1     /* extension: flux-form WENO-Z (5th order, 3rd/1st at walls) */
+};
+
+/* Arithmetic of the DST-I Helmholtz inversion. */
+enum {
+  SOMAXHIP_PREC_FP32 = 0,   /* plain: FP32 MFMA sine-transform GEMMs */
+  SOMAXHIP_PREC_MIXED = 1   /* BF16 hi/lo 3-product MFMA, FP32 accumulate, one
+                               iterative-refinement pass on the Helmholtz residual */
+};
+
+/* ------------------------------------------------------------------------ */
+/* Library                                                                  */
+/* ------------------------------------------------------------------------ */
+
+/* Copies the last error message of the calling thread (NUL-terminated,
+ * truncated to len).  Returns the full message length. */
+size_t somaxhip_last_error(char* buf, size_t len);
+
+/* Checks that the current device is gfx950 and warms the runtime.
+ * No reference counterpart (JAX initialises XLA lazily). */
+int somaxhip_init(void);
+
+/* Library version as major*10000 + minor*100 + patch. */
+int somaxhip_version(void);
+
+/* ------------------------------------------------------------------------ */
+/* Multilayer / barotropic QG                                               */
+/* ------------------------------------------------------------------------ */
+
+/* Immutable per-model tables (sine matrices, eigenvalue tables, modal
+ * matrices, forcing profiles, homogeneous solutions).  Built once per model
+ * by the host from create() (baroclinic.py:231-332, barotropic.py:186-248). */
+typedef struct somaxhip_qg_plan_s* somaxhip_qg_plan;
+
+typedef struct {
+  int nx, ny;                 /* interior cells */
+  int nl;                     /* layers (1 for BarotropicQG) */
+  double dx, dy;              /* grid spacing (ArakawaCGrid2D.from_interior) */
+  const double* cl2m;         /* nl*nl, row-major: modal[m] = sum_l cl2m[m*nl+l]*layer[l] */
+  const double* cm2l;         /* nl*nl, row-major: layer[l] = sum_m cm2l[l*nl+m]*modal[m] */
+  const double* lambdas;      /* nl Helmholtz parameters, f0^2 * eigenvalue (baroclinic.py:296) */
+  const double* beta_y_row;   /* ny+2 values beta*(y_j - Ly/2), y_j = j*dy (baroclinic.py:307-311) */
+  const double* wind_row;     /* ny+2 values of the wind-curl pattern (baroclinic.py:313-318) */
+  const double* wind_scale;   /* nl multipliers of tau0*pattern: 1/H0 on layer 0 (baroclinic.py:181)
+                                 or 1 (barotropic.py:143); 0 elsewhere */
+  int zero_psi_ghost;         /* 1: zero_boundaries(psi) after to_layer (baroclinic.py:158);
+                                 0: keep the DST ghost values (barotropic.py:119-121) */
+  int advection;              /* SOMAXHIP_ADV_* */
+  int precision;              /* SOMAXHIP_PREC_* */
+  int mass_correction;        /* extension: add alpha_m * homsol_m so each mode has zero mean */
+  const float* homsol;        /* nl*(ny+2)*(nx+2) host array, required iff mass_correction */
+  const double* homsol_mean;  /* nl means over the full array, required iff mass_correction */
+} somaxhip_qg_desc;
+
+/* Differentiable parameters (BaroclinicQGParams, baroclinic.py:37-48), passed
+ * per call so they may change between calls without a new plan. */
+typedef struct {
+  double nu;     /* lateral_viscosity: + nu * laplacian(q) (baroclinic.py:188) */
+  double nu4;    /* extension: - nu4 * laplacian(laplacian(q)) */
+  double kappa;  /* bottom_drag: - kappa * laplacian(psi[-1]) (baroclinic.py:184-185) */
+  double tau0;   /* wind_amplitude (baroclinic.py:181) */
+} somaxhip_qg_params;
+
+/* Builds the plan on the current device (host tables are copied). */
+int somaxhip_qg_plan_create(const somaxhip_qg_desc* desc, somaxhip_qg_plan* plan);
+int somaxhip_qg_plan_destroy(somaxhip_qg_plan plan);
+
+/* Bytes of caller-provided device workspace that the calls below need. */
+size_t somaxhip_qg_workspace_bytes(somaxhip_qg_plan plan);
+
+/* psi = _invert_pv(q), or _invert_pv(apply_boundary_conditions(q)) when
+ * apply_bc != 0 (the form every RHS evaluation uses).
+ * Replaces BaroclinicQG._invert_pv (baroclinic.py:135-159) /
+ * BarotropicQG._invert_pv (barotropic.py:113-121). */
+int somaxhip_qg_invert(somaxhip_qg_plan plan, const float* q, float* psi, int apply_bc,
+                       void* workspace, somaxhip_stream_t stream);
+
+/* dq = vector_field(t, state).q with state.q = q, or, when apply_bc != 0,
+ * vector_field(t, apply_boundary_conditions(state)).q as composed by
+ * SomaxModel.build_terms (core/model.py:47-49).  psi_out (nullable) receives
+ * the streamfunction.  Replaces BaroclinicQG.vector_field
+ * (baroclinic.py:161-190) / BarotropicQG.vector_field (barotropic.py:123-152). */
+int somaxhip_qg_rhs(somaxhip_qg_plan plan, const float* q, float* dq, float* psi_out,
+                    const somaxhip_qg_params* params, int apply_bc, void* workspace,
+                    somaxhip_stream_t stream);
+
+/* Ghost ring of nl fields of (ny+2) x (nx+2) set to 0 in place.
+ * Replaces finitevolx.zero_boundaries as used by
+ * BaroclinicQG.apply_boundary_conditions (baroclinic.py:192-195). */
+int somaxhip_zero_ghost(float* field, int nl, int ny, int nx, somaxhip_stream_t stream);
+
+/* Advances q in place by nsteps SSP-RK3 steps of size dt (Shu-Osher form of
+ * the ssrk3_update_stage_{1,2,3} scheme, notebooks/dev/QG_Multilayer/
+ * mqg_demo_retangular.ipynb cell 31), applying the boundary condition before
+ * every RHS like SomaxModel.build_terms.  Stands in for the hot loop of
+ * SomaxModel.integrate (core/model.py:53-88).  psi_out (nullable) receives
+ * the streamfunction of the last stage's input state. */
+int somaxhip_qg_step(somaxhip_qg_plan plan, float* q, float* psi_out, int nsteps, double dt,
+                     const somaxhip_qg_params* params, void* workspace,
+                     somaxhip_stream_t stream);
+
+/* ------------------------------------------------------------------------ */
+/* Measurement                                                              */
+/* ------------------------------------------------------------------------ */
+
+/* Per-kernel-class totals of one profiled run: launches, summed duration
+ * (HIP events around every launch on the caller's stream) and the
+ * algorithmic flops and bytes of those launches. */
+typedef struct {
+  char name[32];
+  int launches;
+  double total_ms;
+  double flops;
+  double bytes;
+} somaxhip_kernel_stat;
+
+/* somaxhip_qg_step with every launch bracketed by HIP events; synchronises
+ * the stream before returning.  No reference counterpart (the reference only
+ * times whole chunks with time.perf_counter, cli/_run.py:289-307). */
+int somaxhip_qg_step_profile(somaxhip_qg_plan plan, float* q, int nsteps, double dt,
+                             const somaxhip_qg_params* params, void* workspace,
+                             somaxhip_stream_t stream, somaxhip_kernel_stat* stats,
+                             int max_stats, int* nstats);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SOMAXHIP_H */

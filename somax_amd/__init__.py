@@ -1,0 +1,39 @@
+"""somax_amd: MI355X-native (gfx950 HIP) QG / shallow-water time step of somax.
+
+Public names mirror ``somax.models`` and ``somax.core``
+(somax/models/__init__.py, somax/core/__init__.py) for the models this
+library accelerates.
+"""
+
+from .core import (
+    ArakawaCGrid2D,
+    Diagnostics,
+    ModalTransform,
+    Params,
+    PhysConsts,
+    Solution,
+    SomaxModel,
+    State,
+    StratificationProfile,
+)
+from .qg import (
+    BaroclinicQG,
+    BaroclinicQGDiagnostics,
+    BaroclinicQGParams,
+    BaroclinicQGPhysConsts,
+    BaroclinicQGState,
+    BarotropicQG,
+    BarotropicQGDiagnostics,
+    BarotropicQGParams,
+    BarotropicQGPhysConsts,
+    BarotropicQGState,
+)
+from .testcases import doublegyre_baroclinic_qg, doublegyre_qg
+
+__all__ = [
+    "ArakawaCGrid2D", "Diagnostics", "ModalTransform", "Params", "PhysConsts", "Solution", "SomaxModel",
+    "State", "StratificationProfile", "BaroclinicQG", "BaroclinicQGDiagnostics", "BaroclinicQGParams",
+    "BaroclinicQGPhysConsts", "BaroclinicQGState", "BarotropicQG", "BarotropicQGDiagnostics",
+    "BarotropicQGParams", "BarotropicQGPhysConsts", "BarotropicQGState", "doublegyre_baroclinic_qg",
+    "doublegyre_qg",
+]

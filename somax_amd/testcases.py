@@ -1,0 +1,136 @@
+"""Test-case factories and synthetic initial conditions.
+
+``doublegyre_qg`` and ``doublegyre_baroclinic_qg`` mirror
+``somax/_src/models/gfd_testcases.py:184-287`` (same defaults; the reference
+starts from rest).  ``random_streamfunction`` and the ``config*`` builders are
+extensions for the benchmark workloads: seeded isotropic k^-3 random fields
+and the basin/forcing parameters of the benchmark configurations.  All
+set-up arithmetic is FP64 NumPy on the host; the returned states are float32
+device tensors.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+
+from .qg import BaroclinicQG, BaroclinicQGState, BarotropicQG, BarotropicQGState
+
+RHO0 = 1000.0
+BASIN = 5.12e6
+F0 = 9.375e-5
+BETA = 1.754e-11
+TAU0 = 0.08
+DRAG = 3.6e-8
+NU4_128 = 2e11  # biharmonic viscosity at 128^2, scaled by (128/n)^4
+
+
+def _torch():
+    import torch
+
+    return torch
+
+
+def doublegyre_qg(nx=64, ny=64, Lx=1e6, Ly=1e6, f0=1e-4, beta=1.6e-11, lateral_viscosity=500.0,
+                  bottom_drag=1e-7, wind_amplitude=1e-12, **kw):
+    """gfd_testcases.py:184-228: barotropic double gyre from rest."""
+    model = BarotropicQG.create(nx=nx, ny=ny, Lx=Lx, Ly=Ly, f0=f0, beta=beta, lateral_viscosity=lateral_viscosity,
+                                bottom_drag=bottom_drag, wind_amplitude=wind_amplitude, wind_profile="doublegyre",
+                                **kw)
+    torch = _torch()
+    q0 = torch.zeros((model.grid.Ny, model.grid.Nx), dtype=torch.float32, device=model.device)
+    return model, BarotropicQGState(q=q0)
+
+
+def doublegyre_baroclinic_qg(nx=128, ny=128, Lx=4e6, Ly=4e6, f0=9.375e-5, beta=1.754e-11, n_layers=3,
+                             H=(400.0, 1100.0, 2600.0), g_prime=(9.81, 0.025, 0.0125), lateral_viscosity=15.0,
+                             bottom_drag=1e-7, wind_amplitude=1.3e-10, **kw):
+    """gfd_testcases.py:231-287: multilayer double gyre from rest."""
+    model = BaroclinicQG.create(nx=nx, ny=ny, Lx=Lx, Ly=Ly, f0=f0, beta=beta, n_layers=n_layers, H=H,
+                                g_prime=g_prime, lateral_viscosity=lateral_viscosity, bottom_drag=bottom_drag,
+                                wind_amplitude=wind_amplitude, wind_profile="doublegyre", **kw)
+    torch = _torch()
+    q0 = torch.zeros((model.consts.n_layers, model.grid.Ny, model.grid.Nx), dtype=torch.float32,
+                     device=model.device)
+    return model, BaroclinicQGState(q=q0)
+
+
+# ---------------------------------------------------------------------------
+# synthetic initial conditions (extension)
+# ---------------------------------------------------------------------------
+def _sine(n: int) -> np.ndarray:
+    j = np.arange(1, n + 1, dtype=np.float64)
+    return np.sqrt(2.0 / (n + 1)) * np.sin(np.pi * np.outer(j, j) / (n + 1))
+
+
+def random_streamfunction(nx: int, ny: int, Lx: float, Ly: float, nl: int, seed: int, urms: float = 0.05,
+                          slope: float = -3.0) -> np.ndarray:
+    """Seeded Gaussian random psi, ``(nl, ny+2, nx+2)`` FP64, zero ghost ring.
+
+    Built in the interior DST-I basis (so psi = 0 on the ghost ring) with an
+    isotropic kinetic-energy spectrum ``E(k) ~ k**slope``: the coefficient of
+    mode ``(ky, kx)`` has standard deviation ``k**((slope - 3)/2)``.  Each
+    layer is scaled to an rms speed ``urms`` of the forward-difference
+    geostrophic velocity ``u = -dpsi/dy``, ``v = dpsi/dx`` on the interior.
+    """
+    rng = np.random.default_rng(seed)
+    dx, dy = Lx / nx, Ly / ny
+    kx = np.pi * np.arange(1, nx + 1) / ((nx + 1) * dx)
+    ky = np.pi * np.arange(1, ny + 1) / ((ny + 1) * dy)
+    K = np.sqrt(ky[:, None] ** 2 + kx[None, :] ** 2)
+    amp = K ** ((slope - 3.0) / 2.0)
+    Sx, Sy = _sine(nx), _sine(ny)
+    psi = np.zeros((nl, ny + 2, nx + 2))
+    for l in range(nl):
+        c = rng.standard_normal((ny, nx)) * amp
+        p = np.zeros((ny + 2, nx + 2))
+        p[1:-1, 1:-1] = Sy @ c @ Sx
+        u = -(p[2:, 1:-1] - p[1:-1, 1:-1]) / dy
+        v = (p[1:-1, 2:] - p[1:-1, 1:-1]) / dx
+        s = np.sqrt(np.mean(u * u + v * v))
+        psi[l] = p * (urms / s)
+    return psi
+
+
+def pv_from_streamfunction(psi: np.ndarray, dx: float, dy: float, lambdas, Cl2m, Cm2l) -> np.ndarray:
+    """``q = lap(psi) - B psi`` with ``B = Cm2l diag(lambdas) Cl2m`` (= f0^2 A for
+    consistent eigenvalues); 5-point Laplacian on the interior, ghost ring 0."""
+    psi = np.asarray(psi, dtype=np.float64)
+    q = np.zeros_like(psi)
+    c = psi[:, 1:-1, 1:-1]
+    q[:, 1:-1, 1:-1] = (psi[:, 1:-1, 2:] - 2 * c + psi[:, 1:-1, :-2]) / dx**2 + (
+        psi[:, 2:, 1:-1] - 2 * c + psi[:, :-2, 1:-1]
+    ) / dy**2
+    B = np.asarray(Cm2l) @ np.diag(np.asarray(lambdas)) @ np.asarray(Cl2m)
+    q[:, 1:-1, 1:-1] -= np.einsum("lk,kyx->lyx", B, c)
+    return q
+
+
+def random_pv(model, seed: int, urms: float = 0.05) -> np.ndarray:
+    """FP64 PV field whose inversion is a seeded k^-3 random streamfunction."""
+    g = model.grid
+    psi = random_streamfunction(g.nx, g.ny, g.Lx, g.Ly, model._nl, seed, urms)
+    return pv_from_streamfunction(psi, g.dx, g.dy, model._lambdas, model._Cl2m, model._Cm2l)
+
+
+# ---------------------------------------------------------------------------
+# benchmark configurations (basin 5120 km, tau0 = 0.08 N/m^2, drag 3.6e-8 /s)
+# ---------------------------------------------------------------------------
+def wind_amplitude(tau0: float = TAU0, Ly: float = BASIN) -> float:
+    """``tau0 * 2 pi / (rho0 Ly)`` (gfd_testcases.py:250-251)."""
+    return tau0 * 2.0 * np.pi / (RHO0 * Ly)
+
+
+def config_barotropic(n: int = 64, depth: float = 4000.0, **kw) -> BarotropicQG:
+    """1-layer double gyre: DST-I Poisson inversion, biharmonic 2e11 m^4/s."""
+    return BarotropicQG.create(nx=n, ny=n, Lx=BASIN, Ly=BASIN, f0=F0, beta=BETA, bottom_drag=DRAG,
+                               wind_amplitude=wind_amplitude() / depth, biharmonic_viscosity=NU4_128, **kw)
+
+
+def config_baroclinic(n: int = 128, nl: int = 3, tau0: float = TAU0, **kw) -> BaroclinicQG:
+    """Multilayer double gyre: H = 400/1100/2600 m, g' = 9.81/0.025/0.0125,
+    biharmonic 2e11 * (128/n)^4 m^4/s, bottom drag on the lowest layer."""
+    H = (400.0, 1100.0, 2600.0)[:nl]
+    gp = (9.81, 0.025, 0.0125)[:nl]
+    return BaroclinicQG.create(nx=n, ny=n, Lx=BASIN, Ly=BASIN, f0=F0, beta=BETA, n_layers=nl, H=H, g_prime=gp,
+                               bottom_drag=DRAG, wind_amplitude=wind_amplitude(tau0),
+                               biharmonic_viscosity=NU4_128 * (128.0 / n) ** 4, **kw)

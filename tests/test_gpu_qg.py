@@ -1,0 +1,237 @@
+"""GPU parity of the HIP QG path against the FP64 oracle.
+
+Inputs are seeded; the GPU receives float32 fields and the oracle receives the
+same float32 values widened to FP64, so every difference is the kernel's own.
+Tolerances (relative L2 over the whole array):
+
+* PV inversion: FP32 plain 3e-6; mixed (bf16x3 + refinement) 3e-7.
+* RHS (vector_field): 2e-5 (FP32 stencil arithmetic on cancelling terms).
+* One SSP-RK3 step: q and psi 1e-6 (the north-star 1-step bar).
+* 100 steps: 1e-4 (the north-star 100-step bar).
+"""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from oracle import qg as oqg
+from oracle import stencils as ost
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    a = np.asarray(a, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+def _pair(n, nl=3, precision="fp32", advection="arakawa", ny=None, **kw):
+    """A GPU model and the oracle setup with identical parameters."""
+    from somax_amd import testcases as tc
+
+    ny = n if ny is None else ny
+    H = (400.0, 1100.0, 2600.0)[:nl]
+    gp = (9.81, 0.025, 0.0125)[:nl]
+    nu4 = kw.pop("biharmonic_viscosity", tc.NU4_128 * (128.0 / n) ** 4)
+    args = dict(nx=n, ny=ny, Lx=tc.BASIN, Ly=tc.BASIN * ny / n, f0=tc.F0, beta=tc.BETA, H=H, g_prime=gp,
+                bottom_drag=tc.DRAG, wind_amplitude=tc.wind_amplitude(), biharmonic_viscosity=nu4,
+                lateral_viscosity=kw.pop("lateral_viscosity", 0.0))
+    from somax_amd import BaroclinicQG
+
+    model = BaroclinicQG.create(n_layers=nl, precision=precision, advection=advection, **args, **kw)
+    setup = oqg.baroclinic_setup(advection=advection, **args, **kw)
+    return model, setup
+
+
+def _state(model, seed, ghost_noise=False):
+    import torch
+
+    from somax_amd import testcases as tc
+
+    q = tc.random_pv(model, seed).astype(np.float32)
+    if ghost_noise:
+        rng = np.random.default_rng(seed + 1000)
+        g = rng.standard_normal(q.shape).astype(np.float32) * np.float32(np.abs(q).max())
+        g[:, 1:-1, 1:-1] = 0.0
+        q = q + g
+    return q, torch.from_numpy(q).cuda()
+
+
+def _log(msg):
+    print(msg, flush=True)
+
+
+@pytest.mark.parametrize("n,precision,tol", [(64, "fp32", 3e-6), (64, "mixed", 3e-7), (128, "fp32", 3e-6),
+                                             (128, "mixed", 3e-7), (1024, "fp32", 3e-6), (1024, "mixed", 3e-7)])
+def test_inversion_matches_oracle(gpu, n, precision, tol):
+    _log(f"[inversion] n={n} precision={precision}")
+    model, setup = _pair(n, precision=precision)
+    q, qd = _state(model, seed=11)
+    psi = model._invert_pv(qd).cpu().numpy()
+    ref = oqg.invert_pv(setup, q.astype(np.float64))
+    r = rel(psi, ref)
+    _log(f"  rel-L2 psi = {r:.3e}")
+    assert r < tol
+
+
+def test_inversion_nonsquare(gpu):
+    _log("[inversion] non-square 96 x 64, 2 layers")
+    model, setup = _pair(96, nl=2, ny=64)
+    q, qd = _state(model, seed=5)
+    psi = model._invert_pv(qd).cpu().numpy()
+    assert rel(psi, oqg.invert_pv(setup, q.astype(np.float64))) < 3e-6
+
+
+@pytest.mark.parametrize("precision", ["fp32", "mixed"])
+def test_vector_field_matches_oracle(gpu, precision):
+    _log(f"[rhs] n=128 precision={precision}")
+    from somax_amd import BaroclinicQGState
+
+    model, setup = _pair(128, precision=precision, lateral_viscosity=30.0)
+    q, qd = _state(model, seed=3)
+    dq = model.vector_field(0.0, BaroclinicQGState(q=qd)).q.cpu().numpy()
+    ref = oqg.rhs(setup, q.astype(np.float64))
+    r = rel(dq, ref)
+    _log(f"  rel-L2 dq = {r:.3e}")
+    assert r < 2e-5
+
+
+def test_build_terms_applies_bc(gpu):
+    """build_terms: RHS of the BC'd state, so ghost noise in q changes nothing
+    (core/model.py:47-49); vector_field alone reads the ghost ring."""
+    _log("[rhs] build_terms vs vector_field with ghost noise")
+    from somax_amd import BaroclinicQGState
+
+    model, setup = _pair(64)
+    q, qd = _state(model, seed=4, ghost_noise=True)
+    dq_bc = model.build_terms()(0.0, BaroclinicQGState(q=qd)).q.cpu().numpy()
+    ref_bc = oqg.rhs(setup, oqg.apply_bc(q.astype(np.float64)))
+    assert rel(dq_bc, ref_bc) < 2e-5
+    dq_raw = model.vector_field(0.0, BaroclinicQGState(q=qd)).q.cpu().numpy()
+    ref_raw = oqg.rhs(setup, q.astype(np.float64))
+    assert rel(dq_raw, ref_raw) < 2e-5
+    assert rel(dq_raw, dq_bc) > 1e-3
+
+
+@pytest.mark.parametrize("n,precision", [(64, "fp32"), (128, "fp32"), (128, "mixed"), (1024, "fp32"),
+                                         (1024, "mixed")])
+def test_one_step_matches_oracle(gpu, n, precision):
+    _log(f"[step] 1 SSP-RK3 step n={n} precision={precision}")
+    from somax_amd import BaroclinicQGState
+
+    model, setup = _pair(n, precision=precision)
+    q, qd = _state(model, seed=7)
+    dt = 1200.0
+    q1 = model.step(BaroclinicQGState(q=qd), dt).q
+    ref = oqg.ssprk3_step(setup, q.astype(np.float64), dt)
+    rq = rel(q1.cpu().numpy(), ref)
+    psi = model._invert_pv(q1).cpu().numpy()
+    rp = rel(psi, oqg.invert_pv(setup, ref))
+    _log(f"  rel-L2 q = {rq:.3e}  psi = {rp:.3e}")
+    assert rq < 1e-6 and rp < 1e-6
+
+
+@pytest.mark.parametrize("precision", ["fp32", "mixed"])
+def test_hundred_steps_match_oracle(gpu, precision):
+    _log(f"[step] 100 SSP-RK3 steps n=128 precision={precision}")
+    from somax_amd import BaroclinicQGState
+
+    model, setup = _pair(128, precision=precision)
+    q, qd = _state(model, seed=9)
+    dt = 1200.0
+    q100 = model.step(BaroclinicQGState(q=qd), dt, nsteps=100).q
+    ref = oqg.run(setup, q.astype(np.float64), dt, 100)
+    rq = rel(q100.cpu().numpy(), ref)
+    rp = rel(model._invert_pv(q100).cpu().numpy(), oqg.invert_pv(setup, ref))
+    _log(f"  rel-L2 q = {rq:.3e}  psi = {rp:.3e}")
+    assert rq < 1e-4 and rp < 1e-4
+
+
+def test_integrate_matches_steps(gpu):
+    """integrate() applies the BC to state0, steps, and shortens the last step."""
+    _log("[integrate] t1 not a multiple of dt")
+    from somax_amd import BaroclinicQGState
+
+    model, setup = _pair(64)
+    q, qd = _state(model, seed=8, ghost_noise=True)
+    sol = model.integrate(BaroclinicQGState(q=qd), 0.0, 2.5 * 1200.0, 1200.0)
+    ref = oqg.apply_bc(q.astype(np.float64))
+    ref = oqg.run(setup, ref, 1200.0, 2)
+    ref = oqg.ssprk3_step(setup, ref, 600.0)
+    assert sol.ys.q.shape == (1,) + q.shape
+    assert rel(sol.ys.q[0].cpu().numpy(), ref) < 1e-6
+
+
+# ---------------------------------------------------------------------------
+# behaviours the reference tests assert (tests/models/test_qg_baroclinic.py)
+# ---------------------------------------------------------------------------
+def test_zero_pv_zero_tendency(gpu):
+    import torch
+
+    from somax_amd import BaroclinicQG, BaroclinicQGState
+
+    m = BaroclinicQG.create(nx=16, ny=16, n_layers=2, H=(500.0, 4500.0), g_prime=(9.81, 0.025))
+    dq = m.vector_field(0.0, BaroclinicQGState(q=torch.zeros((2, 18, 18), device="cuda"))).q
+    assert float(dq.abs().max()) < 1e-12
+
+
+def test_wind_forcing_top_layer_only(gpu):
+    import torch
+
+    from somax_amd import BaroclinicQG, BaroclinicQGState
+
+    m = BaroclinicQG.create(nx=16, ny=16, n_layers=2, H=(500.0, 4500.0), g_prime=(9.81, 0.025),
+                            wind_amplitude=1e-10)
+    dq = m.vector_field(0.0, BaroclinicQGState(q=torch.zeros((2, 18, 18), device="cuda"))).q
+    assert float(dq[0].abs().max()) > 0 and float(dq[1].abs().max()) < 1e-15
+
+
+def test_boundary_conditions(gpu):
+    import torch
+
+    from somax_amd import BaroclinicQG, BaroclinicQGState
+
+    m = BaroclinicQG.create(nx=16, ny=16, n_layers=2, H=(500.0, 4500.0), g_prime=(9.81, 0.025))
+    s = m.apply_boundary_conditions(BaroclinicQGState(q=torch.ones((2, 18, 18), device="cuda")))
+    q = s.q.cpu().numpy()
+    assert np.all(q[:, 0] == 0) and np.all(q[:, -1] == 0) and np.all(q[:, :, 0] == 0) and np.all(q[:, :, -1] == 0)
+    assert np.all(q[:, 1:-1, 1:-1] == 1)
+
+
+def test_wind_develops_flow(gpu):
+    """test_qg_baroclinic.py:202-223 with the HIP SSP-RK3 integrator."""
+    import torch
+
+    from somax_amd import BaroclinicQG, BaroclinicQGState
+
+    m = BaroclinicQG.create(nx=16, ny=16, n_layers=2, H=(500.0, 4500.0), g_prime=(9.81, 0.025),
+                            lateral_viscosity=50.0, wind_amplitude=1e-10)
+    sol = m.integrate(BaroclinicQGState(q=torch.zeros((2, 18, 18), device="cuda")), 0.0, 3600.0, 100.0)
+    qf = sol.ys.q[0].cpu().numpy()
+    assert np.all(np.isfinite(qf)) and np.max(np.abs(qf[0, 2:-2, 2:-2])) > 1e-10
+
+
+def test_barotropic_matches_oracle(gpu):
+    _log("[barotropic] rhs and one step, n=64")
+    import torch
+
+    from somax_amd import BarotropicQGState
+    from somax_amd import testcases as tc
+
+    model = tc.config_barotropic(64)
+    setup = oqg.barotropic_setup(nx=64, ny=64, Lx=tc.BASIN, Ly=tc.BASIN, f0=tc.F0, beta=tc.BETA,
+                                 bottom_drag=tc.DRAG, wind_amplitude=tc.wind_amplitude() / 4000.0,
+                                 biharmonic_viscosity=tc.NU4_128)
+    q = tc.random_pv(model, seed=2)[0].astype(np.float32)
+    qd = torch.from_numpy(q).cuda()
+    dq = model.vector_field(0.0, BarotropicQGState(q=qd)).q.cpu().numpy()
+    assert dq.shape == q.shape
+    assert rel(dq, oqg.rhs(setup, q[None].astype(np.float64))[0]) < 2e-5
+    psi = model._invert_pv(qd).cpu().numpy()
+    ref_psi = oqg.invert_pv(setup, q[None].astype(np.float64))[0]
+    assert rel(psi, ref_psi) < 3e-6
+    assert np.abs(ref_psi[0]).max() > 0  # barotropic keeps psi's ghost ring
+    q1 = model.step(BarotropicQGState(q=qd), 4000.0).q.cpu().numpy()
+    assert rel(q1, oqg.ssprk3_step(setup, q[None].astype(np.float64), 4000.0)[0]) < 1e-6
