@@ -1,22 +1,24 @@
 // Dense sine-transform GEMMs on MFMA (gfx950).
 //
 // Every DST-I product of the Helmholtz inversion is written as
-//     O[r][c] = sum_k X[r][k] * S[c][k]          (TRANS_OUT == false)
-//     O[c][r] = sum_k X[r][k] * S[c][k]          (TRANS_OUT == true)
-// where X is a data plane read row-wise (k contiguous) and S is a row-major
-// sine matrix.  Because S is symmetric, X*S == X*S^T, and the transposed
-// output is obtained by feeding S to the MFMA A slot and X to the B slot, so
-// neither operand is ever transposed through LDS.
+//     O[a][b] = sum_k A[a][k] * B[b][k]
+// where one operand is a data plane X (rows = data rows, k contiguous) and the
+// other the symmetric sine matrix S (rows = modes, k contiguous).  With X in
+// the A slot the output is X*S; with S in the A slot it is (X*S)^T.  Both
+// operands are therefore read row-wise; nothing is transposed through LDS.
 //
-// Precisions:
-//   GP_FP32   v_mfma_f32_32x32x2_f32, FP32 operands (the plain step).
-//   GP_BF16X3 v_mfma_f32_32x32x16_bf16, operands split x = hi + lo in bf16,
-//             hi*hi + hi*lo + lo*hi accumulated in one FP32 accumulator.
-//   GP_BF16X1 hi*hi only (used for the refinement correction's inverse).
-// X arrives in FP32 and is split while it is staged to LDS; S is pre-split.
+// Precision is chosen per batch plane (mode):
+//   FP32 kernel   v_mfma_f32_32x32x2_f32 on FP32 planes (the plain step).
+//   BF16 kernel   v_mfma_f32_32x32x16_bf16 on operands split into bf16
+//                 pieces x = x0 + x1 (+ x2), FP32 accumulation:
+//                   NP = 1  x0*s0
+//                   NP = 3  x0*s0 + x0*s1 + x1*s0                 (~2^-17)
+//                   NP = 6  ... + x0*s2 + x1*s1 + x2*s0           (~2^-25)
+// Producers write the operands already split, in aligned zero-padded
+// layouts, so the k-loop is a plain 16-byte copy into double-buffered LDS.
 //
-// Block: 256 threads = 4 waves in a 2x2 layout, block tile 128x128, wave
-// tile 64x64 = 2x2 MFMA tiles of 32x32.
+// Block: 256 threads = 4 waves (2x2), block tile 128x128, wave tile 64x64 =
+// 2x2 MFMA tiles of 32x32.
 
 #include "common.h"
 #include "kernels.h"
@@ -25,92 +27,159 @@ namespace somax {
 
 namespace {
 
-constexpr int BM = 128;  // A-slot rows per block
-constexpr int BN = 128;  // B-slot rows per block
-constexpr int PAD16 = 8;  // bf16 padding per LDS row (16 bytes)
+constexpr int BM = 128;
+constexpr int BN = 128;
+constexpr int BK16 = 32;          // bf16 k-tile
+constexpr int LD16 = BK16 + 8;    // padded LDS row (80 bytes: conflict-free b128 reads)
+constexpr int TILE16 = BM * LD16; // elements per (buffer, piece) tile
+constexpr int BK32 = 16;          // fp32 k-tile
+constexpr int LD32 = BK32 + 4;    // padded LDS row (80 bytes)
+constexpr int TILE32 = BM * LD32;
 
-template <int PREC>
-struct GemmTraits;
-
-template <>
-struct GemmTraits<GP_FP32> {
-  static constexpr int BK = 16;
-};
-template <>
-struct GemmTraits<GP_BF16X3> {
-  static constexpr int BK = 32;
-};
-template <>
-struct GemmTraits<GP_BF16X1> {
-  static constexpr int BK = 32;
-};
-
-__device__ inline uint32_t pack_bf16x2(uint16_t lo16, uint16_t hi16) {
-  return (uint32_t)lo16 | ((uint32_t)hi16 << 16);
+// Epilogue: write one accumulator element; columns co >= RB are zero padding.
+__device__ inline void epi_store(const GemmArgs& a, long z, int ro, int co, float v, int opieces, float lamz) {
+  if (co >= a.RB) v = 0.0f;
+  else if (a.lamA) v = v / (a.lamA[ro] + a.lamB[co] - lamz);
+  const long idx = (long)ro * a.ldo + co;
+  if (opieces == 0) {
+    a.Of[z * a.o_batch + idx] = v;
+    return;
+  }
+  uint16_t* o = a.Ob + z * a.o_batch + idx;
+  const uint16_t p0 = f32_to_bf16_bits(v);
+  o[0] = p0;
+  if (opieces > 1) {
+    const float r1 = v - bf16_bits_to_f32(p0);
+    const uint16_t p1 = f32_to_bf16_bits(r1);
+    o[a.o_piece] = p1;
+    if (opieces > 2) o[2 * a.o_piece] = f32_to_bf16_bits(r1 - bf16_bits_to_f32(p1));
+  }
 }
 
-}  // namespace
+template <int NPROD>
+__device__ inline void mfma_products(const bf16x8 (&fa)[3][2], const bf16x8 (&fb)[3][2], floatx16 (&acc)[2][2]) {
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fa[0][i], fb[0][j], acc[i][j], 0, 0, 0);
+      if (NPROD >= 3) {
+        acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fa[0][i], fb[1][j], acc[i][j], 0, 0, 0);
+        acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fa[1][i], fb[0][j], acc[i][j], 0, 0, 0);
+      }
+      if (NPROD >= 6) {
+        acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fa[0][i], fb[2][j], acc[i][j], 0, 0, 0);
+        acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fa[1][i], fb[1][j], acc[i][j], 0, 0, 0);
+        acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fa[2][i], fb[0][j], acc[i][j], 0, 0, 0);
+      }
+    }
+}
 
-// ---------------------------------------------------------------------------
-// FP32 variant
-// ---------------------------------------------------------------------------
-template <bool TRANS_OUT, int EPI>
-__global__ __launch_bounds__(256) void dst_gemm_f32_kernel(GemmArgs a) {
-  constexpr int BK = GemmTraits<GP_FP32>::BK;
-  constexpr int LD = BK + 1;
-  __shared__ float sA[BM * LD];
-  __shared__ float sB[BN * LD];
+// FP64 running sum: each k-tile's FP32 MFMA partial is added to a double
+// accumulator and the MFMA accumulator restarts from 0, so the FP32 chain
+// per output is one k-tile long instead of the whole contraction.
+__device__ inline void flush64(floatx16 (&acc)[2][2], double (&sum)[2][2][16]) {
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int r = 0; r < 16; ++r) {
+        sum[i][j][r] += (double)acc[i][j][r];
+        acc[i][j][r] = 0.0f;
+      }
+}
 
+// Main loop of the bf16 kernel for NPROD products (block-uniform).
+// lds layout: [buf 2][operand A/B][piece NPC][row 128][LD16]
+template <int NPROD, bool ACC64>
+__device__ inline void bf16_mainloop(const GemmArgs& a, uint16_t* lds, long z, int a0, int b0, bool x_in_a,
+                                     floatx16 (&acc)[2][2], double (&sum)[2][2][16]) {
+  constexpr int NPC = NPROD == 1 ? 1 : (NPROD == 3 ? 2 : 3);
   const int tid = threadIdx.x;
   const int lane = tid & 63;
   const int wave = tid >> 6;
   const int wm = wave >> 1, wn = wave & 1;
+  const uint16_t* X = a.Xb + z * a.x_batch;
+  const uint16_t* S = a.Sb;
+  const int x0 = x_in_a ? a0 : b0;
+  const int s0 = x_in_a ? b0 : a0;
+  // staging: per operand per piece 128 rows x 32 bf16 = 512 chunks of 16 B, 2 per thread
+  const int cr = tid >> 2;        // 0..63
+  const int ck = (tid & 3) * 8;   // 0,8,16,24
+  const int xr0 = x0 + cr < a.xrows ? x0 + cr : a.xrows - 1;  // rows past R are never stored
+  const int xr1 = x0 + cr + 64 < a.xrows ? x0 + cr + 64 : a.xrows - 1;
+  const uint16_t* xg0 = X + (long)xr0 * a.ldx + ck;
+  const uint16_t* xg1 = X + (long)xr1 * a.ldx + ck;
+  const uint16_t* sg0 = S + (long)(s0 + cr) * a.lds + ck;
+  const uint16_t* sg1 = S + (long)(s0 + cr + 64) * a.lds + ck;
+  const int xo = (x_in_a ? 0 : 1) * NPC * TILE16;  // operand slot of X inside a buffer
+  const int so = (x_in_a ? 1 : 0) * NPC * TILE16;
+  const int st0 = cr * LD16 + ck, st1 = (cr + 64) * LD16 + ck;
+  u32x4 rx0[NPC], rx1[NPC], rs0[NPC], rs1[NPC];
 
+#define SOMAX_GLOAD(k0)                                                                   \
+  _Pragma("unroll") for (int p = 0; p < NPC; ++p) {                                       \
+    rx0[p] = *reinterpret_cast<const u32x4*>(xg0 + p * a.x_piece + (k0));                 \
+    rx1[p] = *reinterpret_cast<const u32x4*>(xg1 + p * a.x_piece + (k0));                 \
+    rs0[p] = *reinterpret_cast<const u32x4*>(sg0 + p * a.s_piece + (k0));                 \
+    rs1[p] = *reinterpret_cast<const u32x4*>(sg1 + p * a.s_piece + (k0));                 \
+  }
+#define SOMAX_SSTORE(buf)                                                                 \
+  _Pragma("unroll") for (int p = 0; p < NPC; ++p) {                                       \
+    uint16_t* tb = lds + (buf) * 2 * NPC * TILE16 + p * TILE16;                           \
+    *reinterpret_cast<u32x4*>(tb + xo + st0) = rx0[p];                                    \
+    *reinterpret_cast<u32x4*>(tb + xo + st1) = rx1[p];                                    \
+    *reinterpret_cast<u32x4*>(tb + so + st0) = rs0[p];                                    \
+    *reinterpret_cast<u32x4*>(tb + so + st1) = rs1[p];                                    \
+  }
+
+  const int nk = (a.K + BK16 - 1) / BK16;
+  SOMAX_GLOAD(0)
+  SOMAX_SSTORE(0)
+  __syncthreads();
+  const int fra = (wm * 64 + (lane & 31)) * LD16 + 8 * (lane >> 5);
+  const int frb = (wn * 64 + (lane & 31)) * LD16 + 8 * (lane >> 5);
+  for (int kt = 0; kt < nk; ++kt) {
+    const int buf = kt & 1;
+    if (kt + 1 < nk) {
+      SOMAX_GLOAD((kt + 1) * BK16)
+    }
+    const uint16_t* tA = lds + buf * 2 * NPC * TILE16;
+    const uint16_t* tB = tA + NPC * TILE16;
+#pragma unroll
+    for (int kk = 0; kk < BK16; kk += 16) {
+      bf16x8 fa[3][2], fb[3][2];
+#pragma unroll
+      for (int p = 0; p < NPC; ++p)
+#pragma unroll
+        for (int t = 0; t < 2; ++t) {
+          fa[p][t] = *reinterpret_cast<const bf16x8*>(tA + p * TILE16 + fra + t * 32 * LD16 + kk);
+          fb[p][t] = *reinterpret_cast<const bf16x8*>(tB + p * TILE16 + frb + t * 32 * LD16 + kk);
+        }
+      mfma_products<NPROD>(fa, fb, acc);
+    }
+    if (ACC64) flush64(acc, sum);
+    if (kt + 1 < nk) {
+      SOMAX_SSTORE(buf ^ 1)
+    }
+    __syncthreads();
+  }
+#undef SOMAX_GLOAD
+#undef SOMAX_SSTORE
+}
+
+}  // namespace
+
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) void dst_gemm_bf16_kernel(GemmArgs a) {
+  extern __shared__ __attribute__((aligned(16))) uint16_t lds16[];
   const long z = blockIdx.z;
-  const float* X = a.X + z * a.bsx;
-  float* O = a.O + z * a.bso;
-
-  // A slot / B slot row ranges
+  const bool lo = z < a.z_split;
+  const int np = lo ? a.np_lo : a.np_hi;
+  const int opieces = lo ? a.op_lo : a.op_hi;
   const int a0 = blockIdx.y * BM;
   const int b0 = blockIdx.x * BN;
-  // rows of X and S served by this block
-  const int x0 = TRANS_OUT ? b0 : a0;
-  const int s0 = TRANS_OUT ? a0 : b0;
-  float* sX = TRANS_OUT ? sB : sA;
-  float* sS = TRANS_OUT ? sA : sB;
-
-  // staging registers: X 128x16 and S 128x16 floats, 8 each per thread
-  float rx[8], rs[8];
-  const int ld_row = tid >> 2;        // 0..63
-  const int ld_k = (tid & 3) * 4;     // 0,4,8,12
-
-  auto prefetch = [&](int k0) {
-#pragma unroll
-    for (int p = 0; p < 2; ++p) {
-      const int r = x0 + ld_row + 64 * p;
-#pragma unroll
-      for (int e = 0; e < 4; ++e) {
-        const int k = k0 + ld_k + e;
-        rx[p * 4 + e] = (r < a.R && k < a.K) ? X[(long)r * a.ldx + k] : 0.0f;
-      }
-      const int c = s0 + ld_row + 64 * p;
-#pragma unroll
-      for (int e = 0; e < 4; ++e) {
-        const int k = k0 + ld_k + e;
-        rs[p * 4 + e] = (c < a.C && k < a.K) ? a.Sf[(long)c * a.lds + k] : 0.0f;
-      }
-    }
-  };
-  auto stage = [&]() {
-#pragma unroll
-    for (int p = 0; p < 2; ++p) {
-#pragma unroll
-      for (int e = 0; e < 4; ++e) {
-        sX[(ld_row + 64 * p) * LD + ld_k + e] = rx[p * 4 + e];
-        sS[(ld_row + 64 * p) * LD + ld_k + e] = rs[p * 4 + e];
-      }
-    }
-  };
+  const bool x_in_a = !a.trans;
 
   floatx16 acc[2][2];
 #pragma unroll
@@ -120,20 +189,112 @@ __global__ __launch_bounds__(256) void dst_gemm_f32_kernel(GemmArgs a) {
 #pragma unroll
       for (int r = 0; r < 16; ++r) acc[i][j][r] = 0.0f;
 
-  const int nk = (a.K + BK - 1) / BK;
-  prefetch(0);
-  for (int kt = 0; kt < nk; ++kt) {
-    __syncthreads();
-    stage();
-    __syncthreads();
-    if (kt + 1 < nk) prefetch((kt + 1) * BK);
+  double sum[2][2][16];
 #pragma unroll
-    for (int kk = 0; kk < BK; kk += 2) {
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int r = 0; r < 16; ++r) sum[i][j][r] = 0.0;
+  if (a.acc64) {
+    if (np == 1) bf16_mainloop<1, true>(a, lds16, z, a0, b0, x_in_a, acc, sum);
+    else if (np == 3) bf16_mainloop<3, true>(a, lds16, z, a0, b0, x_in_a, acc, sum);
+    else bf16_mainloop<6, true>(a, lds16, z, a0, b0, x_in_a, acc, sum);
+  } else {
+    if (np == 1) bf16_mainloop<1, false>(a, lds16, z, a0, b0, x_in_a, acc, sum);
+    else if (np == 3) bf16_mainloop<3, false>(a, lds16, z, a0, b0, x_in_a, acc, sum);
+    else bf16_mainloop<6, false>(a, lds16, z, a0, b0, x_in_a, acc, sum);
+  }
+
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int wm = wave >> 1, wn = wave & 1;
+  const float lamz = a.lamA ? a.lam[z] : 0.0f;
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int r = 0; r < 16; ++r) {
+        const int ro = a0 + wm * 64 + i * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+        const int co = b0 + wn * 64 + j * 32 + (lane & 31);
+        if (ro < a.RA && co < a.RBpad) epi_store(a, z, ro, co, (float)(sum[i][j][r] + (double)acc[i][j][r]), opieces, lamz);
+      }
+}
+
+// ---------------------------------------------------------------------------
+// FP32 kernel (plain step)
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 2))) void dst_gemm_f32_kernel(GemmArgs a) {
+  __shared__ __attribute__((aligned(16))) float lds[2 * 2 * TILE32];
+  const long z = blockIdx.z;
+  const int a0 = blockIdx.y * BM;
+  const int b0 = blockIdx.x * BN;
+  const bool x_in_a = !a.trans;
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wave = tid >> 6;
+  const int wm = wave >> 1, wn = wave & 1;
+  const float* X = a.Xf + z * a.x_batch;
+  const float* S = a.Sf;
+  const int x0 = x_in_a ? a0 : b0;
+  const int s0 = x_in_a ? b0 : a0;
+  // staging: 128 rows x 16 floats = 512 float4, 2 per thread per operand
+  const int cr = tid >> 2;
+  const int ck = (tid & 3) * 4;
+  const int xr0 = x0 + cr < a.xrows ? x0 + cr : a.xrows - 1;
+  const int xr1 = x0 + cr + 64 < a.xrows ? x0 + cr + 64 : a.xrows - 1;
+  const float* xg0 = X + (long)xr0 * a.ldx + ck;
+  const float* xg1 = X + (long)xr1 * a.ldx + ck;
+  const float* sg0 = S + (long)(s0 + cr) * a.lds + ck;
+  const float* sg1 = S + (long)(s0 + cr + 64) * a.lds + ck;
+  const int xo = x_in_a ? 0 : TILE32, so = x_in_a ? TILE32 : 0;
+  const int st0 = cr * LD32 + ck, st1 = (cr + 64) * LD32 + ck;
+  float4 rx0, rx1, rs0, rs1;
+  floatx16 acc[2][2];
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int r = 0; r < 16; ++r) acc[i][j][r] = 0.0f;
+
+  double sum[2][2][16];
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int r = 0; r < 16; ++r) sum[i][j][r] = 0.0;
+  const bool acc64 = a.acc64 != 0;
+  const int nk = (a.K + BK32 - 1) / BK32;
+  rx0 = *reinterpret_cast<const float4*>(xg0);
+  rx1 = *reinterpret_cast<const float4*>(xg1);
+  rs0 = *reinterpret_cast<const float4*>(sg0);
+  rs1 = *reinterpret_cast<const float4*>(sg1);
+  *reinterpret_cast<float4*>(lds + xo + st0) = rx0;
+  *reinterpret_cast<float4*>(lds + xo + st1) = rx1;
+  *reinterpret_cast<float4*>(lds + so + st0) = rs0;
+  *reinterpret_cast<float4*>(lds + so + st1) = rs1;
+  __syncthreads();
+  for (int kt = 0; kt < nk; ++kt) {
+    const int buf = kt & 1;
+    if (kt + 1 < nk) {
+      const int k1 = (kt + 1) * BK32;
+      rx0 = *reinterpret_cast<const float4*>(xg0 + k1);
+      rx1 = *reinterpret_cast<const float4*>(xg1 + k1);
+      rs0 = *reinterpret_cast<const float4*>(sg0 + k1);
+      rs1 = *reinterpret_cast<const float4*>(sg1 + k1);
+    }
+    const float* tA = lds + buf * 2 * TILE32;
+    const float* tB = tA + TILE32;
+#pragma unroll
+    for (int kk = 0; kk < BK32; kk += 2) {
       float fa[2], fb[2];
 #pragma unroll
       for (int t = 0; t < 2; ++t) {
-        fa[t] = sA[(wm * 64 + t * 32 + (lane & 31)) * LD + kk + (lane >> 5)];
-        fb[t] = sB[(wn * 64 + t * 32 + (lane & 31)) * LD + kk + (lane >> 5)];
+        fa[t] = tA[(wm * 64 + t * 32 + (lane & 31)) * LD32 + kk + (lane >> 5)];
+        fb[t] = tB[(wn * 64 + t * 32 + (lane & 31)) * LD32 + kk + (lane >> 5)];
       }
 #pragma unroll
       for (int i = 0; i < 2; ++i)
@@ -141,11 +302,18 @@ __global__ __launch_bounds__(256) void dst_gemm_f32_kernel(GemmArgs a) {
         for (int j = 0; j < 2; ++j)
           acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x2f32(fa[i], fb[j], acc[i][j], 0, 0, 0);
     }
+    if (acc64) flush64(acc, sum);
+    if (kt + 1 < nk) {
+      float* tb = lds + (buf ^ 1) * 2 * TILE32;
+      *reinterpret_cast<float4*>(tb + xo + st0) = rx0;
+      *reinterpret_cast<float4*>(tb + xo + st1) = rx1;
+      *reinterpret_cast<float4*>(tb + so + st0) = rs0;
+      *reinterpret_cast<float4*>(tb + so + st1) = rs1;
+    }
+    __syncthreads();
   }
 
-  // epilogue: accumulator row = A-slot row, col = B-slot row
-  const int RA = TRANS_OUT ? a.C : a.R;
-  const int RB = TRANS_OUT ? a.R : a.C;
+  const float lamz = a.lamA ? a.lam[z] : 0.0f;
 #pragma unroll
   for (int i = 0; i < 2; ++i)
 #pragma unroll
@@ -154,179 +322,30 @@ __global__ __launch_bounds__(256) void dst_gemm_f32_kernel(GemmArgs a) {
       for (int r = 0; r < 16; ++r) {
         const int ro = a0 + wm * 64 + i * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
         const int co = b0 + wn * 64 + j * 32 + (lane & 31);
-        if (ro < RA && co < RB) gemm_store<EPI>(a, O, z, ro, co, acc[i][j][r]);
+        if (ro < a.RA && co < a.RBpad) epi_store(a, z, ro, co, (float)(sum[i][j][r] + (double)acc[i][j][r]), 0, lamz);
       }
 }
 
-// ---------------------------------------------------------------------------
-// BF16 split variants
-// ---------------------------------------------------------------------------
-template <int PREC, bool TRANS_OUT, int EPI>
-__global__ __launch_bounds__(256) void dst_gemm_bf16_kernel(GemmArgs a) {
-  constexpr int BK = GemmTraits<PREC>::BK;  // 32
-  constexpr int LD = BK + PAD16;            // 40 bf16 = 80 bytes per row
-  constexpr bool SPLIT = (PREC == GP_BF16X3);
-  __shared__ __attribute__((aligned(16))) uint16_t sAh[BM * LD];
-  __shared__ __attribute__((aligned(16))) uint16_t sBh[BN * LD];
-  __shared__ __attribute__((aligned(16))) uint16_t sAl[SPLIT ? BM * LD : 8];
-  __shared__ __attribute__((aligned(16))) uint16_t sBl[SPLIT ? BN * LD : 8];
+size_t dst_gemm_lds_bytes(int max_np) {
+  const int npc = max_np == 1 ? 1 : (max_np == 3 ? 2 : 3);
+  return (size_t)2 * 2 * npc * TILE16 * sizeof(uint16_t);
+}
 
-  const int tid = threadIdx.x;
-  const int lane = tid & 63;
-  const int wave = tid >> 6;
-  const int wm = wave >> 1, wn = wave & 1;
-
-  const long z = blockIdx.z;
-  const float* X = a.X + z * a.bsx;
-  float* O = a.O + z * a.bso;
-
-  const int a0 = blockIdx.y * BM;
-  const int b0 = blockIdx.x * BN;
-  const int x0 = TRANS_OUT ? b0 : a0;
-  const int s0 = TRANS_OUT ? a0 : b0;
-  uint16_t* sXh = TRANS_OUT ? sBh : sAh;
-  uint16_t* sXl = TRANS_OUT ? sBl : sAl;
-  uint16_t* sSh = TRANS_OUT ? sAh : sBh;
-  uint16_t* sSl = TRANS_OUT ? sAl : sBl;
-
-  // X staging: 128 rows x 32 k floats; thread -> (row = tid>>3 + 32p, k = (tid&7)*4 .. +3)
-  float rx[4][4];
-  const int xr = tid >> 3;
-  const int xk = (tid & 7) * 4;
-  // S staging: 128 rows x 32 k bf16 = 4 chunks of 8 per row; thread -> (row = tid>>2 + 64p, chunk = tid&3)
-  uint4 rsh[2], rsl[2];
-  const int sr = tid >> 2;
-  const int sk = (tid & 3) * 8;
-
-  auto prefetch = [&](int k0) {
-#pragma unroll
-    for (int p = 0; p < 4; ++p) {
-      const int r = x0 + xr + 32 * p;
-#pragma unroll
-      for (int e = 0; e < 4; ++e) {
-        const int k = k0 + xk + e;
-        rx[p][e] = (r < a.R && k < a.K) ? X[(long)r * a.ldx + k] : 0.0f;
-      }
-    }
-#pragma unroll
-    for (int p = 0; p < 2; ++p) {
-      const int c = s0 + sr + 64 * p;
-      // S is zero padded to a multiple of BK columns and of BM rows.
-      const long off = (long)c * a.lds + k0 + sk;
-      rsh[p] = *reinterpret_cast<const uint4*>(a.Sh + off);
-      if (SPLIT) rsl[p] = *reinterpret_cast<const uint4*>(a.Sl + off);
-    }
-  };
-  auto stage = [&]() {
-#pragma unroll
-    for (int p = 0; p < 4; ++p) {
-      uint16_t h[4], l[4];
-#pragma unroll
-      for (int e = 0; e < 4; ++e) {
-        h[e] = f32_to_bf16_bits(rx[p][e]);
-        if (SPLIT) l[e] = f32_to_bf16_bits(rx[p][e] - bf16_bits_to_f32(h[e]));
-      }
-      uint2 vh = make_uint2(pack_bf16x2(h[0], h[1]), pack_bf16x2(h[2], h[3]));
-      *reinterpret_cast<uint2*>(sXh + (xr + 32 * p) * LD + xk) = vh;
-      if (SPLIT) {
-        uint2 vl = make_uint2(pack_bf16x2(l[0], l[1]), pack_bf16x2(l[2], l[3]));
-        *reinterpret_cast<uint2*>(sXl + (xr + 32 * p) * LD + xk) = vl;
-      }
-    }
-#pragma unroll
-    for (int p = 0; p < 2; ++p) {
-      *reinterpret_cast<uint4*>(sSh + (sr + 64 * p) * LD + sk) = rsh[p];
-      if (SPLIT) *reinterpret_cast<uint4*>(sSl + (sr + 64 * p) * LD + sk) = rsl[p];
-    }
-  };
-
-  floatx16 acc[2][2];
-#pragma unroll
-  for (int i = 0; i < 2; ++i)
-#pragma unroll
-    for (int j = 0; j < 2; ++j)
-#pragma unroll
-      for (int r = 0; r < 16; ++r) acc[i][j][r] = 0.0f;
-
-  const int nk = (a.K + BK - 1) / BK;
-  prefetch(0);
-  for (int kt = 0; kt < nk; ++kt) {
-    __syncthreads();
-    stage();
-    __syncthreads();
-    if (kt + 1 < nk) prefetch((kt + 1) * BK);
-#pragma unroll
-    for (int kk = 0; kk < BK; kk += 16) {
-      bf16x8 ah[2], bh[2], al[2], bl[2];
-#pragma unroll
-      for (int t = 0; t < 2; ++t) {
-        const int ra = (wm * 64 + t * 32 + (lane & 31)) * LD + kk + 8 * (lane >> 5);
-        const int rb = (wn * 64 + t * 32 + (lane & 31)) * LD + kk + 8 * (lane >> 5);
-        ah[t] = *reinterpret_cast<const bf16x8*>(sAh + ra);
-        bh[t] = *reinterpret_cast<const bf16x8*>(sBh + rb);
-        if (SPLIT) {
-          al[t] = *reinterpret_cast<const bf16x8*>(sAl + ra);
-          bl[t] = *reinterpret_cast<const bf16x8*>(sBl + rb);
-        }
-      }
-#pragma unroll
-      for (int i = 0; i < 2; ++i)
-#pragma unroll
-        for (int j = 0; j < 2; ++j) {
-          acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[i], bh[j], acc[i][j], 0, 0, 0);
-          if (SPLIT) {
-            acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[i], bl[j], acc[i][j], 0, 0, 0);
-            acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al[i], bh[j], acc[i][j], 0, 0, 0);
-          }
-        }
-    }
+void launch_dst_gemm(const GemmArgs& a, int batch, hipStream_t st) {
+  dim3 grid((a.RB + BN - 1) / BN, (a.RA + BM - 1) / BM, batch);
+  if (a.Xf) {
+    hipLaunchKernelGGL(dst_gemm_f32_kernel, grid, dim3(256), 0, st, a);
+    return;
   }
-
-  const int RA = TRANS_OUT ? a.C : a.R;
-  const int RB = TRANS_OUT ? a.R : a.C;
-#pragma unroll
-  for (int i = 0; i < 2; ++i)
-#pragma unroll
-    for (int j = 0; j < 2; ++j)
-#pragma unroll
-      for (int r = 0; r < 16; ++r) {
-        const int ro = a0 + wm * 64 + i * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
-        const int co = b0 + wn * 64 + j * 32 + (lane & 31);
-        if (ro < RA && co < RB) gemm_store<EPI>(a, O, z, ro, co, acc[i][j][r]);
-      }
-}
-
-// ---------------------------------------------------------------------------
-// host launcher
-// ---------------------------------------------------------------------------
-template <int PREC, bool T, int EPI>
-static void launch_one(const GemmArgs& a, int batch, hipStream_t st) {
-  const int RA = T ? a.C : a.R;
-  const int RB = T ? a.R : a.C;
-  dim3 grid((RB + BN - 1) / BN, (RA + BM - 1) / BM, batch);
-  if (PREC == GP_FP32)
-    hipLaunchKernelGGL((dst_gemm_f32_kernel<T, EPI>), grid, dim3(256), 0, st, a);
-  else
-    hipLaunchKernelGGL((dst_gemm_bf16_kernel<PREC, T, EPI>), grid, dim3(256), 0, st, a);
-}
-
-template <int PREC>
-static void launch_prec(const GemmArgs& a, bool trans, int epi, int batch, hipStream_t st) {
-  if (trans) {
-    if (epi == GE_STORE) launch_one<PREC, true, GE_STORE>(a, batch, st);
-    else launch_one<PREC, true, GE_FINAL>(a, batch, st);
-  } else {
-    if (epi == GE_STORE) launch_one<PREC, false, GE_STORE>(a, batch, st);
-    else launch_one<PREC, false, GE_FINAL>(a, batch, st);
+  static bool lds_attr = false;
+  if (!lds_attr) {
+    // up to 120 KiB of the 160 KiB LDS of a gfx950 CU (3 pieces, double buffered)
+    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(dst_gemm_bf16_kernel),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)dst_gemm_lds_bytes(6));
+    lds_attr = true;
   }
-}
-
-void launch_dst_gemm(const GemmArgs& a, int prec, bool trans, int epi, int batch, hipStream_t st) {
-  switch (prec) {
-    case GP_FP32: launch_prec<GP_FP32>(a, trans, epi, batch, st); break;
-    case GP_BF16X3: launch_prec<GP_BF16X3>(a, trans, epi, batch, st); break;
-    default: launch_prec<GP_BF16X1>(a, trans, epi, batch, st); break;
-  }
+  const int max_np = a.np_lo > a.np_hi ? a.np_lo : a.np_hi;
+  hipLaunchKernelGGL(dst_gemm_bf16_kernel, grid, dim3(256), dst_gemm_lds_bytes(max_np), st, a);
 }
 
 }  // namespace somax

@@ -24,7 +24,7 @@ __global__ __launch_bounds__(256) void qg_stage_arakawa_kernel(StageArgs a) {
   __shared__ float sq[(TY + 4) * QW];
   __shared__ float sp[(TY + 2) * PW];
   __shared__ float sl[BIHARM ? (TY + 2) * PW : 1];
-  __shared__ float sb[TY + 4];
+  __shared__ float sdb[TY + 4];
 
   const int tid = threadIdx.x;
   const int layer = blockIdx.z;
@@ -54,7 +54,7 @@ __global__ __launch_bounds__(256) void qg_stage_arakawa_kernel(StageArgs a) {
   }
   if (tid < TY + 4) {
     const int gy = y0 - 2 + tid;
-    sb[tid] = (gy >= 0 && gy < Ny) ? a.beta_row[gy] : 0.0f;
+    sdb[tid] = (gy >= 0 && gy < Ny) ? a.dbeta_row[gy] : 0.0f;
   }
   __syncthreads();
 
@@ -86,16 +86,21 @@ __global__ __launch_bounds__(256) void qg_stage_arakawa_kernel(StageArgs a) {
     if (gy >= 1 && gy <= Ny - 2 && gx >= 1 && gx <= Nx - 2) {
       const float* q = &sq[(ly + 2) * QW + tx + 2];
       const float* p = &sp[(ly + 1) * PW + tx + 1];
-      const float bN = sb[ly + 3], bC = sb[ly + 2], bS = sb[ly + 1];
-      // total PV q + beta*y around the point (Arakawa reads the ghost ring)
-      const float tE = q[1] + bC, tW = q[-1] + bC, tN = q[QW] + bN, tS = q[-QW] + bS;
-      const float tNE = q[QW + 1] + bN, tNW = q[QW - 1] + bN, tSE = q[-QW + 1] + bS, tSW = q[-QW - 1] + bS;
+      // J(psi, q + b) = J(psi, q) + J(psi, b), b = beta*(y - y0) a function of
+      // the row only.  Adding b to q first would cancel most of q's bits in
+      // the differences; for b the Arakawa sum reduces exactly to
+      //   2 (pE - pW)(bN - bS) + dN (pNE - pNW) + dS (pSE - pSW)
+      // with the row increments dN = bN - bC, dS = bC - bS.
+      const float dN = sdb[ly + 2], dS = sdb[ly + 1];
+      const float tE = q[1], tW = q[-1], tN = q[QW], tS = q[-QW];
+      const float tNE = q[QW + 1], tNW = q[QW - 1], tSE = q[-QW + 1], tSW = q[-QW - 1];
       const float pE = p[1], pW = p[-1], pN = p[PW], pS = p[-PW];
       const float pNE = p[PW + 1], pNW = p[PW - 1], pSE = p[-PW + 1], pSW = p[-PW - 1];
       const float jpp = (pE - pW) * (tN - tS) - (pN - pS) * (tE - tW);
       const float jpx = pE * (tNE - tSE) - pW * (tNW - tSW) - pN * (tNE - tNW) + pS * (tSE - tSW);
       const float jxp = tN * (pNE - pNW) - tS * (pSE - pSW) - tE * (pNE - pSE) + tW * (pNW - pSW);
-      rhs = -(jpp + jpx + jxp) * a.iarakawa;
+      const float jb = 2.0f * (pE - pW) * (dN + dS) + dN * (pNE - pNW) + dS * (pSE - pSW);
+      rhs = -((jpp + jpx + jxp) + jb) * a.iarakawa;
       if (a.nu != 0.0f) rhs += a.nu * ((q[1] - 2.0f * q[0] + q[-1]) * a.idx2 + (q[QW] - 2.0f * q[0] + q[-QW]) * a.idy2);
       if (BIHARM) {
         const float* l = &sl[(ly + 1) * PW + tx + 1];
@@ -105,9 +110,17 @@ __global__ __launch_bounds__(256) void qg_stage_arakawa_kernel(StageArgs a) {
     }
     rhs += wc * a.wind_row[gy];
     const long idx = off + (long)gy * Nx + gx;
-    float o = a.c2 * rhs;
-    if (a.c1 != 0.0f) o += a.c1 * a.qs[idx];
-    if (a.c0 != 0.0f) o += a.c0 * a.qn[idx];
+    // Stage combination qn + c1 (qs - qn) + c2 rhs: the FP32 weights of the
+    // Shu-Osher form (1/3 + 2/3 != 1 in FP32) would otherwise scale q by
+    // 1 + 3e-8 per step.  Without qn: c1 qs + c2 rhs.
+    float o;
+    if (a.qn) {
+      const float qn = a.qn[idx];
+      o = qn + a.c1 * (a.qs[idx] - qn) + a.c2 * rhs;
+    } else {
+      o = a.c2 * rhs;
+      if (a.c1 != 0.0f) o += a.c1 * a.qs[idx];
+    }
     a.out[idx] = o;
   }
 }
@@ -143,59 +156,106 @@ void launch_zero_ghost(float* f, int nl, int Ny, int Nx, hipStream_t st) {
 }
 
 // ---------------------------------------------------------------------------
-// spectral operator (in place)
-// ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void spectral_kernel(SpectralArgs a) {
-  const int j = blockIdx.x * blockDim.x + threadIdx.x;  // y mode
-  const int i = blockIdx.y;                              // x mode
-  if (j >= a.nym) return;
-  const float lxy = a.lamx[i] + a.lamy[j];
-  float z[MAX_LAYERS], t[MAX_LAYERS];
-  const long base = (long)i * a.ld + j;
-  for (int k = 0; k < a.nl; ++k) z[k] = a.Z[k * a.bs + base];
-  for (int m = 0; m < a.nl; ++m) {
-    float s = 0.0f;
-    for (int k = 0; k < a.nl; ++k) s += a.cl2m[m * MAX_LAYERS + k] * z[k];
-    t[m] = s / (lxy - a.lam[m]);
-  }
-  for (int l = 0; l < a.nl; ++l) {
-    float s = 0.0f;
-    for (int m = 0; m < a.nl; ++m) s += a.cm2l[l * MAX_LAYERS + m] * t[m];
-    a.Z[l * a.bs + base] = s;
-  }
-}
-
-void launch_spectral(const SpectralArgs& a, hipStream_t st) {
-  dim3 grid((a.nym + 255) / 256, a.nxm, 1);
-  hipLaunchKernelGGL(spectral_kernel, grid, dim3(256), 0, st, a);
-}
-
-// ---------------------------------------------------------------------------
 // Helmholtz residual for iterative refinement
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void residual_kernel(ResidualArgs a) {
-  const int x = blockIdx.x * blockDim.x + threadIdx.x;
-  const int y = blockIdx.y;
-  if (x >= a.Nx) return;
-  float pc[MAX_LAYERS];
-  for (int k = 0; k < a.nl; ++k) pc[k] = a.psi[k * a.p_bs + (long)y * a.p_ld + x];
-  const bool inner = (y >= 1 && y <= a.Ny - 2 && x >= 1 && x <= a.Nx - 2);
-  for (int l = 0; l < a.nl; ++l) {
-    const float* p = a.psi + l * a.p_bs + (long)y * a.p_ld + x;
-    const float e = (x + 1 < a.Nx) ? p[1] : 0.0f;
-    const float w = (x > 0) ? p[-1] : 0.0f;
-    const float n = (y + 1 < a.Ny) ? p[a.p_ld] : 0.0f;
-    const float s = (y > 0) ? p[-a.p_ld] : 0.0f;
-    float v = (inner || !a.apply_bc) ? a.q[l * a.q_bs + (long)y * a.q_ld + x] : 0.0f;
-    v -= (e - 2.0f * pc[l] + w) * a.idx2 + (n - 2.0f * pc[l] + s) * a.idy2;
-    for (int k = 0; k < a.nl; ++k) v += a.B[l * MAX_LAYERS + k] * pc[k];
-    a.r[l * a.r_bs + (long)y * a.r_ld + x] = v;
+namespace {
+__device__ inline void store_pieces(uint16_t* o, long piece, int n, float v) {
+  const uint16_t p0 = f32_to_bf16_bits(v);
+  o[0] = p0;
+  if (n > 1) {
+    const float r1 = v - bf16_bits_to_f32(p0);
+    const uint16_t p1 = f32_to_bf16_bits(r1);
+    o[piece] = p1;
+    if (n > 2) o[2 * piece] = f32_to_bf16_bits(r1 - bf16_bits_to_f32(p1));
   }
+}
+}  // namespace
+
+__global__ __launch_bounds__(256) void residual_kernel(ResidualArgs a) {
+  const int x = blockIdx.x * blockDim.x + threadIdx.x;  // up to op_kpad
+  const int y = blockIdx.y;
+  const int m = blockIdx.z;
+  if (x >= a.op_kpad) return;
+  uint16_t* o = a.r + m * a.op_batch + (long)y * a.op_ld + x;
+  if (x >= a.Nx) {
+    store_pieces(o, a.op_piece, 2, 0.0f);
+    return;
+  }
+  const long s = m * a.plane + (long)y * a.Nx + x;
+  const float* p = a.psi + s;
+  const float c = p[0];
+  const float e = (x + 1 < a.Nx) ? p[1] : 0.0f;
+  const float w = (x > 0) ? p[-1] : 0.0f;
+  const float n = (y + 1 < a.Ny) ? p[a.Nx] : 0.0f;
+  const float so = (y > 0) ? p[-a.Nx] : 0.0f;
+  const float r = a.q[s] - ((e - 2.0f * c + w) * a.idx2 + (n - 2.0f * c + so) * a.idy2) + a.lam[m] * c;
+  store_pieces(o, a.op_piece, 2, r);
 }
 
 void launch_residual(const ResidualArgs& a, hipStream_t st) {
-  dim3 grid((a.Nx + 255) / 256, a.Ny, 1);
+  dim3 grid((a.op_kpad + 255) / 256, a.Ny, a.nl);
   hipLaunchKernelGGL(residual_kernel, grid, dim3(256), 0, st, a);
+}
+
+// ---------------------------------------------------------------------------
+// layer -> mode projection in FP64, writing the first GEMM operand
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void to_modal_kernel(ModalArgs a) {
+  const int x = blockIdx.x * blockDim.x + threadIdx.x;  // state column
+  const int y = blockIdx.y;
+  const int off = a.apply_bc ? 1 : 0;
+  const int rows = a.Ny - 2 * off, cols = a.Nx - 2 * off;
+  const int ox = x - off, oy = y - off;  // operand coordinates
+  const bool inside = x < a.Nx;
+  const bool wop = oy >= 0 && oy < rows && ox >= 0 && ox < a.op_kpad;
+  if (!inside && !wop) return;
+  const long s = (long)y * a.Nx + x;
+  const bool ghost = x == 0 || y == 0 || x == a.Nx - 1 || y == a.Ny - 1;
+  double qk[MAX_LAYERS];
+  for (int k = 0; k < a.nl; ++k)
+    qk[k] = (!inside || (ghost && a.apply_bc)) ? 0.0 : (double)a.q[k * a.plane + s];
+  for (int m = 0; m < a.nl; ++m) {
+    double acc = 0.0;
+    for (int k = 0; k < a.nl; ++k) acc += a.cl2m[m * MAX_LAYERS + k] * qk[k];
+    const float v = (float)acc;
+    if (inside) a.qm[m * a.plane + s] = v;
+    if (wop) {
+      const float w = ox < cols ? v : 0.0f;  // zero padding past the operand width
+      const long o = m * a.op_batch + (long)oy * a.op_ld + ox;
+      if (a.op_pieces == 0) static_cast<float*>(a.op)[o] = w;
+      else store_pieces(static_cast<uint16_t*>(a.op) + o, a.op_piece, a.op_pieces, w);
+    }
+  }
+}
+
+void launch_to_modal(const ModalArgs& a, hipStream_t st) {
+  const int off = a.apply_bc ? 1 : 0;
+  const int width = a.Nx > a.op_kpad + off ? a.Nx : a.op_kpad + off;
+  dim3 grid((width + 255) / 256, a.Ny, 1);
+  hipLaunchKernelGGL(to_modal_kernel, grid, dim3(256), 0, st, a);
+}
+
+// ---------------------------------------------------------------------------
+// mode -> layer projection of psi
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void to_layer_kernel(LayerArgs a) {
+  const int x = blockIdx.x * blockDim.x + threadIdx.x;
+  const int y = blockIdx.y;
+  if (x >= a.Nx) return;
+  const long s = (long)y * a.Nx + x;
+  const bool ghost = a.mask && (x == 0 || y == 0 || x == a.Nx - 1 || y == a.Ny - 1);
+  float pm[MAX_LAYERS];
+  for (int m = 0; m < a.nl; ++m) pm[m] = a.p[m * a.plane + s] + (a.d ? a.d[m * a.plane + s] : 0.0f);
+  for (int l = 0; l < a.nl; ++l) {
+    float v = 0.0f;
+    for (int m = 0; m < a.nl; ++m) v += a.cm2l[l * MAX_LAYERS + m] * pm[m];
+    a.psi[l * a.plane + s] = ghost ? 0.0f : v;
+  }
+}
+
+void launch_to_layer(const LayerArgs& a, hipStream_t st) {
+  dim3 grid((a.Nx + 255) / 256, a.Ny, 1);
+  hipLaunchKernelGGL(to_layer_kernel, grid, dim3(256), 0, st, a);
 }
 
 }  // namespace somax
