@@ -176,10 +176,6 @@ int somaxhip_qg_plan_create(const somaxhip_qg_desc* d, somaxhip_qg_plan* out) {
     set_error("somaxhip_qg_plan_create: unknown precision");
     return SOMAXHIP_ERR_ARG;
   }
-  if (d->advection == SOMAXHIP_ADV_WENO) {
-    set_error("somaxhip_qg_plan_create: WENO advection is not available in this build");
-    return SOMAXHIP_ERR_ARG;
-  }
   if (d->mass_correction) {
     set_error("somaxhip_qg_plan_create: mass correction is not available in this build");
     return SOMAXHIP_ERR_ARG;

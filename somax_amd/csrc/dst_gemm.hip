@@ -15,10 +15,12 @@
 //                   NP = 3  x0*s0 + x0*s1 + x1*s0                 (~2^-17)
 //                   NP = 6  ... + x0*s2 + x1*s1 + x2*s0           (~2^-25)
 // Producers write the operands already split, in aligned zero-padded
-// layouts, so the k-loop is a plain 16-byte copy into double-buffered LDS.
+// layouts.  With acc64 the FP32 MFMA partial of every few k-tiles is added to
+// an FP64 running sum, so the FP32 rounding chain per output stays short.
 //
 // Block: 256 threads = 4 waves (2x2), block tile 128x128, wave tile 64x64 =
-// 2x2 MFMA tiles of 32x32.
+// 2x2 MFMA tiles of 32x32.  Grids are 1-D and remapped so that each XCD
+// works on a contiguous run of tiles (shared operand rows in its L2).
 
 #include "common.h"
 #include "kernels.h"
@@ -29,9 +31,6 @@ namespace {
 
 constexpr int BM = 128;
 constexpr int BN = 128;
-constexpr int BK16 = 32;          // bf16 k-tile
-constexpr int LD16 = BK16 + 8;    // padded LDS row (80 bytes: conflict-free b128 reads)
-constexpr int TILE16 = BM * LD16; // elements per (buffer, piece) tile
 constexpr int BK32 = 16;          // fp32 k-tile
 constexpr int LD32 = BK32 + 4;    // padded LDS row (80 bytes)
 constexpr int TILE32 = BM * LD32;
@@ -56,6 +55,66 @@ __device__ inline void epi_store(const GemmArgs& a, long z, int ro, int co, floa
   }
 }
 
+// FP64 running sum: the MFMA partial is added to a double accumulator and
+// the MFMA accumulator restarts from 0.
+__device__ inline void flush64(floatx16 (&acc)[2][2], double (&sum)[2][2][16]) {
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int r = 0; r < 16; ++r) {
+        sum[i][j][r] += (double)acc[i][j][r];
+        acc[i][j][r] = 0.0f;
+      }
+}
+
+__device__ inline void zero_acc(floatx16 (&acc)[2][2], double (&sum)[2][2][16]) {
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int r = 0; r < 16; ++r) {
+        acc[i][j][r] = 0.0f;
+        sum[i][j][r] = 0.0;
+      }
+}
+
+// Linear block -> (plane, A-tile, B-tile).  Blocks are dealt to the 8 XCDs
+// round-robin; this remap hands each XCD a contiguous run of tiles (same
+// plane, neighbouring A rows, all B tiles) so they share its L2.
+__device__ inline void tile_of_block(int GA, int GB, int& z, int& ta, int& tb) {
+  const int T = gridDim.x;
+  const int b = blockIdx.x;
+  const int xcd = b & 7, slot = b >> 3;
+  const int per = T >> 3, extra = T & 7;
+  const int t = xcd * per + (xcd < extra ? xcd : extra) + slot;
+  z = t / (GA * GB);
+  const int rem = t - z * GA * GB;
+  ta = rem / GB;
+  tb = rem - ta * GB;
+}
+
+__device__ inline void store_tile(const GemmArgs& a, long z, int a0, int b0, const floatx16 (&acc)[2][2],
+                                  const double (&sum)[2][2][16], int opieces) {
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int wm = wave >> 1, wn = wave & 1;
+  const float lamz = a.lamA ? a.lam[z] : 0.0f;
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int r = 0; r < 16; ++r) {
+        const int ro = a0 + wm * 64 + i * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+        const int co = b0 + wn * 64 + j * 32 + (lane & 31);
+        if (ro < a.RA && co < a.RBpad)
+          epi_store(a, z, ro, co, (float)(sum[i][j][r] + (double)acc[i][j][r]), opieces, lamz);
+      }
+}
+
 template <int NPROD>
 __device__ inline void mfma_products(const bf16x8 (&fa)[3][2], const bf16x8 (&fb)[3][2], floatx16 (&acc)[2][2]) {
 #pragma unroll
@@ -75,27 +134,39 @@ __device__ inline void mfma_products(const bf16x8 (&fa)[3][2], const bf16x8 (&fb
     }
 }
 
-// FP64 running sum: each k-tile's FP32 MFMA partial is added to a double
-// accumulator and the MFMA accumulator restarts from 0, so the FP32 chain
-// per output is one k-tile long instead of the whole contraction.
-__device__ inline void flush64(floatx16 (&acc)[2][2], double (&sum)[2][2][16]) {
-#pragma unroll
-  for (int i = 0; i < 2; ++i)
-#pragma unroll
-    for (int j = 0; j < 2; ++j)
-#pragma unroll
-      for (int r = 0; r < 16; ++r) {
-        sum[i][j][r] += (double)acc[i][j][r];
-        acc[i][j][r] = 0.0f;
-      }
+// ---------------------------------------------------------------------------
+// BF16 kernel
+//
+// Tiles of 128 rows x 32 k per operand piece live in LDS unpadded (64 B per
+// row) with the 16-byte chunk index XOR-swizzled by swz(row), which keeps the
+// b128 fragment reads of 8 or 16 consecutive rows on distinct banks.  They are
+// filled by global_load_lds (16 bytes per lane straight into LDS, no staging
+// registers); each wave instruction fills 16 rows (1 KiB, linear, so the
+// writes are conflict free).  NSTAGE buffers keep NSTAGE-1 k-tiles in flight.
+// ---------------------------------------------------------------------------
+__device__ inline int swz(int r) { return ((r >> 1) ^ (r >> 2)) & 3; }
+
+template <int NPROD>
+struct Bf16Cfg {
+  static constexpr int NPC = NPROD == 1 ? 1 : (NPROD == 3 ? 2 : 3);
+  static constexpr int NSTAGE = NPC == 3 ? 3 : 4;
+  static constexpr int TILE = 128 * 32;         // bf16 elements per operand piece
+  static constexpr int STAGE = 2 * NPC * TILE;  // A pieces then B pieces
+  static constexpr int LPT = 2 * NPC * 2;       // global_load_lds per thread per k-tile
+  static constexpr int FLUSH = 2;               // k-tiles between FP64 flushes
+};
+
+template <int N>
+__device__ inline void wait_vmcnt() {
+  // s_waitcnt vmcnt(N); expcnt and lgkmcnt fields at their maxima (no wait)
+  __builtin_amdgcn_s_waitcnt((N & 0xF) | (0x7 << 4) | (0xF << 8) | ((N >> 4) << 14));
 }
 
-// Main loop of the bf16 kernel for NPROD products (block-uniform).
-// lds layout: [buf 2][operand A/B][piece NPC][row 128][LD16]
 template <int NPROD, bool ACC64>
 __device__ inline void bf16_mainloop(const GemmArgs& a, uint16_t* lds, long z, int a0, int b0, bool x_in_a,
                                      floatx16 (&acc)[2][2], double (&sum)[2][2][16]) {
-  constexpr int NPC = NPROD == 1 ? 1 : (NPROD == 3 ? 2 : 3);
+  using C = Bf16Cfg<NPROD>;
+  constexpr int NPC = C::NPC;
   const int tid = threadIdx.x;
   const int lane = tid & 63;
   const int wave = tid >> 6;
@@ -104,98 +175,103 @@ __device__ inline void bf16_mainloop(const GemmArgs& a, uint16_t* lds, long z, i
   const uint16_t* S = a.Sb;
   const int x0 = x_in_a ? a0 : b0;
   const int s0 = x_in_a ? b0 : a0;
-  // staging: per operand per piece 128 rows x 32 bf16 = 512 chunks of 16 B, 2 per thread
-  const int cr = tid >> 2;        // 0..63
-  const int ck = (tid & 3) * 8;   // 0,8,16,24
-  const int xr0 = x0 + cr < a.xrows ? x0 + cr : a.xrows - 1;  // rows past R are never stored
-  const int xr1 = x0 + cr + 64 < a.xrows ? x0 + cr + 64 : a.xrows - 1;
-  const uint16_t* xg0 = X + (long)xr0 * a.ldx + ck;
-  const uint16_t* xg1 = X + (long)xr1 * a.ldx + ck;
-  const uint16_t* sg0 = S + (long)(s0 + cr) * a.lds + ck;
-  const uint16_t* sg1 = S + (long)(s0 + cr + 64) * a.lds + ck;
-  const int xo = (x_in_a ? 0 : 1) * NPC * TILE16;  // operand slot of X inside a buffer
-  const int so = (x_in_a ? 1 : 0) * NPC * TILE16;
-  const int st0 = cr * LD16 + ck, st1 = (cr + 64) * LD16 + ck;
-  u32x4 rx0[NPC], rx1[NPC], rs0[NPC], rs1[NPC];
+  const int xslot = x_in_a ? 0 : NPC;  // piece-tile index of X inside a stage
+  const int sslot = x_in_a ? NPC : 0;
 
-#define SOMAX_GLOAD(k0)                                                                   \
-  _Pragma("unroll") for (int p = 0; p < NPC; ++p) {                                       \
-    rx0[p] = *reinterpret_cast<const u32x4*>(xg0 + p * a.x_piece + (k0));                 \
-    rx1[p] = *reinterpret_cast<const u32x4*>(xg1 + p * a.x_piece + (k0));                 \
-    rs0[p] = *reinterpret_cast<const u32x4*>(sg0 + p * a.s_piece + (k0));                 \
-    rs1[p] = *reinterpret_cast<const u32x4*>(sg1 + p * a.s_piece + (k0));                 \
-  }
-#define SOMAX_SSTORE(buf)                                                                 \
-  _Pragma("unroll") for (int p = 0; p < NPC; ++p) {                                       \
-    uint16_t* tb = lds + (buf) * 2 * NPC * TILE16 + p * TILE16;                           \
-    *reinterpret_cast<u32x4*>(tb + xo + st0) = rx0[p];                                    \
-    *reinterpret_cast<u32x4*>(tb + xo + st1) = rx1[p];                                    \
-    *reinterpret_cast<u32x4*>(tb + so + st0) = rs0[p];                                    \
-    *reinterpret_cast<u32x4*>(tb + so + st1) = rs1[p];                                    \
-  }
-
-  const int nk = (a.K + BK16 - 1) / BK16;
-  SOMAX_GLOAD(0)
-  SOMAX_SSTORE(0)
-  __syncthreads();
-  const int fra = (wm * 64 + (lane & 31)) * LD16 + 8 * (lane >> 5);
-  const int frb = (wn * 64 + (lane & 31)) * LD16 + 8 * (lane >> 5);
-  for (int kt = 0; kt < nk; ++kt) {
-    const int buf = kt & 1;
-    if (kt + 1 < nk) {
-      SOMAX_GLOAD((kt + 1) * BK16)
-    }
-    const uint16_t* tA = lds + buf * 2 * NPC * TILE16;
-    const uint16_t* tB = tA + NPC * TILE16;
+  // this lane's two rows of each 128-row tile and the swizzled source chunk
+  const uint16_t* xsrc[2];
+  const uint16_t* ssrc[2];
+  int ldst[2];
 #pragma unroll
-    for (int kk = 0; kk < BK16; kk += 16) {
+  for (int j = 0; j < 2; ++j) {
+    const int r0 = (wave * 2 + j) * 16;  // first row filled by this wave instruction
+    const int row = r0 + (lane >> 2);
+    const int chunk = (lane & 3) ^ swz(row);
+    const int xr = x0 + row < a.xrows ? x0 + row : a.xrows - 1;  // rows past R are never stored
+    xsrc[j] = X + (long)xr * a.ldx + chunk * 8;
+    ssrc[j] = S + (long)(s0 + row) * a.lds + chunk * 8;
+    ldst[j] = r0 * 32;  // wave-uniform LDS element offset of the 16-row block
+  }
+
+#define SOMAX_ISSUE(KT, STG)                                                                         \
+  {                                                                                                  \
+    uint16_t* sb_ = lds + (STG) * C::STAGE;                                                          \
+    const int k0_ = (KT) * 32;                                                                       \
+    _Pragma("unroll") for (int p = 0; p < NPC; ++p)                                                 \
+      _Pragma("unroll") for (int j = 0; j < 2; ++j) {                                                \
+        __builtin_amdgcn_global_load_lds(xsrc[j] + p * a.x_piece + k0_, sb_ + (xslot + p) * C::TILE + ldst[j], 16, \
+                                         0, 0);                                                      \
+        __builtin_amdgcn_global_load_lds(ssrc[j] + p * a.s_piece + k0_, sb_ + (sslot + p) * C::TILE + ldst[j], 16, \
+                                         0, 0);                                                      \
+      }                                                                                              \
+  }
+
+  const int nk = (a.K + 31) / 32;
+#pragma unroll
+  for (int s = 0; s < C::NSTAGE - 1; ++s)
+    if (s < nk) SOMAX_ISSUE(s, s)
+
+  int ra[2], rb[2];
+#pragma unroll
+  for (int t = 0; t < 2; ++t) {
+    ra[t] = wm * 64 + t * 32 + (lane & 31);
+    rb[t] = wn * 64 + t * 32 + (lane & 31);
+  }
+  for (int kt = 0; kt < nk; ++kt) {
+    // tile kt has landed once at most min(NSTAGE-2, nk-1-kt) later tiles are outstanding
+    const int later = (nk - 1 - kt) < (C::NSTAGE - 2) ? (nk - 1 - kt) : (C::NSTAGE - 2);
+    if (later >= 2) wait_vmcnt<2 * C::LPT>();
+    else if (later == 1) wait_vmcnt<C::LPT>();
+    else wait_vmcnt<0>();
+    // Own LDS reads of the previous tile done, then a bare barrier: the
+    // __syncthreads fence would wait for every in-flight tile (vmcnt(0)).
+    __builtin_amdgcn_s_waitcnt((0xF) | (0x7 << 4) | (0x0 << 8) | (0x3 << 14));  // lgkmcnt(0)
+    __builtin_amdgcn_s_barrier();
+    if (kt + C::NSTAGE - 1 < nk) SOMAX_ISSUE(kt + C::NSTAGE - 1, (kt + C::NSTAGE - 1) % C::NSTAGE)
+    const uint16_t* sb = lds + (kt % C::NSTAGE) * C::STAGE;
+#pragma unroll
+    for (int kk = 0; kk < 2; ++kk) {
+      const int c = kk * 2 + (lane >> 5);  // logical 16-byte chunk of this lane's k-slice
       bf16x8 fa[3][2], fb[3][2];
 #pragma unroll
       for (int p = 0; p < NPC; ++p)
 #pragma unroll
         for (int t = 0; t < 2; ++t) {
-          fa[p][t] = *reinterpret_cast<const bf16x8*>(tA + p * TILE16 + fra + t * 32 * LD16 + kk);
-          fb[p][t] = *reinterpret_cast<const bf16x8*>(tB + p * TILE16 + frb + t * 32 * LD16 + kk);
+          fa[p][t] = *reinterpret_cast<const bf16x8*>(sb + p * C::TILE + ra[t] * 32 + ((c ^ swz(ra[t])) << 3));
+          fb[p][t] =
+              *reinterpret_cast<const bf16x8*>(sb + (NPC + p) * C::TILE + rb[t] * 32 + ((c ^ swz(rb[t])) << 3));
         }
       mfma_products<NPROD>(fa, fb, acc);
     }
-    if (ACC64) flush64(acc, sum);
-    if (kt + 1 < nk) {
-      SOMAX_SSTORE(buf ^ 1)
-    }
-    __syncthreads();
+    if (ACC64 && ((kt + 1) % C::FLUSH == 0)) flush64(acc, sum);
   }
-#undef SOMAX_GLOAD
-#undef SOMAX_SSTORE
+#undef SOMAX_ISSUE
 }
 
 }  // namespace
 
+size_t dst_gemm_lds_bytes(int max_np) {
+  const int npc = max_np == 1 ? 1 : (max_np == 3 ? 2 : 3);
+  const int nstage = npc == 3 ? 3 : 4;
+  return (size_t)nstage * 2 * npc * 128 * 32 * sizeof(uint16_t);
+}
+
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) void dst_gemm_bf16_kernel(GemmArgs a) {
   extern __shared__ __attribute__((aligned(16))) uint16_t lds16[];
-  const long z = blockIdx.z;
+  const int GA = (a.RA + BM - 1) / BM, GB = (a.RB + BN - 1) / BN;
+  int zi, ta, tb;
+  tile_of_block(GA, GB, zi, ta, tb);
+  const long z = zi;
   const bool lo = z < a.z_split;
   const int np = lo ? a.np_lo : a.np_hi;
   const int opieces = lo ? a.op_lo : a.op_hi;
-  const int a0 = blockIdx.y * BM;
-  const int b0 = blockIdx.x * BN;
+  const int a0 = ta * BM;
+  const int b0 = tb * BN;
   const bool x_in_a = !a.trans;
 
   floatx16 acc[2][2];
-#pragma unroll
-  for (int i = 0; i < 2; ++i)
-#pragma unroll
-    for (int j = 0; j < 2; ++j)
-#pragma unroll
-      for (int r = 0; r < 16; ++r) acc[i][j][r] = 0.0f;
-
   double sum[2][2][16];
-#pragma unroll
-  for (int i = 0; i < 2; ++i)
-#pragma unroll
-    for (int j = 0; j < 2; ++j)
-#pragma unroll
-      for (int r = 0; r < 16; ++r) sum[i][j][r] = 0.0;
+  zero_acc(acc, sum);
   if (a.acc64) {
     if (np == 1) bf16_mainloop<1, true>(a, lds16, z, a0, b0, x_in_a, acc, sum);
     else if (np == 3) bf16_mainloop<3, true>(a, lds16, z, a0, b0, x_in_a, acc, sum);
@@ -205,31 +281,21 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
     else if (np == 3) bf16_mainloop<3, false>(a, lds16, z, a0, b0, x_in_a, acc, sum);
     else bf16_mainloop<6, false>(a, lds16, z, a0, b0, x_in_a, acc, sum);
   }
-
-  const int lane = threadIdx.x & 63;
-  const int wave = threadIdx.x >> 6;
-  const int wm = wave >> 1, wn = wave & 1;
-  const float lamz = a.lamA ? a.lam[z] : 0.0f;
-#pragma unroll
-  for (int i = 0; i < 2; ++i)
-#pragma unroll
-    for (int j = 0; j < 2; ++j)
-#pragma unroll
-      for (int r = 0; r < 16; ++r) {
-        const int ro = a0 + wm * 64 + i * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
-        const int co = b0 + wn * 64 + j * 32 + (lane & 31);
-        if (ro < a.RA && co < a.RBpad) epi_store(a, z, ro, co, (float)(sum[i][j][r] + (double)acc[i][j][r]), opieces, lamz);
-      }
+  store_tile(a, z, a0, b0, acc, sum, opieces);
 }
 
 // ---------------------------------------------------------------------------
-// FP32 kernel (plain step)
+// FP32 kernel (plain step): register-staged double buffer, FP64 flush every
+// 4 k-tiles of 16.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 2))) void dst_gemm_f32_kernel(GemmArgs a) {
   __shared__ __attribute__((aligned(16))) float lds[2 * 2 * TILE32];
-  const long z = blockIdx.z;
-  const int a0 = blockIdx.y * BM;
-  const int b0 = blockIdx.x * BN;
+  const int GA = (a.RA + BM - 1) / BM, GB = (a.RB + BN - 1) / BN;
+  int zi, ta, tb;
+  tile_of_block(GA, GB, zi, ta, tb);
+  const long z = zi;
+  const int a0 = ta * BM;
+  const int b0 = tb * BN;
   const bool x_in_a = !a.trans;
   const int tid = threadIdx.x;
   const int lane = tid & 63;
@@ -252,20 +318,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 2))) voi
   const int st0 = cr * LD32 + ck, st1 = (cr + 64) * LD32 + ck;
   float4 rx0, rx1, rs0, rs1;
   floatx16 acc[2][2];
-#pragma unroll
-  for (int i = 0; i < 2; ++i)
-#pragma unroll
-    for (int j = 0; j < 2; ++j)
-#pragma unroll
-      for (int r = 0; r < 16; ++r) acc[i][j][r] = 0.0f;
-
   double sum[2][2][16];
-#pragma unroll
-  for (int i = 0; i < 2; ++i)
-#pragma unroll
-    for (int j = 0; j < 2; ++j)
-#pragma unroll
-      for (int r = 0; r < 16; ++r) sum[i][j][r] = 0.0;
+  zero_acc(acc, sum);
   const bool acc64 = a.acc64 != 0;
   const int nk = (a.K + BK32 - 1) / BK32;
   rx0 = *reinterpret_cast<const float4*>(xg0);
@@ -302,7 +356,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 2))) voi
         for (int j = 0; j < 2; ++j)
           acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x2f32(fa[i], fb[j], acc[i][j], 0, 0, 0);
     }
-    if (acc64) flush64(acc, sum);
+    if (acc64 && ((kt & 3) == 3)) flush64(acc, sum);
     if (kt + 1 < nk) {
       float* tb = lds + (buf ^ 1) * 2 * TILE32;
       *reinterpret_cast<float4*>(tb + xo + st0) = rx0;
@@ -312,40 +366,24 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 2))) voi
     }
     __syncthreads();
   }
-
-  const float lamz = a.lamA ? a.lam[z] : 0.0f;
-#pragma unroll
-  for (int i = 0; i < 2; ++i)
-#pragma unroll
-    for (int j = 0; j < 2; ++j)
-#pragma unroll
-      for (int r = 0; r < 16; ++r) {
-        const int ro = a0 + wm * 64 + i * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
-        const int co = b0 + wn * 64 + j * 32 + (lane & 31);
-        if (ro < a.RA && co < a.RBpad) epi_store(a, z, ro, co, (float)(sum[i][j][r] + (double)acc[i][j][r]), 0, lamz);
-      }
-}
-
-size_t dst_gemm_lds_bytes(int max_np) {
-  const int npc = max_np == 1 ? 1 : (max_np == 3 ? 2 : 3);
-  return (size_t)2 * 2 * npc * TILE16 * sizeof(uint16_t);
+  store_tile(a, z, a0, b0, acc, sum, 0);
 }
 
 void launch_dst_gemm(const GemmArgs& a, int batch, hipStream_t st) {
-  dim3 grid((a.RB + BN - 1) / BN, (a.RA + BM - 1) / BM, batch);
+  const int tiles = ((a.RB + BN - 1) / BN) * ((a.RA + BM - 1) / BM) * batch;
   if (a.Xf) {
-    hipLaunchKernelGGL(dst_gemm_f32_kernel, grid, dim3(256), 0, st, a);
+    hipLaunchKernelGGL(dst_gemm_f32_kernel, dim3(tiles), dim3(256), 0, st, a);
     return;
   }
   static bool lds_attr = false;
   if (!lds_attr) {
-    // up to 120 KiB of the 160 KiB LDS of a gfx950 CU (3 pieces, double buffered)
+    // up to 144 KiB of the 160 KiB LDS of a gfx950 CU (3 pieces, 3 stages)
     (void)hipFuncSetAttribute(reinterpret_cast<const void*>(dst_gemm_bf16_kernel),
                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)dst_gemm_lds_bytes(6));
     lds_attr = true;
   }
   const int max_np = a.np_lo > a.np_hi ? a.np_lo : a.np_hi;
-  hipLaunchKernelGGL(dst_gemm_bf16_kernel, grid, dim3(256), dst_gemm_lds_bytes(max_np), st, a);
+  hipLaunchKernelGGL(dst_gemm_bf16_kernel, dim3(tiles), dim3(256), dst_gemm_lds_bytes(max_np), st, a);
 }
 
 }  // namespace somax

@@ -125,7 +125,229 @@ __global__ __launch_bounds__(256) void qg_stage_arakawa_kernel(StageArgs a) {
   }
 }
 
+// ---------------------------------------------------------------------------
+// WENO-Z flux-form advection stage (extension of the reference, which only
+// has the Arakawa Jacobian)
+//
+// Face velocities come from psi at cell corners (4-point average inside the
+// interior block, the wall value psi_wall on its boundary lines), so they are
+// discretely divergence free and vanish through the walls.  Face values of
+// total PV q + b(y) are upwind WENO-Z: 5th order where the five upwind-biased
+// cells are interior, 3rd order where three are, else first-order upwind.
+// The reconstruction works on differences: along x b is constant on a row
+// (WENO commutes with adding a constant), along y the row increments of b are
+// added to the q increments, and the b_row * div(u) term of the flux
+// divergence is dropped because div(u) = 0 identically.
+// ---------------------------------------------------------------------------
+namespace {
+constexpr int WX = 64;
+constexpr int WY = 16;
+constexpr int WQ = WX + 6;  // q tile pitch (halo 3)
+constexpr float WENO_EPS = 1e-20f;
+
+// Face value minus the upwind cell's value, from the four increments
+// d1..d4 around upwind cell i (d2 = q_i - q_{i-1}, d3 = q_{i+1} - q_i).
+__device__ inline float weno5_corr(float d1, float d2, float d3, float d4) {
+  const float b0 = (13.0f / 12.0f) * (d2 - d1) * (d2 - d1) + 0.25f * (3.0f * d2 - d1) * (3.0f * d2 - d1);
+  const float b1 = (13.0f / 12.0f) * (d3 - d2) * (d3 - d2) + 0.25f * (d2 + d3) * (d2 + d3);
+  const float b2 = (13.0f / 12.0f) * (d4 - d3) * (d4 - d3) + 0.25f * (d4 - 3.0f * d3) * (d4 - 3.0f * d3);
+  const float tau = fabsf(b0 - b2);
+  const float a0 = 0.1f * (1.0f + tau / (b0 + WENO_EPS));
+  const float a1 = 0.6f * (1.0f + tau / (b1 + WENO_EPS));
+  const float a2 = 0.3f * (1.0f + tau / (b2 + WENO_EPS));
+  const float c0 = (5.0f * d2 - 2.0f * d1) * (1.0f / 6.0f);
+  const float c1 = (d2 + 2.0f * d3) * (1.0f / 6.0f);
+  const float c2 = (4.0f * d3 - d4) * (1.0f / 6.0f);
+  return (a0 * c0 + a1 * c1 + a2 * c2) / (a0 + a1 + a2);
+}
+
+// 3rd-order WENO-Z correction from d2 = q_i - q_{i-1}, d3 = q_{i+1} - q_i.
+__device__ inline float weno3_corr(float d2, float d3) {
+  const float b0 = d2 * d2, b1 = d3 * d3;
+  const float tau = fabsf(b0 - b1);
+  const float a0 = (1.0f / 3.0f) * (1.0f + tau / (b0 + WENO_EPS));
+  const float a1 = (2.0f / 3.0f) * (1.0f + tau / (b1 + WENO_EPS));
+  return (a0 * (0.5f * d2) + a1 * (0.5f * d3)) / (a0 + a1);
+}
+
+// Upwind face value minus the upwind cell value for the face between cells
+// f and f+1 (1-based interior indices 1..n along the line).  dm[k] holds the
+// increment q_{f+k} - q_{f+k-1} for k = -1..3 (dm index k+1), i.e. the
+// increments from cell f-2 up to cell f+3.  vel > 0: upwind cell f.
+__device__ inline float face_corr(int f, int n, float vel, const float* dm) {
+  if (vel > 0.0f) {
+    if (f - 2 >= 1 && f + 2 <= n) return weno5_corr(dm[0], dm[1], dm[2], dm[3]);
+    if (f - 1 >= 1 && f + 1 <= n) return weno3_corr(dm[1], dm[2]);
+    return 0.0f;
+  }
+  const int g = f + 1;  // mirror image around cell f+1
+  if (g - 2 >= 1 && g + 2 <= n) return weno5_corr(-dm[4], -dm[3], -dm[2], -dm[1]);
+  if (g - 1 >= 1 && g + 1 <= n) return weno3_corr(-dm[3], -dm[2]);
+  return 0.0f;
+}
+}  // namespace
+
+template <bool BIHARM>
+__global__ __launch_bounds__(256) void qg_stage_weno_kernel(StageArgs a) {
+  __shared__ float sq[(WY + 6) * WQ];           // q (BC), halo 3
+  __shared__ float sp[(WY + 2) * (WX + 2)];     // psi, halo 1
+  __shared__ float sc[(WY + 1) * (WX + 1)];     // corner psi for corners (y0-1.., x0-1..)
+  __shared__ float sfx[WY * (WX + 1)];          // x-face fluxes u*qface
+  __shared__ float sfy[(WY + 1) * WX];          // y-face fluxes v*(qface - b_row)
+  __shared__ float sl[BIHARM ? (WY + 2) * (WX + 2) : 1];
+  __shared__ float sdb[WY + 6];                 // b[j+1] - b[j] for rows y0-3 ..
+
+  const int tid = threadIdx.x;
+  const int layer = blockIdx.z;
+  const int x0 = blockIdx.x * WX;
+  const int y0 = blockIdx.y * WY;
+  const int Nx = a.Nx, Ny = a.Ny;
+  const int nxi = Nx - 2, nyi = Ny - 2;
+  const long off = (long)layer * a.plane;
+  const float* qs = a.qs + off;
+  const float* psi = a.psi + off;
+  const float pw = a.psi_wall[layer];
+  constexpr int PW2 = WX + 2;
+
+  for (int i = tid; i < (WY + 6) * WQ; i += 256) {
+    const int ly = i / WQ, lx = i - ly * WQ;
+    const int gy = y0 - 3 + ly, gx = x0 - 3 + lx;
+    float v = 0.0f;
+    const bool inside = gy >= 0 && gy < Ny && gx >= 0 && gx < Nx;
+    const bool ghost = gy == 0 || gx == 0 || gy == Ny - 1 || gx == Nx - 1;
+    if (inside && !(ghost && a.apply_bc)) v = qs[(long)gy * Nx + gx];
+    sq[i] = v;
+  }
+  for (int i = tid; i < (WY + 2) * PW2; i += 256) {
+    const int ly = i / PW2, lx = i - ly * PW2;
+    const int gy = y0 - 1 + ly, gx = x0 - 1 + lx;
+    float v = 0.0f;
+    if (gy >= 0 && gy < Ny && gx >= 0 && gx < Nx) v = psi[(long)gy * Nx + gx];
+    sp[i] = v;
+  }
+  if (tid < WY + 6) {
+    const int gy = y0 - 3 + tid;
+    sdb[tid] = (gy >= 0 && gy < Ny) ? a.dbeta_row[gy] : 0.0f;
+  }
+  __syncthreads();
+
+  // corners: corner (cy, cx) lies between T rows cy, cy+1 and columns cx, cx+1;
+  // on the boundary lines of the interior block it takes the wall value
+  for (int i = tid; i < (WY + 1) * (WX + 1); i += 256) {
+    const int ly = i / (WX + 1), lx = i - ly * (WX + 1);
+    const int cy = y0 - 1 + ly, cx = x0 - 1 + lx;
+    float v = pw;
+    if (cy >= 1 && cy <= Ny - 3 && cx >= 1 && cx <= Nx - 3) {
+      const float* p = &sp[ly * PW2 + lx];  // T (cy, cx) at local (ly, lx) of the psi tile
+      v = 0.25f * (p[0] + p[1] + p[PW2] + p[PW2 + 1]);
+    }
+    sc[i] = v;
+  }
+  if (BIHARM) {
+    for (int i = tid; i < (WY + 2) * PW2; i += 256) {
+      const int ly = i / PW2, lx = i - ly * PW2;
+      const int gy = y0 - 1 + ly, gx = x0 - 1 + lx;
+      float v = 0.0f;
+      if (gy >= 1 && gy <= Ny - 2 && gx >= 1 && gx <= Nx - 2) {
+        const float* c = &sq[(ly + 2) * WQ + lx + 2];
+        v = (c[1] - 2.0f * c[0] + c[-1]) * a.idx2 + (c[WQ] - 2.0f * c[0] + c[-WQ]) * a.idy2;
+      }
+      sl[i] = v;
+    }
+  }
+  __syncthreads();
+
+  // x-face fluxes: face between T columns f and f+1 on row gy, f = x0-1 .. x0+WX-1
+  for (int i = tid; i < WY * (WX + 1); i += 256) {
+    const int ly = i / (WX + 1), lf = i - ly * (WX + 1);
+    const int gy = y0 + ly, f = x0 - 1 + lf;
+    float flux = 0.0f;
+    if (gy >= 1 && gy <= nyi && f >= 1 && f <= nxi - 1) {
+      // u = -(C(gy, f) - C(gy-1, f)) / dy ; corner (gy, f) at local (ly+1, lf)
+      const float u = -(sc[(ly + 1) * (WX + 1) + lf] - sc[ly * (WX + 1) + lf]) * a.idy;
+      const float* qr = &sq[(ly + 3) * WQ + (f - x0 + 3)];  // cell f
+      float dm[5];
+#pragma unroll
+      for (int k = 0; k < 5; ++k) dm[k] = qr[k - 1] - qr[k - 2];  // q_{f+k-1} - q_{f+k-2}
+      const float qup = u > 0.0f ? qr[0] : qr[1];
+      flux = u * (qup + face_corr(f, nxi, u, dm));
+    }
+    sfx[i] = flux;
+  }
+  // y-face fluxes: face between T rows g and g+1 on column gx, g = y0-1 .. y0+WY-1;
+  // stored as v * (face value of q + b minus b of the lower cell g)
+  for (int i = tid; i < (WY + 1) * WX; i += 256) {
+    const int lg = i / WX, lx = i - lg * WX;
+    const int g = y0 - 1 + lg, gx = x0 + lx;
+    float flux = 0.0f;
+    if (gx >= 1 && gx <= nxi && g >= 1 && g <= nyi - 1) {
+      // v = (C(g, gx) - C(g, gx-1)) / dx ; corner (g, gx) at local (lg, lx+1)
+      const float v = (sc[lg * (WX + 1) + lx + 1] - sc[lg * (WX + 1) + lx]) * a.idx;
+      const int lq = g - y0 + 3;  // local q row of cell g
+      float dm[5];
+#pragma unroll
+      for (int k = 0; k < 5; ++k) {
+        // increment of q + b from cell g+k-2 to g+k-1
+        const int r = lq + k - 1;
+        dm[k] = (sq[r * WQ + lx + 3] - sq[(r - 1) * WQ + lx + 3]) + sdb[r - 1];
+      }
+      // face value of q+b minus b_g: upwind g -> q_g + corr ; upwind g+1 -> q_{g+1} + db_g + corr
+      const float base = v > 0.0f ? sq[lq * WQ + lx + 3] : sq[(lq + 1) * WQ + lx + 3] + sdb[lq];
+      flux = v * (base + face_corr(g, nyi, v, dm));
+    }
+    sfy[i] = flux;
+  }
+  __syncthreads();
+
+  const int tx = tid & 63;
+  const int tyb = tid >> 6;
+  const float wc = a.wind_coef[layer];
+  const bool bottom = (layer == a.nl - 1) && (a.kappa != 0.0f);
+#pragma unroll
+  for (int r = 0; r < WY / 4; ++r) {
+    const int ly = tyb + 4 * r;
+    const int gy = y0 + ly, gx = x0 + tx;
+    if (gy >= Ny || gx >= Nx) continue;
+    float rhs = 0.0f;
+    if (gy >= 1 && gy <= Ny - 2 && gx >= 1 && gx <= Nx - 2) {
+      const float fe = sfx[ly * (WX + 1) + tx + 1], fw = sfx[ly * (WX + 1) + tx];
+      // y-faces above/below cell gy: stored relative to b of their lower cell;
+      // the upper face is relative to b_gy already, the lower one to b_{gy-1},
+      // so its b offset db_{gy-1} times v_s completes the b_gy reference.
+      const float gn = sfy[(ly + 1) * WX + tx], gs = sfy[ly * WX + tx];
+      const float vs = (sc[ly * (WX + 1) + tx + 1] - sc[ly * (WX + 1) + tx]) * a.idx;
+      const float gs_rel = gs - vs * sdb[ly + 2];
+      rhs = -((fe - fw) * a.idx + (gn - gs_rel) * a.idy);
+      const float* q = &sq[(ly + 3) * WQ + tx + 3];
+      const float* p = &sp[(ly + 1) * PW2 + tx + 1];
+      if (a.nu != 0.0f) rhs += a.nu * ((q[1] - 2.0f * q[0] + q[-1]) * a.idx2 + (q[WQ] - 2.0f * q[0] + q[-WQ]) * a.idy2);
+      if (BIHARM) {
+        const float* l = &sl[(ly + 1) * PW2 + tx + 1];
+        rhs -= a.nu4 * ((l[1] - 2.0f * l[0] + l[-1]) * a.idx2 + (l[PW2] - 2.0f * l[0] + l[-PW2]) * a.idy2);
+      }
+      if (bottom) rhs -= a.kappa * ((p[1] - 2.0f * p[0] + p[-1]) * a.idx2 + (p[PW2] - 2.0f * p[0] + p[-PW2]) * a.idy2);
+    }
+    rhs += wc * a.wind_row[gy];
+    const long idx = off + (long)gy * Nx + gx;
+    float o;
+    if (a.qn) {
+      const float qn = a.qn[idx];
+      o = qn + a.c1 * (a.qs[idx] - qn) + a.c2 * rhs;
+    } else {
+      o = a.c2 * rhs;
+      if (a.c1 != 0.0f) o += a.c1 * a.qs[idx];
+    }
+    a.out[idx] = o;
+  }
+}
+
 void launch_qg_stage(const StageArgs& a, hipStream_t st) {
+  if (a.advection == 1) {
+    dim3 g((a.Nx + WX - 1) / WX, (a.Ny + WY - 1) / WY, a.nl);
+    if (a.nu4 != 0.0f) hipLaunchKernelGGL(qg_stage_weno_kernel<true>, g, dim3(256), 0, st, a);
+    else hipLaunchKernelGGL(qg_stage_weno_kernel<false>, g, dim3(256), 0, st, a);
+    return;
+  }
   dim3 grid((a.Nx + TX - 1) / TX, (a.Ny + TY - 1) / TY, a.nl);
   if (a.nu4 != 0.0f)
     hipLaunchKernelGGL(qg_stage_arakawa_kernel<true>, grid, dim3(256), 0, st, a);

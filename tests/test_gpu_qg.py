@@ -166,6 +166,53 @@ def test_integrate_matches_steps(gpu):
     assert rel(sol.ys.q[0].cpu().numpy(), ref) < 1e-6
 
 
+@pytest.mark.parametrize("n,precision", [(64, "fp32"), (128, "mixed")])
+def test_weno_rhs_matches_oracle(gpu, n, precision):
+    _log(f"[weno] rhs n={n} precision={precision}")
+    from somax_amd import BaroclinicQGState
+
+    model, setup = _pair(n, precision=precision, advection="weno")
+    q, qd = _state(model, seed=21)
+    dq = model.vector_field(0.0, BaroclinicQGState(q=qd)).q.cpu().numpy()
+    ref = oqg.rhs(setup, q.astype(np.float64))
+    r = rel(dq, ref)
+    _log(f"  rel-L2 dq = {r:.3e}")
+    assert r < 2e-5
+
+
+def test_weno_conserves_pv(gpu):
+    """Flux form with zero wall flux: the interior sum of the advective
+    tendency vanishes (no forcing, no dissipation)."""
+    from somax_amd import BaroclinicQG, BaroclinicQGState
+    from somax_amd import testcases as tc
+
+    m = BaroclinicQG.create(nx=64, ny=64, Lx=tc.BASIN, Ly=tc.BASIN, advection="weno")
+    q = tc.random_pv(m, 4).astype(np.float32)
+    import torch
+
+    dq = m.vector_field(0.0, BaroclinicQGState(q=torch.from_numpy(q).cuda())).q.cpu().numpy().astype(np.float64)
+    s = np.abs(dq[:, 1:-1, 1:-1].sum(axis=(-2, -1)))
+    assert np.all(s < 1e-5 * np.abs(dq).sum(axis=(-2, -1)))
+
+
+@pytest.mark.parametrize("precision", ["fp32", "mixed"])
+def test_weno_steps_match_oracle(gpu, precision):
+    _log(f"[weno] 1 and 20 steps n=128 precision={precision}")
+    from somax_amd import BaroclinicQGState
+
+    model, setup = _pair(128, precision=precision, advection="weno")
+    q, qd = _state(model, seed=23)
+    q1 = model.step(BaroclinicQGState(q=qd), 1200.0).q
+    ref1 = oqg.ssprk3_step(setup, q.astype(np.float64), 1200.0)
+    r1 = rel(q1.cpu().numpy(), ref1)
+    p1 = rel(model._invert_pv(q1).cpu().numpy(), oqg.invert_pv(setup, ref1))
+    q20 = model.step(BaroclinicQGState(q=q1), 1200.0, nsteps=19).q
+    ref20 = oqg.run(setup, ref1, 1200.0, 19)
+    r20 = rel(q20.cpu().numpy(), ref20)
+    _log(f"  1 step: q {r1:.3e} psi {p1:.3e}; 20 steps: q {r20:.3e}")
+    assert r1 < 1e-6 and p1 < 1e-6 and r20 < 1e-5
+
+
 # ---------------------------------------------------------------------------
 # behaviours the reference tests assert (tests/models/test_qg_baroclinic.py)
 # ---------------------------------------------------------------------------
