@@ -121,6 +121,69 @@ def cpu_baseline(n: int, q64, dt: float, steps: int) -> dict:
                       f"(FFT DST-I), single thread, {el:.1f} s"}
 
 
+def bench_ensemble(dt: float, steps: int, warmup: int) -> dict:
+    """The 256-member uneven ensemble (96 x 64^2, 96 x 128^2, 48 x 256^2,
+    16 x 512^2, layers cycling 1/2/3, tau0 ~ U[0.04, 0.12]) advanced as one
+    hipGraph-captured ensemble step with the mixed inversion, against the
+    same members stepped one after another with their own plain FP32 plans."""
+    import numpy as np
+    import torch
+
+    from somax_amd import QGEnsemble
+    from somax_amd import testcases as tc
+
+    log("[bench] ensemble: building 256 members")
+    models, qs = tc.config_ensemble(seed=7, precision="mixed")
+    cells = sum(m._nl * m.grid.nx * m.grid.ny for m in models)
+    out = {"members": len(models), "layer_cells": cells,
+           "counts": {f"{n}x{n}": c for n, c in tc.ENSEMBLE_COUNTS}}
+
+    def timed(fn, k):
+        torch.cuda.synchronize()
+        st = torch.cuda.current_stream()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record(st)
+        fn(k)
+        e1.record(st)
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / k
+
+    for label, conc, graph in (("graph", True, True), ("eager", False, False)):
+        ens = QGEnsemble(models, concurrent=conc, graph=graph)
+        packed = ens.pack(qs)
+        ens.step(packed, dt, max(warmup, 1))
+        ms = timed(lambda k: ens.step(packed, dt, k), steps)
+        if not torch.isfinite(packed).all():
+            raise RuntimeError("non-finite ensemble state")
+        out[f"mixed_{label}"] = {"ms_per_step": round(ms, 4), "value": round(cells / (ms * 1e-3), 1)}
+        log(f"[bench] ensemble mixed {label}: {ms:.3f} ms/step")
+        if label == "graph":
+            prof = ens.profile(packed, dt, 1)
+            out["kernel_profile_eager"] = {k: {"launches": v[0], "total_us": round(1e3 * v[1], 1)}
+                                           for k, v in prof.items()}
+        del ens, packed
+
+    log("[bench] ensemble: per-member plain FP32 steps")
+    plain = [tc.ensemble_member(m.grid.nx, m._nl, 0.0, precision="fp32") for m in models]
+    for p, m in zip(plain, models):
+        p.params = m.params
+    qd = [torch.from_numpy(q.astype(np.float32)).cuda() for q in qs]
+    for p, q in zip(plain, qd):
+        p._step_inplace(q, dt, 1)
+
+    def per_member(k):
+        for _ in range(k):
+            for p, q in zip(plain, qd):
+                p._step_inplace(q, dt, 1)
+
+    ms = timed(per_member, steps)
+    out["plain_per_member"] = {"ms_per_step": round(ms, 4), "value": round(cells / (ms * 1e-3), 1)}
+    out["speedup_vs_plain_per_member"] = round(ms / out["mixed_graph"]["ms_per_step"], 3)
+    log(f"[bench] per-member plain: {ms:.3f} ms/step")
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -131,6 +194,7 @@ def main() -> int:
     ap.add_argument("--advection", default="arakawa", choices=["arakawa", "weno"])
     ap.add_argument("--profile-steps", type=int, default=2)
     ap.add_argument("--cpu-steps", type=int, default=2, help="oracle steps timed for cpu_baseline (0 = skip)")
+    ap.add_argument("--ensemble-steps", type=int, default=10, help="steps timed for the 256-member ensemble (0 = skip)")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -170,6 +234,8 @@ def main() -> int:
     dom_p = max(prof_plain, key=lambda k: prof_plain[k]["total_ms"])
     roof_plain = roofline_entry(dom_p, prof_plain[dom_p])
 
+    ens = bench_ensemble(dt, args.ensemble_steps, 2) if args.ensemble_steps > 0 else None
+
     cpu = None
     if args.cpu_steps > 0:
         log("[bench] cpu baseline (oracle)")
@@ -198,6 +264,7 @@ def main() -> int:
         "roofline_stencil": roof_stencil,
         "kernel_profile_mixed": {k: {"launches": v["launches"], "avg_us": round(1e3 * v["total_ms"] / v["launches"], 2)}
                                  for k, v in prof_mixed.items()},
+        "ensemble": ens,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

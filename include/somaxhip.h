@@ -144,6 +144,45 @@ int somaxhip_qg_step(somaxhip_qg_plan plan, float* q, float* psi_out, int nsteps
                      somaxhip_stream_t stream);
 
 /* ------------------------------------------------------------------------ */
+/* Ensembles                                                                */
+/* ------------------------------------------------------------------------ */
+
+/* Extension with no reference counterpart: the reference advances one model
+ * per SomaxModel.integrate call (core/model.py:53-88); an ensemble there is a
+ * Python loop (or jax.vmap over identical grids).  Here members of any grid
+ * size and layer count advance together.  Members with the same grid, advection
+ * and precision form a class; each class runs every kernel of the step in one
+ * launch over all its planes, and classes are queued largest grid first. */
+typedef struct somaxhip_ens_s* somaxhip_ens;
+
+enum {
+  SOMAXHIP_ENS_CONCURRENT = 1, /* each class on its own stream, forked from and joined to the caller's */
+  SOMAXHIP_ENS_GRAPH = 2       /* capture one step as a hipGraph and replay it */
+};
+
+/* members[i] / params[i], i < n: one QG model each (as for plan_create);
+ * flags: SOMAXHIP_ENS_* bits. */
+int somaxhip_ens_create(const somaxhip_qg_desc* members, const somaxhip_qg_params* params, int n, int flags,
+                        somaxhip_ens* ens);
+int somaxhip_ens_destroy(somaxhip_ens ens);
+
+/* The packed ensemble state: one float32 buffer of state_floats elements;
+ * member i's field float32[nl][ny+2][nx+2] starts at member_offset(i). */
+size_t somaxhip_ens_state_floats(somaxhip_ens ens);
+long somaxhip_ens_member_offset(somaxhip_ens ens, int member); /* -1 on error */
+size_t somaxhip_ens_workspace_bytes(somaxhip_ens ens);
+int somaxhip_ens_num_classes(somaxhip_ens ens);
+
+/* Replaces every member's parameters (n entries, in member order). */
+int somaxhip_ens_set_params(somaxhip_ens ens, const somaxhip_qg_params* params, somaxhip_stream_t stream);
+
+/* Advances every member by nsteps SSP-RK3 steps of size dt, in place, with
+ * the arithmetic of somaxhip_qg_step.  Under SOMAXHIP_ENS_GRAPH the captured
+ * step is kept and replayed while q, workspace and dt stay the same. */
+int somaxhip_ens_step(somaxhip_ens ens, float* q, int nsteps, double dt, void* workspace,
+                      somaxhip_stream_t stream);
+
+/* ------------------------------------------------------------------------ */
 /* Measurement                                                              */
 /* ------------------------------------------------------------------------ */
 
@@ -165,6 +204,12 @@ int somaxhip_qg_step_profile(somaxhip_qg_plan plan, float* q, int nsteps, double
                              const somaxhip_qg_params* params, void* workspace,
                              somaxhip_stream_t stream, somaxhip_kernel_stat* stats,
                              int max_stats, int* nstats);
+
+/* somaxhip_ens_step run eagerly on the caller's stream, classes one after
+ * another, every launch timed; synchronises the stream before returning. */
+int somaxhip_ens_step_profile(somaxhip_ens ens, float* q, int nsteps, double dt, void* workspace,
+                              somaxhip_stream_t stream, somaxhip_kernel_stat* stats, int max_stats,
+                              int* nstats);
 
 #ifdef __cplusplus
 }

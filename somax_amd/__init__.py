@@ -28,6 +28,7 @@ from .qg import (
     BarotropicQGPhysConsts,
     BarotropicQGState,
 )
+from .ensemble import QGEnsemble
 from .testcases import doublegyre_baroclinic_qg, doublegyre_qg
 
 __all__ = [
@@ -35,5 +36,5 @@ __all__ = [
     "State", "StratificationProfile", "BaroclinicQG", "BaroclinicQGDiagnostics", "BaroclinicQGParams",
     "BaroclinicQGPhysConsts", "BaroclinicQGState", "BarotropicQG", "BarotropicQGDiagnostics",
     "BarotropicQGParams", "BarotropicQGPhysConsts", "BarotropicQGState", "doublegyre_baroclinic_qg",
-    "doublegyre_qg",
+    "doublegyre_qg", "QGEnsemble",
 ]

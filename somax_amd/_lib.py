@@ -18,6 +18,8 @@ ADV_ARAKAWA = 0
 ADV_WENO = 1
 PREC_FP32 = 0
 PREC_MIXED = 1
+ENS_CONCURRENT = 1
+ENS_GRAPH = 2
 
 
 class SomaxHipError(RuntimeError):
@@ -89,6 +91,22 @@ _SIGNATURES = {
         ctypes.c_int,
         [_vp, _vp, ctypes.c_int, ctypes.c_double, ctypes.POINTER(QGParams), _vp, _vp, ctypes.POINTER(KernelStat),
          ctypes.c_int, ctypes.POINTER(ctypes.c_int)],
+    ),
+    "somaxhip_ens_create": (
+        ctypes.c_int,
+        [ctypes.POINTER(QGDesc), ctypes.POINTER(QGParams), ctypes.c_int, ctypes.c_int, ctypes.POINTER(_vp)],
+    ),
+    "somaxhip_ens_destroy": (ctypes.c_int, [_vp]),
+    "somaxhip_ens_state_floats": (ctypes.c_size_t, [_vp]),
+    "somaxhip_ens_member_offset": (ctypes.c_long, [_vp, ctypes.c_int]),
+    "somaxhip_ens_workspace_bytes": (ctypes.c_size_t, [_vp]),
+    "somaxhip_ens_num_classes": (ctypes.c_int, [_vp]),
+    "somaxhip_ens_set_params": (ctypes.c_int, [_vp, ctypes.POINTER(QGParams), _vp]),
+    "somaxhip_ens_step": (ctypes.c_int, [_vp, _vp, ctypes.c_int, ctypes.c_double, _vp, _vp]),
+    "somaxhip_ens_step_profile": (
+        ctypes.c_int,
+        [_vp, _vp, ctypes.c_int, ctypes.c_double, _vp, _vp, ctypes.POINTER(KernelStat), ctypes.c_int,
+         ctypes.POINTER(ctypes.c_int)],
     ),
 }
 

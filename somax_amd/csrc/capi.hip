@@ -1,8 +1,14 @@
-// C ABI of the somax HIP library: plans, workspace layout and the QG entry
-// points declared in include/somaxhip.h.
+// C ABI of the somax HIP library: batches, plans, ensembles, workspace
+// layout and the QG entry points declared in include/somaxhip.h.
+//
+// Everything runs on batches (kernels.h): the members of one grid stacked as
+// planes.  A plan is the one-member batch of a single model; an ensemble is a
+// set of batches, one per grid class, over one packed state buffer.
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <memory>
 #include <new>
 #include <string>
 #include <vector>
@@ -36,6 +42,9 @@ struct SineSet {
 
 struct DeviceBlock {
   std::vector<void*> ptrs;
+  DeviceBlock() = default;
+  DeviceBlock(const DeviceBlock&) = delete;
+  DeviceBlock& operator=(const DeviceBlock&) = delete;
   ~DeviceBlock() {
     for (void* p : ptrs) (void)hipFree(p);
   }
@@ -47,6 +56,12 @@ struct DeviceBlock {
     ptrs.push_back(p);
     *out = static_cast<T*>(p);
     return hipSuccess;
+  }
+  template <class T>
+  hipError_t upload(T** out, const std::vector<T>& v) {
+    hipError_t e = alloc(out, v.size());
+    if (e != hipSuccess || v.empty()) return e;
+    return hipMemcpy(*out, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice);
   }
 };
 
@@ -89,18 +104,148 @@ static hipError_t build_sines(DeviceBlock& mem, SineSet& s, int N, double d) {
     s.lam_max = std::fabs(l) > s.lam_max ? std::fabs(l) : s.lam_max;
   }
   hipError_t e;
-#define SOMAX_UP(dst, src, n)                                                            \
-  if ((e = mem.alloc(&dst, n)) != hipSuccess) return e;                                 \
-  if ((e = hipMemcpy(dst, src.data(), (n) * sizeof(src[0]), hipMemcpyHostToDevice)) != \
-      hipSuccess)                                                                        \
+  if ((e = mem.upload(&s.f32_full, ff)) != hipSuccess) return e;
+  if ((e = mem.upload(&s.f32_int, fi)) != hipSuccess) return e;
+  if ((e = mem.upload(&s.b_full, bf)) != hipSuccess) return e;
+  if ((e = mem.upload(&s.b_int, bi)) != hipSuccess) return e;
+  return mem.upload(&s.lam, lam);
+}
+
+// The members of one grid, stacked as planes, with their device tables and
+// the layout of the batch's workspace.
+struct Batch {
+  int nx = 0, ny = 0, Nx = 0, Ny = 0;
+  double dx = 0.0, dy = 0.0;
+  int M = 0, P = 0;
+  int advection = 0, precision = 0;
+  DeviceBlock mem;
+  SineSet sx, sy;
+  Tables t{};
+  float* dbeta_rows = nullptr;
+  float* wind_rows = nullptr;
+  std::vector<int> first, nl;  // host copies of the member tables
+  // workspace layout (byte offsets), all float planes
+  long wpitch = 0;  // pitch of GEMM operand planes (elements)
+  long wpiece = 0;  // elements of one bf16 piece plane (maxN * wpitch)
+  long splane = 0;  // Ny*Nx
+  size_t off_OP1 = 0, off_OP2 = 0, off_QM = 0, off_P0 = 0, off_D = 0, off_Q1 = 0, off_Q2 = 0, off_PSI = 0;
+  size_t ws_bytes = 0;
+};
+
+// Checks one model description; empty string when valid.
+static std::string check_desc(const somaxhip_qg_desc* d) {
+  if (!d) return "null description";
+  if (d->nx < 2 || d->ny < 2 || d->nl < 1 || d->nl > MAX_LAYERS || !(d->dx > 0) || !(d->dy > 0))
+    return "need nx, ny >= 2, 1 <= nl <= 8, dx, dy > 0";
+  if (!d->cl2m || !d->cm2l || !d->lambdas || !d->beta_y_row || !d->wind_row || !d->wind_scale)
+    return "a required table pointer is null";
+  if (d->advection != SOMAXHIP_ADV_ARAKAWA && d->advection != SOMAXHIP_ADV_WENO) return "unknown advection";
+  if (d->precision != SOMAXHIP_PREC_FP32 && d->precision != SOMAXHIP_PREC_MIXED) return "unknown precision";
+  if (d->mass_correction) return "mass correction is not available in this build";
+  return "";
+}
+
+static bool same_class(const somaxhip_qg_desc* a, const somaxhip_qg_desc* b) {
+  return a->nx == b->nx && a->ny == b->ny && a->dx == b->dx && a->dy == b->dy && a->advection == b->advection &&
+         a->precision == b->precision;
+}
+
+// Builds batch `b` from M descriptions of one class (validated by the caller).
+static hipError_t batch_build(Batch& b, const std::vector<const somaxhip_qg_desc*>& ds) {
+  const somaxhip_qg_desc* d0 = ds[0];
+  b.nx = d0->nx;
+  b.ny = d0->ny;
+  b.Nx = d0->nx + 2;
+  b.Ny = d0->ny + 2;
+  b.dx = d0->dx;
+  b.dy = d0->dy;
+  b.advection = d0->advection;
+  b.precision = d0->precision;
+  b.M = (int)ds.size();
+  hipError_t e;
+  if ((e = build_sines(b.mem, b.sx, b.Nx, b.dx)) != hipSuccess) return e;
+  if ((e = build_sines(b.mem, b.sy, b.Ny, b.dy)) != hipSuccess) return e;
+
+  // Poisson-like modes (small lambda: the Laplacian's conditioning is not
+  // capped) need the 3-piece inverse transform in the mixed solve.
+  const double amp_num = b.sx.lam_max + b.sy.lam_max;
+  const double amp_den0 = b.sx.lam_min + b.sy.lam_min;
+
+  std::vector<int> mfirst, mnl, member, layer;
+  std::vector<double> cl2m((size_t)b.M * 64, 0.0);
+  std::vector<float> cm2l((size_t)b.M * 64, 0.0f), lam, wscale;
+  std::vector<unsigned char> mask, hp;
+  std::vector<float> dbeta((size_t)b.M * b.Ny, 0.0f), wind((size_t)b.M * b.Ny, 0.0f);
+  int P = 0;
+  for (int m = 0; m < b.M; ++m) {
+    const somaxhip_qg_desc* d = ds[m];
+    const int nl = d->nl;
+    mfirst.push_back(P);
+    mnl.push_back(nl);
+    mask.push_back(d->zero_psi_ghost ? 1 : 0);
+    for (int i = 0; i < nl; ++i)
+      for (int k = 0; k < nl; ++k) {
+        cl2m[(size_t)m * 64 + i * MAX_LAYERS + k] = d->cl2m[i * nl + k];
+        cm2l[(size_t)m * 64 + i * MAX_LAYERS + k] = (float)d->cm2l[i * nl + k];
+      }
+    for (int l = 0; l < nl; ++l) {
+      member.push_back(m);
+      layer.push_back(l);
+      lam.push_back((float)d->lambdas[l]);
+      const double den = amp_den0 + d->lambdas[l];
+      hp.push_back(!(den > 0.0) || amp_num / den > 1.0e4 ? 1 : 0);
+      wscale.push_back((float)d->wind_scale[l]);
+    }
+    // row increments of beta*(y - y0), formed in FP64 (the stage kernel never
+    // adds beta*y to q before differencing)
+    for (int j = 0; j < b.Ny; ++j) {
+      if (j + 1 < b.Ny) dbeta[(size_t)m * b.Ny + j] = (float)(d->beta_y_row[j + 1] - d->beta_y_row[j]);
+      wind[(size_t)m * b.Ny + j] = (float)d->wind_row[j];
+    }
+    P += nl;
+  }
+  b.P = P;
+  b.first = mfirst;
+  b.nl = mnl;
+  int *dfirst, *dnl, *dmember, *dlayer;
+  double* dcl2m;
+  float *dcm2l, *dlam, *dws;
+  unsigned char *dmask, *dhp;
+  if ((e = b.mem.upload(&dfirst, mfirst)) != hipSuccess || (e = b.mem.upload(&dnl, mnl)) != hipSuccess ||
+      (e = b.mem.upload(&dcl2m, cl2m)) != hipSuccess || (e = b.mem.upload(&dcm2l, cm2l)) != hipSuccess ||
+      (e = b.mem.upload(&dmask, mask)) != hipSuccess || (e = b.mem.upload(&dmember, member)) != hipSuccess ||
+      (e = b.mem.upload(&dlayer, layer)) != hipSuccess || (e = b.mem.upload(&dlam, lam)) != hipSuccess ||
+      (e = b.mem.upload(&dhp, hp)) != hipSuccess || (e = b.mem.upload(&dws, wscale)) != hipSuccess ||
+      (e = b.mem.upload(&b.dbeta_rows, dbeta)) != hipSuccess || (e = b.mem.upload(&b.wind_rows, wind)) != hipSuccess)
     return e;
-  SOMAX_UP(s.f32_full, ff, count);
-  SOMAX_UP(s.f32_int, fi, count);
-  SOMAX_UP(s.b_full, bf, 3 * count);
-  SOMAX_UP(s.b_int, bi, 3 * count);
-  SOMAX_UP(s.lam, lam, (size_t)N);
-#undef SOMAX_UP
-  return hipSuccess;
+  b.t = Tables{dfirst, dnl, dcl2m, dcm2l, dmask, dmember, dlayer, dlam, dhp, dws, nullptr};
+
+  // workspace: two GEMM operand buffers (3 bf16 pieces or 1.5 FP32 planes
+  // per plane), FP32 modal planes, stage states and psi.
+  const long maxN = b.Nx > b.Ny ? b.Nx : b.Ny;
+  b.wpitch = round_up(maxN, 64);
+  b.wpiece = maxN * b.wpitch;
+  b.splane = (long)b.Ny * b.Nx;
+  size_t off = 0;
+  auto take = [&](size_t bytes) {
+    const size_t o = off;
+    off += round_up((long)bytes, 256);
+    return o;
+  };
+  b.off_OP1 = take((size_t)P * 3 * b.wpiece * sizeof(uint16_t));
+  b.off_OP2 = take((size_t)P * 3 * b.wpiece * sizeof(uint16_t));
+  b.off_QM = take((size_t)P * b.splane * sizeof(float));
+  b.off_P0 = take((size_t)P * b.splane * sizeof(float));
+  if (b.precision == SOMAXHIP_PREC_MIXED) b.off_D = take((size_t)P * b.splane * sizeof(float));
+  b.off_Q1 = take((size_t)P * b.splane * sizeof(float));
+  b.off_Q2 = take((size_t)P * b.splane * sizeof(float));
+  b.off_PSI = take((size_t)P * b.splane * sizeof(float));
+  b.ws_bytes = off;
+  return prepare_dst_gemm();
+}
+
+static MemberParams to_member_params(const somaxhip_qg_params& p) {
+  return MemberParams{(float)p.nu, (float)p.nu4, (float)p.kappa, (float)p.tau0};
 }
 
 }  // namespace somax
@@ -108,24 +253,41 @@ static hipError_t build_sines(DeviceBlock& mem, SineSet& s, int N, double d) {
 using namespace somax;
 
 struct somaxhip_qg_plan_s {
-  int nx, ny, nl, Nx, Ny;
-  double dx, dy;
-  int zero_psi_ghost, advection, precision, mass_correction;
-  std::vector<double> cl2m, cm2l, lambdas, wind_scale;
-  DeviceBlock mem;
-  SineSet sx, sy;
-  float* dbeta_row = nullptr;
-  float* lam_dev = nullptr;  // Helmholtz parameter per mode (FP32)
-  float* wind_row = nullptr;
-  float* homsol = nullptr;
-  std::vector<double> homsol_mean;
-  // workspace layout (bytes offsets), all float planes
-  long wpitch = 0;  // pitch of GEMM operand planes (elements)
-  long wpiece = 0;  // elements of one bf16 piece plane (maxN * wpitch)
-  long splane = 0;  // Ny*Nx
-  int z_split = 0;  // modes [0, z_split) use the 3-piece inverse in the mixed solve
-  size_t off_OP1 = 0, off_OP2 = 0, off_QM = 0, off_P0 = 0, off_D = 0, off_Q1 = 0, off_Q2 = 0, off_PSI = 0;
-  size_t ws_bytes = 0;
+  Batch b;
+};
+
+struct somaxhip_ens_s {
+  std::vector<std::unique_ptr<Batch>> cls;  // grid classes, largest first
+  std::vector<int> mclass;                  // input member -> class
+  std::vector<long> moff;                   // input member -> float offset in the packed state
+  std::vector<size_t> cstate;               // class -> float offset of its planes
+  std::vector<size_t> cws;                  // class -> byte offset of its workspace
+  std::vector<std::vector<int>> cmembers;   // class -> input member of each local member
+  std::vector<MemberParams*> cparams;       // class -> device per-member parameters
+  std::vector<std::vector<MemberParams>> hparams;
+  std::vector<int> cbiharm;
+  size_t state_floats = 0, ws_bytes = 0;
+  int flags = 0;
+  // concurrency and graph state
+  hipStream_t main = nullptr;
+  std::vector<hipStream_t> side;
+  hipEvent_t ev_in = nullptr, ev_out = nullptr, ev_fork = nullptr;
+  std::vector<hipEvent_t> ev_join;
+  hipGraphExec_t exec = nullptr;
+  const void* g_q = nullptr;
+  const void* g_ws = nullptr;
+  double g_dt = 0.0;
+  bool stale = false;
+  ~somaxhip_ens_s() {
+    if (main) (void)hipStreamSynchronize(main);
+    if (exec) (void)hipGraphExecDestroy(exec);
+    for (hipStream_t s : side) (void)hipStreamDestroy(s);
+    if (main) (void)hipStreamDestroy(main);
+    for (hipEvent_t e : ev_join) (void)hipEventDestroy(e);
+    if (ev_in) (void)hipEventDestroy(ev_in);
+    if (ev_out) (void)hipEventDestroy(ev_out);
+    if (ev_fork) (void)hipEventDestroy(ev_fork);
+  }
 };
 
 extern "C" {
@@ -140,7 +302,7 @@ size_t somaxhip_last_error(char* buf, size_t len) {
   return m.size();
 }
 
-int somaxhip_version(void) { return 100; }
+int somaxhip_version(void) { return 200; }
 
 int somaxhip_init(void) {
   int dev = 0;
@@ -160,28 +322,9 @@ int somaxhip_qg_plan_create(const somaxhip_qg_desc* d, somaxhip_qg_plan* out) {
     return SOMAXHIP_ERR_ARG;
   }
   *out = nullptr;
-  if (d->nx < 2 || d->ny < 2 || d->nl < 1 || d->nl > MAX_LAYERS || !(d->dx > 0) || !(d->dy > 0)) {
-    set_error("somaxhip_qg_plan_create: need nx, ny >= 2, 1 <= nl <= 8, dx, dy > 0");
-    return SOMAXHIP_ERR_ARG;
-  }
-  if (!d->cl2m || !d->cm2l || !d->lambdas || !d->beta_y_row || !d->wind_row || !d->wind_scale) {
-    set_error("somaxhip_qg_plan_create: a required table pointer is null");
-    return SOMAXHIP_ERR_ARG;
-  }
-  if (d->advection != SOMAXHIP_ADV_ARAKAWA && d->advection != SOMAXHIP_ADV_WENO) {
-    set_error("somaxhip_qg_plan_create: unknown advection");
-    return SOMAXHIP_ERR_ARG;
-  }
-  if (d->precision != SOMAXHIP_PREC_FP32 && d->precision != SOMAXHIP_PREC_MIXED) {
-    set_error("somaxhip_qg_plan_create: unknown precision");
-    return SOMAXHIP_ERR_ARG;
-  }
-  if (d->mass_correction) {
-    set_error("somaxhip_qg_plan_create: mass correction is not available in this build");
-    return SOMAXHIP_ERR_ARG;
-  }
-  if (d->mass_correction && (!d->homsol || !d->homsol_mean)) {
-    set_error("somaxhip_qg_plan_create: mass_correction needs homsol and homsol_mean");
+  const std::string bad = check_desc(d);
+  if (!bad.empty()) {
+    set_error("somaxhip_qg_plan_create: " + bad);
     return SOMAXHIP_ERR_ARG;
   }
   somaxhip_qg_plan p = new (std::nothrow) somaxhip_qg_plan_s();
@@ -189,93 +332,12 @@ int somaxhip_qg_plan_create(const somaxhip_qg_desc* d, somaxhip_qg_plan* out) {
     set_error("somaxhip_qg_plan_create: out of host memory");
     return SOMAXHIP_ERR_ALLOC;
   }
-  p->nx = d->nx;
-  p->ny = d->ny;
-  p->nl = d->nl;
-  p->Nx = d->nx + 2;
-  p->Ny = d->ny + 2;
-  p->dx = d->dx;
-  p->dy = d->dy;
-  p->zero_psi_ghost = d->zero_psi_ghost ? 1 : 0;
-  p->advection = d->advection;
-  p->precision = d->precision;
-  p->mass_correction = d->mass_correction ? 1 : 0;
-  const int nl = d->nl;
-  p->cl2m.assign(d->cl2m, d->cl2m + nl * nl);
-  p->cm2l.assign(d->cm2l, d->cm2l + nl * nl);
-  p->lambdas.assign(d->lambdas, d->lambdas + nl);
-  p->wind_scale.assign(d->wind_scale, d->wind_scale + nl);
-
-  hipError_t e;
-  if ((e = build_sines(p->mem, p->sx, p->Nx, p->dx)) != hipSuccess ||
-      (e = build_sines(p->mem, p->sy, p->Ny, p->dy)) != hipSuccess) {
+  const hipError_t e = batch_build(p->b, {d});
+  if (e != hipSuccess) {
     set_error(std::string("somaxhip_qg_plan_create: ") + hipGetErrorString(e));
     delete p;
     return SOMAXHIP_ERR_ALLOC;
   }
-  // row increments of beta*(y - y0), formed in FP64 (the stage kernel never
-  // adds beta*y to q before differencing)
-  std::vector<float> dbeta(p->Ny, 0.0f), wind(p->Ny);
-  for (int j = 0; j < p->Ny; ++j) {
-    if (j + 1 < p->Ny) dbeta[j] = (float)(d->beta_y_row[j + 1] - d->beta_y_row[j]);
-    wind[j] = (float)d->wind_row[j];
-  }
-  if ((e = p->mem.alloc(&p->dbeta_row, p->Ny)) != hipSuccess ||
-      (e = hipMemcpy(p->dbeta_row, dbeta.data(), p->Ny * sizeof(float), hipMemcpyHostToDevice)) != hipSuccess ||
-      (e = p->mem.alloc(&p->wind_row, p->Ny)) != hipSuccess ||
-      (e = hipMemcpy(p->wind_row, wind.data(), p->Ny * sizeof(float), hipMemcpyHostToDevice)) != hipSuccess) {
-    set_error(std::string("somaxhip_qg_plan_create: ") + hipGetErrorString(e));
-    delete p;
-    return SOMAXHIP_ERR_ALLOC;
-  }
-  {
-    std::vector<float> lf(nl);
-    for (int m = 0; m < nl; ++m) lf[m] = (float)p->lambdas[m];
-    if ((e = p->mem.alloc(&p->lam_dev, nl)) != hipSuccess ||
-        (e = hipMemcpy(p->lam_dev, lf.data(), nl * sizeof(float), hipMemcpyHostToDevice)) != hipSuccess) {
-      set_error(std::string("somaxhip_qg_plan_create: ") + hipGetErrorString(e));
-      delete p;
-      return SOMAXHIP_ERR_ALLOC;
-    }
-  }
-  if (p->mass_correction) {
-    const size_t cnt = (size_t)nl * p->Ny * p->Nx;
-    if ((e = p->mem.alloc(&p->homsol, cnt)) != hipSuccess ||
-        (e = hipMemcpy(p->homsol, d->homsol, cnt * sizeof(float), hipMemcpyHostToDevice)) != hipSuccess) {
-      set_error(std::string("somaxhip_qg_plan_create: ") + hipGetErrorString(e));
-      delete p;
-      return SOMAXHIP_ERR_ALLOC;
-    }
-    p->homsol_mean.assign(d->homsol_mean, d->homsol_mean + nl);
-  }
-
-  // workspace layout: two GEMM operand buffers (3 bf16 pieces or 1.5 FP32
-  // planes per mode), FP32 modal planes, stage states and psi.
-  const long maxN = p->Nx > p->Ny ? p->Nx : p->Ny;
-  p->wpitch = round_up(maxN, 64);
-  p->wpiece = maxN * p->wpitch;
-  p->splane = (long)p->Ny * p->Nx;
-  size_t off = 0;
-  auto take = [&](size_t bytes) {
-    const size_t o = off;
-    off += round_up((long)bytes, 256);
-    return o;
-  };
-  p->off_OP1 = take((size_t)nl * 3 * p->wpiece * sizeof(uint16_t));
-  p->off_OP2 = take((size_t)nl * 3 * p->wpiece * sizeof(uint16_t));
-  p->off_QM = take((size_t)nl * p->splane * sizeof(float));
-  p->off_P0 = take((size_t)nl * p->splane * sizeof(float));
-  if (p->precision == SOMAXHIP_PREC_MIXED) p->off_D = take((size_t)nl * p->splane * sizeof(float));
-  p->off_Q1 = take((size_t)nl * p->splane * sizeof(float));
-  p->off_Q2 = take((size_t)nl * p->splane * sizeof(float));
-  p->off_PSI = take((size_t)nl * p->splane * sizeof(float));
-  // Poisson-like modes (small lambda: the Laplacian's conditioning is not
-  // capped) need the 3-piece inverse transform; modes come in ascending order.
-  const double amp_num = p->sx.lam_max + p->sy.lam_max;
-  p->z_split = 0;
-  for (int m = 0; m < nl; ++m)
-    if (amp_num / (p->sx.lam_min + p->sy.lam_min + p->lambdas[m]) > 1.0e4) p->z_split = m + 1;
-  p->ws_bytes = off;
   *out = p;
   return SOMAXHIP_OK;
 }
@@ -285,7 +347,7 @@ int somaxhip_qg_plan_destroy(somaxhip_qg_plan p) {
   return SOMAXHIP_OK;
 }
 
-size_t somaxhip_qg_workspace_bytes(somaxhip_qg_plan p) { return p ? p->ws_bytes : 0; }
+size_t somaxhip_qg_workspace_bytes(somaxhip_qg_plan p) { return p ? p->b.ws_bytes : 0; }
 
 }  // extern "C"
 
@@ -336,34 +398,30 @@ void gemm(const GemmArgs& a, int batch, hipStream_t st) {
   launch_dst_gemm(a, batch, st);
 }
 
-// One DST-I layer-space solve of (lap_h - B) psi = x.
-//   interior_input: the ghost ring of x is zero (q with its boundary
-//   condition), so the forward transform contracts over interior points only.
-//   out: psi in state layout (ld Nx); Oadd/mask as in GE_FINAL.
-// Products per precision class and the bf16 pieces they read.
+// Products of a GEMM class: NP_F32 selects the FP32 kernels; 1, 3, 6 bf16.
 constexpr int NP_F32 = 0;
 inline int pieces_for(int np) { return np == 1 ? 1 : (np == 3 ? 2 : 3); }
 
-// One DST-I solve per mode of (lap_h - lambda_m) y_m = x_m, from the GEMM
-// operand already in OP1 (interior rows/cols when interior_input) to the
-// FP32 modal planes `out` (state layout).  np_fwd: products of the forward
-// transforms; np_inv_lo / np_inv_hi: products of the inverse transforms for
-// modes below / from z_split.  NP_F32 selects the FP32 kernels throughout.
-void dst_solve(somaxhip_qg_plan p, bool interior_input, int np_fwd, int np_inv_lo, int np_inv_hi, float* out,
+// One DST-I solve per plane of (lap_h - lambda_z) y_z = x_z, from the GEMM
+// operand already in OP1 (interior rows/cols when interior_input: the ghost
+// ring of x is zero, so the forward transform contracts over interior points
+// only) to the FP32 modal planes `out` (state layout).  np_fwd: products of
+// the forward transforms; np_inv_lo / np_inv_hi: products of the inverse
+// transforms of Poisson-like / other planes.
+void dst_solve(const Batch& p, bool interior_input, int np_fwd, int np_inv_lo, int np_inv_hi, float* out,
                void* ws, hipStream_t st, bool acc64_fwd = true, bool acc64_inv = true) {
-  const int Nx = p->Nx, Ny = p->Ny, nl = p->nl;
+  const int Nx = p.Nx, Ny = p.Ny, P = p.P;
   const bool f32 = np_fwd == NP_F32;
-  char* OP1 = static_cast<char*>(ws) + p->off_OP1;
-  char* OP2 = static_cast<char*>(ws) + p->off_OP2;
-  const long pb16 = 3 * p->wpiece;      // bf16 batch stride (elements)
-  const long pb32 = 3 * p->wpiece / 2;  // fp32 batch stride (floats)
+  char* OP1 = static_cast<char*>(ws) + p.off_OP1;
+  char* OP2 = static_cast<char*>(ws) + p.off_OP2;
+  const long pb16 = 3 * p.wpiece;      // bf16 batch stride (elements)
+  const long pb32 = 3 * p.wpiece / 2;  // fp32 batch stride (floats)
   const int R0 = interior_input ? Ny - 2 : Ny;
   const int K0 = interior_input ? Nx - 2 : Nx;
-  const long count_x = p->sx.rows * p->sx.ld, count_y = p->sy.rows * p->sy.ld;
 
-  auto base = [&](GemmArgs& g, char* X, int xrows, int K, const SineSet& s, bool interior_s, int np_lo, int np_hi,
-                  int zs) {
-    g.ldx = p->wpitch;
+  auto base = [&](GemmArgs& g, char* X, int xrows, int K, const SineSet& s, bool interior_s, int np_lo,
+                  int np_hi) {
+    g.ldx = p.wpitch;
     g.xrows = xrows;
     g.K = K;
     if (f32) {
@@ -372,24 +430,24 @@ void dst_solve(somaxhip_qg_plan p, bool interior_input, int np_fwd, int np_inv_l
       g.Sf = interior_s ? s.f32_int : s.f32_full;
     } else {
       g.Xb = reinterpret_cast<const uint16_t*>(X);
-      g.x_piece = p->wpiece;
+      g.x_piece = p.wpiece;
       g.x_batch = pb16;
       g.Sb = interior_s ? s.b_int : s.b_full;
       g.s_piece = s.rows * s.ld;
     }
     g.lds = s.ld;
-    g.z_split = zs;
+    g.hp = p.t.hp;
     g.np_lo = np_lo;
     g.np_hi = np_hi;
   };
   auto out_op = [&](GemmArgs& g, char* O, int op_lo, int op_hi) {
-    g.ldo = p->wpitch;
+    g.ldo = p.wpitch;
     if (f32) {
       g.Of = reinterpret_cast<float*>(O);
       g.o_batch = pb32;
     } else {
       g.Ob = reinterpret_cast<uint16_t*>(O);
-      g.o_piece = p->wpiece;
+      g.o_piece = p.wpiece;
       g.o_batch = pb16;
       g.op_lo = op_lo;
       g.op_hi = op_hi;
@@ -398,163 +456,184 @@ void dst_solve(somaxhip_qg_plan p, bool interior_input, int np_fwd, int np_inv_l
   const int pf = f32 ? 0 : pieces_for(np_fwd);
   const int pil = f32 ? 0 : pieces_for(np_inv_lo);
   const int pih = f32 ? 0 : pieces_for(np_inv_hi);
-  const int zs = p->z_split;
-  (void)count_x;
-  (void)count_y;
 
   // F1: T[xm][y] = sum_x X[y][x] Sx[xm][x]          (S in the A slot)
   GemmArgs g{};
-  base(g, OP1, R0, K0, p->sx, interior_input, np_fwd, np_fwd, 0);
+  base(g, OP1, R0, K0, p.sx, interior_input, np_fwd, np_fwd);
   g.trans = 1;
   out_op(g, OP2, pf, pf);
   g.acc64 = acc64_fwd;
   g.RA = Nx;
   g.RB = R0;
   g.RBpad = (int)round_up(R0, 32);
-  gemm(g, nl, st);
+  gemm(g, P, st);
 
   // F2: W[xm][ym] = sum_y T[xm][y] Sy[ym][y] / (lamx + lamy - lambda)
   GemmArgs h{};
-  base(h, OP2, Nx, R0, p->sy, interior_input, np_fwd, np_fwd, 0);
+  base(h, OP2, Nx, R0, p.sy, interior_input, np_fwd, np_fwd);
   h.trans = 0;
   out_op(h, OP1, pil, pih);
   h.acc64 = acc64_fwd;
-  h.z_split = zs;
   h.RA = Nx;
   h.RB = Ny;
   h.RBpad = (int)round_up(Ny, 32);
-  h.lamA = p->sx.lam;
-  h.lamB = p->sy.lam;
-  h.lam = p->lam_dev;
-  gemm(h, nl, st);
+  h.lamA = p.sx.lam;
+  h.lamB = p.sy.lam;
+  h.lam = p.t.lam;
+  gemm(h, P, st);
 
   // I1: V[y][xm] = sum_ym Sy[y][ym] W[xm][ym]        (S in the A slot)
   GemmArgs u{};
-  base(u, OP1, Nx, Ny, p->sy, false, np_inv_lo, np_inv_hi, zs);
+  base(u, OP1, Nx, Ny, p.sy, false, np_inv_lo, np_inv_hi);
   u.trans = 1;
   out_op(u, OP2, pil, pih);
   u.acc64 = acc64_inv;
   u.RA = Ny;
   u.RB = Nx;
   u.RBpad = (int)round_up(Nx, 32);
-  gemm(u, nl, st);
+  gemm(u, P, st);
 
   // I2: out[y][x] = sum_xm V[y][xm] Sx[x][xm]
   GemmArgs w{};
-  base(w, OP2, Ny, Nx, p->sx, false, np_inv_lo, np_inv_hi, zs);
+  base(w, OP2, Ny, Nx, p.sx, false, np_inv_lo, np_inv_hi);
   w.trans = 0;
   w.Of = out;
   w.Ob = nullptr;
   w.ldo = Nx;
-  w.o_batch = p->splane;
+  w.o_batch = p.splane;
   w.op_lo = w.op_hi = 0;
   w.acc64 = acc64_inv;
   w.RA = Ny;
   w.RB = Nx;
   w.RBpad = Nx;
-  gemm(w, nl, st);
+  gemm(w, P, st);
 }
 
-void invert(somaxhip_qg_plan p, const float* q, float* psi, void* ws, hipStream_t st, bool apply_bc = true) {
-  const int nl = p->nl;
-  const bool mixed = p->precision == SOMAXHIP_PREC_MIXED;
-  float* QM = wsf(ws, p->off_QM);
-  float* P0 = wsf(ws, p->off_P0);
+void invert(const Batch& p, const float* q, float* psi, void* ws, hipStream_t st, bool apply_bc = true) {
+  const bool mixed = p.precision == SOMAXHIP_PREC_MIXED;
+  float* QM = wsf(ws, p.off_QM);
+  float* P0 = wsf(ws, p.off_P0);
   ModalArgs ma{};
   ma.q = q;
   ma.qm = QM;
-  ma.Nx = p->Nx;
-  ma.Ny = p->Ny;
-  ma.nl = nl;
-  ma.plane = p->splane;
+  ma.Nx = p.Nx;
+  ma.Ny = p.Ny;
+  ma.M = p.M;
+  ma.plane = p.splane;
   ma.apply_bc = apply_bc ? 1 : 0;
-  for (int a = 0; a < nl; ++a)
-    for (int b = 0; b < nl; ++b) ma.cl2m[a * MAX_LAYERS + b] = p->cl2m[a * nl + b];
-  ma.op = static_cast<char*>(ws) + p->off_OP1;
+  ma.t = p.t;
+  ma.op = static_cast<char*>(ws) + p.off_OP1;
   ma.op_pieces = mixed ? 2 : 0;
-  ma.op_ld = p->wpitch;
-  ma.op_piece = p->wpiece;
-  ma.op_batch = mixed ? 3 * p->wpiece : 3 * p->wpiece / 2;
-  ma.op_kpad = (int)round_up(apply_bc ? p->Nx - 2 : p->Nx, 32);
+  ma.op_ld = p.wpitch;
+  ma.op_piece = p.wpiece;
+  ma.op_batch = mixed ? 3 * p.wpiece : 3 * p.wpiece / 2;
+  ma.op_kpad = (int)round_up(apply_bc ? p.Nx - 2 : p.Nx, 32);
   {
-    ProfScope ps("to_modal", 0.0, 8.0 * nl * (double)p->splane, st);
+    ProfScope ps("to_modal", 0.0, 8.0 * p.P * (double)p.splane, st);
     launch_to_modal(ma, st);
   }
   LayerArgs la{};
-  la.p = P0;
+  la.pm = P0;
   la.psi = psi;
-  la.Nx = p->Nx;
-  la.Ny = p->Ny;
-  la.nl = nl;
-  la.plane = p->splane;
-  la.mask = p->zero_psi_ghost;
-  for (int a = 0; a < nl; ++a)
-    for (int b = 0; b < nl; ++b) la.cm2l[a * MAX_LAYERS + b] = (float)p->cm2l[a * nl + b];
+  la.Nx = p.Nx;
+  la.Ny = p.Ny;
+  la.M = p.M;
+  la.plane = p.splane;
+  la.t = p.t;
   if (!mixed) {
     dst_solve(p, apply_bc, NP_F32, NP_F32, NP_F32, P0, ws, st);
   } else {
     // bf16x3 forward, x6 (Poisson-like modes) / x3 inverse; FP32 per-mode
     // Helmholtz residual; correction with bf16x3 forward and bf16 inverse.
     dst_solve(p, apply_bc, 3, 6, 3, P0, ws, st);
-    float* D = wsf(ws, p->off_D);
+    float* D = wsf(ws, p.off_D);
     ResidualArgs r{};
     r.q = QM;
     r.psi = P0;
-    r.r = reinterpret_cast<uint16_t*>(static_cast<char*>(ws) + p->off_OP1);
-    r.op_ld = p->wpitch;
-    r.op_piece = p->wpiece;
-    r.op_batch = 3 * p->wpiece;
-    r.op_kpad = (int)round_up(p->Nx, 32);
-    r.plane = p->splane;
-    r.Nx = p->Nx;
-    r.Ny = p->Ny;
-    r.nl = nl;
-    r.idx2 = (float)(1.0 / (p->dx * p->dx));
-    r.idy2 = (float)(1.0 / (p->dy * p->dy));
-    for (int m = 0; m < nl; ++m) r.lam[m] = (float)p->lambdas[m];
+    r.r = reinterpret_cast<uint16_t*>(static_cast<char*>(ws) + p.off_OP1);
+    r.op_ld = p.wpitch;
+    r.op_piece = p.wpiece;
+    r.op_batch = 3 * p.wpiece;
+    r.op_kpad = (int)round_up(p.Nx, 32);
+    r.plane = p.splane;
+    r.Nx = p.Nx;
+    r.Ny = p.Ny;
+    r.P = p.P;
+    r.idx2 = (float)(1.0 / (p.dx * p.dx));
+    r.idy2 = (float)(1.0 / (p.dy * p.dy));
+    r.lam = p.t.lam;
     {
-      ProfScope ps("residual", 0.0, 12.0 * nl * (double)p->splane, st);
+      ProfScope ps("residual", 0.0, 12.0 * p.P * (double)p.splane, st);
       launch_residual(r, st);
     }
     dst_solve(p, false, 3, 1, 1, D, ws, st, true, false);
     la.d = D;
   }
   {
-    ProfScope ps("to_layer", 0.0, (mixed ? 12.0 : 8.0) * nl * (double)p->splane, st);
+    ProfScope ps("to_layer", 0.0, (mixed ? 12.0 : 8.0) * p.P * (double)p.splane, st);
     launch_to_layer(la, st);
   }
 }
 
-void stage(somaxhip_qg_plan p, const float* qs, const float* psi, const float* qn, float* out,
-           const somaxhip_qg_params* prm, float c1, float c2, bool apply_bc, hipStream_t st) {
+// Parameters of a call: per member (device array) or one uniform set.
+struct CallParams {
+  const MemberParams* mp;
+  MemberParams u;
+  int biharm;
+};
+
+void stage(const Batch& p, const float* qs, const float* psi, const float* qn, float* out, const CallParams& cp,
+           float c1, float c2, bool apply_bc, hipStream_t st) {
   StageArgs a{};
   a.qs = qs;
   a.psi = psi;
   a.qn = qn;
   a.out = out;
-  a.Nx = p->Nx;
-  a.Ny = p->Ny;
-  a.nl = p->nl;
-  a.plane = p->splane;
-  a.dbeta_row = p->dbeta_row;
-  a.wind_row = p->wind_row;
-  for (int l = 0; l < p->nl; ++l) a.wind_coef[l] = (float)(prm->tau0 * p->wind_scale[l]);
-  a.nu = (float)prm->nu;
-  a.nu4 = (float)prm->nu4;
-  a.kappa = (float)prm->kappa;
-  a.idx2 = (float)(1.0 / (p->dx * p->dx));
-  a.idy2 = (float)(1.0 / (p->dy * p->dy));
-  a.iarakawa = (float)(1.0 / (12.0 * p->dx * p->dy));
-  a.idx = (float)(1.0 / p->dx);
-  a.idy = (float)(1.0 / p->dy);
+  a.Nx = p.Nx;
+  a.Ny = p.Ny;
+  a.P = p.P;
+  a.plane = p.splane;
+  a.dbeta_rows = p.dbeta_rows;
+  a.wind_rows = p.wind_rows;
+  a.t = p.t;
+  a.mp = cp.mp;
+  a.u = cp.u;
+  a.biharm = cp.biharm;
+  a.idx2 = (float)(1.0 / (p.dx * p.dx));
+  a.idy2 = (float)(1.0 / (p.dy * p.dy));
+  a.iarakawa = (float)(1.0 / (12.0 * p.dx * p.dy));
+  a.idx = (float)(1.0 / p.dx);
+  a.idy = (float)(1.0 / p.dy);
   a.c1 = c1;
   a.c2 = c2;
-  a.advection = p->advection;
+  a.advection = p.advection;
   a.apply_bc = apply_bc ? 1 : 0;
   // algorithmic bytes: read qs and psi, read qn when combined, write out
-  const double bytes = 4.0 * p->nl * (double)p->nx * p->ny * (3.0 + (qn ? 1.0 : 0.0));
+  const double bytes = 4.0 * p.P * (double)p.nx * p.ny * (3.0 + (qn ? 1.0 : 0.0));
   ProfScope ps("qg_stage", 0.0, bytes, st);
   launch_qg_stage(a, st);
+}
+
+// nsteps SSP-RK3 steps of the batch in place (Shu-Osher form, boundary
+// condition before every RHS).
+void rk3(const Batch& p, float* q, float* psi_last, int nsteps, double dt, const CallParams& cp, void* ws,
+         hipStream_t st) {
+  float* psi = psi_last ? psi_last : wsf(ws, p.off_PSI);
+  float* q1 = wsf(ws, p.off_Q1);
+  float* q2 = wsf(ws, p.off_Q2);
+  const float h = (float)dt;
+  for (int n = 0; n < nsteps; ++n) {
+    invert(p, q, psi, ws, st);
+    stage(p, q, psi, nullptr, q1, cp, 1.0f, h, true, st);
+    invert(p, q1, psi, ws, st);
+    stage(p, q1, psi, q, q2, cp, 0.25f, 0.25f * h, true, st);
+    invert(p, q2, psi, ws, st);
+    stage(p, q2, psi, q, q, cp, 2.0f / 3.0f, 2.0f / 3.0f * h, true, st);
+  }
+}
+
+CallParams uniform_params(const somaxhip_qg_params* prm) {
+  return CallParams{nullptr, to_member_params(*prm), prm->nu4 != 0.0 ? 1 : 0};
 }
 
 int check_call(somaxhip_qg_plan p, const void* a, const void* b, void* ws, const char* who) {
@@ -565,88 +644,9 @@ int check_call(somaxhip_qg_plan p, const void* a, const void* b, void* ws, const
   return SOMAXHIP_OK;
 }
 
-}  // namespace
-
-extern "C" {
-
-int somaxhip_qg_invert(somaxhip_qg_plan p, const float* q, float* psi, int apply_bc, void* ws,
-                       somaxhip_stream_t stream) {
-  int rc = check_call(p, q, psi, ws, "somaxhip_qg_invert");
-  if (rc) return rc;
-  hipStream_t st = reinterpret_cast<hipStream_t>(stream);
-  invert(p, q, psi, ws, st, apply_bc != 0);
-  SOMAX_HIP_CHECK(hipGetLastError());
-  return SOMAXHIP_OK;
-}
-
-int somaxhip_qg_rhs(somaxhip_qg_plan p, const float* q, float* dq, float* psi_out,
-                    const somaxhip_qg_params* prm, int apply_bc, void* ws, somaxhip_stream_t stream) {
-  int rc = check_call(p, q, dq, ws, "somaxhip_qg_rhs");
-  if (rc) return rc;
-  if (!prm) {
-    set_error("somaxhip_qg_rhs: null params");
-    return SOMAXHIP_ERR_ARG;
-  }
-  hipStream_t st = reinterpret_cast<hipStream_t>(stream);
-  float* psi = psi_out ? psi_out : wsf(ws, p->off_PSI);
-  invert(p, q, psi, ws, st, apply_bc != 0);
-  stage(p, q, psi, nullptr, dq, prm, 0.0f, 1.0f, apply_bc != 0, st);
-  SOMAX_HIP_CHECK(hipGetLastError());
-  return SOMAXHIP_OK;
-}
-
-int somaxhip_qg_step(somaxhip_qg_plan p, float* q, float* psi_out, int nsteps, double dt,
-                     const somaxhip_qg_params* prm, void* ws, somaxhip_stream_t stream) {
-  int rc = check_call(p, q, q, ws, "somaxhip_qg_step");
-  if (rc) return rc;
-  if (!prm || nsteps < 0) {
-    set_error("somaxhip_qg_step: null params or negative nsteps");
-    return SOMAXHIP_ERR_ARG;
-  }
-  hipStream_t st = reinterpret_cast<hipStream_t>(stream);
-  float* psi = wsf(ws, p->off_PSI);
-  float* q1 = wsf(ws, p->off_Q1);
-  float* q2 = wsf(ws, p->off_Q2);
-  const float h = (float)dt;
-  for (int n = 0; n < nsteps; ++n) {
-    invert(p, q, psi, ws, st);
-    stage(p, q, psi, nullptr, q1, prm, 1.0f, h, true, st);
-    invert(p, q1, psi, ws, st);
-    stage(p, q1, psi, q, q2, prm, 0.25f, 0.25f * h, true, st);
-    invert(p, q2, psi, ws, st);
-    stage(p, q2, psi, q, q, prm, 2.0f / 3.0f, 2.0f / 3.0f * h, true, st);
-  }
-  if (psi_out && nsteps > 0)
-    SOMAX_HIP_CHECK(hipMemcpyAsync(psi_out, psi, (size_t)p->nl * p->splane * sizeof(float),
-                                   hipMemcpyDeviceToDevice, st));
-  SOMAX_HIP_CHECK(hipGetLastError());
-  return SOMAXHIP_OK;
-}
-
-}  // extern "C"
-
-extern "C" int somaxhip_zero_ghost(float* f, int nl, int ny, int nx, somaxhip_stream_t stream) {
-  if (!f || nl < 1 || nx < 1 || ny < 1) {
-    set_error("somaxhip_zero_ghost: null field or bad shape");
-    return SOMAXHIP_ERR_ARG;
-  }
-  launch_zero_ghost(f, nl, ny + 2, nx + 2, reinterpret_cast<hipStream_t>(stream));
-  SOMAX_HIP_CHECK(hipGetLastError());
-  return SOMAXHIP_OK;
-}
-
-extern "C" int somaxhip_qg_step_profile(somaxhip_qg_plan p, float* q, int nsteps, double dt,
-                                        const somaxhip_qg_params* prm, void* ws, somaxhip_stream_t stream,
-                                        somaxhip_kernel_stat* stats, int max_stats, int* nstats) {
-  if (!stats || !nstats || max_stats < 1) {
-    set_error("somaxhip_qg_step_profile: null stats");
-    return SOMAXHIP_ERR_ARG;
-  }
-  Profile prof;
-  g_prof = &prof;
-  const int rc = somaxhip_qg_step(p, q, nullptr, nsteps, dt, prm, ws, stream);
-  g_prof = nullptr;
-  hipError_t e = hipStreamSynchronize(reinterpret_cast<hipStream_t>(stream));
+// Sums the events of a profiled run into per-name totals.
+int collect_profile(Profile& prof, hipStream_t st, somaxhip_kernel_stat* stats, int max_stats, int* nstats) {
+  hipError_t e = hipStreamSynchronize(st);
   int n = 0;
   for (auto& en : prof.entries) {
     float ms = 0.0f;
@@ -667,7 +667,318 @@ extern "C" int somaxhip_qg_step_profile(somaxhip_qg_plan p, float* q, int nsteps
     stats[k].bytes += en.bytes;
   }
   *nstats = n;
-  if (rc) return rc;
   SOMAX_HIP_CHECK(e);
   return SOMAXHIP_OK;
 }
+
+}  // namespace
+
+extern "C" {
+
+int somaxhip_qg_invert(somaxhip_qg_plan p, const float* q, float* psi, int apply_bc, void* ws,
+                       somaxhip_stream_t stream) {
+  int rc = check_call(p, q, psi, ws, "somaxhip_qg_invert");
+  if (rc) return rc;
+  hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+  invert(p->b, q, psi, ws, st, apply_bc != 0);
+  SOMAX_HIP_CHECK(hipGetLastError());
+  return SOMAXHIP_OK;
+}
+
+int somaxhip_qg_rhs(somaxhip_qg_plan p, const float* q, float* dq, float* psi_out,
+                    const somaxhip_qg_params* prm, int apply_bc, void* ws, somaxhip_stream_t stream) {
+  int rc = check_call(p, q, dq, ws, "somaxhip_qg_rhs");
+  if (rc) return rc;
+  if (!prm) {
+    set_error("somaxhip_qg_rhs: null params");
+    return SOMAXHIP_ERR_ARG;
+  }
+  hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+  float* psi = psi_out ? psi_out : wsf(ws, p->b.off_PSI);
+  invert(p->b, q, psi, ws, st, apply_bc != 0);
+  stage(p->b, q, psi, nullptr, dq, uniform_params(prm), 0.0f, 1.0f, apply_bc != 0, st);
+  SOMAX_HIP_CHECK(hipGetLastError());
+  return SOMAXHIP_OK;
+}
+
+int somaxhip_qg_step(somaxhip_qg_plan p, float* q, float* psi_out, int nsteps, double dt,
+                     const somaxhip_qg_params* prm, void* ws, somaxhip_stream_t stream) {
+  int rc = check_call(p, q, q, ws, "somaxhip_qg_step");
+  if (rc) return rc;
+  if (!prm || nsteps < 0) {
+    set_error("somaxhip_qg_step: null params or negative nsteps");
+    return SOMAXHIP_ERR_ARG;
+  }
+  hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+  rk3(p->b, q, psi_out, nsteps, dt, uniform_params(prm), ws, st);
+  SOMAX_HIP_CHECK(hipGetLastError());
+  return SOMAXHIP_OK;
+}
+
+int somaxhip_zero_ghost(float* f, int nl, int ny, int nx, somaxhip_stream_t stream) {
+  if (!f || nl < 1 || nx < 1 || ny < 1) {
+    set_error("somaxhip_zero_ghost: null field or bad shape");
+    return SOMAXHIP_ERR_ARG;
+  }
+  launch_zero_ghost(f, nl, ny + 2, nx + 2, reinterpret_cast<hipStream_t>(stream));
+  SOMAX_HIP_CHECK(hipGetLastError());
+  return SOMAXHIP_OK;
+}
+
+int somaxhip_qg_step_profile(somaxhip_qg_plan p, float* q, int nsteps, double dt, const somaxhip_qg_params* prm,
+                             void* ws, somaxhip_stream_t stream, somaxhip_kernel_stat* stats, int max_stats,
+                             int* nstats) {
+  if (!stats || !nstats || max_stats < 1) {
+    set_error("somaxhip_qg_step_profile: null stats");
+    return SOMAXHIP_ERR_ARG;
+  }
+  Profile prof;
+  g_prof = &prof;
+  const int rc = somaxhip_qg_step(p, q, nullptr, nsteps, dt, prm, ws, stream);
+  g_prof = nullptr;
+  const int rc2 = collect_profile(prof, reinterpret_cast<hipStream_t>(stream), stats, max_stats, nstats);
+  return rc ? rc : rc2;
+}
+
+// ---------------------------------------------------------------------------
+// Ensembles
+// ---------------------------------------------------------------------------
+
+int somaxhip_ens_create(const somaxhip_qg_desc* members, const somaxhip_qg_params* params, int n, int flags,
+                        somaxhip_ens* out) {
+  if (!members || !params || !out || n < 1) {
+    set_error("somaxhip_ens_create: null argument or no members");
+    return SOMAXHIP_ERR_ARG;
+  }
+  *out = nullptr;
+  if (flags & ~(SOMAXHIP_ENS_CONCURRENT | SOMAXHIP_ENS_GRAPH)) {
+    set_error("somaxhip_ens_create: unknown flags");
+    return SOMAXHIP_ERR_ARG;
+  }
+  for (int i = 0; i < n; ++i) {
+    const std::string bad = check_desc(&members[i]);
+    if (!bad.empty()) {
+      set_error("somaxhip_ens_create: member " + std::to_string(i) + ": " + bad);
+      return SOMAXHIP_ERR_ARG;
+    }
+  }
+  std::unique_ptr<somaxhip_ens_s> E(new (std::nothrow) somaxhip_ens_s());
+  if (!E) {
+    set_error("somaxhip_ens_create: out of host memory");
+    return SOMAXHIP_ERR_ALLOC;
+  }
+  E->flags = flags;
+  // group members into classes of one grid; classes ordered by plane size,
+  // largest first, so the longest launches are queued first
+  std::vector<int> rep;  // class -> representative member
+  E->mclass.assign(n, -1);
+  for (int i = 0; i < n; ++i) {
+    int c = 0;
+    while (c < (int)rep.size() && !same_class(&members[rep[c]], &members[i])) ++c;
+    if (c == (int)rep.size()) rep.push_back(i);
+    E->mclass[i] = c;
+  }
+  const int C = (int)rep.size();
+  std::vector<int> order(C);
+  for (int c = 0; c < C; ++c) order[c] = c;
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+    const long sa = (long)(members[rep[a]].nx + 2) * (members[rep[a]].ny + 2);
+    const long sb = (long)(members[rep[b]].nx + 2) * (members[rep[b]].ny + 2);
+    return sa > sb;
+  });
+  std::vector<int> rank(C);
+  for (int k = 0; k < C; ++k) rank[order[k]] = k;
+  for (int i = 0; i < n; ++i) E->mclass[i] = rank[E->mclass[i]];
+  E->cmembers.assign(C, {});
+  for (int i = 0; i < n; ++i) E->cmembers[E->mclass[i]].push_back(i);
+
+  E->moff.assign(n, 0);
+  size_t state = 0, ws = 0;
+  for (int c = 0; c < C; ++c) {
+    std::vector<const somaxhip_qg_desc*> ds;
+    for (int i : E->cmembers[c]) ds.push_back(&members[i]);
+    std::unique_ptr<Batch> b(new (std::nothrow) Batch());
+    if (!b) {
+      set_error("somaxhip_ens_create: out of host memory");
+      return SOMAXHIP_ERR_ALLOC;
+    }
+    hipError_t e = batch_build(*b, ds);
+    MemberParams* dp = nullptr;
+    std::vector<MemberParams> hp;
+    int biharm = 0;
+    for (int i : E->cmembers[c]) {
+      hp.push_back(to_member_params(params[i]));
+      biharm |= params[i].nu4 != 0.0;
+    }
+    if (e == hipSuccess) e = b->mem.upload(&dp, hp);
+    if (e != hipSuccess) {
+      set_error(std::string("somaxhip_ens_create: ") + hipGetErrorString(e));
+      return SOMAXHIP_ERR_ALLOC;
+    }
+    E->cstate.push_back(state);
+    E->cws.push_back(ws);
+    for (int k = 0; k < b->M; ++k) E->moff[E->cmembers[c][k]] = (long)(state + (size_t)b->first[k] * b->splane);
+    state += (size_t)b->P * b->splane;
+    state = (size_t)round_up((long)state, 64);  // 256-byte aligned class blocks
+    ws += (size_t)round_up((long)b->ws_bytes, 256);
+    E->cparams.push_back(dp);
+    E->hparams.push_back(hp);
+    E->cbiharm.push_back(biharm);
+    E->cls.push_back(std::move(b));
+  }
+  E->state_floats = state;
+  E->ws_bytes = ws;
+  // streams (default priority) and events for the class branches
+  SOMAX_HIP_CHECK(hipStreamCreateWithFlags(&E->main, hipStreamNonBlocking));
+  SOMAX_HIP_CHECK(hipEventCreateWithFlags(&E->ev_in, hipEventDisableTiming));
+  SOMAX_HIP_CHECK(hipEventCreateWithFlags(&E->ev_out, hipEventDisableTiming));
+  SOMAX_HIP_CHECK(hipEventCreateWithFlags(&E->ev_fork, hipEventDisableTiming));
+  for (int c = 0; c < C; ++c) {
+    hipStream_t s;
+    hipEvent_t ev;
+    SOMAX_HIP_CHECK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    E->side.push_back(s);
+    SOMAX_HIP_CHECK(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+    E->ev_join.push_back(ev);
+  }
+  *out = E.release();
+  return SOMAXHIP_OK;
+}
+
+int somaxhip_ens_destroy(somaxhip_ens e) {
+  delete e;
+  return SOMAXHIP_OK;
+}
+
+size_t somaxhip_ens_state_floats(somaxhip_ens e) { return e ? e->state_floats : 0; }
+
+size_t somaxhip_ens_workspace_bytes(somaxhip_ens e) { return e ? e->ws_bytes : 0; }
+
+long somaxhip_ens_member_offset(somaxhip_ens e, int member) {
+  if (!e || member < 0 || member >= (int)e->moff.size()) {
+    set_error("somaxhip_ens_member_offset: bad ensemble or member index");
+    return -1;
+  }
+  return e->moff[member];
+}
+
+int somaxhip_ens_num_classes(somaxhip_ens e) { return e ? (int)e->cls.size() : 0; }
+
+int somaxhip_ens_set_params(somaxhip_ens e, const somaxhip_qg_params* params, somaxhip_stream_t stream) {
+  if (!e || !params) {
+    set_error("somaxhip_ens_set_params: null argument");
+    return SOMAXHIP_ERR_ARG;
+  }
+  hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+  for (size_t c = 0; c < e->cls.size(); ++c) {
+    int biharm = 0;
+    for (size_t k = 0; k < e->cmembers[c].size(); ++k) {
+      const somaxhip_qg_params& p = params[e->cmembers[c][k]];
+      e->hparams[c][k] = to_member_params(p);
+      biharm |= p.nu4 != 0.0;
+    }
+    if (biharm != e->cbiharm[c]) e->stale = true;  // the captured step picked the other stage kernel
+    e->cbiharm[c] = biharm;
+    SOMAX_HIP_CHECK(hipMemcpyAsync(e->cparams[c], e->hparams[c].data(), e->hparams[c].size() * sizeof(MemberParams),
+                                   hipMemcpyHostToDevice, st));
+  }
+  // the pageable source is reused by the next call: wait for the copies
+  SOMAX_HIP_CHECK(hipStreamSynchronize(st));
+  return SOMAXHIP_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+// One ensemble step queued behind `st`: each class advances on its own
+// stream when concurrent (fork/join through events, which stream capture
+// turns into parallel graph branches), else all on `st`.
+int ens_queue(somaxhip_ens e, float* q, int nsteps, double dt, void* ws, hipStream_t st) {
+  const int C = (int)e->cls.size();
+  const bool conc = (e->flags & SOMAXHIP_ENS_CONCURRENT) && C > 1;
+  if (conc) {
+    SOMAX_HIP_CHECK(hipEventRecord(e->ev_fork, st));
+    for (int c = 0; c < C; ++c) SOMAX_HIP_CHECK(hipStreamWaitEvent(e->side[c], e->ev_fork, 0));
+  }
+  for (int c = 0; c < C; ++c) {
+    const Batch& b = *e->cls[c];
+    const CallParams cp{e->cparams[c], MemberParams{0, 0, 0, 0}, e->cbiharm[c]};
+    rk3(b, q + e->cstate[c], nullptr, nsteps, dt, cp, static_cast<char*>(ws) + e->cws[c], conc ? e->side[c] : st);
+  }
+  if (conc) {
+    for (int c = 0; c < C; ++c) {
+      SOMAX_HIP_CHECK(hipEventRecord(e->ev_join[c], e->side[c]));
+      SOMAX_HIP_CHECK(hipStreamWaitEvent(st, e->ev_join[c], 0));
+    }
+  }
+  SOMAX_HIP_CHECK(hipGetLastError());
+  return SOMAXHIP_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int somaxhip_ens_step(somaxhip_ens e, float* q, int nsteps, double dt, void* ws, somaxhip_stream_t stream) {
+  if (!e || !q || !ws || nsteps < 0) {
+    set_error("somaxhip_ens_step: null ensemble, state or workspace, or negative nsteps");
+    return SOMAXHIP_ERR_ARG;
+  }
+  hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+  if (nsteps == 0) return SOMAXHIP_OK;
+  if (!(e->flags & SOMAXHIP_ENS_GRAPH)) return ens_queue(e, q, nsteps, dt, ws, st);
+
+  // capture one step on the private stream and replay it there, ordered
+  // after the caller's stream and before its later work by two events
+  if (!e->exec || e->stale || e->g_q != q || e->g_ws != ws || e->g_dt != dt) {
+    if (e->exec) {
+      SOMAX_HIP_CHECK(hipStreamSynchronize(e->main));  // replays of the old graph may be queued
+      (void)hipGraphExecDestroy(e->exec);
+      e->exec = nullptr;
+    }
+    e->stale = false;
+    hipGraph_t graph = nullptr;
+    SOMAX_HIP_CHECK(hipStreamBeginCapture(e->main, hipStreamCaptureModeThreadLocal));
+    const int rc = ens_queue(e, q, 1, dt, ws, e->main);
+    const hipError_t ec = hipStreamEndCapture(e->main, &graph);
+    if (rc) {
+      if (graph) (void)hipGraphDestroy(graph);
+      return rc;
+    }
+    SOMAX_HIP_CHECK(ec);
+    const hipError_t ei = hipGraphInstantiate(&e->exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    SOMAX_HIP_CHECK(ei);
+    e->g_q = q;
+    e->g_ws = ws;
+    e->g_dt = dt;
+  }
+  SOMAX_HIP_CHECK(hipEventRecord(e->ev_in, st));
+  SOMAX_HIP_CHECK(hipStreamWaitEvent(e->main, e->ev_in, 0));
+  for (int n = 0; n < nsteps; ++n) SOMAX_HIP_CHECK(hipGraphLaunch(e->exec, e->main));
+  SOMAX_HIP_CHECK(hipEventRecord(e->ev_out, e->main));
+  SOMAX_HIP_CHECK(hipStreamWaitEvent(st, e->ev_out, 0));
+  return SOMAXHIP_OK;
+}
+
+int somaxhip_ens_step_profile(somaxhip_ens e, float* q, int nsteps, double dt, void* ws, somaxhip_stream_t stream,
+                              somaxhip_kernel_stat* stats, int max_stats, int* nstats) {
+  if (!e || !q || !ws || nsteps < 0 || !stats || !nstats || max_stats < 1) {
+    set_error("somaxhip_ens_step_profile: null argument or negative nsteps");
+    return SOMAXHIP_ERR_ARG;
+  }
+  // serial and eager, so every launch is timed on the caller's stream
+  hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+  const int saved = e->flags;
+  e->flags = 0;
+  Profile prof;
+  g_prof = &prof;
+  const int rc = ens_queue(e, q, nsteps, dt, ws, st);
+  g_prof = nullptr;
+  e->flags = saved;
+  const int rc2 = collect_profile(prof, st, stats, max_stats, nstats);
+  return rc ? rc : rc2;
+}
+
+}  // extern "C"

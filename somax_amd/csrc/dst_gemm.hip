@@ -262,7 +262,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
   int zi, ta, tb;
   tile_of_block(GA, GB, zi, ta, tb);
   const long z = zi;
-  const bool lo = z < a.z_split;
+  const bool lo = a.hp && a.hp[z];
   const int np = lo ? a.np_lo : a.np_hi;
   const int opieces = lo ? a.op_lo : a.op_hi;
   const int a0 = ta * BM;
@@ -369,18 +369,20 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 2))) voi
   store_tile(a, z, a0, b0, acc, sum, 0);
 }
 
+hipError_t prepare_dst_gemm() {
+  // up to 144 KiB of the 160 KiB LDS of a gfx950 CU (3 pieces, 3 stages);
+  // set outside any stream capture, before the first launch
+  static hipError_t status = hipFuncSetAttribute(reinterpret_cast<const void*>(dst_gemm_bf16_kernel),
+                                                 hipFuncAttributeMaxDynamicSharedMemorySize,
+                                                 (int)dst_gemm_lds_bytes(6));
+  return status;
+}
+
 void launch_dst_gemm(const GemmArgs& a, int batch, hipStream_t st) {
   const int tiles = ((a.RB + BN - 1) / BN) * ((a.RA + BM - 1) / BM) * batch;
   if (a.Xf) {
     hipLaunchKernelGGL(dst_gemm_f32_kernel, dim3(tiles), dim3(256), 0, st, a);
     return;
-  }
-  static bool lds_attr = false;
-  if (!lds_attr) {
-    // up to 144 KiB of the 160 KiB LDS of a gfx950 CU (3 pieces, 3 stages)
-    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(dst_gemm_bf16_kernel),
-                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)dst_gemm_lds_bytes(6));
-    lds_attr = true;
   }
   const int max_np = a.np_lo > a.np_hi ? a.np_lo : a.np_hi;
   hipLaunchKernelGGL(dst_gemm_bf16_kernel, dim3(tiles), dim3(256), dst_gemm_lds_bytes(max_np), st, a);

@@ -1,9 +1,38 @@
 // Internal kernel interfaces of the somax HIP library.
+//
+// A batch holds the members that share one grid: their fields are stacked as
+// P planes of Ny x Nx (member m owns planes first[m] .. first[m] + nl[m] - 1),
+// and every kernel runs over all planes of the batch in one launch, reading
+// the member and plane tables below.  A single model is the one-member batch.
 #pragma once
 
 #include "common.h"
 
 namespace somax {
+
+constexpr int MAX_LAYERS = 8;
+
+// Member and plane tables of a batch (device arrays).
+struct Tables {
+  // per member (M entries)
+  const int* mfirst;           // first plane
+  const int* mnl;              // layer count
+  const double* cl2m;          // [M*64] Cl2m of the member, row p at p*8 (mode p from layers)
+  const float* cm2l;           // [M*64] Cm2l of the member, row l at l*8 (layer l from modes)
+  const unsigned char* mask;   // 1: ghost ring of psi set to 0 after the solve
+  // per plane (P entries)
+  const int* member;           // plane -> member
+  const int* layer;            // plane -> layer (or mode) index within the member
+  const float* lam;            // Helmholtz parameter of the plane's mode
+  const unsigned char* hp;     // 1: Poisson-like mode (high-precision inverse)
+  const float* wind_scale;     // multiplier of tau0 * wind pattern for the layer
+  const float* psi_wall;       // WENO wall value of psi (null: 0)
+};
+
+// Physical parameters of one member: nu, nu4, kappa, tau0.
+struct MemberParams {
+  float nu, nu4, kappa, tau0;
+};
 
 // O[a][b] = sum_k A[a][k] B[b][k] with the data plane X in the A slot
 // (trans == 0: O = X S) or in the B slot (trans == 1: O = (X S)^T), S the
@@ -20,10 +49,11 @@ struct GemmArgs {
   long lds, s_piece;
   int K;
   int trans;
-  // per batch plane precision: planes z < z_split use np_lo products and
-  // write op_lo output pieces, the rest np_hi / op_hi (pieces 0 = FP32 out)
-  int z_split, np_lo, np_hi, op_lo, op_hi;
-  int acc64;  // add each k-tile's FP32 partial into an FP64 running sum
+  // per plane precision: planes with hp[z] != 0 use np_lo products and write
+  // op_lo output pieces, the others np_hi / op_hi (pieces 0 = FP32 out)
+  const unsigned char* hp;
+  int np_lo, np_hi, op_lo, op_hi;
+  int acc64;  // add the FP32 partial of every few k-tiles into an FP64 running sum
   // output O[ro][co], ro < RA, co < RB; columns [RB, RBpad) written as 0
   float* Of;
   uint16_t* Ob;
@@ -32,26 +62,26 @@ struct GemmArgs {
   // optional spectral epilogue: O /= (lamA[ro] + lamB[co] - lam[z])
   const float* lamA;
   const float* lamB;
-  const float* lam;  // device array, one value per batch plane
+  const float* lam;
 };
 
 void launch_dst_gemm(const GemmArgs& a, int batch, hipStream_t st);
-
-constexpr int MAX_LAYERS = 8;
+// Raises the bf16 kernel's dynamic LDS limit; call before the first launch.
+hipError_t prepare_dst_gemm();
 
 // Layer -> mode projection with FP64 accumulation (the stretching term makes
 // layer PV much larger than the barotropic modal PV, so an FP32 sum cancels):
-//   qm[m] = sum_k cl2m[m][k] * q[k]   (ghost ring of q read as 0 when apply_bc)
+//   qm[first + p] = sum_k cl2m[p][k] * q[first + k]   (ghost ring of q read as 0 when apply_bc)
 // qm is written in FP32 (state layout) and as the first GEMM's operand:
 // interior or full rows/cols (interior == apply_bc), pitch op_ld, zero padded
 // to op_kpad columns, as op_pieces bf16 pieces or FP32 (op_pieces == 0).
 struct ModalArgs {
   const float* q;
   float* qm;
-  int Nx, Ny, nl;
+  int Nx, Ny, M;
   long plane;
   int apply_bc;
-  double cl2m[MAX_LAYERS * MAX_LAYERS];
+  Tables t;
   void* op;
   int op_pieces;
   long op_ld, op_piece, op_batch;
@@ -60,7 +90,7 @@ struct ModalArgs {
 void launch_to_modal(const ModalArgs& a, hipStream_t st);
 
 // Per-mode Helmholtz residual on the full ghosted array,
-//   r_m = qm_m - lap_h(psi_m) + lambda_m psi_m,
+//   r_z = qm_z - lap_h(psi_z) + lambda_z psi_z,
 // lap_h with zero values outside the array (the operator DST-I diagonalises),
 // written as 2 bf16 pieces in the GEMM operand layout (pitch op_ld, zero
 // padded to op_kpad columns).
@@ -71,22 +101,22 @@ struct ResidualArgs {
   long op_ld, op_piece, op_batch;
   int op_kpad;
   long plane;
-  int Nx, Ny, nl;
+  int Nx, Ny, P;
   float idx2, idy2;
-  float lam[MAX_LAYERS];
+  const float* lam;
 };
 void launch_residual(const ResidualArgs& a, hipStream_t st);
 
 // Mode -> layer projection of the streamfunction:
-//   psi_l = sum_m cm2l[l][m] * (p_m + d_m)   (d nullable), ghost ring 0 if mask
+//   psi[first + l] = sum_p cm2l[l][p] * (pm[first + p] + d[first + p])   (d nullable),
+// ghost ring 0 where the member's mask is set.
 struct LayerArgs {
-  const float* p;
+  const float* pm;
   const float* d;
   float* psi;
-  int Nx, Ny, nl;
+  int Nx, Ny, M;
   long plane;
-  int mask;
-  float cm2l[MAX_LAYERS * MAX_LAYERS];
+  Tables t;
 };
 void launch_to_layer(const LayerArgs& a, hipStream_t st);
 
@@ -96,17 +126,18 @@ void launch_to_layer(const LayerArgs& a, hipStream_t st);
 struct StageArgs {
   const float* qs;   // stage state (raw, ghost values kept in the combination)
   const float* psi;  // streamfunction of BC(qs)
-  const float* qn;   // step start state (may be null when c0 == 0)
+  const float* qn;   // step start state (nullable)
   float* out;
-  int Nx, Ny, nl;
-  long plane;        // Ny*Nx
-  const float* dbeta_row;  // Ny: beta_y[j+1] - beta_y[j] (last entry unused)
-  const float* wind_row;  // Ny
-  float wind_coef[MAX_LAYERS];  // tau0 * wind_scale[l]
-  float nu, nu4, kappa;         // kappa acts on layer nl-1 only
-  float idx2, idy2, iarakawa;   // 1/dx^2, 1/dy^2, 1/(12 dx dy)
-  float idx, idy;               // 1/dx, 1/dy (WENO)
-  float psi_wall[MAX_LAYERS];   // WENO corner value on the walls
+  int Nx, Ny, P;
+  long plane;               // Ny*Nx
+  const float* dbeta_rows;  // [M*Ny] beta_y[j+1] - beta_y[j] per member (last entry unused)
+  const float* wind_rows;   // [M*Ny] wind pattern per member
+  Tables t;
+  const MemberParams* mp;   // per-member parameters (null: the uniform set u)
+  MemberParams u;
+  int biharm;               // some member has nu4 != 0
+  float idx2, idy2, iarakawa;  // 1/dx^2, 1/dy^2, 1/(12 dx dy)
+  float idx, idy;              // 1/dx, 1/dy (WENO)
   float c1, c2;
   int advection;
   int apply_bc;  // 1: ghost ring of qs read as 0 in the RHS (SomaxModel.build_terms)

@@ -27,13 +27,17 @@ __global__ __launch_bounds__(256) void qg_stage_arakawa_kernel(StageArgs a) {
   __shared__ float sdb[TY + 4];
 
   const int tid = threadIdx.x;
-  const int layer = blockIdx.z;
+  const int z = blockIdx.z;
   const int x0 = blockIdx.x * TX;
   const int y0 = blockIdx.y * TY;
   const int Nx = a.Nx, Ny = a.Ny;
-  const long off = (long)layer * a.plane;
+  const long off = (long)z * a.plane;
   const float* qs = a.qs + off;
   const float* psi = a.psi + off;
+  const int m = a.t.member[z];
+  const MemberParams pr = a.mp ? a.mp[m] : a.u;
+  const float* dbeta_row = a.dbeta_rows + (long)m * Ny;
+  const float* wind_row = a.wind_rows + (long)m * Ny;
 
   // q with the boundary condition (ghost ring and outside the array read 0)
   for (int i = tid; i < (TY + 4) * QW; i += 256) {
@@ -54,7 +58,7 @@ __global__ __launch_bounds__(256) void qg_stage_arakawa_kernel(StageArgs a) {
   }
   if (tid < TY + 4) {
     const int gy = y0 - 2 + tid;
-    sdb[tid] = (gy >= 0 && gy < Ny) ? a.dbeta_row[gy] : 0.0f;
+    sdb[tid] = (gy >= 0 && gy < Ny) ? dbeta_row[gy] : 0.0f;
   }
   __syncthreads();
 
@@ -75,8 +79,8 @@ __global__ __launch_bounds__(256) void qg_stage_arakawa_kernel(StageArgs a) {
 
   const int tx = tid & 63;
   const int tyb = tid >> 6;
-  const float wc = a.wind_coef[layer];
-  const bool bottom = (layer == a.nl - 1) && (a.kappa != 0.0f);
+  const float wc = pr.tau0 * a.t.wind_scale[z];
+  const bool bottom = (a.t.layer[z] == a.t.mnl[m] - 1) && (pr.kappa != 0.0f);
 #pragma unroll
   for (int r = 0; r < TY / 4; ++r) {
     const int ly = tyb + 4 * r;
@@ -101,14 +105,14 @@ __global__ __launch_bounds__(256) void qg_stage_arakawa_kernel(StageArgs a) {
       const float jxp = tN * (pNE - pNW) - tS * (pSE - pSW) - tE * (pNE - pSE) + tW * (pNW - pSW);
       const float jb = 2.0f * (pE - pW) * (dN + dS) + dN * (pNE - pNW) + dS * (pSE - pSW);
       rhs = -((jpp + jpx + jxp) + jb) * a.iarakawa;
-      if (a.nu != 0.0f) rhs += a.nu * ((q[1] - 2.0f * q[0] + q[-1]) * a.idx2 + (q[QW] - 2.0f * q[0] + q[-QW]) * a.idy2);
+      if (pr.nu != 0.0f) rhs += pr.nu * ((q[1] - 2.0f * q[0] + q[-1]) * a.idx2 + (q[QW] - 2.0f * q[0] + q[-QW]) * a.idy2);
       if (BIHARM) {
         const float* l = &sl[(ly + 1) * PW + tx + 1];
-        rhs -= a.nu4 * ((l[1] - 2.0f * l[0] + l[-1]) * a.idx2 + (l[PW] - 2.0f * l[0] + l[-PW]) * a.idy2);
+        rhs -= pr.nu4 * ((l[1] - 2.0f * l[0] + l[-1]) * a.idx2 + (l[PW] - 2.0f * l[0] + l[-PW]) * a.idy2);
       }
-      if (bottom) rhs -= a.kappa * ((pE - 2.0f * p[0] + pW) * a.idx2 + (pN - 2.0f * p[0] + pS) * a.idy2);
+      if (bottom) rhs -= pr.kappa * ((pE - 2.0f * p[0] + pW) * a.idx2 + (pN - 2.0f * p[0] + pS) * a.idy2);
     }
-    rhs += wc * a.wind_row[gy];
+    rhs += wc * wind_row[gy];
     const long idx = off + (long)gy * Nx + gx;
     // Stage combination qn + c1 (qs - qn) + c2 rhs: the FP32 weights of the
     // Shu-Osher form (1/3 + 2/3 != 1 in FP32) would otherwise scale q by
@@ -198,15 +202,19 @@ __global__ __launch_bounds__(256) void qg_stage_weno_kernel(StageArgs a) {
   __shared__ float sdb[WY + 6];                 // b[j+1] - b[j] for rows y0-3 ..
 
   const int tid = threadIdx.x;
-  const int layer = blockIdx.z;
+  const int z = blockIdx.z;
   const int x0 = blockIdx.x * WX;
   const int y0 = blockIdx.y * WY;
   const int Nx = a.Nx, Ny = a.Ny;
   const int nxi = Nx - 2, nyi = Ny - 2;
-  const long off = (long)layer * a.plane;
+  const long off = (long)z * a.plane;
   const float* qs = a.qs + off;
   const float* psi = a.psi + off;
-  const float pw = a.psi_wall[layer];
+  const int m = a.t.member[z];
+  const MemberParams pr = a.mp ? a.mp[m] : a.u;
+  const float* dbeta_row = a.dbeta_rows + (long)m * Ny;
+  const float* wind_row = a.wind_rows + (long)m * Ny;
+  const float pw = a.t.psi_wall ? a.t.psi_wall[z] : 0.0f;
   constexpr int PW2 = WX + 2;
 
   for (int i = tid; i < (WY + 6) * WQ; i += 256) {
@@ -227,7 +235,7 @@ __global__ __launch_bounds__(256) void qg_stage_weno_kernel(StageArgs a) {
   }
   if (tid < WY + 6) {
     const int gy = y0 - 3 + tid;
-    sdb[tid] = (gy >= 0 && gy < Ny) ? a.dbeta_row[gy] : 0.0f;
+    sdb[tid] = (gy >= 0 && gy < Ny) ? dbeta_row[gy] : 0.0f;
   }
   __syncthreads();
 
@@ -301,8 +309,8 @@ __global__ __launch_bounds__(256) void qg_stage_weno_kernel(StageArgs a) {
 
   const int tx = tid & 63;
   const int tyb = tid >> 6;
-  const float wc = a.wind_coef[layer];
-  const bool bottom = (layer == a.nl - 1) && (a.kappa != 0.0f);
+  const float wc = pr.tau0 * a.t.wind_scale[z];
+  const bool bottom = (a.t.layer[z] == a.t.mnl[m] - 1) && (pr.kappa != 0.0f);
 #pragma unroll
   for (int r = 0; r < WY / 4; ++r) {
     const int ly = tyb + 4 * r;
@@ -320,14 +328,14 @@ __global__ __launch_bounds__(256) void qg_stage_weno_kernel(StageArgs a) {
       rhs = -((fe - fw) * a.idx + (gn - gs_rel) * a.idy);
       const float* q = &sq[(ly + 3) * WQ + tx + 3];
       const float* p = &sp[(ly + 1) * PW2 + tx + 1];
-      if (a.nu != 0.0f) rhs += a.nu * ((q[1] - 2.0f * q[0] + q[-1]) * a.idx2 + (q[WQ] - 2.0f * q[0] + q[-WQ]) * a.idy2);
+      if (pr.nu != 0.0f) rhs += pr.nu * ((q[1] - 2.0f * q[0] + q[-1]) * a.idx2 + (q[WQ] - 2.0f * q[0] + q[-WQ]) * a.idy2);
       if (BIHARM) {
         const float* l = &sl[(ly + 1) * PW2 + tx + 1];
-        rhs -= a.nu4 * ((l[1] - 2.0f * l[0] + l[-1]) * a.idx2 + (l[PW2] - 2.0f * l[0] + l[-PW2]) * a.idy2);
+        rhs -= pr.nu4 * ((l[1] - 2.0f * l[0] + l[-1]) * a.idx2 + (l[PW2] - 2.0f * l[0] + l[-PW2]) * a.idy2);
       }
-      if (bottom) rhs -= a.kappa * ((p[1] - 2.0f * p[0] + p[-1]) * a.idx2 + (p[PW2] - 2.0f * p[0] + p[-PW2]) * a.idy2);
+      if (bottom) rhs -= pr.kappa * ((p[1] - 2.0f * p[0] + p[-1]) * a.idx2 + (p[PW2] - 2.0f * p[0] + p[-PW2]) * a.idy2);
     }
-    rhs += wc * a.wind_row[gy];
+    rhs += wc * wind_row[gy];
     const long idx = off + (long)gy * Nx + gx;
     float o;
     if (a.qn) {
@@ -343,13 +351,13 @@ __global__ __launch_bounds__(256) void qg_stage_weno_kernel(StageArgs a) {
 
 void launch_qg_stage(const StageArgs& a, hipStream_t st) {
   if (a.advection == 1) {
-    dim3 g((a.Nx + WX - 1) / WX, (a.Ny + WY - 1) / WY, a.nl);
-    if (a.nu4 != 0.0f) hipLaunchKernelGGL(qg_stage_weno_kernel<true>, g, dim3(256), 0, st, a);
+    dim3 g((a.Nx + WX - 1) / WX, (a.Ny + WY - 1) / WY, a.P);
+    if (a.biharm) hipLaunchKernelGGL(qg_stage_weno_kernel<true>, g, dim3(256), 0, st, a);
     else hipLaunchKernelGGL(qg_stage_weno_kernel<false>, g, dim3(256), 0, st, a);
     return;
   }
-  dim3 grid((a.Nx + TX - 1) / TX, (a.Ny + TY - 1) / TY, a.nl);
-  if (a.nu4 != 0.0f)
+  dim3 grid((a.Nx + TX - 1) / TX, (a.Ny + TY - 1) / TY, a.P);
+  if (a.biharm)
     hipLaunchKernelGGL(qg_stage_arakawa_kernel<true>, grid, dim3(256), 0, st, a);
   else
     hipLaunchKernelGGL(qg_stage_arakawa_kernel<false>, grid, dim3(256), 0, st, a);
@@ -396,54 +404,65 @@ __device__ inline void store_pieces(uint16_t* o, long piece, int n, float v) {
 __global__ __launch_bounds__(256) void residual_kernel(ResidualArgs a) {
   const int x = blockIdx.x * blockDim.x + threadIdx.x;  // up to op_kpad
   const int y = blockIdx.y;
-  const int m = blockIdx.z;
+  const int z = blockIdx.z;
   if (x >= a.op_kpad) return;
-  uint16_t* o = a.r + m * a.op_batch + (long)y * a.op_ld + x;
+  uint16_t* o = a.r + z * a.op_batch + (long)y * a.op_ld + x;
   if (x >= a.Nx) {
     store_pieces(o, a.op_piece, 2, 0.0f);
     return;
   }
-  const long s = m * a.plane + (long)y * a.Nx + x;
+  const long s = z * a.plane + (long)y * a.Nx + x;
   const float* p = a.psi + s;
   const float c = p[0];
   const float e = (x + 1 < a.Nx) ? p[1] : 0.0f;
   const float w = (x > 0) ? p[-1] : 0.0f;
   const float n = (y + 1 < a.Ny) ? p[a.Nx] : 0.0f;
   const float so = (y > 0) ? p[-a.Nx] : 0.0f;
-  const float r = a.q[s] - ((e - 2.0f * c + w) * a.idx2 + (n - 2.0f * c + so) * a.idy2) + a.lam[m] * c;
+  const float r = a.q[s] - ((e - 2.0f * c + w) * a.idx2 + (n - 2.0f * c + so) * a.idy2) + a.lam[z] * c;
   store_pieces(o, a.op_piece, 2, r);
 }
 
 void launch_residual(const ResidualArgs& a, hipStream_t st) {
-  dim3 grid((a.op_kpad + 255) / 256, a.Ny, a.nl);
+  dim3 grid((a.op_kpad + 255) / 256, a.Ny, a.P);
   hipLaunchKernelGGL(residual_kernel, grid, dim3(256), 0, st, a);
 }
 
 // ---------------------------------------------------------------------------
 // layer -> mode projection in FP64, writing the first GEMM operand
+// (one thread per cell of one member: its layers are read once)
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void to_modal_kernel(ModalArgs a) {
   const int x = blockIdx.x * blockDim.x + threadIdx.x;  // state column
   const int y = blockIdx.y;
+  const int mem = blockIdx.z;
   const int off = a.apply_bc ? 1 : 0;
   const int rows = a.Ny - 2 * off, cols = a.Nx - 2 * off;
   const int ox = x - off, oy = y - off;  // operand coordinates
   const bool inside = x < a.Nx;
   const bool wop = oy >= 0 && oy < rows && ox >= 0 && ox < a.op_kpad;
   if (!inside && !wop) return;
+  const int first = a.t.mfirst[mem], nl = a.t.mnl[mem];
+  const double* cl2m = a.t.cl2m + mem * MAX_LAYERS * MAX_LAYERS;
   const long s = (long)y * a.Nx + x;
   const bool ghost = x == 0 || y == 0 || x == a.Nx - 1 || y == a.Ny - 1;
+  // fully unrolled over MAX_LAYERS with guards so qk stays in registers
   double qk[MAX_LAYERS];
-  for (int k = 0; k < a.nl; ++k)
-    qk[k] = (!inside || (ghost && a.apply_bc)) ? 0.0 : (double)a.q[k * a.plane + s];
-  for (int m = 0; m < a.nl; ++m) {
+#pragma unroll
+  for (int k = 0; k < MAX_LAYERS; ++k)
+    qk[k] = (k >= nl || !inside || (ghost && a.apply_bc)) ? 0.0 : (double)a.q[(first + k) * a.plane + s];
+#pragma unroll
+  for (int p = 0; p < MAX_LAYERS; ++p) {
+    if (p >= nl) break;
     double acc = 0.0;
-    for (int k = 0; k < a.nl; ++k) acc += a.cl2m[m * MAX_LAYERS + k] * qk[k];
+#pragma unroll
+    for (int k = 0; k < MAX_LAYERS; ++k)
+      if (k < nl) acc += cl2m[p * MAX_LAYERS + k] * qk[k];
     const float v = (float)acc;
-    if (inside) a.qm[m * a.plane + s] = v;
+    const long z = first + p;
+    if (inside) a.qm[z * a.plane + s] = v;
     if (wop) {
       const float w = ox < cols ? v : 0.0f;  // zero padding past the operand width
-      const long o = m * a.op_batch + (long)oy * a.op_ld + ox;
+      const long o = z * a.op_batch + (long)oy * a.op_ld + ox;
       if (a.op_pieces == 0) static_cast<float*>(a.op)[o] = w;
       else store_pieces(static_cast<uint16_t*>(a.op) + o, a.op_piece, a.op_pieces, w);
     }
@@ -453,7 +472,7 @@ __global__ __launch_bounds__(256) void to_modal_kernel(ModalArgs a) {
 void launch_to_modal(const ModalArgs& a, hipStream_t st) {
   const int off = a.apply_bc ? 1 : 0;
   const int width = a.Nx > a.op_kpad + off ? a.Nx : a.op_kpad + off;
-  dim3 grid((width + 255) / 256, a.Ny, 1);
+  dim3 grid((width + 255) / 256, a.Ny, a.M);
   hipLaunchKernelGGL(to_modal_kernel, grid, dim3(256), 0, st, a);
 }
 
@@ -463,20 +482,34 @@ void launch_to_modal(const ModalArgs& a, hipStream_t st) {
 __global__ __launch_bounds__(256) void to_layer_kernel(LayerArgs a) {
   const int x = blockIdx.x * blockDim.x + threadIdx.x;
   const int y = blockIdx.y;
+  const int mem = blockIdx.z;
   if (x >= a.Nx) return;
+  const int first = a.t.mfirst[mem], nl = a.t.mnl[mem];
+  const float* cm2l = a.t.cm2l + mem * MAX_LAYERS * MAX_LAYERS;
   const long s = (long)y * a.Nx + x;
-  const bool ghost = a.mask && (x == 0 || y == 0 || x == a.Nx - 1 || y == a.Ny - 1);
+  const bool ghost = a.t.mask[mem] && (x == 0 || y == 0 || x == a.Nx - 1 || y == a.Ny - 1);
   float pm[MAX_LAYERS];
-  for (int m = 0; m < a.nl; ++m) pm[m] = a.p[m * a.plane + s] + (a.d ? a.d[m * a.plane + s] : 0.0f);
-  for (int l = 0; l < a.nl; ++l) {
+#pragma unroll
+  for (int p = 0; p < MAX_LAYERS; ++p) {
+    pm[p] = 0.0f;
+    if (p < nl) {
+      const long i = (first + p) * a.plane + s;
+      pm[p] = a.pm[i] + (a.d ? a.d[i] : 0.0f);
+    }
+  }
+#pragma unroll
+  for (int l = 0; l < MAX_LAYERS; ++l) {
+    if (l >= nl) break;
     float v = 0.0f;
-    for (int m = 0; m < a.nl; ++m) v += a.cm2l[l * MAX_LAYERS + m] * pm[m];
-    a.psi[l * a.plane + s] = ghost ? 0.0f : v;
+#pragma unroll
+    for (int p = 0; p < MAX_LAYERS; ++p)
+      if (p < nl) v += cm2l[l * MAX_LAYERS + p] * pm[p];
+    a.psi[(first + l) * a.plane + s] = ghost ? 0.0f : v;
   }
 }
 
 void launch_to_layer(const LayerArgs& a, hipStream_t st) {
-  dim3 grid((a.Nx + 255) / 256, a.Ny, 1);
+  dim3 grid((a.Nx + 255) / 256, a.Ny, a.M);
   hipLaunchKernelGGL(to_layer_kernel, grid, dim3(256), 0, st, a);
 }
 

@@ -180,12 +180,9 @@ class _QGBase(SomaxModel):
             self._homsol, self._homsol_mean = homogeneous_solution(grid.Ny, grid.Nx, grid.dx, grid.dy, lambdas)
 
     # -- plan and workspace -------------------------------------------------
-    def _ensure_plan(self):
-        if self._plan is not None:
-            return self._plan
-        torch = _torch()
-        lib = _lib.load()
-        _lib.init_device()
+    def _desc(self):
+        """The C description of this model and the host arrays it points into
+        (keep them alive while the description is in use)."""
         g = self.grid
         keep = []
 
@@ -208,6 +205,15 @@ class _QGBase(SomaxModel):
             keep.append(hs)
             d.homsol = hs.ctypes.data_as(_lib.c_float_p)
             d.homsol_mean = dp(self._homsol_mean)
+        return d, keep
+
+    def _ensure_plan(self):
+        if self._plan is not None:
+            return self._plan
+        torch = _torch()
+        lib = _lib.load()
+        _lib.init_device()
+        d, keep = self._desc()
         plan = ctypes.c_void_p()
         _lib.check(lib.somaxhip_qg_plan_create(ctypes.byref(d), ctypes.byref(plan)), "somaxhip_qg_plan_create")
         self._plan = plan

@@ -120,10 +120,10 @@ def wind_amplitude(tau0: float = TAU0, Ly: float = BASIN) -> float:
     return tau0 * 2.0 * np.pi / (RHO0 * Ly)
 
 
-def config_barotropic(n: int = 64, depth: float = 4000.0, **kw) -> BarotropicQG:
+def config_barotropic(n: int = 64, depth: float = 4000.0, wind_tau0: float = TAU0, **kw) -> BarotropicQG:
     """1-layer double gyre: DST-I Poisson inversion, biharmonic 2e11 m^4/s."""
     return BarotropicQG.create(nx=n, ny=n, Lx=BASIN, Ly=BASIN, f0=F0, beta=BETA, bottom_drag=DRAG,
-                               wind_amplitude=wind_amplitude() / depth, biharmonic_viscosity=NU4_128, **kw)
+                               wind_amplitude=wind_amplitude(wind_tau0) / depth, biharmonic_viscosity=NU4_128, **kw)
 
 
 def config_baroclinic(n: int = 128, nl: int = 3, tau0: float = TAU0, **kw) -> BaroclinicQG:
@@ -134,3 +134,51 @@ def config_baroclinic(n: int = 128, nl: int = 3, tau0: float = TAU0, **kw) -> Ba
     return BaroclinicQG.create(nx=n, ny=n, Lx=BASIN, Ly=BASIN, f0=F0, beta=BETA, n_layers=nl, H=H, g_prime=gp,
                                bottom_drag=DRAG, wind_amplitude=wind_amplitude(tau0),
                                biharmonic_viscosity=NU4_128 * (128.0 / n) ** 4, **kw)
+
+
+# ---------------------------------------------------------------------------
+# ensemble of uneven members (extension)
+# ---------------------------------------------------------------------------
+# Layer thicknesses and reduced gravities per layer count, each 4000 m deep;
+# the 3-layer set is the multilayer double gyre's.
+ENSEMBLE_LAYERS = {
+    1: ((4000.0,), (9.81,)),
+    2: ((400.0, 3600.0), (9.81, 0.025)),
+    3: ((400.0, 1100.0, 2600.0), (9.81, 0.025, 0.0125)),
+}
+# (grid size, members) of the 256-member ensemble
+ENSEMBLE_COUNTS = ((64, 96), (128, 96), (256, 48), (512, 16))
+
+
+def ensemble_member_args(nx: int, nl: int, tau0: float, ny: int | None = None) -> dict:
+    """``BaroclinicQG.create`` keyword arguments of one ensemble member: the
+    double-gyre basin (square cells, Ly scaled with ny), biharmonic
+    2e11 * (128/nx)^4 m^4/s, bottom drag 3.6e-8 /s, wind stress tau0."""
+    ny = nx if ny is None else ny
+    H, gp = ENSEMBLE_LAYERS[nl]
+    return dict(nx=nx, ny=ny, Lx=BASIN, Ly=BASIN * ny / nx, f0=F0, beta=BETA, n_layers=nl, H=H, g_prime=gp,
+                bottom_drag=DRAG, wind_amplitude=wind_amplitude(tau0, BASIN * ny / nx),
+                biharmonic_viscosity=NU4_128 * (128.0 / nx) ** 4)
+
+
+def ensemble_member(nx: int, nl: int, tau0: float, ny: int | None = None, **kw) -> BaroclinicQG:
+    return BaroclinicQG.create(**ensemble_member_args(nx, nl, tau0, ny), **kw)
+
+
+def config_ensemble(seed: int = 0, counts=ENSEMBLE_COUNTS, precision: str = "mixed", **kw):
+    """The uneven ensemble: ``counts`` = (grid size, members) pairs; layer
+    counts cycle 1/2/3 over the members in order; tau0 ~ U[0.04, 0.12] N/m^2
+    and a seeded k^-3 random initial streamfunction per member.  Returns
+    ``(models, member PV fields as float32 host arrays)``."""
+    rng = np.random.default_rng(seed)
+    models, qs = [], []
+    i = 0
+    for n, count in counts:
+        for _ in range(count):
+            nl = 1 + i % 3
+            tau0 = float(rng.uniform(0.04, 0.12))
+            m = ensemble_member(n, nl, tau0, precision=precision, **kw)
+            models.append(m)
+            qs.append(random_pv(m, seed * 100003 + i).astype(np.float32))
+            i += 1
+    return models, qs

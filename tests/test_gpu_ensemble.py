@@ -1,0 +1,197 @@
+"""GPU tests of QG ensembles (members of uneven grid size and layer count).
+
+Each member of an ensemble runs the arithmetic of the single-model step, so
+the bar against ``model.step`` is agreement to FP32 round-off (the kernels are
+deterministic and identical per plane; 1e-7 relative leaves room only for a
+different launch order of identical work).  Against the FP64 oracle the
+one-step bar is the single model's, 1e-6.  Graph replays are checked one by
+one against eager steps, with unrelated eager work and synchronisations in
+between.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from oracle import qg as oqg
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    a = np.asarray(a, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+def _log(msg):
+    print(msg, flush=True)
+
+
+# (nx, ny, nl, kind): kind "bc" = BaroclinicQG, "bt" = BarotropicQG
+MEMBERS = [(64, 64, 1, "bc"), (64, 64, 3, "bc"), (64, 64, 1, "bt"), (96, 64, 2, "bc"), (128, 128, 3, "bc"),
+           (64, 64, 2, "bc"), (96, 64, 2, "bc")]
+
+
+def _specs(seed=5):
+    """(kind, create kwargs) per member; tau0 drawn uniformly in [0.04, 0.12]."""
+    from somax_amd import testcases as tc
+
+    rng = np.random.default_rng(seed)
+    out = []
+    for nx, ny, nl, kind in MEMBERS:
+        tau0 = float(rng.uniform(0.04, 0.12))
+        if kind == "bt":
+            out.append(("bt", dict(nx=nx, ny=ny, Lx=tc.BASIN, Ly=tc.BASIN, f0=tc.F0, beta=tc.BETA,
+                                   bottom_drag=tc.DRAG, wind_amplitude=tc.wind_amplitude(tau0) / 4000.0,
+                                   biharmonic_viscosity=tc.NU4_128)))
+        else:
+            out.append(("bc", tc.ensemble_member_args(nx, nl, tau0, ny=ny)))
+    return out
+
+
+def _models(precision, seed=5):
+    from somax_amd import BaroclinicQG, BarotropicQG
+
+    return [BarotropicQG.create(**a, precision=precision) if k == "bt" else
+            BaroclinicQG.create(**a, precision=precision) for k, a in _specs(seed)]
+
+
+def _oracle(i, seed=5):
+    kind, a = _specs(seed)[i]
+    if kind == "bt":
+        return oqg.barotropic_setup(**a)
+    a = {k: v for k, v in a.items() if k != "n_layers"}
+    return oqg.baroclinic_setup(**a)
+
+
+def _states(models, seed=21):
+    from somax_amd import testcases as tc
+
+    return [tc.random_pv(m, seed + i).astype(np.float32).reshape(m._nl, m.grid.Ny, m.grid.Nx)
+            for i, m in enumerate(models)]
+
+
+@pytest.mark.parametrize("precision", ["fp32", "mixed"])
+@pytest.mark.parametrize("concurrent,graph", [(False, False), (True, False), (True, True)])
+def test_ensemble_matches_single_models(gpu, precision, concurrent, graph):
+    import torch
+
+    from somax_amd import QGEnsemble
+
+    _log(f"[ensemble] precision={precision} concurrent={concurrent} graph={graph}")
+    models = _models(precision)
+    qs = _states(models)
+    ens = QGEnsemble(models, concurrent=concurrent, graph=graph)
+    assert len(ens) == len(MEMBERS)
+    packed = ens.pack(qs)
+    dt, nsteps = 1200.0, 3
+    ens.step(packed, dt, nsteps)
+    torch.cuda.synchronize()
+    worst = 0.0
+    for i, m in enumerate(models):
+        q = torch.from_numpy(qs[i]).cuda()
+        m._step_inplace(q, dt, nsteps)
+        r = rel(ens.member(packed, i).cpu().numpy(), q.cpu().numpy())
+        worst = max(worst, r)
+    _log(f"  worst member rel-L2 vs single model = {worst:.3e}")
+    assert worst < 1e-7
+
+
+def test_ensemble_graph_replays(gpu):
+    import torch
+
+    from somax_amd import QGEnsemble
+
+    _log("[ensemble] graph replays vs eager, mixed")
+    models = _models("mixed")
+    qs = _states(models)
+    eager = QGEnsemble(models, concurrent=False, graph=False)
+    graph = QGEnsemble(models, concurrent=True, graph=True)
+    pe, pg = eager.pack(qs), graph.pack(qs)
+    scratch = torch.randn(1 << 20, device="cuda")
+    for k in range(4):
+        graph.step(pg, 1200.0, 1)
+        scratch.mul_(1.0001).add_(0.5)  # eager work between replays
+        torch.cuda.synchronize()
+        eager.step(pe, 1200.0, 1)
+        torch.cuda.synchronize()
+        r = rel(pg.cpu().numpy(), pe.cpu().numpy())
+        _log(f"  replay {k}: rel-L2 = {r:.3e}")
+        assert r < 1e-7
+    # two replays per call
+    graph.step(pg, 1200.0, 2)
+    eager.step(pe, 1200.0, 2)
+    torch.cuda.synchronize()
+    assert rel(pg.cpu().numpy(), pe.cpu().numpy()) < 1e-7
+
+
+def test_ensemble_one_step_vs_oracle(gpu):
+    import torch
+
+    from somax_amd import QGEnsemble
+
+    _log("[ensemble] one step vs FP64 oracle, mixed")
+    models = _models("mixed")
+    qs = _states(models)
+    ens = QGEnsemble(models)
+    packed = ens.pack(qs)
+    dt = 1200.0
+    ens.step(packed, dt, 1)
+    torch.cuda.synchronize()
+    for i in (0, 1, 2, 4):
+        setup = _oracle(i)
+        q0 = qs[i].astype(np.float64)
+        if setup.nl == 1 and MEMBERS[i][3] == "bt":
+            q0 = q0[0]
+        ref = oqg.ssprk3_step(setup, q0, dt)
+        r = rel(ens.member(packed, i).cpu().numpy().reshape(ref.shape), ref)
+        _log(f"  member {i} {MEMBERS[i]}: rel-L2 q = {r:.3e}")
+        assert r < 1e-6
+
+
+def test_ensemble_set_params(gpu):
+    import dataclasses
+
+    import torch
+
+    from somax_amd import QGEnsemble
+
+    _log("[ensemble] set_params")
+    models = _models("fp32")
+    qs = _states(models)
+    ens = QGEnsemble(models, concurrent=True, graph=True)
+    packed = ens.pack(qs)
+    ens.step(packed, 1200.0, 1)  # captures a graph with the first parameters
+    new = [dataclasses.replace(m.params, wind_amplitude=2.0 * m.params.wind_amplitude, biharmonic_viscosity=0.0)
+           for m in models]
+    ens.set_params(new)
+    packed = ens.pack(qs)
+    ens.step(packed, 1200.0, 2)
+    torch.cuda.synchronize()
+    for i, m in enumerate(models):
+        assert m.params.biharmonic_viscosity == 0.0
+        q = torch.from_numpy(qs[i]).cuda()
+        m._step_inplace(q, 1200.0, 2)
+        assert rel(ens.member(packed, i).cpu().numpy(), q.cpu().numpy()) < 1e-7
+
+
+def test_ensemble_mixed_precision_classes(gpu):
+    import torch
+
+    from somax_amd import QGEnsemble
+    from somax_amd import testcases as tc
+
+    _log("[ensemble] fp32 and mixed members of one grid form two classes")
+    models = [tc.ensemble_member(64, 3, 0.08, precision="fp32"), tc.ensemble_member(64, 3, 0.08, precision="mixed")]
+    ens = QGEnsemble(models)
+    assert ens.num_classes == 2
+    qs = _states(models)
+    packed = ens.pack(qs)
+    ens.step(packed, 1200.0, 1)
+    torch.cuda.synchronize()
+    for i, m in enumerate(models):
+        q = torch.from_numpy(qs[i]).cuda()
+        m._step_inplace(q, 1200.0, 1)
+        assert rel(ens.member(packed, i).cpu().numpy(), q.cpu().numpy()) < 1e-7
