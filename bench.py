@@ -87,6 +87,18 @@ def kernel_profile(model, q0, dt: float, steps: int):
     return out
 
 
+def measured_traffic() -> dict:
+    """HBM bytes per launch of the 1024² mixed step's kernels, measured with
+    rocprofv3 FETCH_SIZE / WRITE_SIZE passes and scaled by a copy of known
+    size (profiles/hbm_traffic_qg1024_mixed.json); empty if absent."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "hbm_traffic_qg1024_mixed.json")
+    try:
+        with open(path) as f:
+            return json.load(f)["bytes_per_launch"]
+    except (OSError, ValueError, KeyError):
+        return {}
+
+
 def roofline_entry(name: str, st: dict) -> dict:
     avg_ms = st["total_ms"] / st["launches"]
     if name.startswith("dst_gemm"):
@@ -233,6 +245,10 @@ def main() -> int:
     roof_stencil = roofline_entry("qg_stage", prof_mixed["qg_stage"])
     dom_p = max(prof_plain, key=lambda k: prof_plain[k]["total_ms"])
     roof_plain = roofline_entry(dom_p, prof_plain[dom_p])
+    if n == 1024 and args.advection == "arakawa":
+        tr = measured_traffic()
+        roof["traffic"] = tr.get(dom)
+        roof_stencil["traffic"] = tr.get("qg_stage")
 
     ens = bench_ensemble(dt, args.ensemble_steps, 2) if args.ensemble_steps > 0 else None
 

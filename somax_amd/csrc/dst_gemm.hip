@@ -256,6 +256,9 @@ size_t dst_gemm_lds_bytes(int max_np) {
   return (size_t)nstage * 2 * npc * 128 * 32 * sizeof(uint16_t);
 }
 
+// One instantiation per largest product count of a launch (x1, x3, x6), so
+// each is a kernel symbol of its own; planes may use fewer products.
+template <int NPMAX>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) void dst_gemm_bf16_kernel(GemmArgs a) {
   extern __shared__ __attribute__((aligned(16))) uint16_t lds16[];
   const int GA = (a.RA + BM - 1) / BM, GB = (a.RB + BN - 1) / BN;
@@ -273,16 +276,19 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
   double sum[2][2][16];
   zero_acc(acc, sum);
   if (a.acc64) {
-    if (np == 1) bf16_mainloop<1, true>(a, lds16, z, a0, b0, x_in_a, acc, sum);
-    else if (np == 3) bf16_mainloop<3, true>(a, lds16, z, a0, b0, x_in_a, acc, sum);
+    if (NPMAX == 1 || np == 1) bf16_mainloop<1, true>(a, lds16, z, a0, b0, x_in_a, acc, sum);
+    else if (NPMAX == 3 || np == 3) bf16_mainloop<3, true>(a, lds16, z, a0, b0, x_in_a, acc, sum);
     else bf16_mainloop<6, true>(a, lds16, z, a0, b0, x_in_a, acc, sum);
   } else {
-    if (np == 1) bf16_mainloop<1, false>(a, lds16, z, a0, b0, x_in_a, acc, sum);
-    else if (np == 3) bf16_mainloop<3, false>(a, lds16, z, a0, b0, x_in_a, acc, sum);
+    if (NPMAX == 1 || np == 1) bf16_mainloop<1, false>(a, lds16, z, a0, b0, x_in_a, acc, sum);
+    else if (NPMAX == 3 || np == 3) bf16_mainloop<3, false>(a, lds16, z, a0, b0, x_in_a, acc, sum);
     else bf16_mainloop<6, false>(a, lds16, z, a0, b0, x_in_a, acc, sum);
   }
   store_tile(a, z, a0, b0, acc, sum, opieces);
 }
+template __global__ void dst_gemm_bf16_kernel<1>(GemmArgs);
+template __global__ void dst_gemm_bf16_kernel<3>(GemmArgs);
+template __global__ void dst_gemm_bf16_kernel<6>(GemmArgs);
 
 // ---------------------------------------------------------------------------
 // FP32 kernel (plain step): register-staged double buffer, FP64 flush every
@@ -372,9 +378,17 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 2))) voi
 hipError_t prepare_dst_gemm() {
   // up to 144 KiB of the 160 KiB LDS of a gfx950 CU (3 pieces, 3 stages);
   // set outside any stream capture, before the first launch
-  static hipError_t status = hipFuncSetAttribute(reinterpret_cast<const void*>(dst_gemm_bf16_kernel),
-                                                 hipFuncAttributeMaxDynamicSharedMemorySize,
-                                                 (int)dst_gemm_lds_bytes(6));
+  static hipError_t status = [] {
+    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(dst_gemm_bf16_kernel<1>),
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)dst_gemm_lds_bytes(1));
+    if (e == hipSuccess)
+      e = hipFuncSetAttribute(reinterpret_cast<const void*>(dst_gemm_bf16_kernel<3>),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)dst_gemm_lds_bytes(3));
+    if (e == hipSuccess)
+      e = hipFuncSetAttribute(reinterpret_cast<const void*>(dst_gemm_bf16_kernel<6>),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)dst_gemm_lds_bytes(6));
+    return e;
+  }();
   return status;
 }
 
@@ -385,7 +399,10 @@ void launch_dst_gemm(const GemmArgs& a, int batch, hipStream_t st) {
     return;
   }
   const int max_np = a.np_lo > a.np_hi ? a.np_lo : a.np_hi;
-  hipLaunchKernelGGL(dst_gemm_bf16_kernel, dim3(tiles), dim3(256), dst_gemm_lds_bytes(max_np), st, a);
+  const size_t lds = dst_gemm_lds_bytes(max_np);
+  if (max_np == 1) hipLaunchKernelGGL(dst_gemm_bf16_kernel<1>, dim3(tiles), dim3(256), lds, st, a);
+  else if (max_np == 3) hipLaunchKernelGGL(dst_gemm_bf16_kernel<3>, dim3(tiles), dim3(256), lds, st, a);
+  else hipLaunchKernelGGL(dst_gemm_bf16_kernel<6>, dim3(tiles), dim3(256), lds, st, a);
 }
 
 }  // namespace somax

@@ -1,0 +1,138 @@
+"""Fixed workloads for hardware-counter and kernel-trace runs.
+
+``python -m somax_amd.profiling --workload qg1024-mixed --steps 2`` first
+copies ``--copy-mib`` MiB between two device buffers with one plain copy
+kernel (its FETCH_SIZE / WRITE_SIZE give the counters' scale against a known
+byte count), then advances the named workload by ``--steps`` SSP-RK3 steps.
+Workloads:
+
+* ``qg1024-mixed`` / ``qg1024-fp32``: 3-layer QG double gyre, 1024 x 1024
+  interior cells, mixed or plain FP32 inversion, Arakawa advection.
+* ``ensemble``: the 256-member uneven ensemble, mixed inversion, one
+  eager step (every launch visible to the profiler).
+
+Each step is preceded by one untimed warm-up step.  Prints one line per
+phase to stdout.
+
+``python -m somax_amd.profiling summarize-trace TRACE.csv`` and
+``summarize-pmc FETCH.csv WRITE.csv COPY_MIB`` turn rocprofv3 CSV output into
+the markdown tables kept with the profiles.
+"""
+
+from __future__ import annotations
+
+import argparse
+import sys
+
+
+def _grid(r) -> int:
+    return int(r["Grid_Size_X"]) * int(r["Grid_Size_Y"]) * int(r["Grid_Size_Z"]) if "Grid_Size_X" in r else int(
+        r["Grid_Size"])
+
+
+def summarize_trace(path: str, top: int = 40) -> str:
+    """Markdown table of a rocprofv3 ``*_kernel_trace.csv``: launches and mean
+    duration per (kernel, grid size in work-items)."""
+    import collections
+    import csv
+
+    agg = collections.defaultdict(list)
+    with open(path) as f:
+        for r in csv.DictReader(f):
+            agg[(r["Kernel_Name"], _grid(r))].append(int(r["End_Timestamp"]) - int(r["Start_Timestamp"]))
+    lines = ["| kernel | grid (work-items) | launches | mean us | total ms |", "|---|---:|---:|---:|---:|"]
+    for (name, g), v in sorted(agg.items(), key=lambda kv: -sum(kv[1]))[:top]:
+        lines.append(f"| `{name[:70]}` | {g} | {len(v)} | {sum(v) / len(v) / 1e3:.2f} | {sum(v) / 1e6:.3f} |")
+    return "\n".join(lines)
+
+
+def summarize_pmc(fetch_csv: str, write_csv: str, copy_mib: int) -> str:
+    """Markdown table of FETCH_SIZE / WRITE_SIZE (KiB per dispatch) per
+    (kernel, grid), scaled by what each counter reads for the elementwise copy
+    of ``copy_mib`` MiB (the multiply kernel with the largest grid)."""
+    import collections
+    import csv
+
+    def load(path):
+        agg = collections.defaultdict(list)
+        with open(path) as f:
+            for r in csv.DictReader(f):
+                agg[(r["Kernel_Name"], _grid(r))].append(float(r["Counter_Value"]))
+        return agg
+
+    fe, wr = load(fetch_csv), load(write_csv)
+    true_kib = copy_mib * 1024.0
+    copy_key = max((k for k in fe if "MulFunctor" in k[0]), key=lambda k: k[1])
+    f_scale = true_kib / (sum(fe[copy_key]) / len(fe[copy_key]))
+    w_scale = true_kib / (sum(wr[copy_key]) / len(wr[copy_key]))
+    lines = [f"copy kernel ({copy_mib} MiB read, {copy_mib} MiB written): FETCH_SIZE scale {f_scale:.3f}, "
+             f"WRITE_SIZE scale {w_scale:.3f}", "",
+             "| kernel | grid | dispatches | FETCH MiB (raw) | WRITE MiB (raw) | HBM MB per launch (scaled) |",
+             "|---|---:|---:|---:|---:|---:|"]
+    for k in sorted(fe, key=lambda k: -sum(fe[k]) / len(fe[k])):
+        if k == copy_key or k not in wr:
+            continue
+        f = sum(fe[k]) / len(fe[k])
+        w = sum(wr[k]) / len(wr[k])
+        mb = (f * f_scale + w * w_scale) * 1024.0 / 1e6
+        lines.append(f"| `{k[0][:70]}` | {k[1]} | {len(fe[k])} | {f / 1024:.2f} | {w / 1024:.2f} | {mb:.2f} |")
+    return "\n".join(lines)
+
+
+def main(argv=None) -> int:
+    if argv is None:
+        argv = sys.argv[1:]
+    if argv and argv[0] == "summarize-trace":
+        print(summarize_trace(argv[1]))
+        return 0
+    if argv and argv[0] == "summarize-pmc":
+        print(summarize_pmc(argv[1], argv[2], int(argv[3])))
+        return 0
+    ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
+    ap.add_argument("--workload", default="qg1024-mixed", choices=["qg1024-mixed", "qg1024-fp32", "ensemble"])
+    ap.add_argument("--steps", type=int, default=2)
+    ap.add_argument("--copy-mib", type=int, default=1024)
+    ap.add_argument("--dt", type=float, default=1200.0)
+    args = ap.parse_args(argv)
+
+    import numpy as np
+    import torch
+
+    from . import testcases as tc
+
+    if args.copy_mib > 0:
+        n = args.copy_mib * (1 << 20) // 4
+        src = torch.ones(n, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        torch.mul(src, 1.0, out=dst)  # an elementwise kernel (not a DMA copy)
+        torch.cuda.synchronize()
+        print(f"copy: {args.copy_mib} MiB read + {args.copy_mib} MiB written", flush=True)
+        del src, dst
+
+    if args.workload == "ensemble":
+        from .ensemble import QGEnsemble
+
+        models, qs = tc.config_ensemble(seed=7, precision="mixed")
+        ens = QGEnsemble(models, concurrent=False, graph=False)
+        packed = ens.pack(qs)
+        ens.step(packed, args.dt, 1)
+        torch.cuda.synchronize()
+        ens.step(packed, args.dt, args.steps)
+        torch.cuda.synchronize()
+        print(f"ensemble: {len(models)} members, {args.steps} steps", flush=True)
+        return 0
+
+    precision = "mixed" if args.workload.endswith("mixed") else "fp32"
+    model = tc.config_baroclinic(1024, precision=precision)
+    q = torch.from_numpy(tc.random_pv(model, seed=2024).astype(np.float32)).cuda()
+    model._step_inplace(q, args.dt, 1)
+    torch.cuda.synchronize()
+    model._step_inplace(q, args.dt, args.steps)
+    torch.cuda.synchronize()
+    print(f"{args.workload}: {args.steps} steps", flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())
