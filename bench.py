@@ -196,6 +196,48 @@ def bench_ensemble(dt: float, steps: int, warmup: int) -> dict:
     return out
 
 
+def bench_swm(steps: int, warmup: int) -> dict:
+    """The 2-layer shallow-water baroclinic jet: 512 x 512 channel (periodic in
+    x, walls in y), f-plane, H = 500/3500 m, g' = 9.81/0.02, balanced 0.5 m/s
+    jet plus 1 m interface noise, SSP-RK3 at CFL 0.5; one fused stage kernel
+    per RK stage."""
+    import numpy as np
+    import torch
+
+    from somax_amd import _lib
+    from somax_amd import testcases as tc
+
+    model, (h, u, v) = tc.baroclinic_jet_swm(512, seed=3, lateral_viscosity=50.0)
+    dt = tc.jet_dt(model)
+    fields = [torch.from_numpy(a.astype(np.float32)).cuda() for a in (h, u, v)]
+    model._step_inplace(*fields, dt, max(warmup, 1))
+    torch.cuda.synchronize()
+    st = torch.cuda.current_stream()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record(st)
+    model._step_inplace(*fields, dt, steps)
+    e1.record(st)
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / steps
+    if not all(torch.isfinite(f).all() for f in fields):
+        raise RuntimeError("non-finite shallow-water state")
+    stats = (_lib.KernelStat * 4)()
+    nst = ctypes.c_int(0)
+    prm = model._c_params()
+    _lib.check(
+        _lib.load().somaxhip_swm_step_profile(model._plan, *(f.data_ptr() for f in fields), 2, dt, ctypes.byref(prm),
+                                             model._ws.data_ptr(), ctypes.c_void_p(st.cuda_stream), stats, 4,
+                                             ctypes.byref(nst)),
+        "somaxhip_swm_step_profile",
+    )
+    k = stats[0]
+    cells = model._nl * model.grid.nx * model.grid.ny
+    roof = roofline_entry("swm_stage", dict(launches=k.launches, total_ms=k.total_ms, flops=0.0, bytes=k.bytes))
+    return {"workload": "swm2x512x512_channel_jet_ssprk3", "dt": round(dt, 3), "ms_per_step": round(ms, 4),
+            "value": round(cells / (ms * 1e-3), 1), "unit": "layer-cell-steps/s", "roofline": roof}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -207,6 +249,7 @@ def main() -> int:
     ap.add_argument("--profile-steps", type=int, default=2)
     ap.add_argument("--cpu-steps", type=int, default=2, help="oracle steps timed for cpu_baseline (0 = skip)")
     ap.add_argument("--ensemble-steps", type=int, default=10, help="steps timed for the 256-member ensemble (0 = skip)")
+    ap.add_argument("--swm-steps", type=int, default=50, help="steps timed for the shallow-water jet (0 = skip)")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -251,6 +294,11 @@ def main() -> int:
         roof_stencil["traffic"] = tr.get("qg_stage")
 
     ens = bench_ensemble(dt, args.ensemble_steps, 2) if args.ensemble_steps > 0 else None
+    swm = None
+    if args.swm_steps > 0:
+        log("[bench] shallow-water jet 2x512x512")
+        swm = bench_swm(args.swm_steps, 2)
+        log(f"[bench] swm {swm['ms_per_step']:.3f} ms/step")
 
     cpu = None
     if args.cpu_steps > 0:
@@ -281,6 +329,7 @@ def main() -> int:
         "kernel_profile_mixed": {k: {"launches": v["launches"], "avg_us": round(1e3 * v["total_ms"] / v["launches"], 2)}
                                  for k, v in prof_mixed.items()},
         "ensemble": ens,
+        "swm": swm,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

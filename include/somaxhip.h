@@ -144,6 +144,60 @@ int somaxhip_qg_step(somaxhip_qg_plan plan, float* q, float* psi_out, int nsteps
                      somaxhip_stream_t stream);
 
 /* ------------------------------------------------------------------------ */
+/* Multilayer shallow water                                                 */
+/* ------------------------------------------------------------------------ */
+
+/* Boundary conditions of MultilayerShallowWater2D (multilayer.py:203-223). */
+enum {
+  SOMAXHIP_BC_PERIODIC = 0, /* enforce_periodic on h, u, v (multilayer.py:213-216) */
+  SOMAXHIP_BC_WALL = 1,     /* free-slip walls (multilayer.py:217-222, 383-410) */
+  SOMAXHIP_BC_CHANNEL = 2   /* extension: periodic in x, the wall rules in y */
+};
+
+typedef struct somaxhip_swm_plan_s* somaxhip_swm_plan;
+
+typedef struct {
+  int nx, ny;               /* interior cells */
+  int nl;                   /* layers */
+  double dx, dy;
+  const double* g_prime;    /* nl reduced gravities; p_k = sum_{j<=k} g'_j h_j (multilayer.py:180-182) */
+  const double* H;          /* nl layer thicknesses; the wind is divided by H[0] (multilayer.py:190) */
+  const double* f_row;      /* ny+2 values f0 + beta*(y_j - Ly/2), y_j = j*dy (multilayer.py:345-349) */
+  const double* wind_row;   /* ny+2 values of the x wind-stress pattern (multilayer.py:352-356) */
+  int bc;                   /* SOMAXHIP_BC_* */
+} somaxhip_swm_desc;
+
+/* MultilayerSW2DParams (multilayer.py:43-55), passed per call. */
+typedef struct {
+  double nu;     /* lateral_viscosity: + nu * laplacian of u and v (multilayer.py:194-195) */
+  double kappa;  /* bottom_drag: - kappa * (u, v) of the bottom layer (multilayer.py:198-199) */
+  double tau0;   /* wind_amplitude (multilayer.py:190-191) */
+} somaxhip_swm_params;
+
+/* Plan of one MultilayerShallowWater2D.create (multilayer.py:259-377). */
+int somaxhip_swm_plan_create(const somaxhip_swm_desc* desc, somaxhip_swm_plan* plan);
+int somaxhip_swm_plan_destroy(somaxhip_swm_plan plan);
+size_t somaxhip_swm_workspace_bytes(somaxhip_swm_plan plan);
+
+/* (dh, du, dv) = vector_field(t, state) (multilayer.py:150-201) with state
+ * (h, u, v), each float32[nl][ny+2][nx+2]; apply_bc != 0 applies
+ * apply_boundary_conditions first, as SomaxModel.build_terms does. */
+int somaxhip_swm_rhs(somaxhip_swm_plan plan, const float* h, const float* u, const float* v, float* dh,
+                     float* du, float* dv, const somaxhip_swm_params* params, int apply_bc,
+                     somaxhip_stream_t stream);
+
+/* (ho, uo, vo) = apply_boundary_conditions(h, u, v) (multilayer.py:203-223);
+ * outputs must not alias inputs. */
+int somaxhip_swm_apply_bc(somaxhip_swm_plan plan, const float* h, const float* u, const float* v, float* ho,
+                          float* uo, float* vo, somaxhip_stream_t stream);
+
+/* Advances (h, u, v) in place by nsteps SSP-RK3 steps of dt with the
+ * boundary condition applied before every RHS (the hot loop of
+ * SomaxModel.integrate, core/model.py:53-88). */
+int somaxhip_swm_step(somaxhip_swm_plan plan, float* h, float* u, float* v, int nsteps, double dt,
+                      const somaxhip_swm_params* params, void* workspace, somaxhip_stream_t stream);
+
+/* ------------------------------------------------------------------------ */
 /* Ensembles                                                                */
 /* ------------------------------------------------------------------------ */
 
@@ -204,6 +258,12 @@ int somaxhip_qg_step_profile(somaxhip_qg_plan plan, float* q, int nsteps, double
                              const somaxhip_qg_params* params, void* workspace,
                              somaxhip_stream_t stream, somaxhip_kernel_stat* stats,
                              int max_stats, int* nstats);
+
+/* somaxhip_swm_step with every stage launch timed (HIP events, one
+ * synchronisation per launch). */
+int somaxhip_swm_step_profile(somaxhip_swm_plan plan, float* h, float* u, float* v, int nsteps, double dt,
+                              const somaxhip_swm_params* params, void* workspace, somaxhip_stream_t stream,
+                              somaxhip_kernel_stat* stats, int max_stats, int* nstats);
 
 /* somaxhip_ens_step run eagerly on the caller's stream, classes one after
  * another, every launch timed; synchronises the stream before returning. */

@@ -1,0 +1,296 @@
+"""FP64 restatement of the multilayer shallow-water model (test infrastructure).
+
+Follows ``somax/_src/models/swm/multilayer.py``: ``vector_field`` (:150-201),
+``apply_boundary_conditions`` (:203-223) with the wall helpers (:383-410),
+``diagnose`` (:225-255) and ``create`` (:259-377).  The arithmetic lives in
+finitevolX v0.0.39 (an un-vendored git dependency that is not installed), so
+the staggering and stencils below are this restatement's stated convention:
+
+* C-grid by index offset on arrays of one shape ``(Ny, Nx)``: ``h[j, i]`` at
+  the T point (cell centre), ``u[j, i]`` on its east face, ``v[j, i]`` on its
+  north face, X (corner) point ``[j, i]`` at its north-east corner.  T row j
+  sits at ``y = j*dy`` (grid pin of the QG models).
+* Every intermediate field (PV at X, mass fluxes, kinetic energy, averages)
+  is evaluated wherever its stencil lies inside the array; tendencies are
+  written on the interior ``[1:-1, 1:-1]`` of each field and are zero on the
+  ghost ring, except the wind (top layer) and bottom drag (bottom layer)
+  terms, which the reference adds to the full arrays (:190-199).
+* Averages are arithmetic means of the nearest points; kinetic energy at T
+  is ``(u_w^2 + u_e^2 + v_s^2 + v_n^2) / 4`` (mean of squares); PV at X is
+  ``(zeta + f_X) / h_X`` with ``f_X`` and ``h_X`` the 4-point T means;
+  ``zeta = dv/dx - du/dy`` by one-sided differences around the corner.
+* Mass flux ``Advection2D(method="upwind1")``: face value of h from the
+  upwind cell (left/lower for positive velocity).
+* ``Diffusion2D(f, nu)``: ``nu`` times the 5-point Laplacian.
+* Periodic ghost copy (``enforce_periodic``): ghost row/column 0 takes the
+  last interior row/column and ghost N-1 the first.
+
+These conventions are "parity unpinned" against the reference: the reference's
+own tests pin only the rest state (zero tendency < 1e-10), forcing locality
+and the wall values (tests/models/test_swm_multilayer.py:94-175), which the
+checks in tests/test_oracle_swm.py reproduce; invariants (mass conservation,
+free-stream preservation, antisymmetry of the Coriolis/PV term's work) pin
+the arithmetic further.
+
+``bc="channel"`` (periodic in x, walls in y) is an extension for the
+baroclinic-jet benchmark: the periodic copy on columns, the wall rules on
+rows.
+"""
+
+from __future__ import annotations
+
+from dataclasses import dataclass
+
+import numpy as np
+
+
+@dataclass
+class SWMSetup:
+    nx: int
+    ny: int
+    Lx: float
+    Ly: float
+    nl: int
+    g_prime: np.ndarray
+    H: np.ndarray
+    f_row: np.ndarray  # (Ny,) Coriolis parameter at T rows
+    wind_row: np.ndarray  # (Ny,) normalised x wind-stress pattern at T rows
+    nu: float = 0.0
+    kappa: float = 0.0
+    tau0: float = 0.0
+    bc: str = "periodic"
+
+    @property
+    def Nx(self):
+        return self.nx + 2
+
+    @property
+    def Ny(self):
+        return self.ny + 2
+
+    @property
+    def dx(self):
+        return self.Lx / self.nx
+
+    @property
+    def dy(self):
+        return self.Ly / self.ny
+
+    @property
+    def shape(self):
+        return (self.nl, self.Ny, self.Nx)
+
+
+def setup(nx=64, ny=64, Lx=4e6, Ly=4e6, f0=9.375e-5, beta=1.754e-11, H=(400.0, 1100.0, 2600.0),
+          g_prime=(9.81, 0.025, 0.0125), lateral_viscosity=0.0, bottom_drag=0.0, wind_amplitude=0.0,
+          wind_profile="doublegyre", bc="periodic") -> SWMSetup:
+    """``MultilayerShallowWater2D.create`` (multilayer.py:259-377) in FP64:
+    ``f = f0 + beta (y - Ly/2)`` (:345-349), ``tau_x = -cos(2 pi y / Ly)``
+    (doublegyre) or ``-cos(pi y / Ly)`` (single) (:352-356)."""
+    H = np.asarray(H, dtype=np.float64)
+    gp = np.asarray(g_prime, dtype=np.float64)
+    if H.size != gp.size:
+        raise ValueError("H and g_prime must have the same length")
+    Ny = ny + 2
+    dy = Ly / ny
+    y = np.arange(Ny, dtype=np.float64) * dy
+    f_row = f0 + beta * (y - Ly / 2.0)
+    if wind_profile == "single":
+        wind = -np.cos(np.pi * y / Ly)
+    else:
+        wind = -np.cos(2.0 * np.pi * y / Ly)
+    if bc not in ("periodic", "wall", "channel"):
+        raise ValueError(f"unknown bc {bc!r}")
+    return SWMSetup(nx=nx, ny=ny, Lx=Lx, Ly=Ly, nl=H.size, g_prime=gp, H=H, f_row=f_row, wind_row=wind,
+                    nu=lateral_viscosity, kappa=bottom_drag, tau0=wind_amplitude, bc=bc)
+
+
+# ---------------------------------------------------------------------------
+# boundary conditions (multilayer.py:203-223, 383-410)
+# ---------------------------------------------------------------------------
+def _periodic_cols(f):
+    f = f.copy()
+    f[..., :, 0] = f[..., :, -2]
+    f[..., :, -1] = f[..., :, 1]
+    return f
+
+
+def _periodic_rows(f):
+    f = f.copy()
+    f[..., 0, :] = f[..., -2, :]
+    f[..., -1, :] = f[..., 1, :]
+    return f
+
+
+def enforce_periodic(f):
+    """Ghost ring from the opposite interior row/column."""
+    return _periodic_cols(_periodic_rows(np.asarray(f, dtype=np.float64)))
+
+
+def wall_bc_u(u):
+    """multilayer.py:386-392: u = 0 on columns 0, -2, -1; rows 0/-1 copy rows 1/-2."""
+    u = np.array(u, dtype=np.float64)
+    u[..., :, 0] = 0.0
+    u[..., :, -2] = 0.0
+    u[..., :, -1] = 0.0
+    u[..., 0, :] = u[..., 1, :]
+    u[..., -1, :] = u[..., -2, :]
+    return u
+
+
+def wall_bc_v(v):
+    """multilayer.py:395-401: v = 0 on rows 0, -2, -1; columns 0/-1 copy 1/-2."""
+    v = np.array(v, dtype=np.float64)
+    v[..., 0, :] = 0.0
+    v[..., -2, :] = 0.0
+    v[..., -1, :] = 0.0
+    v[..., :, 0] = v[..., :, 1]
+    v[..., :, -1] = v[..., :, -2]
+    return v
+
+
+def wall_bc_h(h):
+    """multilayer.py:404-408: zero gradient on all walls."""
+    h = np.array(h, dtype=np.float64)
+    h[..., 0, :] = h[..., 1, :]
+    h[..., -1, :] = h[..., -2, :]
+    h[..., :, 0] = h[..., :, 1]
+    h[..., :, -1] = h[..., :, -2]
+    return h
+
+
+def apply_bc(s: SWMSetup, h, u, v):
+    if s.bc == "periodic":
+        return enforce_periodic(h), enforce_periodic(u), enforce_periodic(v)
+    if s.bc == "wall":
+        return wall_bc_h(h), wall_bc_u(u), wall_bc_v(v)
+    # channel: periodic columns, wall rows
+    h = np.array(h, dtype=np.float64)
+    h[..., 0, :] = h[..., 1, :]
+    h[..., -1, :] = h[..., -2, :]
+    u = np.array(u, dtype=np.float64)
+    u[..., 0, :] = u[..., 1, :]
+    u[..., -1, :] = u[..., -2, :]
+    v = np.array(v, dtype=np.float64)
+    v[..., 0, :] = 0.0
+    v[..., -2, :] = 0.0
+    v[..., -1, :] = 0.0
+    return _periodic_cols(h), _periodic_cols(u), _periodic_cols(v)
+
+
+# ---------------------------------------------------------------------------
+# intermediate fields (each where its stencil lies inside the array)
+# ---------------------------------------------------------------------------
+def relative_vorticity(s: SWMSetup, u, v):
+    """zeta at X points [j, i], j < Ny-1, i < Nx-1; 0 elsewhere."""
+    z = np.zeros(np.shape(u))
+    z[..., :-1, :-1] = (v[..., :-1, 1:] - v[..., :-1, :-1]) / s.dx - (u[..., 1:, :-1] - u[..., :-1, :-1]) / s.dy
+    return z
+
+
+def t_to_x(f):
+    out = np.zeros(np.shape(f))
+    out[..., :-1, :-1] = 0.25 * (f[..., :-1, :-1] + f[..., :-1, 1:] + f[..., 1:, :-1] + f[..., 1:, 1:])
+    return out
+
+
+def potential_vorticity(s: SWMSetup, u, v, h):
+    """q = (zeta + f_X) / h_X at X points (multilayer.py:164)."""
+    q = np.zeros(np.shape(u))
+    fX = 0.5 * (s.f_row[:-1] + s.f_row[1:])[:, None]
+    hX = t_to_x(h)[..., :-1, :-1]
+    zeta = relative_vorticity(s, u, v)[..., :-1, :-1]
+    q[..., :-1, :-1] = (zeta + fX) / hX
+    return q
+
+
+def kinetic_energy(u, v):
+    """KE at T points [j, i], j >= 1, i >= 1 (mean of the four face squares)."""
+    ke = np.zeros(np.shape(u))
+    ke[..., 1:, 1:] = 0.25 * (u[..., 1:, 1:] ** 2 + u[..., 1:, :-1] ** 2 + v[..., 1:, 1:] ** 2 + v[..., :-1, 1:] ** 2)
+    return ke
+
+
+def _interior(a):
+    return a[..., 1:-1, 1:-1]
+
+
+def rhs(s: SWMSetup, h, u, v):
+    """``vector_field`` (multilayer.py:150-201) on the state as given."""
+    h = np.asarray(h, dtype=np.float64)
+    u = np.asarray(u, dtype=np.float64)
+    v = np.asarray(v, dtype=np.float64)
+    dx, dy = s.dx, s.dy
+    # 1. mass: upwind fluxes on every face between two cells of the array
+    F = np.zeros_like(u)
+    F[..., :, :-1] = u[..., :, :-1] * np.where(u[..., :, :-1] > 0, h[..., :, :-1], h[..., :, 1:])
+    G = np.zeros_like(v)
+    G[..., :-1, :] = v[..., :-1, :] * np.where(v[..., :-1, :] > 0, h[..., :-1, :], h[..., 1:, :])
+    dh = np.zeros_like(h)
+    dh[..., 1:-1, 1:-1] = -((F[..., 1:-1, 1:-1] - F[..., 1:-1, :-2]) / dx + (G[..., 1:-1, 1:-1] - G[..., :-2, 1:-1]) / dy)
+    # 2-3. PV at X and mass fluxes at U / V
+    q = potential_vorticity(s, u, v, h)
+    uh = np.zeros_like(u)
+    uh[..., :, :-1] = 0.5 * (h[..., :, :-1] + h[..., :, 1:]) * u[..., :, :-1]
+    vh = np.zeros_like(v)
+    vh[..., :-1, :] = 0.5 * (h[..., :-1, :] + h[..., 1:, :]) * v[..., :-1, :]
+    # 4. Bernoulli potential: KE + cumulative g' h (multilayer.py:178-183)
+    P = kinetic_energy(u, v) + np.cumsum(s.g_prime[:, None, None] * h, axis=0)
+    # 5. momentum on interior U / V points
+    qU = 0.5 * (q[..., 1:-1, 1:-1] + q[..., :-2, 1:-1])
+    vhU = 0.25 * (vh[..., 1:-1, 1:-1] + vh[..., 1:-1, 2:] + vh[..., :-2, 1:-1] + vh[..., :-2, 2:])
+    qV = 0.5 * (q[..., 1:-1, 1:-1] + q[..., 1:-1, :-2])
+    uhV = 0.25 * (uh[..., 1:-1, 1:-1] + uh[..., 1:-1, :-2] + uh[..., 2:, 1:-1] + uh[..., 2:, :-2])
+    du = np.zeros_like(u)
+    dv = np.zeros_like(v)
+    du[..., 1:-1, 1:-1] = qU * vhU - (P[..., 1:-1, 2:] - P[..., 1:-1, 1:-1]) / dx
+    dv[..., 1:-1, 1:-1] = -qV * uhV - (P[..., 2:, 1:-1] - P[..., 1:-1, 1:-1]) / dy
+    # 6. wind on the top layer, full array (multilayer.py:190-191)
+    du[0] += s.tau0 * s.wind_row[:, None] / s.H[0]
+    # 7. diffusion (all layers)
+    if s.nu != 0.0:
+        for f, d in ((u, du), (v, dv)):
+            d[..., 1:-1, 1:-1] += s.nu * ((f[..., 1:-1, 2:] - 2 * _interior(f) + f[..., 1:-1, :-2]) / dx**2
+                                          + (f[..., 2:, 1:-1] - 2 * _interior(f) + f[..., :-2, 1:-1]) / dy**2)
+    # 8. bottom drag on the bottom layer, full array (multilayer.py:198-199)
+    du[-1] += -s.kappa * u[-1]
+    dv[-1] += -s.kappa * v[-1]
+    return dh, du, dv
+
+
+def ssprk3_step(s: SWMSetup, h, u, v, dt):
+    """One SSP-RK3 step (Shu-Osher form) with the BC applied before each RHS,
+    as ``SomaxModel.build_terms`` composes them (core/model.py:47-49)."""
+    y0 = [np.asarray(a, dtype=np.float64) for a in (h, u, v)]
+
+    def f(y):
+        return rhs(s, *apply_bc(s, *y))
+
+    k = f(y0)
+    y1 = [a + dt * b for a, b in zip(y0, k)]
+    k = f(y1)
+    y2 = [0.75 * a + 0.25 * (b + dt * c) for a, b, c in zip(y0, y1, k)]
+    k = f(y2)
+    return tuple(a / 3.0 + 2.0 / 3.0 * (b + dt * c) for a, b, c in zip(y0, y2, k))
+
+
+def run(s: SWMSetup, h, u, v, dt, nsteps):
+    y = (h, u, v)
+    for _ in range(nsteps):
+        y = ssprk3_step(s, *y, dt)
+    return y
+
+
+def diagnose(s: SWMSetup, h, u, v):
+    """``diagnose`` (multilayer.py:225-255): per-layer energy (KE + 0.5 g' h^2)
+    and potential enstrophy 0.5 q^2 h_X, summed over the interior x dx dy."""
+    h = np.asarray(h, dtype=np.float64)
+    ke = kinetic_energy(u, v)
+    q = potential_vorticity(s, u, v, h)
+    hX = t_to_x(h)
+    area = s.dx * s.dy
+    energy = (_interior(ke).sum(axis=(-2, -1)) + 0.5 * s.g_prime * (_interior(h) ** 2).sum(axis=(-2, -1))) * area
+    enstrophy = 0.5 * (_interior(q) ** 2 * _interior(hX)).sum(axis=(-2, -1)) * area
+    return {"energy": energy, "total_energy": energy.sum(), "enstrophy": enstrophy,
+            "total_enstrophy": enstrophy.sum(), "potential_vorticity": q,
+            "relative_vorticity": relative_vorticity(s, u, v), "kinetic_energy_field": ke}

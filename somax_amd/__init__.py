@@ -29,6 +29,13 @@ from .qg import (
     BarotropicQGState,
 )
 from .ensemble import QGEnsemble
+from .swm import (
+    MultilayerShallowWater2D,
+    MultilayerSW2DDiagnostics,
+    MultilayerSW2DParams,
+    MultilayerSW2DPhysConsts,
+    MultilayerSW2DState,
+)
 from .testcases import doublegyre_baroclinic_qg, doublegyre_qg
 
 __all__ = [
@@ -36,5 +43,6 @@ __all__ = [
     "State", "StratificationProfile", "BaroclinicQG", "BaroclinicQGDiagnostics", "BaroclinicQGParams",
     "BaroclinicQGPhysConsts", "BaroclinicQGState", "BarotropicQG", "BarotropicQGDiagnostics",
     "BarotropicQGParams", "BarotropicQGPhysConsts", "BarotropicQGState", "doublegyre_baroclinic_qg",
-    "doublegyre_qg", "QGEnsemble",
+    "doublegyre_qg", "QGEnsemble", "MultilayerShallowWater2D", "MultilayerSW2DDiagnostics", "MultilayerSW2DParams",
+    "MultilayerSW2DPhysConsts", "MultilayerSW2DState",
 ]

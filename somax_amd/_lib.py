@@ -20,6 +20,9 @@ PREC_FP32 = 0
 PREC_MIXED = 1
 ENS_CONCURRENT = 1
 ENS_GRAPH = 2
+BC_PERIODIC = 0
+BC_WALL = 1
+BC_CHANNEL = 2
 
 
 class SomaxHipError(RuntimeError):
@@ -61,6 +64,29 @@ class QGParams(ctypes.Structure):
     ]
 
 
+class SWMDesc(ctypes.Structure):
+    _fields_ = [
+        ("nx", ctypes.c_int),
+        ("ny", ctypes.c_int),
+        ("nl", ctypes.c_int),
+        ("dx", ctypes.c_double),
+        ("dy", ctypes.c_double),
+        ("g_prime", c_double_p),
+        ("H", c_double_p),
+        ("f_row", c_double_p),
+        ("wind_row", c_double_p),
+        ("bc", ctypes.c_int),
+    ]
+
+
+class SWMParams(ctypes.Structure):
+    _fields_ = [
+        ("nu", ctypes.c_double),
+        ("kappa", ctypes.c_double),
+        ("tau0", ctypes.c_double),
+    ]
+
+
 class KernelStat(ctypes.Structure):
     _fields_ = [
         ("name", ctypes.c_char * 32),
@@ -91,6 +117,20 @@ _SIGNATURES = {
         ctypes.c_int,
         [_vp, _vp, ctypes.c_int, ctypes.c_double, ctypes.POINTER(QGParams), _vp, _vp, ctypes.POINTER(KernelStat),
          ctypes.c_int, ctypes.POINTER(ctypes.c_int)],
+    ),
+    "somaxhip_swm_plan_create": (ctypes.c_int, [ctypes.POINTER(SWMDesc), ctypes.POINTER(_vp)]),
+    "somaxhip_swm_plan_destroy": (ctypes.c_int, [_vp]),
+    "somaxhip_swm_workspace_bytes": (ctypes.c_size_t, [_vp]),
+    "somaxhip_swm_rhs": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, ctypes.POINTER(SWMParams), ctypes.c_int, _vp]),
+    "somaxhip_swm_apply_bc": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "somaxhip_swm_step": (
+        ctypes.c_int,
+        [_vp, _vp, _vp, _vp, ctypes.c_int, ctypes.c_double, ctypes.POINTER(SWMParams), _vp, _vp],
+    ),
+    "somaxhip_swm_step_profile": (
+        ctypes.c_int,
+        [_vp, _vp, _vp, _vp, ctypes.c_int, ctypes.c_double, ctypes.POINTER(SWMParams), _vp, _vp,
+         ctypes.POINTER(KernelStat), ctypes.c_int, ctypes.POINTER(ctypes.c_int)],
     ),
     "somaxhip_ens_create": (
         ctypes.c_int,

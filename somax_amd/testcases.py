@@ -182,3 +182,85 @@ def config_ensemble(seed: int = 0, counts=ENSEMBLE_COUNTS, precision: str = "mix
             qs.append(random_pv(m, seed * 100003 + i).astype(np.float32))
             i += 1
     return models, qs
+
+
+# ---------------------------------------------------------------------------
+# multilayer shallow water
+# ---------------------------------------------------------------------------
+def baroclinic_instability_swm(nx=64, ny=64, Lx=1e6, Ly=1e6, f0=1e-4, beta=1.6e-11, H=(500.0, 4500.0),
+                               g_prime=(9.81, 0.025), lateral_viscosity=100.0, bottom_drag=1e-7, jet_speed=0.5,
+                               jet_width=5e4, perturbation=0.01, **kw):
+    """gfd_testcases.py:290-362: opposite Gaussian jets in the two layers of a
+    periodic domain, ``v`` perturbed by ``sin(4 pi x / Lx)`` on the jet."""
+    from .swm import MultilayerShallowWater2D, MultilayerSW2DState
+
+    model = MultilayerShallowWater2D.create(nx=nx, ny=ny, Lx=Lx, Ly=Ly, f0=f0, beta=beta, n_layers=2, H=H,
+                                            g_prime=g_prime, lateral_viscosity=lateral_viscosity,
+                                            bottom_drag=bottom_drag, bc="periodic", **kw)
+    g = model.grid
+    x = np.arange(g.Nx) * g.dx
+    y = np.arange(g.Ny) * g.dy
+    X, Y = np.meshgrid(x, y)
+    jet = np.exp(-0.5 * ((Y - Ly / 2.0) / jet_width) ** 2)
+    signs = np.array([1.0, -1.0])
+    u0 = signs[:, None, None] * jet_speed * jet[None]
+    v0 = np.broadcast_to(perturbation * np.sin(4.0 * np.pi * X / Lx)[None] * jet[None], (2, g.Ny, g.Nx)).copy()
+    h0 = np.ones((2, g.Ny, g.Nx)) * np.asarray(model.strat.H)[:, None, None]
+    torch = _torch()
+    dev = model.device
+    state = MultilayerSW2DState(h=torch.tensor(h0, dtype=torch.float32, device=dev),
+                                u=torch.tensor(u0, dtype=torch.float32, device=dev),
+                                v=torch.tensor(v0, dtype=torch.float32, device=dev))
+    return model, state
+
+
+# The 2-layer baroclinic jet: f-plane, H = 500/3500 m, g' = 9.81/0.02, a
+# square channel of 2048 km (periodic in x, walls in y).
+JET_L = 2.048e6
+JET_F0 = 1e-4
+
+
+def baroclinic_jet_swm(n: int = 512, seed: int = 0, jet_speed: float = 0.5, jet_width: float = 2e5,
+                       noise: float = 1.0, L: float = JET_L, **kw):
+    """Geostrophically balanced zonal jet in the upper layer, lower layer at
+    rest, plus seeded white noise of amplitude ``noise`` metres on the
+    interface (added to h_0, taken from h_1).  Balance with the cumulative
+    pressure ``P_k = sum_{j<=k} g'_j h_j``: ``f u_0 = -g'_0 dh_0/dy`` and
+    ``g'_1 dh_1/dy = -g'_0 dh_0/dy`` (u_1 = 0), integrated from the channel
+    centre.  Returns ``(model, host (h, u, v) FP64 arrays)``."""
+    from .swm import MultilayerShallowWater2D
+
+    H = (500.0, 3500.0)
+    gp = (9.81, 0.02)
+    model = MultilayerShallowWater2D.create(nx=n, ny=n, Lx=L, Ly=L, f0=JET_F0, beta=0.0, n_layers=2, H=H,
+                                            g_prime=gp, bc="channel", **kw)
+    g = model.grid
+    y = np.arange(g.Ny) * g.dy
+    yc = L / 2.0
+    s = (y - yc) / jet_width
+    ujet = jet_speed * np.exp(-0.5 * s**2)
+    # integral of the jet from the centre: U w sqrt(pi/2) erf(s / sqrt 2)
+    from math import erf, pi, sqrt
+
+    integ = jet_speed * jet_width * sqrt(pi / 2.0) * np.array([erf(v / sqrt(2.0)) for v in s])
+    dh0 = -JET_F0 / gp[0] * integ
+    dh1 = -gp[0] / gp[1] * dh0
+    rng = np.random.default_rng(seed)
+    eta = noise * (2.0 * rng.random((g.Ny, g.Nx)) - 1.0)
+    h = np.empty((2, g.Ny, g.Nx))
+    h[0] = H[0] + dh0[:, None] + eta
+    h[1] = H[1] + dh1[:, None] - eta
+    u = np.zeros((2, g.Ny, g.Nx))
+    u[0] = ujet[:, None]
+    v = np.zeros((2, g.Ny, g.Nx))
+    return model, (h, u, v)
+
+
+def jet_dt(model, cfl: float = 0.5) -> float:
+    """CFL-limited step: dt = cfl * min(dx, dy) / (max gravity-wave speed + 0.5 m/s),
+    wave speeds from the eigenvalues of diag(H) times the cumulative pressure matrix."""
+    H = np.asarray(model.strat.H)
+    gp = np.asarray(model.strat.g_prime)
+    M = np.tril(np.ones((H.size, H.size))) * gp[None, :]
+    c = float(np.sqrt(np.max(np.linalg.eigvals(np.diag(H) @ M).real)))
+    return cfl * min(model.grid.dx, model.grid.dy) / (c + 0.5)

@@ -1,0 +1,159 @@
+"""GPU parity of the multilayer shallow-water path against the FP64 oracle.
+
+The GPU receives float32 fields and the oracle the same values widened to
+FP64.  Tolerances (relative L2 over the whole array, per field):
+
+* apply_boundary_conditions: exact (copies and zeros).
+* RHS: dh 1e-5, du and dv 1e-5 (FP32 products of O(1) factors; the pressure
+  gradient is formed from exact FP32 differences of h).
+* One SSP-RK3 step: h, u, v 1e-6; 100 steps: 1e-4.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from oracle import swm as oswm
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    a = np.asarray(a, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+def _log(msg):
+    print(msg, flush=True)
+
+
+def _jet(n, bc="channel", seed=0, **kw):
+    """The baroclinic jet at n^2 (viscosity and drag as given) and its oracle setup."""
+    from somax_amd import testcases as tc
+
+    model, (h, u, v) = tc.baroclinic_jet_swm(n, seed=seed, **kw)
+    if bc != "channel":
+        model = type(model).create(nx=n, ny=n, Lx=tc.JET_L, Ly=tc.JET_L, f0=tc.JET_F0, beta=0.0, n_layers=2,
+                                   H=(500.0, 3500.0), g_prime=(9.81, 0.02), bc=bc, **kw)
+    s = oswm.setup(nx=n, ny=n, Lx=tc.JET_L, Ly=tc.JET_L, f0=tc.JET_F0, beta=0.0, H=(500.0, 3500.0),
+                   g_prime=(9.81, 0.02), bc=bc, **kw)
+    rng = np.random.default_rng(seed + 7)
+    v = 0.05 * rng.standard_normal(v.shape)  # cross-jet flow so every term is active
+    f32 = [np.asarray(a, dtype=np.float32) for a in (h, u, v)]
+    return model, s, f32
+
+
+def _state(f32):
+    import torch
+
+    from somax_amd import MultilayerSW2DState
+
+    h, u, v = (torch.from_numpy(a.copy()).cuda() for a in f32)
+    return MultilayerSW2DState(h=h, u=u, v=v)
+
+
+KW = dict(lateral_viscosity=50.0, bottom_drag=1e-7, wind_amplitude=2e-5)
+
+
+@pytest.mark.parametrize("bc", ["periodic", "wall", "channel"])
+def test_apply_bc_matches_oracle(gpu, bc):
+    _log(f"[swm] apply_bc {bc}")
+    model, s, f32 = _jet(64, bc, **KW)
+    st = model.apply_boundary_conditions(_state(f32))
+    ref = oswm.apply_bc(s, *[a.astype(np.float64) for a in f32])
+    for name, a, b in zip("huv", (st.h, st.u, st.v), ref):
+        assert np.array_equal(a.cpu().numpy().astype(np.float64), b), name
+
+
+@pytest.mark.parametrize("bc", ["periodic", "wall", "channel"])
+@pytest.mark.parametrize("apply_bc", [False, True])
+def test_rhs_matches_oracle(gpu, bc, apply_bc):
+    _log(f"[swm] rhs {bc} apply_bc={apply_bc}")
+    model, s, f32 = _jet(128, bc, **KW)
+    st = _state(f32)
+    d = model.build_terms()(0.0, st) if apply_bc else model.vector_field(0.0, st)
+    x = [a.astype(np.float64) for a in f32]
+    ref = oswm.rhs(s, *(oswm.apply_bc(s, *x) if apply_bc else x))
+    rs = [rel(a.cpu().numpy(), b) for a, b in zip((d.h, d.u, d.v), ref)]
+    _log(f"  rel-L2 dh {rs[0]:.3e} du {rs[1]:.3e} dv {rs[2]:.3e}")
+    assert max(rs) < 1e-5
+
+
+@pytest.mark.parametrize("n,bc", [(128, "channel"), (128, "wall"), (128, "periodic"), (512, "channel")])
+def test_one_step_matches_oracle(gpu, n, bc):
+    from somax_amd import testcases as tc
+
+    _log(f"[swm] one step n={n} {bc}")
+    model, s, f32 = _jet(n, bc, **KW)
+    dt = tc.jet_dt(model)
+    out = model.step(_state(f32), dt, 1)
+    ref = oswm.ssprk3_step(s, *[a.astype(np.float64) for a in f32], dt)
+    rs = [rel(a.cpu().numpy(), b) for a, b in zip((out.h, out.u, out.v), ref)]
+    _log(f"  dt={dt:.2f}s rel-L2 h {rs[0]:.3e} u {rs[1]:.3e} v {rs[2]:.3e}")
+    assert max(rs) < 1e-6
+
+
+def test_hundred_steps_match_oracle(gpu):
+    from somax_amd import testcases as tc
+
+    _log("[swm] 100 steps n=128 channel")
+    model, s, f32 = _jet(128, "channel", **KW)
+    dt = tc.jet_dt(model)
+    out = model.step(_state(f32), dt, 100)
+    ref = oswm.run(s, *[a.astype(np.float64) for a in f32], dt, 100)
+    rs = [rel(a.cpu().numpy(), b) for a, b in zip((out.h, out.u, out.v), ref)]
+    _log(f"  rel-L2 h {rs[0]:.3e} u {rs[1]:.3e} v {rs[2]:.3e}")
+    assert max(rs) < 1e-4
+
+
+def test_rest_state_and_forcing_locality(gpu):
+    """tests/models/test_swm_multilayer.py:94-127 on the HIP path."""
+    import torch
+
+    from somax_amd import MultilayerShallowWater2D, MultilayerSW2DState
+
+    m = MultilayerShallowWater2D.create(nx=16, ny=16, n_layers=2, H=(500.0, 500.0), g_prime=(9.81, 0.02))
+    shape = (2, m.grid.Ny, m.grid.Nx)
+    h = torch.ones(shape, device="cuda") * torch.tensor([500.0, 500.0], device="cuda")[:, None, None]
+    rest = m.apply_boundary_conditions(MultilayerSW2DState(h=h, u=torch.zeros(shape, device="cuda"),
+                                                           v=torch.zeros(shape, device="cuda")))
+    d = m.vector_field(0.0, rest)
+    assert max(float(d.h.abs().max()), float(d.u.abs().max()), float(d.v.abs().max())) < 1e-10
+    mw = MultilayerShallowWater2D.create(nx=16, ny=16, n_layers=2, H=(500.0, 500.0), g_prime=(9.81, 0.02),
+                                         wind_amplitude=1e-5)
+    d = mw.vector_field(0.0, rest)
+    assert float(d.u[0].abs().max()) > 0 and float(d.u[-1].abs().max()) < 1e-15
+    md = MultilayerShallowWater2D.create(nx=16, ny=16, n_layers=2, H=(500.0, 500.0), g_prime=(9.81, 0.02),
+                                         bottom_drag=1e-4)
+    uni = MultilayerSW2DState(h=rest.h, u=torch.ones(shape, device="cuda"), v=rest.v)
+    d = md.vector_field(0.0, md.apply_boundary_conditions(uni))
+    assert torch.isfinite(d.u).all()
+    assert float((d.u[-1, 1:-1, 1:-1] + 1e-4).abs().max()) < 1e-9  # drag on the bottom layer only
+
+
+@pytest.mark.parametrize("bc", ["periodic", "channel", "wall"])
+def test_mass_conserved_over_steps(gpu, bc):
+    from somax_amd import testcases as tc
+
+    _log(f"[swm] mass over 50 steps, {bc}")
+    model, s, f32 = _jet(128, bc, **KW)
+    st = model.apply_boundary_conditions(_state(f32))
+    m0 = st.h[:, 1:-1, 1:-1].double().sum(dim=(-2, -1)).cpu().numpy()
+    out = model.step(st, tc.jet_dt(model), 50)
+    m1 = out.h[:, 1:-1, 1:-1].double().sum(dim=(-2, -1)).cpu().numpy()
+    r = np.abs(m1 - m0) / m0
+    _log(f"  relative mass change per layer {r}")
+    assert r.max() < 1e-6
+
+
+def test_reference_instability_case_integrates(gpu):
+    """tests/models/test_swm_multilayer.py:254-266."""
+    import torch
+
+    from somax_amd import testcases as tc
+
+    model, state0 = tc.baroclinic_instability_swm(nx=16, ny=16)
+    sol = model.integrate(state0, t0=0.0, t1=1000.0, dt=10.0)
+    assert torch.isfinite(sol.ys.h).all() and sol.ys.h.shape[0] == 1

@@ -81,3 +81,35 @@ def test_config_parameters():
     assert m.params.wind_amplitude == pytest.approx(9.8175e-11, rel=1e-4)
     b = tc.config_barotropic(64)
     assert b.params.wind_amplitude == pytest.approx(2.4544e-14, rel=1e-4)
+
+
+def test_swm_create_and_validation():
+    """tests/models/test_swm_multilayer.py:55-83 on the host side."""
+    from somax_amd import MultilayerShallowWater2D
+
+    assert MultilayerShallowWater2D.create().consts.n_layers == 3
+    m = MultilayerShallowWater2D.create(nx=16, ny=16, n_layers=2, H=(500.0, 500.0), g_prime=(9.81, 0.02))
+    assert isinstance(m, SomaxModel) and m.strat.H.shape == (2,)
+    strat = StratificationProfile.from_N2_constant(N2=1e-5, depth=4000.0, n_layers=4)
+    assert MultilayerShallowWater2D.create(nx=16, ny=16, stratification=strat).consts.n_layers == 4
+    with pytest.raises(ValueError, match="must all be equal"):
+        MultilayerShallowWater2D.create(nx=16, ny=16, n_layers=3, H=(500.0, 500.0), g_prime=(9.81, 0.02))
+    with pytest.raises(ValueError):
+        MultilayerShallowWater2D.create(nx=16, ny=16, bc="open")
+    y = np.arange(18) * (4e6 / 16)
+    np.testing.assert_allclose(m.wind_stress_x[:, 0], -np.cos(2 * np.pi * y / 4e6))
+    np.testing.assert_allclose(m.f_field[:, 0], 9.375e-5 + 1.754e-11 * (y - 2e6))
+
+
+def test_baroclinic_jet_balance():
+    """The jet's layer thicknesses satisfy the discrete-in-y geostrophic
+    balance f u_0 = -g'_0 dh_0/dy to the accuracy of the Gaussian's sampling."""
+    from somax_amd import testcases as tc
+
+    model, (h, u, v) = tc.baroclinic_jet_swm(512, noise=0.0)
+    dy = model.grid.dy
+    dh0 = np.gradient(h[0, :, 5], dy)
+    dh1 = np.gradient(h[1, :, 5], dy)
+    np.testing.assert_allclose(tc.JET_F0 * u[0, :, 5], -9.81 * dh0, atol=1e-3 * tc.JET_F0 * 0.5)
+    np.testing.assert_allclose(9.81 * dh0 + 0.02 * dh1, 0.0, atol=1e-12)
+    assert h.min() > 0 and 20.0 < tc.jet_dt(model) < 500.0

@@ -1,0 +1,143 @@
+"""Pins of the FP64 shallow-water oracle (oracle/swm.py), no GPU.
+
+* The reference's own qualitative tests (tests/models/test_swm_multilayer.py:94-175):
+  rest state has zero tendency, wind acts on the top layer only, a height
+  bump couples into the lower layer's momentum, wall BC values.
+* Textbook C-grid theory: the eigenvalues of the tendency's Jacobian about
+  rest on a periodic f-plane are 0 (geostrophic modes) and +-i*omega with
+  omega^2 = f^2 cos^2(k dx/2) cos^2(l dy/2) + c^2 [(2/dx sin(k dx/2))^2 +
+  (2/dy sin(l dy/2))^2] (Arakawa & Lamb 1977, grid C), where c^2 runs over
+  the eigenvalues of diag(H) M for the pressure matrix M of the cumulative
+  g' h coupling.  A wrong sign, a transposed average or an index shift in
+  any of the linear operators moves these eigenvalues.
+* Flux-form mass conservation for every boundary condition, and the sign of
+  the Coriolis term on uniform flow.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from oracle import swm
+
+
+def _setup(bc="periodic", nl=2, n=16, **kw):
+    H = (500.0, 500.0, 1000.0)[:nl]
+    gp = (9.81, 0.02, 0.01)[:nl]
+    args = dict(nx=n, ny=n, Lx=1e6, Ly=1e6, f0=1e-4, beta=0.0, H=H, g_prime=gp, bc=bc)
+    args.update(kw)
+    return swm.setup(**args)
+
+
+def _rest(s):
+    h = np.ones(s.shape) * s.H[:, None, None]
+    return h, np.zeros(s.shape), np.zeros(s.shape)
+
+
+@pytest.mark.parametrize("bc", ["periodic", "wall", "channel"])
+def test_rest_state_zero_tendency(bc):
+    s = _setup(bc)
+    dh, du, dv = swm.rhs(s, *swm.apply_bc(s, *_rest(s)))
+    assert max(np.abs(dh).max(), np.abs(du).max(), np.abs(dv).max()) < 1e-10
+
+
+def test_wind_top_layer_only():
+    s = _setup(wind_amplitude=1e-5)
+    dh, du, dv = swm.rhs(s, *swm.apply_bc(s, *_rest(s)))
+    assert np.abs(du[0]).max() > 0
+    assert np.abs(du[-1]).max() < 1e-15
+
+
+def test_pressure_couples_layers():
+    s = _setup()
+    h, u, v = _rest(s)
+    h[0, s.Ny // 2, s.Nx // 2] += 1.0
+    dh, du, dv = swm.rhs(s, *swm.apply_bc(s, h, u, v))
+    assert np.abs(du[1]).max() > 0
+
+
+def test_wall_bc_values():
+    s = _setup("wall")
+    h, u, v = _rest(s)
+    u[:] = 1.0
+    v[:] = 1.0
+    hb, ub, vb = swm.apply_bc(s, h, u, v)
+    assert np.all(ub[:, :, 0] == 0) and np.all(ub[:, :, -1] == 0) and np.all(ub[:, :, -2] == 0)
+    assert np.all(vb[:, 0, :] == 0) and np.all(vb[:, -1, :] == 0) and np.all(vb[:, -2, :] == 0)
+    assert np.all(ub[:, 0, 1:-2] == 1.0)  # free slip copies the first interior row
+
+
+def _jacobian_eigs(s):
+    """Eigenvalues of d(rhs)/d(interior state) about rest, by central differences."""
+    h0, u0, v0 = _rest(s)
+    ny, nx = s.ny, s.nx
+    n1 = s.nl * ny * nx
+    idx = [(f, l, j, i) for f in range(3) for l in range(s.nl) for j in range(1, ny + 1) for i in range(1, nx + 1)]
+    J = np.zeros((3 * n1, 3 * n1))
+    eps = 1e-6
+    for c, (f, l, j, i) in enumerate(idx):
+        cols = []
+        for sign in (1.0, -1.0):
+            st = [h0.copy(), u0.copy(), v0.copy()]
+            st[f][l, j, i] += sign * eps * (s.H[l] if f == 0 else 1.0)
+            d = swm.rhs(s, *swm.apply_bc(s, *st))
+            cols.append(np.concatenate([x[:, 1:-1, 1:-1].ravel() for x in d]))
+        col = (cols[0] - cols[1]) / (2 * eps * (s.H[l] if f == 0 else 1.0))
+        J[:, c] = col
+    return np.linalg.eigvals(J)
+
+
+@pytest.mark.parametrize("nl", [1, 2])
+def test_linear_dispersion_matches_c_grid_theory(nl):
+    nx = ny = 6
+    s = _setup("periodic", nl=nl, n=nx, Lx=6e5, Ly=6e5)
+    lam = _jacobian_eigs(s)
+    assert np.abs(lam.real).max() < 1e-9 * np.abs(lam).max()
+    # pressure matrix of the cumulative coupling: P_k = sum_{j<=k} g'_j h_j
+    M = np.tril(np.ones((nl, nl))) * s.g_prime[None, :]
+    c2 = np.sort(np.linalg.eigvals(np.diag(s.H) @ M).real)
+    expect = []
+    for kk in range(nx):
+        for ll in range(ny):
+            k = 2 * np.pi * kk / s.Lx
+            l = 2 * np.pi * ll / s.Ly
+            C = np.cos(k * s.dx / 2) * np.cos(l * s.dy / 2)
+            S2 = (2 / s.dx * np.sin(k * s.dx / 2)) ** 2 + (2 / s.dy * np.sin(l * s.dy / 2)) ** 2
+            for c in c2:
+                w = np.sqrt(1e-4**2 * C**2 + c * S2)
+                expect += [w, -w]
+            expect += [0.0] * nl  # geostrophic modes
+    got = np.sort(lam.imag)
+    np.testing.assert_allclose(got, np.sort(expect), atol=1e-7 * np.abs(expect).max())
+
+
+@pytest.mark.parametrize("bc", ["periodic", "wall", "channel"])
+def test_mass_conserved(bc):
+    s = _setup(bc, nl=2, n=12)
+    rng = np.random.default_rng(3)
+    h, u, v = _rest(s)
+    h = h + rng.standard_normal(h.shape)
+    u = 0.1 * rng.standard_normal(u.shape)
+    v = 0.1 * rng.standard_normal(v.shape)
+    dh, _, _ = swm.rhs(s, *swm.apply_bc(s, h, u, v))
+    scale = np.abs(dh).max()
+    assert np.abs(dh[:, 1:-1, 1:-1].sum(axis=(-2, -1))).max() < 1e-12 * scale * dh[0].size
+
+
+def test_coriolis_turns_uniform_flow_right():
+    """Uniform eastward flow on an f-plane: dv/dt = -f U, du/dt = dh/dt = 0."""
+    s = _setup("periodic", nl=1, n=8)
+    h, u, v = _rest(s)
+    u[:] = 0.3
+    dh, du, dv = swm.rhs(s, *swm.apply_bc(s, h, u, v))
+    np.testing.assert_allclose(dv[0, 1:-1, 1:-1], -1e-4 * 0.3, rtol=1e-12)
+    assert np.abs(du[0, 1:-1, 1:-1]).max() < 1e-15 and np.abs(dh).max() < 1e-9
+
+
+def test_diagnose_rest_energy():
+    s = _setup("wall")
+    d = swm.diagnose(s, *swm.apply_bc(s, *_rest(s)))
+    expect = 0.5 * s.g_prime * s.H**2 * s.nx * s.ny * s.dx * s.dy
+    np.testing.assert_allclose(d["energy"], expect, rtol=1e-12)
+    assert np.all(d["energy"] >= 0)
