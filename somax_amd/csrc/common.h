@@ -32,6 +32,11 @@ __host__ __device__ inline uint16_t f32_to_bf16_bits(float x) {
   return (uint16_t)(r >> 16);
 }
 
+// Device form: the hardware round-to-nearest-even conversion
+// (v_cvt_pk_bf16_f32 on gfx950), branch free; equal to f32_to_bf16_bits
+// for every finite input.
+__device__ inline uint16_t bf16_rne(float x) { return __builtin_bit_cast(uint16_t, static_cast<__bf16>(x)); }
+
 __host__ __device__ inline float bf16_bits_to_f32(uint16_t b) {
   union {
     float f;

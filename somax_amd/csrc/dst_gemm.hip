@@ -35,40 +35,6 @@ constexpr int BK32 = 16;          // fp32 k-tile
 constexpr int LD32 = BK32 + 4;    // padded LDS row (80 bytes)
 constexpr int TILE32 = BM * LD32;
 
-// Epilogue: write one accumulator element; columns co >= RB are zero padding.
-__device__ inline void epi_store(const GemmArgs& a, long z, int ro, int co, float v, int opieces, float lamz) {
-  if (co >= a.RB) v = 0.0f;
-  else if (a.lamA) v = v / (a.lamA[ro] + a.lamB[co] - lamz);
-  const long idx = (long)ro * a.ldo + co;
-  if (opieces == 0) {
-    a.Of[z * a.o_batch + idx] = v;
-    return;
-  }
-  uint16_t* o = a.Ob + z * a.o_batch + idx;
-  const uint16_t p0 = f32_to_bf16_bits(v);
-  o[0] = p0;
-  if (opieces > 1) {
-    const float r1 = v - bf16_bits_to_f32(p0);
-    const uint16_t p1 = f32_to_bf16_bits(r1);
-    o[a.o_piece] = p1;
-    if (opieces > 2) o[2 * a.o_piece] = f32_to_bf16_bits(r1 - bf16_bits_to_f32(p1));
-  }
-}
-
-// FP64 running sum: the MFMA partial is added to a double accumulator and
-// the MFMA accumulator restarts from 0.
-__device__ inline void flush64(floatx16 (&acc)[2][2], double (&sum)[2][2][16]) {
-#pragma unroll
-  for (int i = 0; i < 2; ++i)
-#pragma unroll
-    for (int j = 0; j < 2; ++j)
-#pragma unroll
-      for (int r = 0; r < 16; ++r) {
-        sum[i][j][r] += (double)acc[i][j][r];
-        acc[i][j][r] = 0.0f;
-      }
-}
-
 __device__ inline void zero_acc(floatx16 (&acc)[2][2], double (&sum)[2][2][16]) {
 #pragma unroll
   for (int i = 0; i < 2; ++i)
@@ -96,23 +62,100 @@ __device__ inline void tile_of_block(int GA, int GB, int& z, int& ta, int& tb) {
   tb = rem - ta * GB;
 }
 
+// Epilogue.  The 128 x 128 tile leaves through LDS in two halves of 64 rows
+// (the waves holding that half write their accumulators, FP64 running sum
+// added when SUM64), so global stores are wide and coalesced: FP32 outputs
+// with lanes along a row, bf16 pieces as 16-byte vectors of 8 columns.
+// `stage` needs 64 * EPI_PITCH floats of LDS; the caller's pipeline is done.
+constexpr int EPI_PITCH = 132;
+
+template <bool SUM64>
 __device__ inline void store_tile(const GemmArgs& a, long z, int a0, int b0, const floatx16 (&acc)[2][2],
-                                  const double (&sum)[2][2][16], int opieces) {
-  const int lane = threadIdx.x & 63;
-  const int wave = threadIdx.x >> 6;
+                                  const double (&sum)[2][2][16], int opieces, float* stage) {
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wave = tid >> 6;
   const int wm = wave >> 1, wn = wave & 1;
   const float lamz = a.lamA ? a.lam[z] : 0.0f;
 #pragma unroll
-  for (int i = 0; i < 2; ++i)
+  for (int h = 0; h < 2; ++h) {
+    __syncthreads();  // the LDS is free (pipeline drained / previous half stored)
+    if (wm == h) {
 #pragma unroll
-    for (int j = 0; j < 2; ++j)
+      for (int i = 0; i < 2; ++i)
 #pragma unroll
-      for (int r = 0; r < 16; ++r) {
-        const int ro = a0 + wm * 64 + i * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
-        const int co = b0 + wn * 64 + j * 32 + (lane & 31);
-        if (ro < a.RA && co < a.RBpad)
-          epi_store(a, z, ro, co, (float)(sum[i][j][r] + (double)acc[i][j][r]), opieces, lamz);
+        for (int j = 0; j < 2; ++j)
+#pragma unroll
+          for (int r = 0; r < 16; ++r) {
+            const int lr = i * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+            const int lc = wn * 64 + j * 32 + (lane & 31);
+            stage[lr * EPI_PITCH + lc] = SUM64 ? (float)(sum[i][j][r] + (double)acc[i][j][r]) : acc[i][j][r];
+          }
+    }
+    __syncthreads();
+    const int r0 = a0 + h * 64;
+    if (opieces == 0) {
+      // FP32 (any pitch): 64 lanes on 64 consecutive columns
+      for (int e = tid; e < 64 * 128; e += 256) {
+        const int lr = e >> 7, lc = e & 127;
+        const int ro = r0 + lr, co = b0 + lc;
+        if (ro >= a.RA || co >= a.RBpad) continue;
+        float v = stage[lr * EPI_PITCH + lc];
+        if (a.lamA) v = v / (a.lamA[ro] + a.lamB[co < a.RB ? co : a.RB - 1] - lamz);
+        a.Of[z * a.o_batch + (long)ro * a.ldo + co] = co < a.RB ? v : 0.0f;
       }
+    } else {
+      // bf16 pieces: 8 consecutive columns per thread, 16 rows per pass.
+      // RBpad is a multiple of 32 and b0 of 128, so a vector is all in or out.
+      const int lc = (tid & 15) * 8;
+      const int co = b0 + lc;
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        const int lr = (tid >> 4) + 16 * k;
+        const int ro = r0 + lr;
+        if (ro >= a.RA || co >= a.RBpad) continue;
+        const floatx4 va = *reinterpret_cast<const floatx4*>(stage + lr * EPI_PITCH + lc);
+        const floatx4 vb = *reinterpret_cast<const floatx4*>(stage + lr * EPI_PITCH + lc + 4);
+        float v[8] = {va[0], va[1], va[2], va[3], vb[0], vb[1], vb[2], vb[3]};
+        // branch free per element (columns past RB are zero padding)
+        if (a.lamA) {
+          const float la = a.lamA[ro] - lamz;
+#pragma unroll
+          for (int e = 0; e < 8; ++e) {
+            const int c = co + e < a.RB ? co + e : a.RB - 1;
+            const float q = v[e] / (la + a.lamB[c]);
+            v[e] = co + e < a.RB ? q : 0.0f;
+          }
+        } else {
+#pragma unroll
+          for (int e = 0; e < 8; ++e) v[e] = co + e < a.RB ? v[e] : 0.0f;
+        }
+        u32x4 p0, p1, p2;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          uint32_t w0 = 0, w1 = 0, w2 = 0;
+#pragma unroll
+          for (int q = 0; q < 2; ++q) {
+            const float x = v[2 * e + q];
+            const uint16_t b0_ = bf16_rne(x);
+            const float r1 = x - bf16_bits_to_f32(b0_);
+            const uint16_t b1_ = bf16_rne(r1);
+            const uint16_t b2_ = bf16_rne(r1 - bf16_bits_to_f32(b1_));
+            w0 |= (uint32_t)b0_ << (16 * q);
+            w1 |= (uint32_t)b1_ << (16 * q);
+            w2 |= (uint32_t)b2_ << (16 * q);
+          }
+          p0[e] = w0;
+          p1[e] = w1;
+          p2[e] = w2;
+        }
+        uint16_t* o = a.Ob + z * a.o_batch + (long)ro * a.ldo + co;
+        *reinterpret_cast<u32x4*>(o) = p0;
+        if (opieces > 1) *reinterpret_cast<u32x4*>(o + a.o_piece) = p1;
+        if (opieces > 2) *reinterpret_cast<u32x4*>(o + 2 * a.o_piece) = p2;
+      }
+    }
+  }
 }
 
 template <int NPROD>
@@ -217,34 +260,106 @@ __device__ inline void bf16_mainloop(const GemmArgs& a, uint16_t* lds, long z, i
     ra[t] = wm * 64 + t * 32 + (lane & 31);
     rb[t] = wn * 64 + t * 32 + (lane & 31);
   }
-  for (int kt = 0; kt < nk; ++kt) {
-    // tile kt has landed once at most min(NSTAGE-2, nk-1-kt) later tiles are outstanding
+  // fragments of k16 step kk of the tile in stage buffer sb
+  auto read_frags = [&](const uint16_t* sb, int kk, bf16x8(&fa)[3][2], bf16x8(&fb)[3][2]) {
+    const int c = kk * 2 + (lane >> 5);  // logical 16-byte chunk of this lane's k-slice
+#pragma unroll
+    for (int p = 0; p < NPC; ++p)
+#pragma unroll
+      for (int t = 0; t < 2; ++t) {
+        fa[p][t] = *reinterpret_cast<const bf16x8*>(sb + p * C::TILE + ra[t] * 32 + ((c ^ swz(ra[t])) << 3));
+        fb[p][t] = *reinterpret_cast<const bf16x8*>(sb + (NPC + p) * C::TILE + rb[t] * 32 + ((c ^ swz(rb[t])) << 3));
+      }
+  };
+  // Make tile kt current: its loads (this wave's) have landed once at most
+  // min(NSTAGE-2, nk-1-kt) later tiles are outstanding; after the barrier
+  // every wave's loads of kt are visible and every wave has finished reading
+  // tile kt-1, whose stage then takes tile kt+NSTAGE-1.  The caller has
+  // waited for its own LDS reads (lgkmcnt(0)) first.  The empty asm
+  // statements keep the compiler from moving LDS reads across the barrier.
+  auto enter_tile = [&](int kt) {
     const int later = (nk - 1 - kt) < (C::NSTAGE - 2) ? (nk - 1 - kt) : (C::NSTAGE - 2);
     if (later >= 2) wait_vmcnt<2 * C::LPT>();
     else if (later == 1) wait_vmcnt<C::LPT>();
     else wait_vmcnt<0>();
-    // Own LDS reads of the previous tile done, then a bare barrier: the
-    // __syncthreads fence would wait for every in-flight tile (vmcnt(0)).
     __builtin_amdgcn_s_waitcnt((0xF) | (0x7 << 4) | (0x0 << 8) | (0x3 << 14));  // lgkmcnt(0)
+    asm volatile("" ::: "memory");
     __builtin_amdgcn_s_barrier();
+    asm volatile("" ::: "memory");
     if (kt + C::NSTAGE - 1 < nk) SOMAX_ISSUE(kt + C::NSTAGE - 1, (kt + C::NSTAGE - 1) % C::NSTAGE)
-    const uint16_t* sb = lds + (kt % C::NSTAGE) * C::STAGE;
-#pragma unroll
-    for (int kk = 0; kk < 2; ++kk) {
-      const int c = kk * 2 + (lane >> 5);  // logical 16-byte chunk of this lane's k-slice
-      bf16x8 fa[3][2], fb[3][2];
-#pragma unroll
-      for (int p = 0; p < NPC; ++p)
-#pragma unroll
-        for (int t = 0; t < 2; ++t) {
-          fa[p][t] = *reinterpret_cast<const bf16x8*>(sb + p * C::TILE + ra[t] * 32 + ((c ^ swz(ra[t])) << 3));
-          fb[p][t] =
-              *reinterpret_cast<const bf16x8*>(sb + (NPC + p) * C::TILE + rb[t] * 32 + ((c ^ swz(rb[t])) << 3));
-        }
-      mfma_products<NPROD>(fa, fb, acc);
-    }
-    if (ACC64 && ((kt + 1) % C::FLUSH == 0)) flush64(acc, sum);
+  };
+
+  // Software pipeline: the fragments of the next k16 step are read while the
+  // MFMAs of the current one run; the barrier that hands over the next tile
+  // sits between the two steps of a tile, so its first fragments also load
+  // under the current tile's second-step MFMAs.
+  // Four fragment register sets rotate over the two k16 steps of two
+  // consecutive tiles, so a read never targets registers that a queued MFMA
+  // still has to consume (that write-after-read would hold the read back
+  // until the MFMA starts and serialise LDS latency with MFMA execution).
+  bf16x8 fa0[3][2], fb0[3][2], fa1[3][2], fb1[3][2], fa2[3][2], fb2[3][2], fa3[3][2], fb3[3][2];
+  auto stage_of = [&](int kt) { return lds + (kt % C::NSTAGE) * C::STAGE; };
+  auto lgkm0 = [] { __builtin_amdgcn_s_waitcnt((0xF) | (0x7 << 4) | (0x0 << 8) | (0x3 << 14)); };
+  enter_tile(0);
+  read_frags(stage_of(0), 0, fa0, fb0);
+  // One tile that hands over to its successor: step 0 from (xa, xb), step 1
+  // into (ya, yb), next tile's step 0 into (za, zb).  sched_barrier(0) fences
+  // keep the machine scheduler from sinking reads below the MFMAs they overlap.
+#define SOMAX_TILE_MID(KT, W, XA, XB, YA, YB, ZA, ZB)       \
+  {                                                         \
+    lgkm0();                                                \
+    read_frags(stage_of(KT), 1, YA, YB);                    \
+    __builtin_amdgcn_sched_barrier(0);                      \
+    mfma_products<NPROD>(XA, XB, W);                        \
+    __builtin_amdgcn_sched_barrier(0);                      \
+    enter_tile((KT) + 1);                                   \
+    read_frags(stage_of((KT) + 1), 0, ZA, ZB);              \
+    __builtin_amdgcn_sched_barrier(0);                      \
+    mfma_products<NPROD>(YA, YB, W);                        \
+    __builtin_amdgcn_sched_barrier(0);                      \
   }
+  // two tiles, leaving the fragments of tile kt+2 step 0 in set 0
+  auto pair = [&](int kt, floatx16(&w)[2][2]) {
+    SOMAX_TILE_MID(kt, w, fa0, fb0, fa1, fb1, fa2, fb2)
+    SOMAX_TILE_MID(kt + 1, w, fa2, fb2, fa3, fb3, fa0, fb0)
+  };
+  int kt = 0;
+  if (ACC64) {
+    // FP64 running sum over windows of two k-tiles: each window's MFMA chain
+    // starts from zero and is added to the sum at its end, so no FP32
+    // accumulator is carried around the loop (it stays in AGPRs).
+    static_assert(C::FLUSH == 2, "one window per tile pair");
+    for (; kt + 2 <= nk - 1; kt += 2) {
+      floatx16 w[2][2];
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) w[i][j] = floatx16{};
+      pair(kt, w);
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+#pragma unroll
+          for (int r = 0; r < 16; ++r) sum[i][j][r] += (double)w[i][j][r];
+    }
+  } else {
+    for (; kt + 2 <= nk - 1; kt += 2) pair(kt, acc);
+  }
+  // remaining one or two tiles (the last has no successor); set 0 holds step 0
+  if (kt == nk - 2) {
+    SOMAX_TILE_MID(kt, acc, fa0, fb0, fa1, fb1, fa2, fb2)
+    lgkm0();
+    read_frags(stage_of(nk - 1), 1, fa3, fb3);
+    mfma_products<NPROD>(fa2, fb2, acc);
+    mfma_products<NPROD>(fa3, fb3, acc);
+  } else {
+    lgkm0();
+    read_frags(stage_of(nk - 1), 1, fa1, fb1);
+    mfma_products<NPROD>(fa0, fb0, acc);
+    mfma_products<NPROD>(fa1, fb1, acc);
+  }
+#undef SOMAX_TILE_MID
 #undef SOMAX_ISSUE
 }
 
@@ -275,16 +390,15 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
   floatx16 acc[2][2];
   double sum[2][2][16];
   zero_acc(acc, sum);
-  if (a.acc64) {
-    if (NPMAX == 1 || np == 1) bf16_mainloop<1, true>(a, lds16, z, a0, b0, x_in_a, acc, sum);
-    else if (NPMAX == 3 || np == 3) bf16_mainloop<3, true>(a, lds16, z, a0, b0, x_in_a, acc, sum);
-    else bf16_mainloop<6, true>(a, lds16, z, a0, b0, x_in_a, acc, sum);
-  } else {
-    if (NPMAX == 1 || np == 1) bf16_mainloop<1, false>(a, lds16, z, a0, b0, x_in_a, acc, sum);
-    else if (NPMAX == 3 || np == 3) bf16_mainloop<3, false>(a, lds16, z, a0, b0, x_in_a, acc, sum);
-    else bf16_mainloop<6, false>(a, lds16, z, a0, b0, x_in_a, acc, sum);
-  }
-  store_tile(a, z, a0, b0, acc, sum, opieces);
+  // FP32 accumulation over K ~ 1e3 costs ~2e-7 relative, far below the
+  // 3-product split error (~1e-5): only 6-product planes accumulate in FP64.
+  if (NPMAX == 1 || np == 1) bf16_mainloop<1, false>(a, lds16, z, a0, b0, x_in_a, acc, sum);
+  else if (NPMAX == 3 || np == 3) bf16_mainloop<3, false>(a, lds16, z, a0, b0, x_in_a, acc, sum);
+  else if (a.acc64) bf16_mainloop<6, true>(a, lds16, z, a0, b0, x_in_a, acc, sum);
+  else bf16_mainloop<6, false>(a, lds16, z, a0, b0, x_in_a, acc, sum);
+  float* stage = reinterpret_cast<float*>(lds16);
+  if (NPMAX == 6 && np == 6 && a.acc64) store_tile<true>(a, z, a0, b0, acc, sum, opieces, stage);
+  else store_tile<false>(a, z, a0, b0, acc, sum, opieces, stage);
 }
 template __global__ void dst_gemm_bf16_kernel<1>(GemmArgs);
 template __global__ void dst_gemm_bf16_kernel<3>(GemmArgs);
@@ -337,7 +451,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 2))) voi
   *reinterpret_cast<float4*>(lds + so + st0) = rs0;
   *reinterpret_cast<float4*>(lds + so + st1) = rs1;
   __syncthreads();
-  for (int kt = 0; kt < nk; ++kt) {
+  // one k-tile: prefetch the next into registers, MFMA the current from LDS
+  // into `w`, then park the prefetch in the other LDS buffer
+  auto ktile = [&](int kt, floatx16(&w)[2][2]) {
     const int buf = kt & 1;
     if (kt + 1 < nk) {
       const int k1 = (kt + 1) * BK32;
@@ -360,9 +476,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 2))) voi
       for (int i = 0; i < 2; ++i)
 #pragma unroll
         for (int j = 0; j < 2; ++j)
-          acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x2f32(fa[i], fb[j], acc[i][j], 0, 0, 0);
+          w[i][j] = __builtin_amdgcn_mfma_f32_32x32x2f32(fa[i], fb[j], w[i][j], 0, 0, 0);
     }
-    if (acc64 && ((kt & 3) == 3)) flush64(acc, sum);
     if (kt + 1 < nk) {
       float* tb = lds + (buf ^ 1) * 2 * TILE32;
       *reinterpret_cast<float4*>(tb + xo + st0) = rx0;
@@ -371,8 +486,32 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 2))) voi
       *reinterpret_cast<float4*>(tb + so + st1) = rs1;
     }
     __syncthreads();
+  };
+  if (!acc64) {
+    for (int kt = 0; kt < nk; ++kt) ktile(kt, acc);
+  } else {
+    // FP64 running sum over windows of 4 k-tiles, each window's MFMA chain
+    // starting from zero (no FP32 accumulator carried around the loop)
+    int kt = 0;
+    for (; kt + 4 <= nk; kt += 4) {
+      floatx16 w[2][2];
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) w[i][j] = floatx16{};
+#pragma unroll
+      for (int f = 0; f < 4; ++f) ktile(kt + f, w);
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+#pragma unroll
+          for (int r = 0; r < 16; ++r) sum[i][j][r] += (double)w[i][j][r];
+    }
+    for (; kt < nk; ++kt) ktile(kt, acc);
   }
-  store_tile(a, z, a0, b0, acc, sum, 0);
+  if (acc64) store_tile<true>(a, z, a0, b0, acc, sum, 0, lds);
+  else store_tile<false>(a, z, a0, b0, acc, sum, 0, lds);
 }
 
 hipError_t prepare_dst_gemm() {

@@ -390,13 +390,13 @@ void launch_zero_ghost(float* f, int nl, int Ny, int Nx, hipStream_t st) {
 // ---------------------------------------------------------------------------
 namespace {
 __device__ inline void store_pieces(uint16_t* o, long piece, int n, float v) {
-  const uint16_t p0 = f32_to_bf16_bits(v);
+  const uint16_t p0 = bf16_rne(v);
   o[0] = p0;
   if (n > 1) {
     const float r1 = v - bf16_bits_to_f32(p0);
-    const uint16_t p1 = f32_to_bf16_bits(r1);
+    const uint16_t p1 = bf16_rne(r1);
     o[piece] = p1;
-    if (n > 2) o[2 * piece] = f32_to_bf16_bits(r1 - bf16_bits_to_f32(p1));
+    if (n > 2) o[2 * piece] = bf16_rne(r1 - bf16_bits_to_f32(p1));
   }
 }
 }  // namespace
