@@ -94,19 +94,72 @@ __device__ inline float bc_value(const float* p, int f, int gy, int gx, int Ny, 
   return p[(long)sy * Nx + sx];
 }
 
+
+// Source index of row gy (or column gx) for each field after the boundary
+// condition; -1 where the condition sets the value to 0 or outside the array.
+// The conditions are separable: a row rule and a column rule per field.
+struct BcAxis {
+  int h, u, v;
+};
+__device__ inline BcAxis bc_rows(int gy, int Ny, int bc, int apply) {
+  if (gy < 0 || gy >= Ny) return {-1, -1, -1};
+  if (!apply) return {gy, gy, gy};
+  if (bc == SOMAXHIP_BC_PERIODIC) {
+    const int s = gy == 0 ? Ny - 2 : (gy == Ny - 1 ? 1 : gy);
+    return {s, s, s};
+  }
+  const int s = gy == 0 ? 1 : (gy == Ny - 1 ? Ny - 2 : gy);  // zero gradient / free slip
+  const int sv = (gy == 0 || gy == Ny - 2 || gy == Ny - 1) ? -1 : gy;  // no normal flow
+  return {s, s, sv};
+}
+__device__ inline BcAxis bc_cols(int gx, int Nx, int bc, int apply) {
+  if (gx < 0 || gx >= Nx) return {-1, -1, -1};
+  if (!apply) return {gx, gx, gx};
+  if (bc == SOMAXHIP_BC_PERIODIC || bc == SOMAXHIP_BC_CHANNEL) {
+    const int s = gx == 0 ? Nx - 2 : (gx == Nx - 1 ? 1 : gx);
+    return {s, s, s};
+  }
+  const int s = gx == 0 ? 1 : (gx == Nx - 1 ? Nx - 2 : gx);
+  const int su = (gx == 0 || gx == Nx - 2 || gx == Nx - 1) ? -1 : gx;
+  return {s, su, s};
+}
+
+// Linear block -> tile, each XCD taking a contiguous run of tiles in
+// row-major order (vertically adjacent tiles share their halo rows in L2).
+__device__ inline void swm_tile(int tiles_x, int& ty, int& tx) {
+  const int T = gridDim.x;
+  const int b = blockIdx.x;
+  const int xcd = b & 7, slot = b >> 3;
+  const int per = T >> 3, extra = T & 7;
+  const int t = xcd * per + (xcd < extra ? xcd : extra) + slot;
+  ty = t / tiles_x;
+  tx = t - ty * tiles_x;
+}
+
 }  // namespace
 
 __global__ __launch_bounds__(256) void swm_stage_kernel(SwmArgs a) {
   __shared__ float sh[SPH * SPW];
   __shared__ float su[SPH * SPW];
   __shared__ float sv[SPH * SPW];
+  __shared__ float sfx[STY + 1];  // Coriolis at the corner rows y0-1 .. y0+STY-1
 
   const int tid = threadIdx.x;
   const int tx = tid & 63;
   const int ty = tid >> 6;
-  const int x0 = blockIdx.x * STX;
-  const int y0 = blockIdx.y * STY;
   const int Nx = a.Nx, Ny = a.Ny;
+  int tyi, txi;
+  swm_tile((Nx + STX - 1) / STX, tyi, txi);
+  const int x0 = txi * STX;
+  const int y0 = tyi * STY;
+
+  if (tid <= STY) {
+    const int gy = y0 - 1 + tid;
+    sfx[tid] = (gy >= 0 && gy + 1 < Ny) ? 0.5f * (a.f_row[gy] + a.f_row[gy + 1]) : 0.0f;
+  }
+  // column maps of this lane's (at most two) load columns
+  const BcAxis cm0 = bc_cols(x0 - 1 + tx, Nx, a.bc, a.apply_bc);
+  const BcAxis cm1 = bc_cols(x0 - 1 + tx + 64, Nx, a.bc, a.apply_bc);
 
   // cumulative pressure-gradient sums of this thread's points (U and V)
   float dpx[SRW], dpy[SRW];
@@ -115,13 +168,21 @@ __global__ __launch_bounds__(256) void swm_stage_kernel(SwmArgs a) {
 
   for (int k = 0; k < a.nl; ++k) {
     const long off = (long)k * a.plane;
+    const float* hs = a.hs + off;
+    const float* us = a.us + off;
+    const float* vs = a.vs + off;
     if (k > 0) __syncthreads();  // previous layer's tiles are no longer read
-    for (int i = tid; i < SPH * SPW; i += 256) {
-      const int ly = i / SPW, lx = i - ly * SPW;
-      const int gy = y0 - 1 + ly, gx = x0 - 1 + lx;
-      sh[i] = bc_value(a.hs + off, F_H, gy, gx, Ny, Nx, a.bc, a.apply_bc);
-      su[i] = bc_value(a.us + off, F_U, gy, gx, Ny, Nx, a.bc, a.apply_bc);
-      sv[i] = bc_value(a.vs + off, F_V, gy, gx, Ny, Nx, a.bc, a.apply_bc);
+    for (int r = ty; r < SPH; r += 4) {
+      const BcAxis rm = bc_rows(y0 - 1 + r, Ny, a.bc, a.apply_bc);
+#pragma unroll
+      for (int half = 0; half < 2; ++half) {
+        const int c = tx + 64 * half;
+        if (c >= SPW) break;
+        const BcAxis& cm = half ? cm1 : cm0;
+        sh[r * SPW + c] = (rm.h >= 0 && cm.h >= 0) ? hs[(long)rm.h * Nx + cm.h] : 0.0f;
+        su[r * SPW + c] = (rm.u >= 0 && cm.u >= 0) ? us[(long)rm.u * Nx + cm.u] : 0.0f;
+        sv[r * SPW + c] = (rm.v >= 0 && cm.v >= 0) ? vs[(long)rm.v * Nx + cm.v] : 0.0f;
+      }
     }
     __syncthreads();
     const float gp = a.gprime[k];
@@ -136,8 +197,6 @@ __global__ __launch_bounds__(256) void swm_stage_kernel(SwmArgs a) {
 #define H(dj, di) sh[c + (dj) * SPW + (di)]
 #define U(dj, di) su[c + (dj) * SPW + (di)]
 #define V(dj, di) sv[c + (dj) * SPW + (di)]
-      // pressure-gradient sums need this layer's h differences at every point
-      // whose U / V faces the thread owns (valid for interior points)
       const bool interior = gy >= 1 && gy <= Ny - 2 && gx >= 1 && gx <= Nx - 2;
       float dh = 0.0f, du = 0.0f, dv = 0.0f;
       if (interior) {
@@ -152,9 +211,8 @@ __global__ __launch_bounds__(256) void swm_stage_kernel(SwmArgs a) {
         // PV at X points (dj, di) relative to (gy, gx)
         auto q = [&](int dj, int di) {
           const float zeta = (V(dj, di + 1) - V(dj, di)) * a.idx - (U(dj + 1, di) - U(dj, di)) * a.idy;
-          const float fx = 0.5f * (a.f_row[gy + dj] + a.f_row[gy + dj + 1]);
           const float hx = hk + 0.25f * (H(dj, di) + H(dj, di + 1) + H(dj + 1, di) + H(dj + 1, di + 1));
-          return (zeta + fx) / hx;
+          return (zeta + sfx[ly + 1 + dj]) * __builtin_amdgcn_rcpf(hx);
         };
         auto vh = [&](int dj, int di) { return (hk + 0.5f * (H(dj, di) + H(dj + 1, di))) * V(dj, di); };
         auto uh = [&](int dj, int di) { return (hk + 0.5f * (H(dj, di) + H(dj, di + 1))) * U(dj, di); };
@@ -284,7 +342,7 @@ void swm_stage(somaxhip_swm_plan p, const float* hs, const float* us, const floa
   a.c2 = c2;
   a.bc = p->bc;
   a.apply_bc = apply_bc;
-  dim3 grid((p->Nx + STX - 1) / STX, (p->Ny + STY - 1) / STY, 1);
+  dim3 grid((unsigned)(((p->Nx + STX - 1) / STX) * ((p->Ny + STY - 1) / STY)), 1, 1);
   hipEvent_t e0 = nullptr, e1 = nullptr;
   if (g_swm_prof) {
     (void)hipEventCreate(&e0);
