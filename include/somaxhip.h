@@ -128,6 +128,15 @@ int somaxhip_qg_rhs(somaxhip_qg_plan plan, const float* q, float* dq, float* psi
                     const somaxhip_qg_params* params, int apply_bc, void* workspace,
                     somaxhip_stream_t stream);
 
+/* diagnose(state) (baroclinic.py:197-228 / barotropic.py:159-183): psi =
+ * _invert_pv(q); u = -d(psi)/dy on V faces, v = d(psi)/dx on U faces, zeta =
+ * laplacian(psi) (u, v, zeta nullable); ke_enstrophy (device, 2*nl doubles)
+ * receives per layer the kinetic energy 0.5 sum (uT^2 + vT^2) dx dy and the
+ * enstrophy 0.5 sum q^2 dx dy over the interior, summed in FP64 in a fixed
+ * order. */
+int somaxhip_qg_diagnose(somaxhip_qg_plan plan, const float* q, float* psi, float* u, float* v, float* zeta,
+                         double* ke_enstrophy, void* workspace, somaxhip_stream_t stream);
+
 /* Ghost ring of nl fields of (ny+2) x (nx+2) set to 0 in place.
  * Replaces finitevolx.zero_boundaries as used by
  * BaroclinicQG.apply_boundary_conditions (baroclinic.py:192-195). */
@@ -190,6 +199,15 @@ int somaxhip_swm_rhs(somaxhip_swm_plan plan, const float* h, const float* u, con
  * outputs must not alias inputs. */
 int somaxhip_swm_apply_bc(somaxhip_swm_plan plan, const float* h, const float* u, const float* v, float* ho,
                           float* uo, float* vo, somaxhip_stream_t stream);
+
+/* diagnose(state) (multilayer.py:225-255): PV q = (zeta + f)/h_X and relative
+ * vorticity zeta at X points, kinetic energy at T points (each nullable);
+ * energy_enstrophy (device, 2*nl doubles) receives per layer the energy
+ * sum (KE + 0.5 g' h^2) dx dy and the potential enstrophy sum 0.5 q^2 h_X dx dy
+ * over the interior, summed in FP64 in a fixed order.  Uses the workspace. */
+int somaxhip_swm_diagnose(somaxhip_swm_plan plan, const float* h, const float* u, const float* v, float* q,
+                          float* zeta, float* ke, double* energy_enstrophy, void* workspace,
+                          somaxhip_stream_t stream);
 
 /* Advances (h, u, v) in place by nsteps SSP-RK3 steps of dt with the
  * boundary condition applied before every RHS (the hot loop of

@@ -230,21 +230,32 @@ def run(s: QGSetup, q: np.ndarray, dt: float, nsteps: int) -> np.ndarray:
 
 
 def diagnose(s: QGSetup, q: np.ndarray):
-    """``diagnose`` (baroclinic.py:197-228): psi, KE and enstrophy per layer.
+    """``diagnose`` (baroclinic.py:197-228, barotropic.py:159-183): psi,
+    velocities, kinetic energy and enstrophy per layer, relative vorticity.
 
-    u = -diff_y_T_to_V(psi), v = diff_x_T_to_U(psi) (forward differences,
-    ghost ring 0), averaged back to T (V_to_T, U_to_T: mean of the two faces)."""
+    Convention (finitevolX's index ranges are parity unpinned): every operator
+    is evaluated wherever its stencil lies inside the array.
+    u = -diff_y_T_to_V(psi) on rows 0..Ny-2, v = diff_x_T_to_U(psi) on columns
+    0..Nx-2 (forward differences); V_to_T / U_to_T average the two faces of a
+    T cell, so the T-point velocities are centred differences of psi; KE and
+    enstrophy sum the interior T cells times dx dy; zeta = 5-point Laplacian on
+    the interior."""
+    q = np.asarray(q, dtype=np.float64).reshape(s.nl, s.Ny, s.Nx)
     psi = invert_pv(s, q)
     dx, dy = s.dx, s.dy
     u = np.zeros_like(psi)
     v = np.zeros_like(psi)
-    u[:, 1:-1, 1:-1] = -(psi[:, 2:, 1:-1] - psi[:, 1:-1, 1:-1]) / dy
-    v[:, 1:-1, 1:-1] = (psi[:, 1:-1, 2:] - psi[:, 1:-1, 1:-1]) / dx
-    uT = np.zeros_like(u)
-    vT = np.zeros_like(v)
-    uT[:, 1:-1, 1:-1] = 0.5 * (u[:, 1:-1, 1:-1] + u[:, :-2, 1:-1])
-    vT[:, 1:-1, 1:-1] = 0.5 * (v[:, 1:-1, 1:-1] + v[:, 1:-1, :-2])
+    u[..., :-1, :] = -(psi[..., 1:, :] - psi[..., :-1, :]) / dy
+    v[..., :, :-1] = (psi[..., :, 1:] - psi[..., :, :-1]) / dx
+    uT = 0.5 * (u[..., 1:-1, 1:-1] + u[..., :-2, 1:-1])
+    vT = 0.5 * (v[..., 1:-1, 1:-1] + v[..., 1:-1, :-2])
     area = dx * dy
-    ke = 0.5 * np.sum(uT[:, 1:-1, 1:-1] ** 2 + vT[:, 1:-1, 1:-1] ** 2, axis=(-2, -1)) * area
-    ens = 0.5 * np.sum(np.asarray(q)[:, 1:-1, 1:-1] ** 2, axis=(-2, -1)) * area
-    return {"psi": psi, "u": u, "v": v, "kinetic_energy": ke, "enstrophy": ens}
+    ke = 0.5 * np.sum(uT**2 + vT**2, axis=(-2, -1)) * area
+    ens = 0.5 * np.sum(q[..., 1:-1, 1:-1] ** 2, axis=(-2, -1)) * area
+    zeta = np.zeros_like(psi)
+    c = psi[..., 1:-1, 1:-1]
+    zeta[..., 1:-1, 1:-1] = (psi[..., 1:-1, 2:] - 2 * c + psi[..., 1:-1, :-2]) / dx**2 + (
+        psi[..., 2:, 1:-1] - 2 * c + psi[..., :-2, 1:-1]
+    ) / dy**2
+    return {"psi": psi, "u": u, "v": v, "kinetic_energy": ke, "total_kinetic_energy": ke.sum(),
+            "enstrophy": ens, "total_enstrophy": ens.sum(), "relative_vorticity": zeta}

@@ -113,6 +113,7 @@ _SIGNATURES = {
         [_vp, _vp, _vp, ctypes.c_int, ctypes.c_double, ctypes.POINTER(QGParams), _vp, _vp],
     ),
     "somaxhip_zero_ghost": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _vp]),
+    "somaxhip_qg_diagnose": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "somaxhip_qg_step_profile": (
         ctypes.c_int,
         [_vp, _vp, ctypes.c_int, ctypes.c_double, ctypes.POINTER(QGParams), _vp, _vp, ctypes.POINTER(KernelStat),
@@ -123,6 +124,7 @@ _SIGNATURES = {
     "somaxhip_swm_workspace_bytes": (ctypes.c_size_t, [_vp]),
     "somaxhip_swm_rhs": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, ctypes.POINTER(SWMParams), ctypes.c_int, _vp]),
     "somaxhip_swm_apply_bc": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "somaxhip_swm_diagnose": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "somaxhip_swm_step": (
         ctypes.c_int,
         [_vp, _vp, _vp, _vp, ctypes.c_int, ctypes.c_double, ctypes.POINTER(SWMParams), _vp, _vp],

@@ -129,6 +129,7 @@ struct Batch {
   long wpiece = 0;  // elements of one bf16 piece plane (maxN * wpitch)
   long splane = 0;  // Ny*Nx
   size_t off_OP1 = 0, off_OP2 = 0, off_QM = 0, off_P0 = 0, off_D = 0, off_Q1 = 0, off_Q2 = 0, off_PSI = 0;
+  size_t off_DIAG = 0;  // FP64 partial sums of the diagnostics
   size_t ws_bytes = 0;
 };
 
@@ -240,6 +241,7 @@ static hipError_t batch_build(Batch& b, const std::vector<const somaxhip_qg_desc
   b.off_Q1 = take((size_t)P * b.splane * sizeof(float));
   b.off_Q2 = take((size_t)P * b.splane * sizeof(float));
   b.off_PSI = take((size_t)P * b.splane * sizeof(float));
+  b.off_DIAG = take(qg_diag_partial_doubles(b.Nx, b.Ny, P) * sizeof(double));
   b.ws_bytes = off;
   return prepare_dst_gemm();
 }
@@ -711,6 +713,39 @@ int somaxhip_qg_step(somaxhip_qg_plan p, float* q, float* psi_out, int nsteps, d
   }
   hipStream_t st = reinterpret_cast<hipStream_t>(stream);
   rk3(p->b, q, psi_out, nsteps, dt, uniform_params(prm), ws, st);
+  SOMAX_HIP_CHECK(hipGetLastError());
+  return SOMAXHIP_OK;
+}
+
+int somaxhip_qg_diagnose(somaxhip_qg_plan p, const float* q, float* psi, float* u, float* v, float* zeta,
+                         double* ke_enstrophy, void* ws, somaxhip_stream_t stream) {
+  int rc = check_call(p, q, psi, ws, "somaxhip_qg_diagnose");
+  if (rc) return rc;
+  if (!ke_enstrophy) {
+    set_error("somaxhip_qg_diagnose: null ke_enstrophy");
+    return SOMAXHIP_ERR_ARG;
+  }
+  hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+  const Batch& b = p->b;
+  invert(b, q, psi, ws, st, false);
+  QGDiagArgs a{};
+  a.q = q;
+  a.psi = psi;
+  a.u = u;
+  a.v = v;
+  a.zeta = zeta;
+  a.Nx = b.Nx;
+  a.Ny = b.Ny;
+  a.P = b.P;
+  a.plane = b.splane;
+  a.idx = (float)(1.0 / b.dx);
+  a.idy = (float)(1.0 / b.dy);
+  a.idx2 = (float)(1.0 / (b.dx * b.dx));
+  a.idy2 = (float)(1.0 / (b.dy * b.dy));
+  a.area = b.dx * b.dy;
+  a.partial = reinterpret_cast<double*>(static_cast<char*>(ws) + b.off_DIAG);
+  a.out = ke_enstrophy;
+  launch_qg_diag(a, st);
   SOMAX_HIP_CHECK(hipGetLastError());
   return SOMAXHIP_OK;
 }

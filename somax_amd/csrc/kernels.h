@@ -148,3 +148,32 @@ void launch_qg_stage(const StageArgs& a, hipStream_t st);
 void launch_zero_ghost(float* f, int nl, int Ny, int Nx, hipStream_t st);
 
 }  // namespace somax
+
+namespace somax {
+
+// Diagnostics of the QG models on the planes of a batch (psi already solved):
+//   u = -d(psi)/dy on V faces (rows 0..Ny-2), v = d(psi)/dx on U faces
+//   (columns 0..Nx-2), zeta = 5-point Laplacian (interior), and per plane
+//   KE = 0.5 sum_interior (uT^2 + vT^2) dx dy with uT, vT the averages of the
+//   two faces of a cell, enstrophy = 0.5 sum_interior q^2 dx dy.
+// Sums are FP64, reduced in a fixed order (deterministic).
+struct QGDiagArgs {
+  const float* q;
+  const float* psi;
+  float* u;     // nullable
+  float* v;     // nullable
+  float* zeta;  // nullable
+  int Nx, Ny, P;
+  long plane;
+  float idx, idy, idx2, idy2;
+  double area;
+  double* partial;  // [P][Ny * bx][2] workspace
+  double* out;      // [P][2]: KE, enstrophy (device)
+};
+size_t qg_diag_partial_doubles(int Nx, int Ny, int P);
+void launch_qg_diag(const QGDiagArgs& a, hipStream_t st);
+
+// Fixed-order sum of `n` per-block pairs of every plane into out[p][2] * scale.
+void launch_pair_reduce(const double* partial, int n, int P, double scale, double* out, hipStream_t st);
+
+}  // namespace somax

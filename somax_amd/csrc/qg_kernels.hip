@@ -514,3 +514,98 @@ void launch_to_layer(const LayerArgs& a, hipStream_t st) {
 }
 
 }  // namespace somax
+
+namespace somax {
+
+// ---------------------------------------------------------------------------
+// diagnostics (baroclinic.py:197-228, barotropic.py:159-183)
+// ---------------------------------------------------------------------------
+size_t qg_diag_partial_doubles(int Nx, int Ny, int P) { return (size_t)2 * P * Ny * ((Nx + 255) / 256); }
+
+namespace {
+// block sum of a pair of doubles (256 threads), written by thread 0
+__device__ inline void block_pair_sum(double a, double b, double* dst) {
+  __shared__ double sa[256], sb[256];
+  const int t = threadIdx.x;
+  sa[t] = a;
+  sb[t] = b;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (t < s) {
+      sa[t] += sa[t + s];
+      sb[t] += sb[t + s];
+    }
+    __syncthreads();
+  }
+  if (t == 0) {
+    dst[0] = sa[0];
+    dst[1] = sb[0];
+  }
+}
+}  // namespace
+
+__global__ __launch_bounds__(256) void qg_diag_kernel(QGDiagArgs a) {
+  const int x = blockIdx.x * 256 + threadIdx.x;
+  const int y = blockIdx.y;
+  const int z = blockIdx.z;
+  const int Nx = a.Nx, Ny = a.Ny;
+  double ke = 0.0, en = 0.0;
+  if (x < Nx) {
+    const long s = z * a.plane + (long)y * Nx + x;
+    const float* p = a.psi + s;
+    if (a.u) {
+      float uv = 0.0f;
+      if (y <= Ny - 2) uv = -(p[Nx] - p[0]) * a.idy;
+      a.u[s] = uv;
+    }
+    if (a.v) {
+      float vv = 0.0f;
+      if (x <= Nx - 2) vv = (p[1] - p[0]) * a.idx;
+      a.v[s] = vv;
+    }
+    const bool interior = y >= 1 && y <= Ny - 2 && x >= 1 && x <= Nx - 2;
+    if (a.zeta) {
+      float zv = 0.0f;
+      if (interior) zv = (p[1] - 2.0f * p[0] + p[-1]) * a.idx2 + (p[Nx] - 2.0f * p[0] + p[-Nx]) * a.idy2;
+      a.zeta[s] = zv;
+    }
+    if (interior) {
+      // face averages: -(psi[y+1] - psi[y-1]) / (2 dy), (psi[x+1] - psi[x-1]) / (2 dx)
+      const double uT = -0.5 * ((double)(p[Nx] - p[0]) + (double)(p[0] - p[-Nx])) * a.idy;
+      const double vT = 0.5 * ((double)(p[1] - p[0]) + (double)(p[0] - p[-1])) * a.idx;
+      ke = uT * uT + vT * vT;
+      const double qv = a.q[s];
+      en = qv * qv;
+    }
+  }
+  const long blk = (long)z * gridDim.y * gridDim.x + (long)y * gridDim.x + blockIdx.x;
+  block_pair_sum(ke, en, a.partial + 2 * blk);
+}
+
+__global__ __launch_bounds__(256) void pair_reduce_kernel(const double* partial, int n, double scale, double* out) {
+  const int z = blockIdx.x;
+  const double* p = partial + 2L * n * z;
+  double a = 0.0, b = 0.0;
+  for (int i = threadIdx.x; i < n; i += 256) {
+    a += p[2 * i];
+    b += p[2 * i + 1];
+  }
+  __shared__ double res[2];
+  block_pair_sum(a, b, res);
+  if (threadIdx.x == 0) {
+    out[2 * z] = scale * res[0];
+    out[2 * z + 1] = scale * res[1];
+  }
+}
+
+void launch_qg_diag(const QGDiagArgs& a, hipStream_t st) {
+  const int bx = (a.Nx + 255) / 256;
+  hipLaunchKernelGGL(qg_diag_kernel, dim3(bx, a.Ny, a.P), dim3(256), 0, st, a);
+  launch_pair_reduce(a.partial, bx * a.Ny, a.P, 0.5 * a.area, a.out, st);
+}
+
+void launch_pair_reduce(const double* partial, int n, int P, double scale, double* out, hipStream_t st) {
+  hipLaunchKernelGGL(pair_reduce_kernel, dim3(P), dim3(256), 0, st, partial, n, scale, out);
+}
+
+}  // namespace somax

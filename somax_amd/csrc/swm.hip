@@ -25,6 +25,7 @@
 
 #include "../../include/somaxhip.h"
 #include "common.h"
+#include "kernels.h"
 
 namespace somax {
 
@@ -269,6 +270,59 @@ __global__ __launch_bounds__(256) void swm_bc_kernel(const float* h, const float
   vo[i] = bc_value(v + off, F_V, gy, gx, Ny, Nx, bc, 1);
 }
 
+// Diagnostics (multilayer.py:225-255): PV and relative vorticity at X points,
+// kinetic energy at T points (each where its stencil lies in the array), and
+// per layer the FP64 sums over the interior of KE + 0.5 g' h^2 (energy) and
+// 0.5 q^2 h_X (potential enstrophy); partial sums per block, reduced later
+// in a fixed order.
+__global__ __launch_bounds__(256) void swm_diag_kernel(const float* h, const float* u, const float* v, float* qo,
+                                                       float* zo, float* keo, int Nx, int Ny, long plane,
+                                                       const float* f_row, SwmArgs g, double* partial) {
+  __shared__ double sa[256], sb[256];
+  const int x = blockIdx.x * 256 + threadIdx.x;
+  const int y = blockIdx.y;
+  const int k = blockIdx.z;
+  double e = 0.0, en = 0.0;
+  if (x < Nx) {
+    const long s = k * plane + (long)y * Nx + x;
+    const float* H = h + s;
+    const float* U = u + s;
+    const float* V = v + s;
+    float qv = 0.0f, zv = 0.0f, hx = 0.0f;
+    if (y <= Ny - 2 && x <= Nx - 2) {
+      zv = (V[1] - V[0]) * g.idx - (U[Nx] - U[0]) * g.idy;
+      hx = 0.25f * (H[0] + H[1] + H[Nx] + H[Nx + 1]);
+      qv = (zv + 0.5f * (f_row[y] + f_row[y + 1])) / hx;
+    }
+    float kv = 0.0f;
+    if (y >= 1 && x >= 1) kv = 0.25f * (U[0] * U[0] + U[-1] * U[-1] + V[0] * V[0] + V[-Nx] * V[-Nx]);
+    if (qo) qo[s] = qv;
+    if (zo) zo[s] = zv;
+    if (keo) keo[s] = kv;
+    if (y >= 1 && y <= Ny - 2 && x >= 1 && x <= Nx - 2) {
+      const double hv = H[0];
+      e = (double)kv + 0.5 * (double)g.gprime[k] * hv * hv;
+      en = 0.5 * (double)qv * (double)qv * (double)hx;
+    }
+  }
+  const int t = threadIdx.x;
+  sa[t] = e;
+  sb[t] = en;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (t < w) {
+      sa[t] += sa[t + w];
+      sb[t] += sb[t + w];
+    }
+    __syncthreads();
+  }
+  if (t == 0) {
+    const long blk = (long)k * gridDim.y * gridDim.x + (long)y * gridDim.x + blockIdx.x;
+    partial[2 * blk] = sa[0];
+    partial[2 * blk + 1] = sb[0];
+  }
+}
+
 // f[k] += sign * H[k] over every point of nl planes (total thickness <->
 // deviation from the rest thickness; exact in FP32 while |h - H| <= H).
 __global__ __launch_bounds__(256) void swm_shift_kernel(float* f, long plane, SwmArgs a, float sign) {
@@ -484,6 +538,28 @@ int somaxhip_swm_apply_bc(somaxhip_swm_plan p, const float* h, const float* u, c
   dim3 grid((p->Nx + 255) / 256, p->Ny, p->nl);
   hipLaunchKernelGGL(swm_bc_kernel, grid, dim3(256), 0, reinterpret_cast<hipStream_t>(stream), h, u, v, ho, uo, vo,
                      p->Ny, p->Nx, p->plane, p->bc);
+  SOMAX_HIP_CHECK(hipGetLastError());
+  return SOMAXHIP_OK;
+}
+
+int somaxhip_swm_diagnose(somaxhip_swm_plan p, const float* h, const float* u, const float* v, float* q, float* zeta,
+                          float* ke, double* energy_enstrophy, void* ws, somaxhip_stream_t stream) {
+  int rc = check_plan(p, "somaxhip_swm_diagnose");
+  if (rc) return rc;
+  if (!h || !u || !v || !energy_enstrophy || !ws) {
+    set_error("somaxhip_swm_diagnose: null field, output or workspace");
+    return SOMAXHIP_ERR_ARG;
+  }
+  hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+  SwmArgs g{};
+  for (int k = 0; k < p->nl; ++k) g.gprime[k] = p->gprime[k];
+  g.idx = (float)(1.0 / p->dx);
+  g.idy = (float)(1.0 / p->dy);
+  const int bx = (p->Nx + 255) / 256;
+  double* partial = static_cast<double*>(ws);  // 2 * nl * Ny * bx doubles, well inside the workspace
+  hipLaunchKernelGGL(swm_diag_kernel, dim3(bx, p->Ny, p->nl), dim3(256), 0, st, h, u, v, q, zeta, ke, p->Nx, p->Ny,
+                     p->plane, p->f_row, g, partial);
+  launch_pair_reduce(partial, bx * p->Ny, p->nl, p->dx * p->dy, energy_enstrophy, st);
   SOMAX_HIP_CHECK(hipGetLastError());
   return SOMAXHIP_OK;
 }

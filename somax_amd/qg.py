@@ -349,6 +349,20 @@ class _QGBase(SomaxModel):
 
         return _rhs
 
+    def _diagnose(self, q):
+        """psi, u, v, zeta fields and per-layer (KE, enstrophy) on the device."""
+        torch = _torch()
+        plan = self._ensure_plan()
+        q = self._field(q)
+        psi, u, v, zeta = (torch.empty_like(q) for _ in range(4))
+        sums = torch.empty((self._nl, 2), dtype=torch.float64, device=q.device)
+        _lib.check(
+            _lib.load().somaxhip_qg_diagnose(plan, q.data_ptr(), psi.data_ptr(), u.data_ptr(), v.data_ptr(),
+                                            zeta.data_ptr(), sums.data_ptr(), self._ws.data_ptr(), self._stream()),
+            "somaxhip_qg_diagnose",
+        )
+        return psi, u, v, zeta, sums[:, 0], sums[:, 1]
+
     def _invert_pv(self, q):
         """``_invert_pv`` (baroclinic.py:135-159 / barotropic.py:113-121) on q as given."""
         return self._invert(q, apply_bc=False).reshape(q.shape)
@@ -426,7 +440,17 @@ class BaroclinicQG(_QGBase):
                             device=device)
 
     def diagnose(self, state) -> BaroclinicQGDiagnostics:
-        raise NotImplementedError("diagnose() is not available in this build")
+        """baroclinic.py:197-228: psi, geostrophic velocities on V / U faces,
+        kinetic energy and enstrophy per layer (FP64 sums), relative vorticity."""
+        torch = _torch()
+        psi, u, v, zeta, ke, ens = self._diagnose(state.q)
+        shape = (self._nl, self.grid.Ny, self.grid.Nx)
+        return BaroclinicQGDiagnostics(
+            psi=psi.reshape(shape), u=u.reshape(shape), v=v.reshape(shape), kinetic_energy=ke,
+            total_kinetic_energy=ke.sum(), enstrophy=ens, total_enstrophy=ens.sum(),
+            relative_vorticity=zeta.reshape(shape),
+            rossby_radii=torch.as_tensor(np.asarray(self.modal.rossby_radii), dtype=torch.float64),
+        )
 
 
 class BarotropicQG(_QGBase):
@@ -473,4 +497,8 @@ class BarotropicQG(_QGBase):
                             precision=precision, device=device)
 
     def diagnose(self, state) -> BarotropicQGDiagnostics:
-        raise NotImplementedError("diagnose() is not available in this build")
+        """barotropic.py:159-183 (fields 2-D, energies scalar)."""
+        psi, u, v, zeta, ke, ens = self._diagnose(state.q)
+        shape = (self.grid.Ny, self.grid.Nx)
+        return BarotropicQGDiagnostics(psi=psi.reshape(shape), u=u.reshape(shape), v=v.reshape(shape),
+                                       kinetic_energy=ke[0], enstrophy=ens[0], relative_vorticity=zeta.reshape(shape))

@@ -249,7 +249,23 @@ class MultilayerShallowWater2D(SomaxModel):
         return Solution(ts=torch.tensor(ts, dtype=torch.float64), ys=ys)
 
     def diagnose(self, state) -> MultilayerSW2DDiagnostics:
-        raise NotImplementedError("diagnose() is not available in this build")
+        """multilayer.py:225-255: energy and potential enstrophy per layer
+        (FP64 sums), PV and relative vorticity at X points, KE at T points."""
+        torch = _torch()
+        plan = self._ensure_plan()
+        h, u, v = self._fields(state)
+        q, zeta, ke = (torch.empty_like(h) for _ in range(3))
+        sums = torch.empty((self._nl, 2), dtype=torch.float64, device=h.device)
+        _lib.check(
+            _lib.load().somaxhip_swm_diagnose(plan, h.data_ptr(), u.data_ptr(), v.data_ptr(), q.data_ptr(),
+                                             zeta.data_ptr(), ke.data_ptr(), sums.data_ptr(), self._ws.data_ptr(),
+                                             self._stream()),
+            "somaxhip_swm_diagnose",
+        )
+        energy, enstrophy = sums[:, 0], sums[:, 1]
+        return MultilayerSW2DDiagnostics(energy=energy, total_energy=energy.sum(), enstrophy=enstrophy,
+                                         total_enstrophy=enstrophy.sum(), potential_vorticity=q,
+                                         relative_vorticity=zeta, kinetic_energy_field=ke)
 
     @staticmethod
     def create(nx: int = 64, ny: int = 64, Lx: float = 4e6, Ly: float = 4e6, g: float = 9.81, f0: float = 9.375e-5,

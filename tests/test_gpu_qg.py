@@ -284,3 +284,45 @@ def test_barotropic_matches_oracle(gpu):
     assert np.abs(ref_psi[0]).max() > 0  # barotropic keeps psi's ghost ring
     q1 = model.step(BarotropicQGState(q=qd), 4000.0).q.cpu().numpy()
     assert rel(q1, oqg.ssprk3_step(setup, q[None].astype(np.float64), 4000.0)[0]) < 1e-6
+
+
+@pytest.mark.parametrize("precision", ["fp32", "mixed"])
+def test_diagnose_matches_oracle(gpu, precision):
+    """diagnose (baroclinic.py:197-228).  psi to the inversion bar (3e-6);
+    the differenced fields inherit psi's FP32 error amplified by the ratio of
+    grid-scale to field-scale wavenumber (the error is white, the k^-3 signal
+    is not): u, v 5e-5 (one difference), zeta 5e-4 (two); the FP64-summed
+    per-layer KE 1e-5 and enstrophy 1e-6."""
+    _log(f"[diagnose] n=128 precision={precision}")
+    model, setup = _pair(128, precision=precision)
+    q, qd = _state(model, seed=31)
+    d = model.diagnose(type(model)._state_cls(q=qd))
+    ref = oqg.diagnose(setup, q.astype(np.float64))
+    r_psi = rel(d.psi.cpu().numpy(), ref["psi"])
+    r_u = rel(d.u.cpu().numpy(), ref["u"])
+    r_v = rel(d.v.cpu().numpy(), ref["v"])
+    r_z = rel(d.relative_vorticity.cpu().numpy(), ref["relative_vorticity"])
+    r_ke = rel(d.kinetic_energy.cpu().numpy(), ref["kinetic_energy"])
+    r_en = rel(d.enstrophy.cpu().numpy(), ref["enstrophy"])
+    _log(f"  psi {r_psi:.2e} u {r_u:.2e} v {r_v:.2e} zeta {r_z:.2e} KE {r_ke:.2e} enstrophy {r_en:.2e}")
+    assert r_psi < 3e-6 and r_u < 5e-5 and r_v < 5e-5 and r_z < 5e-4 and r_ke < 1e-5 and r_en < 1e-6
+    assert float(d.total_kinetic_energy) == pytest.approx(float(d.kinetic_energy.sum()))
+    assert d.rossby_radii.shape == (3,)
+
+
+def test_barotropic_diagnose(gpu):
+    import torch
+
+    from somax_amd import testcases as tc
+
+    _log("[diagnose] barotropic n=64")
+    model = tc.config_barotropic(64)
+    setup = oqg.barotropic_setup(nx=64, ny=64, Lx=tc.BASIN, Ly=tc.BASIN, f0=tc.F0, beta=tc.BETA,
+                                 bottom_drag=tc.DRAG, wind_amplitude=tc.wind_amplitude() / 4000.0,
+                                 biharmonic_viscosity=tc.NU4_128)
+    q = tc.random_pv(model, 5).astype(np.float32)[0]
+    d = model.diagnose(type(model)._state_cls(q=torch.from_numpy(q).cuda()))
+    ref = oqg.diagnose(setup, q.astype(np.float64))
+    assert d.psi.shape == q.shape and d.kinetic_energy.ndim == 0
+    assert rel(d.kinetic_energy.cpu().numpy(), ref["kinetic_energy"][0]) < 1e-5
+    assert rel(d.enstrophy.cpu().numpy(), ref["enstrophy"][0]) < 1e-6

@@ -157,3 +157,20 @@ def test_reference_instability_case_integrates(gpu):
     model, state0 = tc.baroclinic_instability_swm(nx=16, ny=16)
     sol = model.integrate(state0, t0=0.0, t1=1000.0, dt=10.0)
     assert torch.isfinite(sol.ys.h).all() and sol.ys.h.shape[0] == 1
+
+
+@pytest.mark.parametrize("bc", ["channel", "wall"])
+def test_diagnose_matches_oracle(gpu, bc):
+    """diagnose (multilayer.py:225-255): energy and enstrophy per layer to 1e-6
+    relative (FP64 sums), PV / vorticity / KE fields to 1e-5 relative L2."""
+    _log(f"[swm] diagnose {bc}")
+    model, s, f32 = _jet(128, bc, **KW)
+    d = model.diagnose(_state(f32))
+    ref = oswm.diagnose(s, *[a.astype(np.float64) for a in f32])
+    rs = {k: rel(getattr(d, a).cpu().numpy(), ref[k]) for k, a in
+          (("energy", "energy"), ("enstrophy", "enstrophy"), ("potential_vorticity", "potential_vorticity"),
+           ("relative_vorticity", "relative_vorticity"), ("kinetic_energy_field", "kinetic_energy_field"))}
+    _log("  " + " ".join(f"{k} {v:.2e}" for k, v in rs.items()))
+    assert rs["energy"] < 1e-6 and rs["enstrophy"] < 1e-6
+    assert rs["potential_vorticity"] < 1e-5 and rs["relative_vorticity"] < 1e-5 and rs["kinetic_energy_field"] < 1e-5
+    assert np.all(d.energy.cpu().numpy() >= 0)

@@ -352,3 +352,31 @@ def test_qg_weno_rhs_conserves_pv_without_forcing():
     q = stencils.zero_boundaries(np.random.default_rng(8).standard_normal((2, 18, 18)) * 1e-6)
     dq = qg.rhs(s, q)
     np.testing.assert_allclose(dq[:, 1:-1, 1:-1].sum(axis=(-2, -1)), 0.0, atol=1e-18)
+
+
+def test_qg_diagnose_closed_form():
+    """Quadratic psi = c j^2 (barotropic: the DST solve returns it exactly from
+    q = its zero-padded Laplacian): the T-point velocity is the centred
+    difference -2 c j / dy, so KE = 0.5 sum_j (2 c j / dy)^2 nx dx dy over the
+    interior rows; a forward or backward difference would give (2j +- 1).
+    The wall-face velocity u[0] = -c/dy is included; u on the last row is 0."""
+    from oracle import dst
+    from oracle import qg as oqg
+
+    n = 12
+    s = oqg.barotropic_setup(nx=n, ny=n, Lx=1.2e5, Ly=1.2e5)
+    j = np.arange(s.Ny, dtype=np.float64)
+    c = 1e3
+    psi = np.broadcast_to((c * j**2)[:, None], (s.Ny, s.Nx)).copy()
+    q = dst.helmholtz_apply(psi, s.dx, s.dy)
+    d = oqg.diagnose(s, q)
+    np.testing.assert_allclose(d["psi"][0], psi, rtol=1e-9, atol=1e-9 * psi.max())
+    jj = np.arange(1, n + 1, dtype=np.float64)
+    ke = 0.5 * np.sum((2 * c * jj / s.dy) ** 2) * n * s.dx * s.dy
+    np.testing.assert_allclose(d["kinetic_energy"], ke, rtol=1e-9)
+    np.testing.assert_allclose(d["u"][0, 0, 3], -c / s.dy, rtol=1e-9)
+    assert np.all(d["u"][0, -1] == 0) and np.abs(d["v"]).max() < 1e-6 * c / s.dx
+    np.testing.assert_allclose(d["enstrophy"], 0.5 * np.sum(q[1:-1, 1:-1] ** 2) * s.dx * s.dy)
+    # reflection symmetry of the summed energy
+    d2 = oqg.diagnose(s, q[::-1, ::-1].copy())
+    np.testing.assert_allclose(d2["kinetic_energy"], d["kinetic_energy"], rtol=1e-9)
