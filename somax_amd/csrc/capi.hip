@@ -129,8 +129,13 @@ struct Batch {
   long wpiece = 0;  // elements of one bf16 piece plane (maxN * wpitch)
   long splane = 0;  // Ny*Nx
   size_t off_OP1 = 0, off_OP2 = 0, off_QM = 0, off_P0 = 0, off_D = 0, off_Q1 = 0, off_Q2 = 0, off_PSI = 0;
-  size_t off_DIAG = 0;  // FP64 partial sums of the diagnostics
+  size_t off_DIAG = 0;  // FP64 partial sums of the diagnostics and the mass constraint
+  size_t off_MSUM = 0, off_ALPHA = 0, off_PWALL = 0;  // per-plane sums, alpha, wall psi
   size_t ws_bytes = 0;
+  // mass correction: homogeneous solutions (P planes) and their means (0: none)
+  int any_mass = 0;
+  float* homsol = nullptr;
+  double* hmean = nullptr;
 };
 
 // Checks one model description; empty string when valid.
@@ -142,7 +147,8 @@ static std::string check_desc(const somaxhip_qg_desc* d) {
     return "a required table pointer is null";
   if (d->advection != SOMAXHIP_ADV_ARAKAWA && d->advection != SOMAXHIP_ADV_WENO) return "unknown advection";
   if (d->precision != SOMAXHIP_PREC_FP32 && d->precision != SOMAXHIP_PREC_MIXED) return "unknown precision";
-  if (d->mass_correction) return "mass correction is not available in this build";
+  if (d->mass_correction && (!d->homsol || !d->homsol_mean))
+    return "mass_correction needs homsol and homsol_mean";
   return "";
 }
 
@@ -221,6 +227,21 @@ static hipError_t batch_build(Batch& b, const std::vector<const somaxhip_qg_desc
     return e;
   b.t = Tables{dfirst, dnl, dcl2m, dcm2l, dmask, dmember, dlayer, dlam, dhp, dws, nullptr};
 
+  // mass correction: homogeneous solution and its mean per plane
+  for (const somaxhip_qg_desc* d : ds) b.any_mass |= d->mass_correction ? 1 : 0;
+  if (b.any_mass) {
+    const size_t plane = (size_t)b.Ny * b.Nx;
+    std::vector<float> hs((size_t)P * plane, 0.0f);
+    std::vector<double> hm(P, 0.0);
+    for (int m = 0; m < b.M; ++m) {
+      const somaxhip_qg_desc* d = ds[m];
+      if (!d->mass_correction) continue;
+      std::copy(d->homsol, d->homsol + (size_t)d->nl * plane, hs.begin() + (size_t)mfirst[m] * plane);
+      for (int l = 0; l < d->nl; ++l) hm[mfirst[m] + l] = d->homsol_mean[l];
+    }
+    if ((e = b.mem.upload(&b.homsol, hs)) != hipSuccess || (e = b.mem.upload(&b.hmean, hm)) != hipSuccess) return e;
+  }
+
   // workspace: two GEMM operand buffers (3 bf16 pieces or 1.5 FP32 planes
   // per plane), FP32 modal planes, stage states and psi.
   const long maxN = b.Nx > b.Ny ? b.Nx : b.Ny;
@@ -242,6 +263,9 @@ static hipError_t batch_build(Batch& b, const std::vector<const somaxhip_qg_desc
   b.off_Q2 = take((size_t)P * b.splane * sizeof(float));
   b.off_PSI = take((size_t)P * b.splane * sizeof(float));
   b.off_DIAG = take(qg_diag_partial_doubles(b.Nx, b.Ny, P) * sizeof(double));
+  b.off_MSUM = take((size_t)2 * P * sizeof(double));
+  b.off_ALPHA = take((size_t)P * sizeof(double));
+  b.off_PWALL = take((size_t)P * sizeof(float));
   b.ws_bytes = off;
   return prepare_dst_gemm();
 }
@@ -571,6 +595,19 @@ void invert(const Batch& p, const float* q, float* psi, void* ws, hipStream_t st
     dst_solve(p, false, 3, 1, 1, D, ws, st, true, false);
     la.d = D;
   }
+  if (p.any_mass) {
+    // FP64 domain mean of every modal streamfunction, alpha = -mean / mean(homsol)
+    double* sums = reinterpret_cast<double*>(static_cast<char*>(ws) + p.off_MSUM);
+    double* alpha = reinterpret_cast<double*>(static_cast<char*>(ws) + p.off_ALPHA);
+    {
+      ProfScope ps("mass_constraint", 0.0, (mixed ? 8.0 : 4.0) * p.P * (double)p.splane, st);
+      launch_plane_sums(P0, la.d, p.Nx, p.Ny, p.P, p.splane,
+                        reinterpret_cast<double*>(static_cast<char*>(ws) + p.off_DIAG), sums, st);
+      launch_mass_alpha(sums, p.hmean, p.t, p.P, 1.0 / (double)p.splane, alpha, wsf(ws, p.off_PWALL), st);
+    }
+    la.homsol = p.homsol;
+    la.alpha = alpha;
+  }
   {
     ProfScope ps("to_layer", 0.0, (mixed ? 12.0 : 8.0) * p.P * (double)p.splane, st);
     launch_to_layer(la, st);
@@ -585,7 +622,7 @@ struct CallParams {
 };
 
 void stage(const Batch& p, const float* qs, const float* psi, const float* qn, float* out, const CallParams& cp,
-           float c1, float c2, bool apply_bc, hipStream_t st) {
+           float c1, float c2, bool apply_bc, void* ws, hipStream_t st) {
   StageArgs a{};
   a.qs = qs;
   a.psi = psi;
@@ -598,6 +635,7 @@ void stage(const Batch& p, const float* qs, const float* psi, const float* qn, f
   a.dbeta_rows = p.dbeta_rows;
   a.wind_rows = p.wind_rows;
   a.t = p.t;
+  if (p.any_mass) a.t.psi_wall = wsf(ws, p.off_PWALL);  // wall value of psi from the last inversion
   a.mp = cp.mp;
   a.u = cp.u;
   a.biharm = cp.biharm;
@@ -626,11 +664,11 @@ void rk3(const Batch& p, float* q, float* psi_last, int nsteps, double dt, const
   const float h = (float)dt;
   for (int n = 0; n < nsteps; ++n) {
     invert(p, q, psi, ws, st);
-    stage(p, q, psi, nullptr, q1, cp, 1.0f, h, true, st);
+    stage(p, q, psi, nullptr, q1, cp, 1.0f, h, true, ws, st);
     invert(p, q1, psi, ws, st);
-    stage(p, q1, psi, q, q2, cp, 0.25f, 0.25f * h, true, st);
+    stage(p, q1, psi, q, q2, cp, 0.25f, 0.25f * h, true, ws, st);
     invert(p, q2, psi, ws, st);
-    stage(p, q2, psi, q, q, cp, 2.0f / 3.0f, 2.0f / 3.0f * h, true, st);
+    stage(p, q2, psi, q, q, cp, 2.0f / 3.0f, 2.0f / 3.0f * h, true, ws, st);
   }
 }
 
@@ -698,7 +736,7 @@ int somaxhip_qg_rhs(somaxhip_qg_plan p, const float* q, float* dq, float* psi_ou
   hipStream_t st = reinterpret_cast<hipStream_t>(stream);
   float* psi = psi_out ? psi_out : wsf(ws, p->b.off_PSI);
   invert(p->b, q, psi, ws, st, apply_bc != 0);
-  stage(p->b, q, psi, nullptr, dq, uniform_params(prm), 0.0f, 1.0f, apply_bc != 0, st);
+  stage(p->b, q, psi, nullptr, dq, uniform_params(prm), 0.0f, 1.0f, apply_bc != 0, ws, st);
   SOMAX_HIP_CHECK(hipGetLastError());
   return SOMAXHIP_OK;
 }

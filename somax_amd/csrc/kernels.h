@@ -117,6 +117,9 @@ struct LayerArgs {
   int Nx, Ny, M;
   long plane;
   Tables t;
+  // mass correction (nullable): pm += alpha[p] * homsol[p] before the projection
+  const float* homsol;
+  const double* alpha;
 };
 void launch_to_layer(const LayerArgs& a, hipStream_t st);
 
@@ -175,5 +178,16 @@ void launch_qg_diag(const QGDiagArgs& a, hipStream_t st);
 
 // Fixed-order sum of `n` per-block pairs of every plane into out[p][2] * scale.
 void launch_pair_reduce(const double* partial, int n, int P, double scale, double* out, hipStream_t st);
+
+// Mass correction of the modal streamfunction (homogeneous solution method):
+// per plane the FP64 sum of (a + b) over the full array (b nullable) into
+// out[p][0] (fixed order, via launch_pair_reduce), then
+//   alpha[p] = -sum[p] / (Nx*Ny) / hmean[p]   (0 where hmean[p] == 0)
+// and the wall value of the layer streamfunction psi_wall[first + l] =
+// sum_p cm2l[l][p] * alpha[first + p] for the WENO corner streamfunction.
+void launch_plane_sums(const float* a, const float* b, int Nx, int Ny, int P, long plane, double* partial,
+                       double* out, hipStream_t st);
+void launch_mass_alpha(const double* sums, const double* hmean, Tables t, int P, double inv_count, double* alpha,
+                       float* psi_wall, hipStream_t st);
 
 }  // namespace somax

@@ -495,6 +495,7 @@ __global__ __launch_bounds__(256) void to_layer_kernel(LayerArgs a) {
     if (p < nl) {
       const long i = (first + p) * a.plane + s;
       pm[p] = a.pm[i] + (a.d ? a.d[i] : 0.0f);
+      if (a.homsol) pm[p] += (float)a.alpha[first + p] * a.homsol[i];
     }
   }
 #pragma unroll
@@ -606,6 +607,55 @@ void launch_qg_diag(const QGDiagArgs& a, hipStream_t st) {
 
 void launch_pair_reduce(const double* partial, int n, int P, double scale, double* out, hipStream_t st) {
   hipLaunchKernelGGL(pair_reduce_kernel, dim3(P), dim3(256), 0, st, partial, n, scale, out);
+}
+
+}  // namespace somax
+
+namespace somax {
+
+// ---------------------------------------------------------------------------
+// mass correction (homogeneous solution method)
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void plane_sums_kernel(const float* a, const float* b, int Nx, long plane,
+                                                          double* partial) {
+  const int x = blockIdx.x * 256 + threadIdx.x;
+  const int y = blockIdx.y;
+  const int z = blockIdx.z;
+  double v = 0.0;
+  if (x < Nx) {
+    const long i = z * plane + (long)y * Nx + x;
+    v = (double)a[i] + (b ? (double)b[i] : 0.0);
+  }
+  const long blk = (long)z * gridDim.y * gridDim.x + (long)y * gridDim.x + blockIdx.x;
+  block_pair_sum(v, 0.0, partial + 2 * blk);
+}
+
+__global__ __launch_bounds__(256) void mass_alpha_kernel(const double* sums, const double* hmean, Tables t, int P,
+                                                          double inv_count, double* alpha, float* psi_wall) {
+  for (int z = threadIdx.x; z < P; z += 256) {
+    const double hm = hmean[z];
+    alpha[z] = hm != 0.0 ? -sums[2 * z] * inv_count / hm : 0.0;
+  }
+  __syncthreads();
+  for (int z = threadIdx.x; z < P; z += 256) {
+    const int m = t.member[z], l = t.layer[z];
+    const int first = t.mfirst[m], nl = t.mnl[m];
+    double w = 0.0;
+    for (int p = 0; p < nl; ++p) w += (double)t.cm2l[m * MAX_LAYERS * MAX_LAYERS + l * MAX_LAYERS + p] * alpha[first + p];
+    psi_wall[z] = (float)w;
+  }
+}
+
+void launch_plane_sums(const float* a, const float* b, int Nx, int Ny, int P, long plane, double* partial,
+                       double* out, hipStream_t st) {
+  const int bx = (Nx + 255) / 256;
+  hipLaunchKernelGGL(plane_sums_kernel, dim3(bx, Ny, P), dim3(256), 0, st, a, b, Nx, plane, partial);
+  launch_pair_reduce(partial, bx * Ny, P, 1.0, out, st);
+}
+
+void launch_mass_alpha(const double* sums, const double* hmean, Tables t, int P, double inv_count, double* alpha,
+                       float* psi_wall, hipStream_t st) {
+  hipLaunchKernelGGL(mass_alpha_kernel, dim3(1), dim3(256), 0, st, sums, hmean, t, P, inv_count, alpha, psi_wall);
 }
 
 }  // namespace somax

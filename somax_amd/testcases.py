@@ -63,7 +63,7 @@ def _sine(n: int) -> np.ndarray:
 
 
 def random_streamfunction(nx: int, ny: int, Lx: float, Ly: float, nl: int, seed: int, urms: float = 0.05,
-                          slope: float = -3.0) -> np.ndarray:
+                          slope: float = -3.0, zero_mean: bool = False) -> np.ndarray:
     """Seeded Gaussian random psi, ``(nl, ny+2, nx+2)`` FP64, zero ghost ring.
 
     Built in the interior DST-I basis (so psi = 0 on the ghost ring) with an
@@ -71,6 +71,11 @@ def random_streamfunction(nx: int, ny: int, Lx: float, Ly: float, nl: int, seed:
     mode ``(ky, kx)`` has standard deviation ``k**((slope - 3)/2)``.  Each
     layer is scaled to an rms speed ``urms`` of the forward-difference
     geostrophic velocity ``u = -dpsi/dy``, ``v = dpsi/dx`` on the interior.
+
+    ``zero_mean`` removes each layer's mean over the array with the gravest
+    basin mode ``sin(pi x / Lx) sin(pi y / Ly)`` (also 0 on the ghost ring): a
+    state the mass-conservation correction leaves unchanged, as a spun-up
+    basin is, instead of one it turns into a grid-scale wall current.
     """
     rng = np.random.default_rng(seed)
     dx, dy = Lx / nx, Ly / ny
@@ -84,6 +89,10 @@ def random_streamfunction(nx: int, ny: int, Lx: float, Ly: float, nl: int, seed:
         c = rng.standard_normal((ny, nx)) * amp
         p = np.zeros((ny + 2, nx + 2))
         p[1:-1, 1:-1] = Sy @ c @ Sx
+        if zero_mean:
+            g = np.zeros((ny + 2, nx + 2))
+            g[1:-1, 1:-1] = np.outer(Sy[:, 0], Sx[0, :])
+            p -= (p.mean() / g.mean()) * g
         u = -(p[2:, 1:-1] - p[1:-1, 1:-1]) / dy
         v = (p[1:-1, 2:] - p[1:-1, 1:-1]) / dx
         s = np.sqrt(np.mean(u * u + v * v))
@@ -105,10 +114,10 @@ def pv_from_streamfunction(psi: np.ndarray, dx: float, dy: float, lambdas, Cl2m,
     return q
 
 
-def random_pv(model, seed: int, urms: float = 0.05) -> np.ndarray:
+def random_pv(model, seed: int, urms: float = 0.05, zero_mean: bool = False) -> np.ndarray:
     """FP64 PV field whose inversion is a seeded k^-3 random streamfunction."""
     g = model.grid
-    psi = random_streamfunction(g.nx, g.ny, g.Lx, g.Ly, model._nl, seed, urms)
+    psi = random_streamfunction(g.nx, g.ny, g.Lx, g.Ly, model._nl, seed, urms, zero_mean=zero_mean)
     return pv_from_streamfunction(psi, g.dx, g.dy, model._lambdas, model._Cl2m, model._Cm2l)
 
 

@@ -45,12 +45,12 @@ def _pair(n, nl=3, precision="fp32", advection="arakawa", ny=None, **kw):
     return model, setup
 
 
-def _state(model, seed, ghost_noise=False):
+def _state(model, seed, ghost_noise=False, zero_mean=False):
     import torch
 
     from somax_amd import testcases as tc
 
-    q = tc.random_pv(model, seed).astype(np.float32)
+    q = tc.random_pv(model, seed, zero_mean=zero_mean).astype(np.float32)
     if ghost_noise:
         rng = np.random.default_rng(seed + 1000)
         g = rng.standard_normal(q.shape).astype(np.float32) * np.float32(np.abs(q).max())
@@ -326,3 +326,48 @@ def test_barotropic_diagnose(gpu):
     assert d.psi.shape == q.shape and d.kinetic_energy.ndim == 0
     assert rel(d.kinetic_energy.cpu().numpy(), ref["kinetic_energy"][0]) < 1e-5
     assert rel(d.enstrophy.cpu().numpy(), ref["enstrophy"][0]) < 1e-6
+
+
+@pytest.mark.parametrize("precision,tol", [("fp32", 3e-6), ("mixed", 3e-7)])
+def test_mass_corrected_inversion(gpu, precision, tol):
+    """Homogeneous-solution mass correction (mqg_demo_retangular.ipynb cells
+    18-19): psi matches the oracle to the inversion bar, and every modal
+    streamfunction has zero mean over the array (FP64 constraint)."""
+    _log(f"[mass] inversion n=128 precision={precision}")
+    model, setup = _pair(128, precision=precision, mass_correction=True)
+    q, qd = _state(model, seed=41)
+    psi = model._invert_pv(qd).cpu().numpy().astype(np.float64)
+    ref = oqg.invert_pv(setup, q.astype(np.float64))
+    r = rel(psi, ref)
+    modal = np.einsum("ml,lyx->myx", setup.Cl2m, psi)
+    means = np.abs(modal.mean(axis=(-2, -1))) / np.abs(modal).mean(axis=(-2, -1))
+    _log(f"  rel-L2 psi = {r:.3e}, |modal mean| / mean|modal| = {means}")
+    assert r < tol
+    assert means.max() < 1e-6
+
+
+@pytest.mark.parametrize("precision", ["fp32", "mixed"])
+def test_mass_corrected_weno_step(gpu, precision):
+    """WENO-Z with mass correction: the corner streamfunction takes the wall
+    value Cm2l @ alpha; one step and 10 steps against the oracle.
+
+    The initial streamfunction has zero layer means (as a spun-up basin has).
+    Started from one with large means, the correction puts a 16 m/s current
+    into a wall layer one cell wide, and the FP64 oracle itself blows up
+    within 10 steps at dt = 1200 s."""
+    import torch
+
+    _log(f"[mass] WENO step n=128 precision={precision}")
+    model, setup = _pair(128, precision=precision, advection="weno", mass_correction=True)
+    q, qd = _state(model, seed=43, zero_mean=True)
+    dt = 1200.0
+    q1 = qd.clone()
+    model._step_inplace(q1, dt, 1)
+    ref1 = oqg.ssprk3_step(setup, q.astype(np.float64), dt)
+    q10 = qd.clone()
+    model._step_inplace(q10, dt, 10)
+    torch.cuda.synchronize()
+    ref10 = oqg.run(setup, q.astype(np.float64), dt, 10)
+    r1, r10 = rel(q1.cpu().numpy(), ref1), rel(q10.cpu().numpy(), ref10)
+    _log(f"  1 step q {r1:.3e}; 10 steps q {r10:.3e}")
+    assert r1 < 1e-6 and r10 < 1e-5
