@@ -7,6 +7,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <new>
@@ -38,6 +39,15 @@ struct SineSet {
   uint16_t* b_int = nullptr;
   float* lam = nullptr;  // DST-I eigenvalues (2cos(pi(k+1)/(N+1)) - 2)/d^2
   double lam_min = 0.0, lam_max = 0.0;  // smallest / largest |eigenvalue|
+  // folded half-length operators (even N, h = N/2): variants v = 0, 1 forward
+  // S_p[i][k'] = S[2i+p][fold_phys(k')] and v = 2, 3 inverse S_p^T, each
+  // frows x fld (FP32) or 3 bf16 pieces; eigenvalues lam_fold[p*h + i] = lam[2i+p]
+  int h = 0;
+  long fld = 0, frows = 0;
+  float* f32_fold = nullptr;
+  uint16_t* b_fold = nullptr;
+  float* lam_fold = nullptr;
+  std::vector<float> lam_host;
 };
 
 struct DeviceBlock {
@@ -108,7 +118,49 @@ static hipError_t build_sines(DeviceBlock& mem, SineSet& s, int N, double d) {
   if ((e = mem.upload(&s.f32_int, fi)) != hipSuccess) return e;
   if ((e = mem.upload(&s.b_full, bf)) != hipSuccess) return e;
   if ((e = mem.upload(&s.b_int, bi)) != hipSuccess) return e;
+  s.lam_host = lam;
   return mem.upload(&s.lam, lam);
+}
+
+// Folded operators of an even-length axis (S[N-1-n][a] = (-1)^a S[n][a]).
+static hipError_t build_fold(DeviceBlock& mem, SineSet& s) {
+  const int N = s.N, h = N / 2;
+  s.h = h;
+  s.fld = round_up(h, 32);
+  s.frows = round_up(h, 128);
+  const size_t var = (size_t)s.frows * s.fld;
+  std::vector<float> ff(4 * var, 0.0f), lf(2 * (size_t)h);
+  std::vector<uint16_t> bf(4 * 3 * var, 0);
+  const double c = std::sqrt(2.0 / (N + 1));
+  auto S = [&](int a, int n) { return c * std::sin(M_PI * (double)(a + 1) * (double)(n + 1) / (double)(N + 1)); };
+  for (int p = 0; p < 2; ++p)
+    for (int i = 0; i < h; ++i)
+      for (int k = 0; k < h; ++k) {
+        const double v = S(2 * i + p, fold_phys(k, h));
+        const size_t fwd = (size_t)p * var + (size_t)i * s.fld + k;       // rows = outputs i
+        const size_t inv = (size_t)(2 + p) * var + (size_t)k * s.fld + i;  // rows = outputs k'
+        ff[fwd] = (float)v;
+        ff[inv] = (float)v;
+        split3(v, &bf[(size_t)p * 3 * var + (size_t)i * s.fld + k], (long)var);
+        split3(v, &bf[(size_t)(2 + p) * 3 * var + (size_t)k * s.fld + i], (long)var);
+      }
+  for (int p = 0; p < 2; ++p)
+    for (int i = 0; i < h; ++i) lf[(size_t)p * h + i] = s.lam_host[2 * i + p];
+  hipError_t e;
+  if ((e = mem.upload(&s.f32_fold, ff)) != hipSuccess) return e;
+  if ((e = mem.upload(&s.b_fold, bf)) != hipSuccess) return e;
+  return mem.upload(&s.lam_fold, lf);
+}
+
+// Folded transforms halve the GEMM work; below h = 128 the four quarter
+// blocks cost more tiles than the plain transform.  SOMAXHIP_FOLD=0 / 1
+// turns them off / on for every even grid.
+static bool use_fold(int Nx, int Ny) {
+  if (Nx % 2 || Ny % 2) return false;
+  const char* env = std::getenv("SOMAXHIP_FOLD");
+  if (env && env[0] == '0') return false;
+  if (env && env[0] == '1') return Nx >= 4 && Ny >= 4;
+  return Nx / 2 >= 128 && Ny / 2 >= 128;
 }
 
 // The members of one grid, stacked as planes, with their device tables and
@@ -118,6 +170,8 @@ struct Batch {
   double dx = 0.0, dy = 0.0;
   int M = 0, P = 0;
   int advection = 0, precision = 0;
+  int fold = 0;           // folded transforms (4 parity blocks of (Ny/2) x (Nx/2) per plane)
+  long fplane = 0;        // folded: (Ny/2) * (Nx/2)
   DeviceBlock mem;
   SineSet sx, sy;
   Tables t{};
@@ -172,6 +226,12 @@ static hipError_t batch_build(Batch& b, const std::vector<const somaxhip_qg_desc
   hipError_t e;
   if ((e = build_sines(b.mem, b.sx, b.Nx, b.dx)) != hipSuccess) return e;
   if ((e = build_sines(b.mem, b.sy, b.Ny, b.dy)) != hipSuccess) return e;
+  b.fold = use_fold(b.Nx, b.Ny) ? 1 : 0;
+  if (b.fold) {
+    if ((e = build_fold(b.mem, b.sx)) != hipSuccess) return e;
+    if ((e = build_fold(b.mem, b.sy)) != hipSuccess) return e;
+    b.fplane = (long)b.sx.h * b.sy.h;
+  }
 
   // Poisson-like modes (small lambda: the Laplacian's conditioning is not
   // capped) need the 3-piece inverse transform in the mixed solve.
@@ -244,18 +304,21 @@ static hipError_t batch_build(Batch& b, const std::vector<const somaxhip_qg_desc
 
   // workspace: two GEMM operand buffers (3 bf16 pieces or 1.5 FP32 planes
   // per plane), FP32 modal planes, stage states and psi.
+  // (folded: four blocks per plane, each at most (maxN/2) x pitch)
   const long maxN = b.Nx > b.Ny ? b.Nx : b.Ny;
-  b.wpitch = round_up(maxN, 64);
-  b.wpiece = maxN * b.wpitch;
+  const long rowsN = b.fold ? maxN / 2 : maxN;
+  b.wpitch = round_up(rowsN, 64);
+  b.wpiece = rowsN * b.wpitch;
   b.splane = (long)b.Ny * b.Nx;
+  const int opl = b.fold ? 4 * P : P;  // GEMM operand planes
   size_t off = 0;
   auto take = [&](size_t bytes) {
     const size_t o = off;
     off += round_up((long)bytes, 256);
     return o;
   };
-  b.off_OP1 = take((size_t)P * 3 * b.wpiece * sizeof(uint16_t));
-  b.off_OP2 = take((size_t)P * 3 * b.wpiece * sizeof(uint16_t));
+  b.off_OP1 = take((size_t)opl * 3 * b.wpiece * sizeof(uint16_t));
+  b.off_OP2 = take((size_t)opl * 3 * b.wpiece * sizeof(uint16_t));
   b.off_QM = take((size_t)P * b.splane * sizeof(float));
   b.off_P0 = take((size_t)P * b.splane * sizeof(float));
   if (b.precision == SOMAXHIP_PREC_MIXED) b.off_D = take((size_t)P * b.splane * sizeof(float));
@@ -428,14 +491,131 @@ void gemm(const GemmArgs& a, int batch, hipStream_t st) {
 constexpr int NP_F32 = 0;
 inline int pieces_for(int np) { return np == 1 ? 1 : (np == 3 ? 2 : 3); }
 
+// Folded solve: the four parity blocks of every plane as 4P half-size planes
+// (plane b * P + m).  In: OP1 holds the fold sums (interior input: the ghost
+// pair, last in the folded order, is zero and skipped).  Out: `out` receives
+// the parity components of the solution, 4P planes of hy x hx.
+void dst_solve_folded(const Batch& p, bool interior_input, int np_fwd, int np_inv_lo, int np_inv_hi, float* out,
+                      void* ws, hipStream_t st, bool acc64_fwd, bool acc64_inv) {
+  const int hx = p.sx.h, hy = p.sy.h, P = p.P;
+  const bool f32 = np_fwd == NP_F32;
+  char* OP1 = static_cast<char*>(ws) + p.off_OP1;
+  char* OP2 = static_cast<char*>(ws) + p.off_OP2;
+  const long pb16 = 3 * p.wpiece;
+  const long pb32 = 3 * p.wpiece / 2;
+  const int R0 = interior_input ? hy - 1 : hy;  // F1 data rows = F2 k extent
+  const int K0 = interior_input ? hx - 1 : hx;  // F1 k extent
+
+  // variant v of an axis: 0/1 forward parity, 2/3 inverse parity
+  auto base = [&](GemmArgs& g, char* X, int xrows, int K, const SineSet& s, int inverse, int s_par, int np_lo,
+                  int np_hi) {
+    const long var = s.frows * s.fld;
+    g.ldx = p.wpitch;
+    g.xrows = xrows;
+    g.K = K;
+    if (f32) {
+      g.Xf = reinterpret_cast<const float*>(X);
+      g.x_batch = pb32;
+      g.Sf = s.f32_fold + (inverse ? 2 * var : 0);
+      g.s_var = var;
+    } else {
+      g.Xb = reinterpret_cast<const uint16_t*>(X);
+      g.x_piece = p.wpiece;
+      g.x_batch = pb16;
+      g.Sb = s.b_fold + (inverse ? 2 * 3 * var : 0);
+      g.s_piece = var;
+      g.s_var = 3 * var;
+    }
+    g.lds = s.fld;
+    g.hp = p.t.hp;
+    g.np_lo = np_lo;
+    g.np_hi = np_hi;
+    g.fold_P = P;
+    g.s_par = s_par;
+  };
+  auto out_op = [&](GemmArgs& g, char* O, int op_lo, int op_hi) {
+    g.ldo = p.wpitch;
+    if (f32) {
+      g.Of = reinterpret_cast<float*>(O);
+      g.o_batch = pb32;
+    } else {
+      g.Ob = reinterpret_cast<uint16_t*>(O);
+      g.o_piece = p.wpiece;
+      g.o_batch = pb16;
+      g.op_lo = op_lo;
+      g.op_hi = op_hi;
+    }
+  };
+  const int pf = f32 ? 0 : pieces_for(np_fwd);
+  const int pil = f32 ? 0 : pieces_for(np_inv_lo);
+  const int pih = f32 ? 0 : pieces_for(np_inv_hi);
+
+  // F1: T[ix][y'] = sum_x' Sx_q[ix][x'] X[y'][x']
+  GemmArgs g{};
+  base(g, OP1, R0, K0, p.sx, 0, 0, np_fwd, np_fwd);
+  g.trans = 1;
+  out_op(g, OP2, pf, pf);
+  g.acc64 = acc64_fwd;
+  g.RA = hx;
+  g.RB = R0;
+  g.RBpad = (int)round_up(R0, 32);
+  gemm(g, 4 * P, st);
+
+  // F2: W[ix][iy] = sum_y' T[ix][y'] Sy_p[iy][y'] / (lamx_q[ix] + lamy_p[iy] - lambda)
+  GemmArgs h{};
+  base(h, OP2, hx, R0, p.sy, 0, 1, np_fwd, np_fwd);
+  h.trans = 0;
+  out_op(h, OP1, pil, pih);
+  h.acc64 = acc64_fwd;
+  h.RA = hx;
+  h.RB = hy;
+  h.RBpad = (int)round_up(hy, 32);
+  h.lamA = p.sx.lam_fold;
+  h.lamB = p.sy.lam_fold;
+  h.lam_var = hx;
+  h.lamB_var = hy;
+  h.lam = p.t.lam;
+  gemm(h, 4 * P, st);
+
+  // I1: V[y'][ix] = sum_iy Sy_p^T[y'][iy] W[ix][iy]
+  GemmArgs u{};
+  base(u, OP1, hx, hy, p.sy, 1, 1, np_inv_lo, np_inv_hi);
+  u.trans = 1;
+  out_op(u, OP2, pil, pih);
+  u.acc64 = acc64_inv;
+  u.RA = hy;
+  u.RB = hx;
+  u.RBpad = (int)round_up(hx, 32);
+  gemm(u, 4 * P, st);
+
+  // I2: out[y'][x'] = sum_ix V[y'][ix] Sx_q^T[x'][ix]
+  GemmArgs w{};
+  base(w, OP2, hy, hx, p.sx, 1, 0, np_inv_lo, np_inv_hi);
+  w.trans = 0;
+  w.Of = out;
+  w.Ob = nullptr;
+  w.ldo = hx;
+  w.o_batch = p.fplane;
+  w.op_lo = w.op_hi = 0;
+  w.acc64 = acc64_inv;
+  w.RA = hy;
+  w.RB = hx;
+  w.RBpad = hx;
+  gemm(w, 4 * P, st);
+}
+
 // One DST-I solve per plane of (lap_h - lambda_z) y_z = x_z, from the GEMM
 // operand already in OP1 (interior rows/cols when interior_input: the ghost
 // ring of x is zero, so the forward transform contracts over interior points
-// only) to the FP32 modal planes `out` (state layout).  np_fwd: products of
-// the forward transforms; np_inv_lo / np_inv_hi: products of the inverse
-// transforms of Poisson-like / other planes.
+// only) to the FP32 modal planes `out` (state layout; folded batches: parity
+// components).  np_fwd: products of the forward transforms; np_inv_lo /
+// np_inv_hi: products of the inverse transforms of Poisson-like / other planes.
 void dst_solve(const Batch& p, bool interior_input, int np_fwd, int np_inv_lo, int np_inv_hi, float* out,
                void* ws, hipStream_t st, bool acc64_fwd = true, bool acc64_inv = true) {
+  if (p.fold) {
+    dst_solve_folded(p, interior_input, np_fwd, np_inv_lo, np_inv_hi, out, ws, st, acc64_fwd, acc64_inv);
+    return;
+  }
   const int Nx = p.Nx, Ny = p.Ny, P = p.P;
   const bool f32 = np_fwd == NP_F32;
   char* OP1 = static_cast<char*>(ws) + p.off_OP1;
@@ -554,6 +734,11 @@ void invert(const Batch& p, const float* q, float* psi, void* ws, hipStream_t st
   ma.op_piece = p.wpiece;
   ma.op_batch = mixed ? 3 * p.wpiece : 3 * p.wpiece / 2;
   ma.op_kpad = (int)round_up(apply_bc ? p.Nx - 2 : p.Nx, 32);
+  if (p.fold) {
+    ma.fold = 1;
+    ma.P = p.P;
+    ma.op_kpad = (int)round_up(apply_bc ? p.sx.h - 1 : p.sx.h, 32);
+  }
   {
     ProfScope ps("to_modal", 0.0, 8.0 * p.P * (double)p.splane, st);
     launch_to_modal(ma, st);
@@ -566,6 +751,8 @@ void invert(const Batch& p, const float* q, float* psi, void* ws, hipStream_t st
   la.M = p.M;
   la.plane = p.splane;
   la.t = p.t;
+  la.fold = p.fold;
+  la.P = p.P;
   if (!mixed) {
     dst_solve(p, apply_bc, NP_F32, NP_F32, NP_F32, P0, ws, st);
   } else {
@@ -580,7 +767,8 @@ void invert(const Batch& p, const float* q, float* psi, void* ws, hipStream_t st
     r.op_ld = p.wpitch;
     r.op_piece = p.wpiece;
     r.op_batch = 3 * p.wpiece;
-    r.op_kpad = (int)round_up(p.Nx, 32);
+    r.op_kpad = (int)round_up(p.fold ? p.sx.h : p.Nx, 32);
+    r.fold = p.fold;
     r.plane = p.splane;
     r.Nx = p.Nx;
     r.Ny = p.Ny;
@@ -601,9 +789,12 @@ void invert(const Batch& p, const float* q, float* psi, void* ws, hipStream_t st
     double* alpha = reinterpret_cast<double*>(static_cast<char*>(ws) + p.off_ALPHA);
     {
       ProfScope ps("mass_constraint", 0.0, (mixed ? 8.0 : 4.0) * p.P * (double)p.splane, st);
-      launch_plane_sums(P0, la.d, p.Nx, p.Ny, p.P, p.splane,
+      // folded: the array sum is 4x the sum of the even-even block (planes 0..P-1)
+      launch_plane_sums(P0, la.d, p.fold ? p.sx.h : p.Nx, p.fold ? p.sy.h : p.Ny, p.P,
+                        p.fold ? p.fplane : p.splane,
                         reinterpret_cast<double*>(static_cast<char*>(ws) + p.off_DIAG), sums, st);
-      launch_mass_alpha(sums, p.hmean, p.t, p.P, 1.0 / (double)p.splane, alpha, wsf(ws, p.off_PWALL), st);
+      launch_mass_alpha(sums, p.hmean, p.t, p.P, (p.fold ? 4.0 : 1.0) / (double)p.splane, alpha,
+                        wsf(ws, p.off_PWALL), st);
     }
     la.homsol = p.homsol;
     la.alpha = alpha;

@@ -50,8 +50,7 @@ __device__ inline void zero_acc(floatx16 (&acc)[2][2], double (&sum)[2][2][16]) 
 // Linear block -> (plane, A-tile, B-tile).  Blocks are dealt to the 8 XCDs
 // round-robin; this remap hands each XCD a contiguous run of tiles (same
 // plane, neighbouring A rows, all B tiles) so they share its L2.
-__device__ inline void tile_of_block(int GA, int GB, int& z, int& ta, int& tb) {
-  const int T = gridDim.x;
+__device__ inline void tile_of_block(int GA, int GB, int T, int& z, int& ta, int& tb) {
   const int b = blockIdx.x;
   const int xcd = b & 7, slot = b >> 3;
   const int per = T >> 3, extra = T & 7;
@@ -69,14 +68,21 @@ __device__ inline void tile_of_block(int GA, int GB, int& z, int& ta, int& tb) {
 // `stage` needs 64 * EPI_PITCH floats of LDS; the caller's pipeline is done.
 constexpr int EPI_PITCH = 132;
 
+// Eigenvalue rows of the spectral epilogue for one batch plane (null: none).
+struct Spectral {
+  const float* lamA;
+  const float* lamB;
+  float lamz;
+};
+
 template <bool SUM64>
 __device__ inline void store_tile(const GemmArgs& a, long z, int a0, int b0, const floatx16 (&acc)[2][2],
-                                  const double (&sum)[2][2][16], int opieces, float* stage) {
+                                  const double (&sum)[2][2][16], int opieces, float* stage, const Spectral& sp) {
   const int tid = threadIdx.x;
   const int lane = tid & 63;
   const int wave = tid >> 6;
   const int wm = wave >> 1, wn = wave & 1;
-  const float lamz = a.lamA ? a.lam[z] : 0.0f;
+  const float lamz = sp.lamz;
 #pragma unroll
   for (int h = 0; h < 2; ++h) {
     __syncthreads();  // the LDS is free (pipeline drained / previous half stored)
@@ -101,7 +107,7 @@ __device__ inline void store_tile(const GemmArgs& a, long z, int a0, int b0, con
         const int ro = r0 + lr, co = b0 + lc;
         if (ro >= a.RA || co >= a.RBpad) continue;
         float v = stage[lr * EPI_PITCH + lc];
-        if (a.lamA) v = v / (a.lamA[ro] + a.lamB[co < a.RB ? co : a.RB - 1] - lamz);
+        if (sp.lamA) v = v / (sp.lamA[ro] + sp.lamB[co < a.RB ? co : a.RB - 1] - lamz);
         a.Of[z * a.o_batch + (long)ro * a.ldo + co] = co < a.RB ? v : 0.0f;
       }
     } else {
@@ -118,12 +124,12 @@ __device__ inline void store_tile(const GemmArgs& a, long z, int a0, int b0, con
         const floatx4 vb = *reinterpret_cast<const floatx4*>(stage + lr * EPI_PITCH + lc + 4);
         float v[8] = {va[0], va[1], va[2], va[3], vb[0], vb[1], vb[2], vb[3]};
         // branch free per element (columns past RB are zero padding)
-        if (a.lamA) {
-          const float la = a.lamA[ro] - lamz;
+        if (sp.lamA) {
+          const float la = sp.lamA[ro] - lamz;
 #pragma unroll
           for (int e = 0; e < 8; ++e) {
             const int c = co + e < a.RB ? co + e : a.RB - 1;
-            const float q = v[e] / (la + a.lamB[c]);
+            const float q = v[e] / (la + sp.lamB[c]);
             v[e] = co + e < a.RB ? q : 0.0f;
           }
         } else {
@@ -154,6 +160,171 @@ __device__ inline void store_tile(const GemmArgs& a, long z, int a0, int b0, con
         if (opieces > 1) *reinterpret_cast<u32x4*>(o + a.o_piece) = p1;
         if (opieces > 2) *reinterpret_cast<u32x4*>(o + 2 * a.o_piece) = p2;
       }
+    }
+  }
+}
+
+// Where batch plane z reads its tables and sine operand (folded or plain).
+struct PlaneSel {
+  int m;        // table plane (mode)
+  long s_off;   // element offset of the sine operand's parity variant
+  Spectral sp;  // eigenvalue rows
+};
+
+__device__ inline PlaneSel select_plane(const GemmArgs& a, long z) {
+  PlaneSel s;
+  int blk = 0;
+  s.m = (int)z;
+  if (a.fold_P) {
+    s.m = (int)(z % a.fold_P);
+    blk = (int)(z / a.fold_P);
+  }
+  const int px = blk & 1, py = blk >> 1;
+  s.s_off = a.fold_P ? (long)(a.s_par ? py : px) * a.s_var : 0;
+  s.sp.lamA = a.lamA ? a.lamA + px * a.lam_var : nullptr;
+  s.sp.lamB = a.lamB ? a.lamB + py * a.lamB_var : nullptr;
+  s.sp.lamz = a.lamA ? a.lam[s.m] : 0.0f;
+  return s;
+}
+
+// One output element of a launch: spectral division, then FP32 or bf16 pieces.
+__device__ inline void store_one(const GemmArgs& a, long z, int r, int c, float v, int opieces, const Spectral& sp) {
+  if (sp.lamA) v = v / (sp.lamA[r] + sp.lamB[c] - sp.lamz);
+  const long o = z * a.o_batch + (long)r * a.ldo + c;
+  if (opieces == 0) {
+    a.Of[o] = v;
+    return;
+  }
+  const uint16_t b0 = bf16_rne(v);
+  a.Ob[o] = b0;
+  if (opieces > 1) {
+    const float r1 = v - bf16_bits_to_f32(b0);
+    const uint16_t b1 = bf16_rne(r1);
+    a.Ob[o + a.o_piece] = b1;
+    if (opieces > 2) a.Ob[o + 2 * a.o_piece] = bf16_rne(r1 - bf16_bits_to_f32(b1));
+  }
+}
+
+// Fringe workgroup `part` of `fpw` for batch plane z: the dot products of
+// rows [RAm, RA) x columns [0, RB) and rows [0, RAm) x columns [RBm, RB),
+// eight lanes per dot along k with FP64 accumulation of the reconstructed
+// operands (sum of the NPC bf16 pieces, or FP32), then its share of the zero
+// padding [RB, RBpad) of the rows that no tile covers.  The loads of KB
+// 64-element k-chunks are issued before any is used, so a pass costs one
+// memory latency rather than one per chunk.
+template <bool F32, int NPC>
+__device__ void gemm_fringe(const GemmArgs& a, long z, int part, int fpw, const PlaneSel& ps, int opieces) {
+  constexpr int KB = 9;
+  const int RAm = gemm_tile_cut(a.RA), RBm = gemm_tile_cut(a.RB);
+  const int fa = a.RA - RAm, fb = a.RB - RBm;
+  const long nrow = (long)fa * a.RB;
+  const long dots = nrow + (long)RAm * fb;
+  const int tid = threadIdx.x, g = tid >> 3, l8 = tid & 7;
+  const bool x_in_a = !a.trans;
+  const char* X = F32 ? reinterpret_cast<const char*>(a.Xf + z * a.x_batch)
+                      : reinterpret_cast<const char*>(a.Xb + z * a.x_batch);
+  const char* S = F32 ? reinterpret_cast<const char*>(a.Sf + ps.s_off) : reinterpret_cast<const char*>(a.Sb + ps.s_off);
+  const int esz = F32 ? 4 : 2;
+  auto xrow = [&](int r) { return X + (long)(r < a.xrows ? r : a.xrows - 1) * a.ldx * esz; };
+  auto srow = [&](int r) { return S + (long)r * a.lds * esz; };
+  const long pa = x_in_a ? a.x_piece : a.s_piece, pb = x_in_a ? a.s_piece : a.x_piece;  // bf16 piece strides
+  const long per = (dots + fpw - 1) / fpw;
+  const long d0 = (long)part * per;
+  const long d1 = d0 + per < dots ? d0 + per : dots;
+  for (long base = d0; base < d1; base += 32) {
+    const long d = base + g;
+    const bool live = d < d1;
+    int r = 0, c = 0;
+    if (live) {
+      if (d < nrow) {
+        r = RAm + (int)(d / a.RB);
+        c = (int)(d % a.RB);
+      } else {
+        const long e = d - nrow;
+        r = (int)(e / fb);
+        c = RBm + (int)(e % fb);
+      }
+    }
+    double acc = 0.0;
+    if (live) {
+      const char* ra = x_in_a ? xrow(r) : srow(r);
+      const char* rb = x_in_a ? srow(c) : xrow(c);
+      for (int kb = 0; kb < a.K; kb += 64 * KB) {
+        if constexpr (F32) {
+          floatx4 va[KB][2], vb[KB][2];
+#pragma unroll
+          for (int j = 0; j < KB; ++j) {
+            const int k0 = kb + 64 * j + l8 * 8;
+            const float* fa = reinterpret_cast<const float*>(ra) + k0;
+            const float* fb = reinterpret_cast<const float*>(rb) + k0;
+#pragma unroll
+            for (int w = 0; w < 2; ++w) {
+              va[j][w] = k0 < a.K ? *reinterpret_cast<const floatx4*>(fa + 4 * w) : floatx4{0.0f, 0.0f, 0.0f, 0.0f};
+              vb[j][w] = k0 < a.K ? *reinterpret_cast<const floatx4*>(fb + 4 * w) : floatx4{0.0f, 0.0f, 0.0f, 0.0f};
+            }
+          }
+#pragma unroll
+          for (int j = 0; j < KB; ++j)
+#pragma unroll
+            for (int w = 0; w < 2; ++w)
+#pragma unroll
+              for (int e = 0; e < 4; ++e) acc += (double)va[j][w][e] * (double)vb[j][w][e];
+        } else {
+          u32x4 va[KB][NPC], vb[KB][NPC];
+#pragma unroll
+          for (int j = 0; j < KB; ++j) {
+            const int k0 = kb + 64 * j + l8 * 8;
+            const uint16_t* ba = reinterpret_cast<const uint16_t*>(ra) + k0;
+            const uint16_t* bb = reinterpret_cast<const uint16_t*>(rb) + k0;
+#pragma unroll
+            for (int w = 0; w < NPC; ++w) {
+              va[j][w] = k0 < a.K ? *reinterpret_cast<const u32x4*>(ba + w * pa) : u32x4{0u, 0u, 0u, 0u};
+              vb[j][w] = k0 < a.K ? *reinterpret_cast<const u32x4*>(bb + w * pb) : u32x4{0u, 0u, 0u, 0u};
+            }
+          }
+#pragma unroll
+          for (int j = 0; j < KB; ++j) {
+            double xa[8], xb[8];
+#pragma unroll
+            for (int e = 0; e < 8; ++e) xa[e] = xb[e] = 0.0;
+#pragma unroll
+            for (int w = 0; w < NPC; ++w)
+#pragma unroll
+              for (int e = 0; e < 4; ++e) {
+                const uint32_t ua = va[j][w][e], ub = vb[j][w][e];
+                xa[2 * e] += (double)__builtin_bit_cast(float, ua << 16);
+                xa[2 * e + 1] += (double)__builtin_bit_cast(float, ua & 0xffff0000u);
+                xb[2 * e] += (double)__builtin_bit_cast(float, ub << 16);
+                xb[2 * e + 1] += (double)__builtin_bit_cast(float, ub & 0xffff0000u);
+              }
+#pragma unroll
+            for (int e = 0; e < 8; ++e) acc += xa[e] * xb[e];
+          }
+        }
+      }
+    }
+    acc += __shfl_xor(acc, 1);
+    acc += __shfl_xor(acc, 2);
+    acc += __shfl_xor(acc, 4);
+    if (live && l8 == 0) store_one(a, z, r, c, (float)acc, opieces, ps.sp);
+  }
+  // zero padding: rows [RAm, RA), and every row when the columns have a fringe
+  const int padw = a.RBpad - a.RB;
+  const int nzr = fb > 0 ? a.RA : fa;
+  const long nz = (long)nzr * padw;
+  if (nz == 0) return;
+  const long zper = (nz + fpw - 1) / fpw;
+  const long z0 = (long)part * zper;
+  const long z1 = z0 + zper < nz ? z0 + zper : nz;
+  for (long e = z0 + tid; e < z1; e += 256) {
+    const int rr = (int)(e / padw);
+    const int r = fb > 0 ? rr : RAm + rr;
+    const int c = a.RB + (int)(e % padw);
+    const long o = z * a.o_batch + (long)r * a.ldo + c;
+    if (opieces == 0) {
+      a.Of[o] = 0.0f;
+    } else {
+      for (int p = 0; p < opieces; ++p) a.Ob[o + p * a.o_piece] = 0;
     }
   }
 }
@@ -206,8 +377,8 @@ __device__ inline void wait_vmcnt() {
 }
 
 template <int NPROD, bool ACC64>
-__device__ inline void bf16_mainloop(const GemmArgs& a, uint16_t* lds, long z, int a0, int b0, bool x_in_a,
-                                     floatx16 (&acc)[2][2], double (&sum)[2][2][16]) {
+__device__ inline void bf16_mainloop(const GemmArgs& a, const uint16_t* S, uint16_t* lds, long z, int a0, int b0,
+                                     bool x_in_a, floatx16 (&acc)[2][2], double (&sum)[2][2][16]) {
   using C = Bf16Cfg<NPROD>;
   constexpr int NPC = C::NPC;
   const int tid = threadIdx.x;
@@ -215,7 +386,6 @@ __device__ inline void bf16_mainloop(const GemmArgs& a, uint16_t* lds, long z, i
   const int wave = tid >> 6;
   const int wm = wave >> 1, wn = wave & 1;
   const uint16_t* X = a.Xb + z * a.x_batch;
-  const uint16_t* S = a.Sb;
   const int x0 = x_in_a ? a0 : b0;
   const int s0 = x_in_a ? b0 : a0;
   const int xslot = x_in_a ? 0 : NPC;  // piece-tile index of X inside a stage
@@ -376,29 +546,45 @@ size_t dst_gemm_lds_bytes(int max_np) {
 template <int NPMAX>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) void dst_gemm_bf16_kernel(GemmArgs a) {
   extern __shared__ __attribute__((aligned(16))) uint16_t lds16[];
-  const int GA = (a.RA + BM - 1) / BM, GB = (a.RB + BN - 1) / BN;
-  int zi, ta, tb;
-  tile_of_block(GA, GB, zi, ta, tb);
+  const int GA = (gemm_tile_cut(a.RA) + BM - 1) / BM, GB = (gemm_tile_cut(a.RB) + BN - 1) / BN;
+  const int T = GA * GB * a.nplanes;
+  const bool fringe = (int)blockIdx.x >= T;
+  int zi, ta = 0, tb = 0, part = 0, fpw = 0;
+  if (fringe) {
+    fpw = gemm_fringe_wgs(a.RA, a.RB);
+    zi = ((int)blockIdx.x - T) / fpw;
+    part = ((int)blockIdx.x - T) - zi * fpw;
+  } else {
+    tile_of_block(GA, GB, T, zi, ta, tb);
+  }
   const long z = zi;
-  const bool lo = a.hp && a.hp[z];
+  const PlaneSel ps = select_plane(a, z);
+  const bool lo = a.hp && a.hp[ps.m];
   const int np = lo ? a.np_lo : a.np_hi;
   const int opieces = lo ? a.op_lo : a.op_hi;
+  if (fringe) {
+    if (np == 1) gemm_fringe<false, 1>(a, z, part, fpw, ps, opieces);
+    else if (NPMAX == 3 || np == 3) gemm_fringe<false, 2>(a, z, part, fpw, ps, opieces);
+    else gemm_fringe<false, 3>(a, z, part, fpw, ps, opieces);
+    return;
+  }
   const int a0 = ta * BM;
   const int b0 = tb * BN;
   const bool x_in_a = !a.trans;
+  const uint16_t* S = a.Sb + ps.s_off;
 
   floatx16 acc[2][2];
   double sum[2][2][16];
   zero_acc(acc, sum);
   // FP32 accumulation over K ~ 1e3 costs ~2e-7 relative, far below the
   // 3-product split error (~1e-5): only 6-product planes accumulate in FP64.
-  if (NPMAX == 1 || np == 1) bf16_mainloop<1, false>(a, lds16, z, a0, b0, x_in_a, acc, sum);
-  else if (NPMAX == 3 || np == 3) bf16_mainloop<3, false>(a, lds16, z, a0, b0, x_in_a, acc, sum);
-  else if (a.acc64) bf16_mainloop<6, true>(a, lds16, z, a0, b0, x_in_a, acc, sum);
-  else bf16_mainloop<6, false>(a, lds16, z, a0, b0, x_in_a, acc, sum);
+  if (NPMAX == 1 || np == 1) bf16_mainloop<1, false>(a, S, lds16, z, a0, b0, x_in_a, acc, sum);
+  else if (NPMAX == 3 || np == 3) bf16_mainloop<3, false>(a, S, lds16, z, a0, b0, x_in_a, acc, sum);
+  else if (a.acc64) bf16_mainloop<6, true>(a, S, lds16, z, a0, b0, x_in_a, acc, sum);
+  else bf16_mainloop<6, false>(a, S, lds16, z, a0, b0, x_in_a, acc, sum);
   float* stage = reinterpret_cast<float*>(lds16);
-  if (NPMAX == 6 && np == 6 && a.acc64) store_tile<true>(a, z, a0, b0, acc, sum, opieces, stage);
-  else store_tile<false>(a, z, a0, b0, acc, sum, opieces, stage);
+  if (NPMAX == 6 && np == 6 && a.acc64) store_tile<true>(a, z, a0, b0, acc, sum, opieces, stage, ps.sp);
+  else store_tile<false>(a, z, a0, b0, acc, sum, opieces, stage, ps.sp);
 }
 template __global__ void dst_gemm_bf16_kernel<1>(GemmArgs);
 template __global__ void dst_gemm_bf16_kernel<3>(GemmArgs);
@@ -410,10 +596,18 @@ template __global__ void dst_gemm_bf16_kernel<6>(GemmArgs);
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 2))) void dst_gemm_f32_kernel(GemmArgs a) {
   __shared__ __attribute__((aligned(16))) float lds[2 * 2 * TILE32];
-  const int GA = (a.RA + BM - 1) / BM, GB = (a.RB + BN - 1) / BN;
+  const int GA = (gemm_tile_cut(a.RA) + BM - 1) / BM, GB = (gemm_tile_cut(a.RB) + BN - 1) / BN;
+  const int T = GA * GB * a.nplanes;
+  if ((int)blockIdx.x >= T) {
+    const int fpw = gemm_fringe_wgs(a.RA, a.RB);
+    const int zf = ((int)blockIdx.x - T) / fpw;
+    gemm_fringe<true, 1>(a, zf, ((int)blockIdx.x - T) - zf * fpw, fpw, select_plane(a, zf), 0);
+    return;
+  }
   int zi, ta, tb;
-  tile_of_block(GA, GB, zi, ta, tb);
+  tile_of_block(GA, GB, T, zi, ta, tb);
   const long z = zi;
+  const PlaneSel ps = select_plane(a, z);
   const int a0 = ta * BM;
   const int b0 = tb * BN;
   const bool x_in_a = !a.trans;
@@ -422,7 +616,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 2))) voi
   const int wave = tid >> 6;
   const int wm = wave >> 1, wn = wave & 1;
   const float* X = a.Xf + z * a.x_batch;
-  const float* S = a.Sf;
+  const float* S = a.Sf + ps.s_off;
   const int x0 = x_in_a ? a0 : b0;
   const int s0 = x_in_a ? b0 : a0;
   // staging: 128 rows x 16 floats = 512 float4, 2 per thread per operand
@@ -510,8 +704,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 2))) voi
     }
     for (; kt < nk; ++kt) ktile(kt, acc);
   }
-  if (acc64) store_tile<true>(a, z, a0, b0, acc, sum, 0, lds);
-  else store_tile<false>(a, z, a0, b0, acc, sum, 0, lds);
+  if (acc64) store_tile<true>(a, z, a0, b0, acc, sum, 0, lds, ps.sp);
+  else store_tile<false>(a, z, a0, b0, acc, sum, 0, lds, ps.sp);
 }
 
 hipError_t prepare_dst_gemm() {
@@ -531,17 +725,20 @@ hipError_t prepare_dst_gemm() {
   return status;
 }
 
-void launch_dst_gemm(const GemmArgs& a, int batch, hipStream_t st) {
-  const int tiles = ((a.RB + BN - 1) / BN) * ((a.RA + BM - 1) / BM) * batch;
+void launch_dst_gemm(const GemmArgs& args, int batch, hipStream_t st) {
+  GemmArgs a = args;
+  a.nplanes = batch;
+  const int tiles = ((gemm_tile_cut(a.RB) + BN - 1) / BN) * ((gemm_tile_cut(a.RA) + BM - 1) / BM) * batch;
+  const int blocks = tiles + gemm_fringe_wgs(a.RA, a.RB) * batch;
   if (a.Xf) {
-    hipLaunchKernelGGL(dst_gemm_f32_kernel, dim3(tiles), dim3(256), 0, st, a);
+    hipLaunchKernelGGL(dst_gemm_f32_kernel, dim3(blocks), dim3(256), 0, st, a);
     return;
   }
   const int max_np = a.np_lo > a.np_hi ? a.np_lo : a.np_hi;
   const size_t lds = dst_gemm_lds_bytes(max_np);
-  if (max_np == 1) hipLaunchKernelGGL(dst_gemm_bf16_kernel<1>, dim3(tiles), dim3(256), lds, st, a);
-  else if (max_np == 3) hipLaunchKernelGGL(dst_gemm_bf16_kernel<3>, dim3(tiles), dim3(256), lds, st, a);
-  else hipLaunchKernelGGL(dst_gemm_bf16_kernel<6>, dim3(tiles), dim3(256), lds, st, a);
+  if (max_np == 1) hipLaunchKernelGGL(dst_gemm_bf16_kernel<1>, dim3(blocks), dim3(256), lds, st, a);
+  else if (max_np == 3) hipLaunchKernelGGL(dst_gemm_bf16_kernel<3>, dim3(blocks), dim3(256), lds, st, a);
+  else hipLaunchKernelGGL(dst_gemm_bf16_kernel<6>, dim3(blocks), dim3(256), lds, st, a);
 }
 
 }  // namespace somax

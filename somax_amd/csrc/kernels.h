@@ -35,8 +35,21 @@ struct MemberParams {
 };
 
 // O[a][b] = sum_k A[a][k] B[b][k] with the data plane X in the A slot
-// (trans == 0: O = X S) or in the B slot (trans == 1: O = (X S)^T), S the
-// symmetric sine matrix.  FP32 operands when Xf != null, else bf16 pieces.
+// (trans == 0: O = X S) or in the B slot (trans == 1: O = (X S)^T), S a sine
+// matrix (rows = outputs, k contiguous).  FP32 operands when Xf != null, else
+// bf16 pieces.
+//
+// Folded transforms (fold_P > 0): with S[N-1-n][a] = (-1)^a S[n][a], a DST-I
+// of even length N = 2h splits into two independent half-length products by
+// output parity, and a 2-D transform into four blocks b = 2p + q (p: parity
+// along y, q: along x).  Batch plane z is block z / fold_P of table plane
+// z % fold_P; the sine operand and the eigenvalue rows come in two parity
+// variants, s_var and lam_var elements apart.
+//
+// Fringe: when RA (RB) exceeds a multiple of 128 by 1 or 2, the 128 x 128
+// tiles stop at that multiple and fringe_wgs extra workgroups per plane
+// compute the leftover rows (columns) as FP64-accumulated dot products, and
+// write the zero padding of columns [RB, RBpad) that no tile covers.
 struct GemmArgs {
   // data operand X: rows xrows, k contiguous, zero padded in k to the k-tile
   const float* Xf;
@@ -63,7 +76,26 @@ struct GemmArgs {
   const float* lamA;
   const float* lamB;
   const float* lam;
+  // folded transforms (see above); 0 = plain
+  int fold_P;
+  int s_par;    // parity choosing the sine variant: 0 = block & 1 (x), 1 = block >> 1 (y)
+  long s_var;   // elements between the sine operand's parity variants
+  int lam_var;   // floats between the parity variants of lamA (x parity)
+  int lamB_var;  // floats between the parity variants of lamB (y parity)
+  int nplanes;   // batch planes of the launch (set by launch_dst_gemm)
 };
+
+// Rows covered by full tiles when R leaves a fringe of 1 or 2 (else R).
+__host__ __device__ inline int gemm_tile_cut(int R) {
+  const int r = R & 127;
+  return (R > 128 && (r == 1 || r == 2)) ? R - r : R;
+}
+// Fringe workgroups per plane of a launch (0: no fringe).
+__host__ __device__ inline int gemm_fringe_wgs(int RA, int RB) {
+  const int am = gemm_tile_cut(RA), bm = gemm_tile_cut(RB);
+  const long dots = (long)(RA - am) * RB + (long)am * (RB - bm);
+  return dots > 0 ? (int)((dots + 255) / 256) : 0;
+}
 
 void launch_dst_gemm(const GemmArgs& a, int batch, hipStream_t st);
 // Raises the bf16 kernel's dynamic LDS limit; call before the first launch.
@@ -86,7 +118,17 @@ struct ModalArgs {
   int op_pieces;
   long op_ld, op_piece, op_batch;
   int op_kpad;
+  // folded layout (fold != 0): qm and op hold, for block b = 2p + q of table
+  // plane m at batch plane b * P + m, the fold sums
+  //   F[k'y][k'x] = sum_{sy,sx} (-1)^(p sy + q sx) v[mirror^sy(y)][mirror^sx(x)],
+  // y = fold_phys(k'y, Ny/2), mirror(y) = Ny-1-y; qm planes are hy x hx
+  // (pitch hx), the operand rows k'y < hy zero padded to op_kpad columns.
+  int fold, P;
 };
+
+// Folded index k' in [0, h) -> physical index in the first half: the pairs
+// (k'+1, N-2-k') first, the ghost pair (0, N-1) last.
+__host__ __device__ inline int fold_phys(int k, int h) { return k == h - 1 ? 0 : k + 1; }
 void launch_to_modal(const ModalArgs& a, hipStream_t st);
 
 // Per-mode Helmholtz residual on the full ghosted array,
@@ -104,6 +146,11 @@ struct ResidualArgs {
   int Nx, Ny, P;
   float idx2, idy2;
   const float* lam;
+  // folded (fold != 0): q = fold sums, psi = parity components (4P planes of
+  // Ny/2 x Nx/2, pitch Nx/2); writes the fold sums q - 4 (lap_h - lam) psi of
+  // the residual.  The components satisfy the Laplacian's zero exterior and
+  // the mirror rule psi[h] = +-psi[h-1] (even / odd parity) at the fold line.
+  int fold;
 };
 void launch_residual(const ResidualArgs& a, hipStream_t st);
 
@@ -120,6 +167,10 @@ struct LayerArgs {
   // mass correction (nullable): pm += alpha[p] * homsol[p] before the projection
   const float* homsol;
   const double* alpha;
+  // folded (fold != 0): pm and d hold parity components (block b = 2p + q of
+  // table plane m at plane b * P + m, Ny/2 x Nx/2, pitch Nx/2); the physical
+  // value at (mirror^sy(y), mirror^sx(x)) is sum_b (-1)^(p sy + q sx) pm_b.
+  int fold, P;
 };
 void launch_to_layer(const LayerArgs& a, hipStream_t st);
 

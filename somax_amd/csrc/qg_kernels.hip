@@ -422,7 +422,43 @@ __global__ __launch_bounds__(256) void residual_kernel(ResidualArgs a) {
   store_pieces(o, a.op_piece, 2, r);
 }
 
+// Folded residual on the parity components (block b = 2p + q of plane m at
+// b * P + m): the 5-point Laplacian of a component on the half array, with
+// zero exterior and the mirror rule at the fold line (the neighbour past the
+// last physical index h-1 is +-the value itself: even / odd parity).
+__global__ __launch_bounds__(256) void residual_fold_kernel(ResidualArgs a) {
+  const int kx = blockIdx.x * blockDim.x + threadIdx.x;  // up to op_kpad
+  const int ky = blockIdx.y;
+  const int z = blockIdx.z;
+  if (kx >= a.op_kpad) return;
+  const int hx = a.Nx / 2, hy = a.Ny / 2;
+  uint16_t* o = a.r + z * a.op_batch + (long)ky * a.op_ld + kx;
+  if (kx >= hx) {
+    store_pieces(o, a.op_piece, 2, 0.0f);
+    return;
+  }
+  const int blk = z / a.P, m = z - blk * a.P;
+  const float sx = (blk & 1) ? -1.0f : 1.0f, sy = (blk >> 1) ? -1.0f : 1.0f;
+  const float* Z = a.psi + (long)z * hx * hy;
+  const float* Zr = Z + (long)ky * hx;
+  const float c = Zr[kx];
+  // physical indices in the first half; neighbours mapped back to folded indices
+  const int x = fold_phys(kx, hx), y = fold_phys(ky, hy);
+  const float e = x == hx - 1 ? sx * c : Zr[x];
+  const float w = x == 0 ? 0.0f : (x == 1 ? Zr[hx - 1] : Zr[x - 2]);
+  const float n = y == hy - 1 ? sy * c : Z[(long)y * hx + kx];
+  const float so = y == 0 ? 0.0f : (y == 1 ? Z[(long)(hy - 1) * hx + kx] : Z[(long)(y - 2) * hx + kx]);
+  const float lap = (e - 2.0f * c + w) * a.idx2 + (n - 2.0f * c + so) * a.idy2;
+  const float r = a.q[(long)z * hx * hy + (long)ky * hx + kx] - 4.0f * (lap - a.lam[m] * c);
+  store_pieces(o, a.op_piece, 2, r);
+}
+
 void launch_residual(const ResidualArgs& a, hipStream_t st) {
+  if (a.fold) {
+    dim3 grid((a.op_kpad + 255) / 256, a.Ny / 2, 4 * a.P);
+    hipLaunchKernelGGL(residual_fold_kernel, grid, dim3(256), 0, st, a);
+    return;
+  }
   dim3 grid((a.op_kpad + 255) / 256, a.Ny, a.P);
   hipLaunchKernelGGL(residual_kernel, grid, dim3(256), 0, st, a);
 }
@@ -469,7 +505,68 @@ __global__ __launch_bounds__(256) void to_modal_kernel(ModalArgs a) {
   }
 }
 
+// Folded variant: one thread per folded point (k'y, k'x) of one member reads
+// the four mirror points of every layer, projects them to modes in FP64 and
+// writes the four parity fold sums of each mode.
+__global__ __launch_bounds__(256) void to_modal_fold_kernel(ModalArgs a) {
+  const int kx = blockIdx.x * blockDim.x + threadIdx.x;
+  const int ky = blockIdx.y;
+  const int mem = blockIdx.z;
+  const int hx = a.Nx / 2, hy = a.Ny / 2;
+  const int K0 = a.apply_bc ? hx - 1 : hx;  // operand columns carrying data
+  const int R0 = a.apply_bc ? hy - 1 : hy;  // operand rows
+  const bool inside = kx < hx;
+  const bool wop = ky < R0 && kx < a.op_kpad;
+  if (!inside && !wop) return;
+  const int first = a.t.mfirst[mem], nl = a.t.mnl[mem];
+  const double* cl2m = a.t.cl2m + mem * MAX_LAYERS * MAX_LAYERS;
+  const int y0 = fold_phys(ky, hy), y1 = a.Ny - 1 - y0;
+  const int x0 = inside ? fold_phys(kx, hx) : 0, x1 = a.Nx - 1 - x0;
+  const long pts[4] = {(long)y0 * a.Nx + x0, (long)y0 * a.Nx + x1, (long)y1 * a.Nx + x0, (long)y1 * a.Nx + x1};
+  // the ghost pair (last folded index) holds only ghost points
+  const bool zero = !inside || (a.apply_bc && (ky == hy - 1 || kx == hx - 1));
+  double qk[MAX_LAYERS][4];
+#pragma unroll
+  for (int k = 0; k < MAX_LAYERS; ++k)
+#pragma unroll
+    for (int c = 0; c < 4; ++c)
+      qk[k][c] = (k >= nl || zero) ? 0.0 : (double)a.q[(first + k) * a.plane + pts[c]];
+  const long fplane = (long)hx * hy;
+#pragma unroll
+  for (int p = 0; p < MAX_LAYERS; ++p) {
+    if (p >= nl) break;
+    double v[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int k = 0; k < MAX_LAYERS; ++k)
+      if (k < nl)
+#pragma unroll
+        for (int c = 0; c < 4; ++c) v[c] += cl2m[p * MAX_LAYERS + k] * qk[k][c];
+    // block b = 2 py + qx: sum over mirror points (sy, sx) of (-1)^(py sy + qx sx)
+    const double f[4] = {v[0] + v[1] + v[2] + v[3], v[0] - v[1] + v[2] - v[3], v[0] + v[1] - v[2] - v[3],
+                         v[0] - v[1] - v[2] + v[3]};
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+      const long z = (long)b * a.P + first + p;
+      const float fv = (float)f[b];
+      if (inside) a.qm[z * fplane + (long)ky * hx + kx] = fv;
+      if (wop) {
+        const float w = kx < K0 ? fv : 0.0f;  // zero padding past the operand width
+        const long o = z * a.op_batch + (long)ky * a.op_ld + kx;
+        if (a.op_pieces == 0) static_cast<float*>(a.op)[o] = w;
+        else store_pieces(static_cast<uint16_t*>(a.op) + o, a.op_piece, a.op_pieces, w);
+      }
+    }
+  }
+}
+
 void launch_to_modal(const ModalArgs& a, hipStream_t st) {
+  if (a.fold) {
+    const int hx = a.Nx / 2;
+    const int width = hx > a.op_kpad ? hx : a.op_kpad;
+    dim3 grid((width + 255) / 256, a.Ny / 2, a.M);
+    hipLaunchKernelGGL(to_modal_fold_kernel, grid, dim3(256), 0, st, a);
+    return;
+  }
   const int off = a.apply_bc ? 1 : 0;
   const int width = a.Nx > a.op_kpad + off ? a.Nx : a.op_kpad + off;
   dim3 grid((width + 255) / 256, a.Ny, a.M);
@@ -509,7 +606,66 @@ __global__ __launch_bounds__(256) void to_layer_kernel(LayerArgs a) {
   }
 }
 
+// Folded variant: one thread per folded point of one member unfolds the
+// parity components of every mode at the four mirror points, adds the mass
+// correction and mixes modes into layers.
+__global__ __launch_bounds__(256) void to_layer_fold_kernel(LayerArgs a) {
+  const int kx = blockIdx.x * blockDim.x + threadIdx.x;
+  const int ky = blockIdx.y;
+  const int mem = blockIdx.z;
+  const int hx = a.Nx / 2, hy = a.Ny / 2;
+  if (kx >= hx) return;
+  const int first = a.t.mfirst[mem], nl = a.t.mnl[mem];
+  const float* cm2l = a.t.cm2l + mem * MAX_LAYERS * MAX_LAYERS;
+  const int y0 = fold_phys(ky, hy), y1 = a.Ny - 1 - y0;
+  const int x0 = fold_phys(kx, hx), x1 = a.Nx - 1 - x0;
+  const long pts[4] = {(long)y0 * a.Nx + x0, (long)y0 * a.Nx + x1, (long)y1 * a.Nx + x0, (long)y1 * a.Nx + x1};
+  const bool ghost = a.t.mask[mem] && (ky == hy - 1 || kx == hx - 1);
+  const long fplane = (long)hx * hy;
+  const long fo = (long)ky * hx + kx;
+  float pm[MAX_LAYERS][4];
+#pragma unroll
+  for (int p = 0; p < MAX_LAYERS; ++p) {
+#pragma unroll
+    for (int c = 0; c < 4; ++c) pm[p][c] = 0.0f;
+    if (p < nl) {
+      float zb[4];
+#pragma unroll
+      for (int b = 0; b < 4; ++b) {
+        const long i = ((long)b * a.P + first + p) * fplane + fo;
+        zb[b] = a.pm[i] + (a.d ? a.d[i] : 0.0f);
+      }
+      pm[p][0] = (zb[0] + zb[1]) + (zb[2] + zb[3]);
+      pm[p][1] = (zb[0] - zb[1]) + (zb[2] - zb[3]);
+      pm[p][2] = (zb[0] + zb[1]) - (zb[2] + zb[3]);
+      pm[p][3] = (zb[0] - zb[1]) - (zb[2] - zb[3]);
+      if (a.homsol) {
+        const float al = (float)a.alpha[first + p];
+#pragma unroll
+        for (int c = 0; c < 4; ++c) pm[p][c] += al * a.homsol[(first + p) * a.plane + pts[c]];
+      }
+    }
+  }
+#pragma unroll
+  for (int l = 0; l < MAX_LAYERS; ++l) {
+    if (l >= nl) break;
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      float v = 0.0f;
+#pragma unroll
+      for (int p = 0; p < MAX_LAYERS; ++p)
+        if (p < nl) v += cm2l[l * MAX_LAYERS + p] * pm[p][c];
+      a.psi[(first + l) * a.plane + pts[c]] = ghost ? 0.0f : v;
+    }
+  }
+}
+
 void launch_to_layer(const LayerArgs& a, hipStream_t st) {
+  if (a.fold) {
+    dim3 grid((a.Nx / 2 + 255) / 256, a.Ny / 2, a.M);
+    hipLaunchKernelGGL(to_layer_fold_kernel, grid, dim3(256), 0, st, a);
+    return;
+  }
   dim3 grid((a.Nx + 255) / 256, a.Ny, a.M);
   hipLaunchKernelGGL(to_layer_kernel, grid, dim3(256), 0, st, a);
 }

@@ -371,3 +371,51 @@ def test_mass_corrected_weno_step(gpu, precision):
     r1, r10 = rel(q1.cpu().numpy(), ref1), rel(q10.cpu().numpy(), ref10)
     _log(f"  1 step q {r1:.3e}; 10 steps q {r10:.3e}")
     assert r1 < 1e-6 and r10 < 1e-5
+
+
+@pytest.fixture
+def fold_mode(monkeypatch):
+    """Sets SOMAXHIP_FOLD (read when a plan is created) for one test."""
+
+    def set_mode(mode):
+        monkeypatch.setenv("SOMAXHIP_FOLD", mode)
+
+    return set_mode
+
+
+# Folded transforms (parity blocks of half size) against plain ones, on shapes
+# the automatic choice would not take: fold forced at small and non-square
+# grids (tile fringes of 0 rows), plain forced at 256^2 and 1024^2 (fringes of
+# 2 rows), and the folded 256^2 default (fringe of 1 row).
+@pytest.mark.parametrize("mode,nx,ny", [("1", 64, 64), ("1", 126, 62), ("1", 254, 254), ("0", 256, 256),
+                                        ("", 256, 256), ("0", 1024, 1024)])
+@pytest.mark.parametrize("precision", ["fp32", "mixed"])
+def test_fold_and_fringe_paths(gpu, fold_mode, mode, nx, ny, precision):
+    fold_mode(mode)
+    _log(f"[fold] SOMAXHIP_FOLD={mode or 'auto'} {nx}x{ny} precision={precision}")
+    model, setup = _pair(nx, ny=ny, precision=precision)
+    q, qd = _state(model, seed=61)
+    psi = model._invert_pv(qd).cpu().numpy()
+    ref = oqg.invert_pv(setup, q.astype(np.float64))
+    q1 = qd.clone()
+    model._step_inplace(q1, 1200.0, 1)
+    ref1 = oqg.ssprk3_step(setup, q.astype(np.float64), 1200.0)
+    r, r1 = rel(psi, ref), rel(q1.cpu().numpy(), ref1)
+    _log(f"  rel-L2 psi {r:.3e}, one step q {r1:.3e}")
+    assert r < (3e-6 if precision == "fp32" else 3e-7)
+    assert r1 < 1e-6
+
+
+@pytest.mark.parametrize("mode", ["1", "0"])
+def test_fold_mass_corrected(gpu, fold_mode, mode):
+    """Mass correction reads the folded even-even block (4x its sum)."""
+    fold_mode(mode)
+    _log(f"[fold] mass correction SOMAXHIP_FOLD={mode} n=128")
+    model, setup = _pair(128, precision="mixed", mass_correction=True)
+    q, qd = _state(model, seed=41)
+    psi = model._invert_pv(qd).cpu().numpy().astype(np.float64)
+    ref = oqg.invert_pv(setup, q.astype(np.float64))
+    modal = np.einsum("ml,lyx->myx", setup.Cl2m, psi)
+    means = np.abs(modal.mean(axis=(-2, -1))) / np.abs(modal).mean(axis=(-2, -1))
+    _log(f"  rel-L2 psi {rel(psi, ref):.3e}, modal means {means}")
+    assert rel(psi, ref) < 3e-7 and means.max() < 1e-6
