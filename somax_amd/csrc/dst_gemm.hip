@@ -22,6 +22,8 @@
 // 2x2 MFMA tiles of 32x32.  Grids are 1-D and remapped so that each XCD
 // works on a contiguous run of tiles (shared operand rows in its L2).
 
+#include <type_traits>
+
 #include "common.h"
 #include "kernels.h"
 
@@ -206,125 +208,189 @@ __device__ inline void store_one(const GemmArgs& a, long z, int r, int c, float 
 }
 
 // Fringe workgroup `part` of `fpw` for batch plane z: the dot products of
-// rows [RAm, RA) x columns [0, RB) and rows [0, RAm) x columns [RBm, RB),
-// eight lanes per dot along k with FP64 accumulation of the reconstructed
-// operands (sum of the NPC bf16 pieces, or FP32), then its share of the zero
-// padding [RB, RBpad) of the rows that no tile covers.  The loads of KB
-// 64-element k-chunks are issued before any is used, so a pass costs one
-// memory latency rather than one per chunk.
+// rows [am, RA) x columns [0, RB) and rows [0, am) x columns [bm, RB), then
+// its share of the zero padding [RB, RBpad) of the rows that no tile covers.
+// Every dot pairs one of the at most four fixed rows (A rows [am, RA), B rows
+// [bm, RB)) with a variable row.  The fixed rows are staged once in LDS,
+// reconstructed from their pieces; eight lanes share a dot along k, two dots
+// per lane group are in flight, and the loads of KB 64-element k-chunks are
+// issued before any is used.  Accumulation is FP64 for FP32 operands and
+// six-product planes, FP32 otherwise (the precision of those products).
 template <bool F32, int NPC>
-__device__ void gemm_fringe(const GemmArgs& a, long z, int part, int fpw, const PlaneSel& ps, int opieces) {
+__device__ void gemm_fringe(const GemmArgs& a, long z, int part, const GemmCut& cut, const PlaneSel& ps, int opieces,
+                            void* lds) {
+  using acc_t = typename std::conditional<(F32 || NPC == 3), double, float>::type;
   constexpr int KB = 9;
-  const int RAm = gemm_tile_cut(a.RA), RBm = gemm_tile_cut(a.RB);
-  const int fa = a.RA - RAm, fb = a.RB - RBm;
-  const long nrow = (long)fa * a.RB;
-  const long dots = nrow + (long)RAm * fb;
+  constexpr int NV = F32 ? 2 : NPC;  // 16-byte loads per variable-row chunk
+  const int fpw = cut.fpw;
+  const int fa = a.RA - cut.am, fb = a.RB - cut.bm;
+  const int nrow = fa * a.RB;
+  const int dots = nrow + cut.am * fb;
   const int tid = threadIdx.x, g = tid >> 3, l8 = tid & 7;
   const bool x_in_a = !a.trans;
+  const int esz = F32 ? 4 : 2;
   const char* X = F32 ? reinterpret_cast<const char*>(a.Xf + z * a.x_batch)
                       : reinterpret_cast<const char*>(a.Xb + z * a.x_batch);
   const char* S = F32 ? reinterpret_cast<const char*>(a.Sf + ps.s_off) : reinterpret_cast<const char*>(a.Sb + ps.s_off);
-  const int esz = F32 ? 4 : 2;
   auto xrow = [&](int r) { return X + (long)(r < a.xrows ? r : a.xrows - 1) * a.ldx * esz; };
   auto srow = [&](int r) { return S + (long)r * a.lds * esz; };
-  const long pa = x_in_a ? a.x_piece : a.s_piece, pb = x_in_a ? a.s_piece : a.x_piece;  // bf16 piece strides
-  const long per = (dots + fpw - 1) / fpw;
-  const long d0 = (long)part * per;
-  const long d1 = d0 + per < dots ? d0 + per : dots;
-  for (long base = d0; base < d1; base += 32) {
-    const long d = base + g;
-    const bool live = d < d1;
-    int r = 0, c = 0;
-    if (live) {
-      if (d < nrow) {
-        r = RAm + (int)(d / a.RB);
-        c = (int)(d % a.RB);
-      } else {
-        const long e = d - nrow;
-        r = (int)(e / fb);
-        c = RBm + (int)(e % fb);
+  // row of the A operand (is_a) or of the B operand, and its piece stride
+  auto row_of = [&](bool is_a, int r, long& piece) {
+    const bool from_x = is_a == x_in_a;
+    piece = from_x ? a.x_piece : a.s_piece;
+    return from_x ? xrow(r) : srow(r);
+  };
+  // 8 reconstructed operand values at k0 (k0 < K, 8-aligned)
+  auto load_chunk = [&](const char* row, long piece, int k0, acc_t (&v)[8]) {
+    if constexpr (F32) {
+      const float* f = reinterpret_cast<const float*>(row) + k0;
+      const floatx4 x0 = *reinterpret_cast<const floatx4*>(f);
+      const floatx4 x1 = *reinterpret_cast<const floatx4*>(f + 4);
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        v[e] = (acc_t)x0[e];
+        v[4 + e] = (acc_t)x1[e];
+      }
+    } else {
+      const uint16_t* b = reinterpret_cast<const uint16_t*>(row) + k0;
+#pragma unroll
+      for (int e = 0; e < 8; ++e) v[e] = (acc_t)0;
+#pragma unroll
+      for (int p = 0; p < NPC; ++p) {
+        const u32x4 w = *reinterpret_cast<const u32x4*>(b + p * piece);
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          const uint32_t u = w[e];
+          v[2 * e] += (acc_t)__builtin_bit_cast(float, u << 16);
+          v[2 * e + 1] += (acc_t)__builtin_bit_cast(float, u & 0xffff0000u);
+        }
       }
     }
-    double acc = 0.0;
-    if (live) {
-      const char* ra = x_in_a ? xrow(r) : srow(r);
-      const char* rb = x_in_a ? srow(c) : xrow(c);
-      for (int kb = 0; kb < a.K; kb += 64 * KB) {
-        if constexpr (F32) {
-          floatx4 va[KB][2], vb[KB][2];
-#pragma unroll
-          for (int j = 0; j < KB; ++j) {
-            const int k0 = kb + 64 * j + l8 * 8;
-            const float* fa = reinterpret_cast<const float*>(ra) + k0;
-            const float* fb = reinterpret_cast<const float*>(rb) + k0;
-#pragma unroll
-            for (int w = 0; w < 2; ++w) {
-              va[j][w] = k0 < a.K ? *reinterpret_cast<const floatx4*>(fa + 4 * w) : floatx4{0.0f, 0.0f, 0.0f, 0.0f};
-              vb[j][w] = k0 < a.K ? *reinterpret_cast<const floatx4*>(fb + 4 * w) : floatx4{0.0f, 0.0f, 0.0f, 0.0f};
-            }
-          }
-#pragma unroll
-          for (int j = 0; j < KB; ++j)
-#pragma unroll
-            for (int w = 0; w < 2; ++w)
-#pragma unroll
-              for (int e = 0; e < 4; ++e) acc += (double)va[j][w][e] * (double)vb[j][w][e];
-        } else {
-          u32x4 va[KB][NPC], vb[KB][NPC];
-#pragma unroll
-          for (int j = 0; j < KB; ++j) {
-            const int k0 = kb + 64 * j + l8 * 8;
-            const uint16_t* ba = reinterpret_cast<const uint16_t*>(ra) + k0;
-            const uint16_t* bb = reinterpret_cast<const uint16_t*>(rb) + k0;
-#pragma unroll
-            for (int w = 0; w < NPC; ++w) {
-              va[j][w] = k0 < a.K ? *reinterpret_cast<const u32x4*>(ba + w * pa) : u32x4{0u, 0u, 0u, 0u};
-              vb[j][w] = k0 < a.K ? *reinterpret_cast<const u32x4*>(bb + w * pb) : u32x4{0u, 0u, 0u, 0u};
-            }
-          }
-#pragma unroll
-          for (int j = 0; j < KB; ++j) {
-            double xa[8], xb[8];
-#pragma unroll
-            for (int e = 0; e < 8; ++e) xa[e] = xb[e] = 0.0;
-#pragma unroll
-            for (int w = 0; w < NPC; ++w)
-#pragma unroll
-              for (int e = 0; e < 4; ++e) {
-                const uint32_t ua = va[j][w][e], ub = vb[j][w][e];
-                xa[2 * e] += (double)__builtin_bit_cast(float, ua << 16);
-                xa[2 * e + 1] += (double)__builtin_bit_cast(float, ua & 0xffff0000u);
-                xb[2 * e] += (double)__builtin_bit_cast(float, ub << 16);
-                xb[2 * e + 1] += (double)__builtin_bit_cast(float, ub & 0xffff0000u);
-              }
-#pragma unroll
-            for (int e = 0; e < 8; ++e) acc += xa[e] * xb[e];
+  };
+
+  // zero padding first (stores need no wait): rows [am, RA), and every row
+  // when the columns have a fringe; 32 lanes per row, columns RB + lane
+  {
+    const int padw = a.RBpad - a.RB;
+    const int nzr = fb > 0 ? a.RA : fa;
+    if (padw > 0 && nzr > 0) {
+      const int zper = (nzr + fpw - 1) / fpw;
+      const int r0 = part * zper, r1 = r0 + zper < nzr ? r0 + zper : nzr;
+      for (int e = r0 * 32 + tid; e < r1 * 32; e += 256) {
+        const int rq = e >> 5;
+        const int r = fb > 0 ? rq : cut.am + rq;
+        for (int cq = e & 31; cq < padw; cq += 32) {
+          const long o = z * a.o_batch + (long)r * a.ldo + a.RB + cq;
+          if (opieces == 0) {
+            a.Of[o] = 0.0f;
+          } else {
+            for (int p = 0; p < opieces; ++p) a.Ob[o + p * a.o_piece] = 0;
           }
         }
       }
     }
-    acc += __shfl_xor(acc, 1);
-    acc += __shfl_xor(acc, 2);
-    acc += __shfl_xor(acc, 4);
-    if (live && l8 == 0) store_one(a, z, r, c, (float)acc, opieces, ps.sp);
   }
-  // zero padding: rows [RAm, RA), and every row when the columns have a fringe
-  const int padw = a.RBpad - a.RB;
-  const int nzr = fb > 0 ? a.RA : fa;
-  const long nz = (long)nzr * padw;
-  if (nz == 0) return;
-  const long zper = (nz + fpw - 1) / fpw;
-  const long z0 = (long)part * zper;
-  const long z1 = z0 + zper < nz ? z0 + zper : nz;
-  for (long e = z0 + tid; e < z1; e += 256) {
-    const int rr = (int)(e / padw);
-    const int r = fb > 0 ? rr : RAm + rr;
-    const int c = a.RB + (int)(e % padw);
-    const long o = z * a.o_batch + (long)r * a.ldo + c;
-    if (opieces == 0) {
-      a.Of[o] = 0.0f;
-    } else {
-      for (int p = 0; p < opieces; ++p) a.Ob[o + p * a.o_piece] = 0;
+  // fixed rows -> LDS: fix[i][k], i < fa: A row am + i; else B row bm + i - fa
+  const int K8 = (a.K + 7) & ~7;
+  acc_t* fix = static_cast<acc_t*>(lds);
+  const int nfix = fa + fb;
+  for (int e = tid; e < nfix * (K8 / 8); e += 256) {
+    const int i = e / (K8 / 8), k0 = (e - i * (K8 / 8)) * 8;
+    long piece;
+    const char* row = i < fa ? row_of(true, cut.am + i, piece) : row_of(false, cut.bm + i - fa, piece);
+    acc_t v[8];
+    load_chunk(row, piece, k0, v);
+#pragma unroll
+    for (int q = 0; q < 8; ++q) fix[i * K8 + k0 + q] = v[q];
+  }
+  __syncthreads();
+
+  const int per = (dots + fpw - 1) / fpw;
+  const int d0 = part * per;
+  const int d1 = d0 + per < dots ? d0 + per : dots;
+  for (int base = d0; base < d1; base += 64) {
+    const char* vrow[2];
+    long vpiece[2];
+    int fixi[2], rr[2], cc[2];
+    bool live[2];
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const int d = base + 32 * h + g;
+      live[h] = d < d1;
+      int r = 0, c = 0, f = 0;
+      bool var_is_b = true;
+      if (live[h]) {
+        if (d < nrow) {
+          const int q = (unsigned)d / (unsigned)a.RB;
+          r = cut.am + q;
+          c = d - q * a.RB;
+          f = r - cut.am;
+        } else {
+          const int e = d - nrow;
+          const int q = fb == 1 ? e : e >> 1;  // fb is 1 or 2
+          r = q;
+          c = cut.bm + (e - q * fb);
+          f = fa + c - cut.bm;
+          var_is_b = false;
+        }
+      }
+      rr[h] = r;
+      cc[h] = c;
+      fixi[h] = f;
+      vrow[h] = var_is_b ? row_of(false, c, vpiece[h]) : row_of(true, r, vpiece[h]);
+    }
+    acc_t acc[2] = {(acc_t)0, (acc_t)0};
+    for (int kb = 0; kb < a.K; kb += 64 * KB) {
+      u32x4 raw[2][KB][NV];
+#pragma unroll
+      for (int h = 0; h < 2; ++h)
+#pragma unroll
+        for (int j = 0; j < KB; ++j) {
+          const int k0 = kb + 64 * j + l8 * 8;
+#pragma unroll
+          for (int w = 0; w < NV; ++w) {
+            const char* src = vrow[h] + (long)k0 * esz + (F32 ? 16 * w : w * vpiece[h] * esz);
+            raw[h][j][w] = (live[h] && k0 < a.K) ? *reinterpret_cast<const u32x4*>(src) : u32x4{0u, 0u, 0u, 0u};
+          }
+        }
+#pragma unroll
+      for (int h = 0; h < 2; ++h)
+#pragma unroll
+        for (int j = 0; j < KB; ++j) {
+          const int k0 = kb + 64 * j + l8 * 8;
+          if (k0 >= a.K) continue;
+          acc_t v[8];
+          if constexpr (F32) {
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+              const uint32_t u0 = raw[h][j][0][e], u1 = raw[h][j][1][e];
+              v[e] = (acc_t)__builtin_bit_cast(float, u0);
+              v[4 + e] = (acc_t)__builtin_bit_cast(float, u1);
+            }
+          } else {
+#pragma unroll
+            for (int e = 0; e < 8; ++e) v[e] = (acc_t)0;
+#pragma unroll
+            for (int w = 0; w < NV; ++w)
+#pragma unroll
+              for (int e = 0; e < 4; ++e) {
+                const uint32_t u = raw[h][j][w][e];
+                v[2 * e] += (acc_t)__builtin_bit_cast(float, u << 16);
+                v[2 * e + 1] += (acc_t)__builtin_bit_cast(float, u & 0xffff0000u);
+              }
+          }
+          const acc_t* fx = fix + fixi[h] * K8 + k0;
+#pragma unroll
+          for (int e = 0; e < 8; ++e) acc[h] += v[e] * fx[e];
+        }
+    }
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      acc_t t = acc[h];
+      t += __shfl_xor(t, 1);
+      t += __shfl_xor(t, 2);
+      t += __shfl_xor(t, 4);
+      if (live[h] && l8 == 0) store_one(a, z, rr[h], cc[h], (float)t, opieces, ps.sp);
     }
   }
 }
@@ -546,14 +612,14 @@ size_t dst_gemm_lds_bytes(int max_np) {
 template <int NPMAX>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) void dst_gemm_bf16_kernel(GemmArgs a) {
   extern __shared__ __attribute__((aligned(16))) uint16_t lds16[];
-  const int GA = (gemm_tile_cut(a.RA) + BM - 1) / BM, GB = (gemm_tile_cut(a.RB) + BN - 1) / BN;
+  const GemmCut cut = gemm_cut(a.RA, a.RB, a.K);
+  const int GA = (cut.am + BM - 1) / BM, GB = (cut.bm + BN - 1) / BN;
   const int T = GA * GB * a.nplanes;
   const bool fringe = (int)blockIdx.x >= T;
-  int zi, ta = 0, tb = 0, part = 0, fpw = 0;
+  int zi, ta = 0, tb = 0, part = 0;
   if (fringe) {
-    fpw = gemm_fringe_wgs(a.RA, a.RB);
-    zi = ((int)blockIdx.x - T) / fpw;
-    part = ((int)blockIdx.x - T) - zi * fpw;
+    zi = ((int)blockIdx.x - T) / cut.fpw;
+    part = ((int)blockIdx.x - T) - zi * cut.fpw;
   } else {
     tile_of_block(GA, GB, T, zi, ta, tb);
   }
@@ -563,9 +629,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
   const int np = lo ? a.np_lo : a.np_hi;
   const int opieces = lo ? a.op_lo : a.op_hi;
   if (fringe) {
-    if (np == 1) gemm_fringe<false, 1>(a, z, part, fpw, ps, opieces);
-    else if (NPMAX == 3 || np == 3) gemm_fringe<false, 2>(a, z, part, fpw, ps, opieces);
-    else gemm_fringe<false, 3>(a, z, part, fpw, ps, opieces);
+    if (np == 1) gemm_fringe<false, 1>(a, z, part, cut, ps, opieces, lds16);
+    else if (NPMAX == 3 || np == 3) gemm_fringe<false, 2>(a, z, part, cut, ps, opieces, lds16);
+    else gemm_fringe<false, 3>(a, z, part, cut, ps, opieces, lds16);
     return;
   }
   const int a0 = ta * BM;
@@ -596,12 +662,12 @@ template __global__ void dst_gemm_bf16_kernel<6>(GemmArgs);
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 2))) void dst_gemm_f32_kernel(GemmArgs a) {
   __shared__ __attribute__((aligned(16))) float lds[2 * 2 * TILE32];
-  const int GA = (gemm_tile_cut(a.RA) + BM - 1) / BM, GB = (gemm_tile_cut(a.RB) + BN - 1) / BN;
+  const GemmCut cut = gemm_cut(a.RA, a.RB, a.K);
+  const int GA = (cut.am + BM - 1) / BM, GB = (cut.bm + BN - 1) / BN;
   const int T = GA * GB * a.nplanes;
   if ((int)blockIdx.x >= T) {
-    const int fpw = gemm_fringe_wgs(a.RA, a.RB);
-    const int zf = ((int)blockIdx.x - T) / fpw;
-    gemm_fringe<true, 1>(a, zf, ((int)blockIdx.x - T) - zf * fpw, fpw, select_plane(a, zf), 0);
+    const int zf = ((int)blockIdx.x - T) / cut.fpw;
+    gemm_fringe<true, 1>(a, zf, ((int)blockIdx.x - T) - zf * cut.fpw, cut, select_plane(a, zf), 0, lds);
     return;
   }
   int zi, ta, tb;
@@ -728,8 +794,9 @@ hipError_t prepare_dst_gemm() {
 void launch_dst_gemm(const GemmArgs& args, int batch, hipStream_t st) {
   GemmArgs a = args;
   a.nplanes = batch;
-  const int tiles = ((gemm_tile_cut(a.RB) + BN - 1) / BN) * ((gemm_tile_cut(a.RA) + BM - 1) / BM) * batch;
-  const int blocks = tiles + gemm_fringe_wgs(a.RA, a.RB) * batch;
+  const GemmCut cut = gemm_cut(a.RA, a.RB, a.K);
+  const int tiles = ((cut.bm + BN - 1) / BN) * ((cut.am + BM - 1) / BM) * batch;
+  const int blocks = tiles + cut.fpw * batch;
   if (a.Xf) {
     hipLaunchKernelGGL(dst_gemm_f32_kernel, dim3(blocks), dim3(256), 0, st, a);
     return;

@@ -85,16 +85,28 @@ struct GemmArgs {
   int nplanes;   // batch planes of the launch (set by launch_dst_gemm)
 };
 
-// Rows covered by full tiles when R leaves a fringe of 1 or 2 (else R).
-__host__ __device__ inline int gemm_tile_cut(int R) {
-  const int r = R & 127;
-  return (R > 128 && (r == 1 || r == 2)) ? R - r : R;
-}
-// Fringe workgroups per plane of a launch (0: no fringe).
-__host__ __device__ inline int gemm_fringe_wgs(int RA, int RB) {
-  const int am = gemm_tile_cut(RA), bm = gemm_tile_cut(RB);
-  const long dots = (long)(RA - am) * RB + (long)am * (RB - bm);
-  return dots > 0 ? (int)((dots + 255) / 256) : 0;
+// Tile cut of a launch: rows (columns) covered by full tiles, am (bm), and
+// fringe workgroups per plane, fpw.  A dimension exceeding a multiple of 128
+// by 1 or 2 leaves that fringe to the fringe workgroups, whose fixed operand
+// rows (at most 4, FP64) are staged in 40 KiB of LDS; otherwise tiles cover
+// the whole dimension (fpw = 0).
+struct GemmCut {
+  int am, bm, fpw;
+};
+__host__ __device__ inline GemmCut gemm_cut(int RA, int RB, int K) {
+  auto cut = [](int R) {
+    const int r = R & 127;
+    return (R > 128 && (r == 1 || r == 2)) ? R - r : R;
+  };
+  GemmCut c{cut(RA), cut(RB), 0};
+  const long k8 = (K + 7) / 8 * 8;
+  if ((long)((RA - c.am) + (RB - c.bm)) * k8 * 8 > 40960) {
+    c.am = RA;
+    c.bm = RB;
+  }
+  const long dots = (long)(RA - c.am) * RB + (long)c.am * (RB - c.bm);
+  c.fpw = dots > 0 ? (int)((dots + 255) / 256) : 0;
+  return c;
 }
 
 void launch_dst_gemm(const GemmArgs& a, int batch, hipStream_t st);
