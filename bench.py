@@ -87,11 +87,11 @@ def kernel_profile(model, q0, dt: float, steps: int):
     return out
 
 
-def measured_traffic() -> dict:
-    """HBM bytes per launch of the 1024² mixed step's kernels, measured with
-    rocprofv3 FETCH_SIZE / WRITE_SIZE passes and scaled by a copy of known
-    size (profiles/hbm_traffic_qg1024_mixed.json); empty if absent."""
-    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "hbm_traffic_qg1024_mixed.json")
+def measured_traffic(name: str = "hbm_traffic_qg1024_mixed.json") -> dict:
+    """HBM bytes per launch of a workload's kernels, measured with rocprofv3
+    FETCH_SIZE / WRITE_SIZE passes and scaled by a copy of known size
+    (profiles/hbm_traffic_*.json); empty if absent."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", name)
     try:
         with open(path) as f:
             return json.load(f)["bytes_per_launch"]
@@ -234,6 +234,7 @@ def bench_swm(steps: int, warmup: int) -> dict:
     k = stats[0]
     cells = model._nl * model.grid.nx * model.grid.ny
     roof = roofline_entry("swm_stage", dict(launches=k.launches, total_ms=k.total_ms, flops=0.0, bytes=k.bytes))
+    roof["traffic"] = measured_traffic("hbm_traffic_swm512.json").get("swm_stage")
     return {"workload": "swm2x512x512_channel_jet_ssprk3", "dt": round(dt, 3), "ms_per_step": round(ms, 4),
             "value": round(cells / (ms * 1e-3), 1), "unit": "layer-cell-steps/s", "roofline": roof}
 
@@ -247,7 +248,7 @@ def main() -> int:
     ap.add_argument("--dt", type=float, default=1200.0)
     ap.add_argument("--advection", default="arakawa", choices=["arakawa", "weno"])
     ap.add_argument("--profile-steps", type=int, default=2)
-    ap.add_argument("--cpu-steps", type=int, default=2, help="oracle steps timed for cpu_baseline (0 = skip)")
+    ap.add_argument("--cpu-steps", type=int, default=15, help="oracle steps timed for cpu_baseline (0 = skip)")
     ap.add_argument("--ensemble-steps", type=int, default=10, help="steps timed for the 256-member ensemble (0 = skip)")
     ap.add_argument("--swm-steps", type=int, default=50, help="steps timed for the shallow-water jet (0 = skip)")
     args = ap.parse_args()

@@ -10,13 +10,16 @@ Workloads:
   interior cells, mixed or plain FP32 inversion, Arakawa advection.
 * ``ensemble``: the 256-member uneven ensemble, mixed inversion, one
   eager step (every launch visible to the profiler).
+* ``swm512``: the 2-layer shallow-water baroclinic jet, 512 x 512 channel,
+  SSP-RK3 at CFL 0.5.
 
 Each step is preceded by one untimed warm-up step.  Prints one line per
 phase to stdout.
 
 ``python -m somax_amd.profiling summarize-trace TRACE.csv`` and
-``summarize-pmc FETCH.csv WRITE.csv COPY_MIB`` turn rocprofv3 CSV output into
-the markdown tables kept with the profiles.
+``summarize-pmc FETCH.csv WRITE.csv COPY_MIB [OUT.json WORKLOAD]`` turn
+rocprofv3 CSV output into the markdown tables kept with the profiles (and the
+per-kernel HBM bytes per launch that ``bench.py`` reports as ``traffic``).
 """
 
 from __future__ import annotations
@@ -46,10 +49,27 @@ def summarize_trace(path: str, top: int = 40) -> str:
     return "\n".join(lines)
 
 
-def summarize_pmc(fetch_csv: str, write_csv: str, copy_mib: int) -> str:
+# rocprof kernel symbol fragment -> the name the step profile (and bench.py) uses
+_BENCH_NAMES = (("dst_gemm_bf16_kernel<1>", "dst_gemm_bf16x1"), ("dst_gemm_bf16_kernel<3>", "dst_gemm_bf16x3"),
+                ("dst_gemm_bf16_kernel<6>", "dst_gemm_bf16x6"), ("dst_gemm_f32_kernel", "dst_gemm_fp32"),
+                ("qg_stage_", "qg_stage"), ("residual", "residual"), ("to_modal", "to_modal"),
+                ("to_layer", "to_layer"), ("swm_stage_kernel", "swm_stage"))
+
+
+def bench_name(kernel: str):
+    for frag, name in _BENCH_NAMES:
+        if frag in kernel:
+            return name
+    return None
+
+
+def summarize_pmc(fetch_csv: str, write_csv: str, copy_mib: int, per_name: dict | None = None) -> str:
     """Markdown table of FETCH_SIZE / WRITE_SIZE (KiB per dispatch) per
     (kernel, grid), scaled by what each counter reads for the elementwise copy
-    of ``copy_mib`` MiB (the multiply kernel with the largest grid)."""
+    of ``copy_mib`` MiB (the multiply kernel with the largest grid).  When
+    ``per_name`` is a dict it receives, per step-profile kernel name, the
+    scaled HBM bytes per launch averaged over all dispatches of that name, and
+    the two scales."""
     import collections
     import csv
 
@@ -69,6 +89,7 @@ def summarize_pmc(fetch_csv: str, write_csv: str, copy_mib: int) -> str:
              f"WRITE_SIZE scale {w_scale:.3f}", "",
              "| kernel | grid | dispatches | FETCH MiB (raw) | WRITE MiB (raw) | HBM MB per launch (scaled) |",
              "|---|---:|---:|---:|---:|---:|"]
+    totals = collections.defaultdict(lambda: [0.0, 0])
     for k in sorted(fe, key=lambda k: -sum(fe[k]) / len(fe[k])):
         if k == copy_key or k not in wr:
             continue
@@ -76,6 +97,15 @@ def summarize_pmc(fetch_csv: str, write_csv: str, copy_mib: int) -> str:
         w = sum(wr[k]) / len(wr[k])
         mb = (f * f_scale + w * w_scale) * 1024.0 / 1e6
         lines.append(f"| `{k[0][:70]}` | {k[1]} | {len(fe[k])} | {f / 1024:.2f} | {w / 1024:.2f} | {mb:.2f} |")
+        name = bench_name(k[0])
+        if name:
+            n = min(len(fe[k]), len(wr[k]))
+            totals[name][0] += (sum(fe[k][:n]) * f_scale + sum(wr[k][:n]) * w_scale) * 1024.0
+            totals[name][1] += n
+    if per_name is not None:
+        per_name["fetch_scale"] = round(f_scale, 3)
+        per_name["write_scale"] = round(w_scale, 3)
+        per_name["bytes_per_launch"] = {n: round(t / c) for n, (t, c) in totals.items() if c}
     return "\n".join(lines)
 
 
@@ -86,10 +116,18 @@ def main(argv=None) -> int:
         print(summarize_trace(argv[1]))
         return 0
     if argv and argv[0] == "summarize-pmc":
-        print(summarize_pmc(argv[1], argv[2], int(argv[3])))
+        # summarize-pmc FETCH.csv WRITE.csv COPY_MIB [OUT.json WORKLOAD]
+        per = {} if len(argv) > 5 else None
+        print(summarize_pmc(argv[1], argv[2], int(argv[3]), per))
+        if per is not None:
+            import json
+
+            per = {"workload": argv[5], "source": "rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE", **per}
+            with open(argv[4], "w") as f:
+                json.dump(per, f, indent=1)
         return 0
     ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
-    ap.add_argument("--workload", default="qg1024-mixed", choices=["qg1024-mixed", "qg1024-fp32", "ensemble"])
+    ap.add_argument("--workload", default="qg1024-mixed", choices=["qg1024-mixed", "qg1024-fp32", "ensemble", "swm512"])
     ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--copy-mib", type=int, default=1024)
     ap.add_argument("--dt", type=float, default=1200.0)
@@ -121,6 +159,20 @@ def main(argv=None) -> int:
         ens.step(packed, args.dt, args.steps)
         torch.cuda.synchronize()
         print(f"ensemble: {len(models)} members, {args.steps} steps", flush=True)
+        return 0
+
+    if args.workload == "swm512":
+        from . import MultilayerSW2DState
+
+        model, (h, u, v) = tc.baroclinic_jet_swm(512, seed=3, lateral_viscosity=50.0)
+        h, u, v = (torch.from_numpy(np.asarray(a, dtype=np.float32)).cuda() for a in (h, u, v))
+        st = MultilayerSW2DState(h=h, u=u, v=v)
+        dt = tc.jet_dt(model)
+        st = model.step(st, dt, 1)
+        torch.cuda.synchronize()
+        st = model.step(st, dt, args.steps)
+        torch.cuda.synchronize()
+        print(f"swm512: {args.steps} steps", flush=True)
         return 0
 
     precision = "mixed" if args.workload.endswith("mixed") else "fp32"
