@@ -137,6 +137,17 @@ int somaxhip_qg_rhs(somaxhip_qg_plan plan, const float* q, float* dq, float* psi
 int somaxhip_qg_diagnose(somaxhip_qg_plan plan, const float* q, float* psi, float* u, float* v, float* zeta,
                          double* ke_enstrophy, void* workspace, somaxhip_stream_t stream);
 
+/* Tsit5 time stepping: the solver SomaxModel.integrate uses by default
+ * (core/model.py:75, diffrax.Tsit5 with ConstantStepSize).  Advances q in
+ * place by nsteps steps of dt with the boundary condition applied before
+ * every RHS; six RHS evaluations per step (first same as last), each an
+ * inversion plus the fused RHS kernel.  tsit5_workspace: a second caller
+ * buffer of somaxhip_qg_tsit5_workspace_bytes (stage state and the seven
+ * stage derivatives). */
+size_t somaxhip_qg_tsit5_workspace_bytes(somaxhip_qg_plan plan);
+int somaxhip_qg_step_tsit5(somaxhip_qg_plan plan, float* q, int nsteps, double dt, const somaxhip_qg_params* params,
+                           void* workspace, void* tsit5_workspace, somaxhip_stream_t stream);
+
 /* Ghost ring of nl fields of (ny+2) x (nx+2) set to 0 in place.
  * Replaces finitevolx.zero_boundaries as used by
  * BaroclinicQG.apply_boundary_conditions (baroclinic.py:192-195). */
@@ -214,6 +225,13 @@ int somaxhip_swm_diagnose(somaxhip_swm_plan plan, const float* h, const float* u
  * SomaxModel.integrate, core/model.py:53-88). */
 int somaxhip_swm_step(somaxhip_swm_plan plan, float* h, float* u, float* v, int nsteps, double dt,
                       const somaxhip_swm_params* params, void* workspace, somaxhip_stream_t stream);
+
+/* Tsit5 time stepping of (h, u, v) in place (the default solver of
+ * SomaxModel.integrate, core/model.py:75), as somaxhip_qg_step_tsit5;
+ * tsit5_workspace of somaxhip_swm_tsit5_workspace_bytes. */
+size_t somaxhip_swm_tsit5_workspace_bytes(somaxhip_swm_plan plan);
+int somaxhip_swm_step_tsit5(somaxhip_swm_plan plan, float* h, float* u, float* v, int nsteps, double dt,
+                            const somaxhip_swm_params* params, void* tsit5_workspace, somaxhip_stream_t stream);
 
 /* ------------------------------------------------------------------------ */
 /* Ensembles                                                                */

@@ -30,5 +30,7 @@ Parity pins (each function's docstring names its own):
                    known answers.
 * ``swm``       -- rest state -> zero tendency, forcing locality, wall BC,
                    exact mass conservation of the flux-form continuity
-                   equation.
+                   equation, C-grid dispersion relation.
+* ``rk``        -- Tsit5 tableau pinned by the 17 order conditions of orders
+                   1-5 and by its dt^5 convergence on y' = lambda y.
 """

@@ -223,6 +223,14 @@ def ssprk3_step(s: QGSetup, q: np.ndarray, dt: float) -> np.ndarray:
     return q0 / 3.0 + 2.0 / 3.0 * (q2 + dt * rhs(s, apply_bc(q2)))
 
 
+def tsit5_step(s: QGSetup, q: np.ndarray, dt: float) -> np.ndarray:
+    """One Tsit5 step (``oracle/rk.py``; ``integrate``'s default solver,
+    core/model.py:75) with the BC applied before each RHS (core/model.py:47-49)."""
+    from . import rk
+
+    return rk.explicit_step(lambda y: (rhs(s, apply_bc(y[0])),), (q,), dt)[0]
+
+
 def run(s: QGSetup, q: np.ndarray, dt: float, nsteps: int) -> np.ndarray:
     for _ in range(nsteps):
         q = ssprk3_step(s, q, dt)

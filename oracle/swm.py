@@ -274,6 +274,14 @@ def ssprk3_step(s: SWMSetup, h, u, v, dt):
     return tuple(a / 3.0 + 2.0 / 3.0 * (b + dt * c) for a, b, c in zip(y0, y2, k))
 
 
+def tsit5_step(s: SWMSetup, h, u, v, dt):
+    """One Tsit5 step (``oracle/rk.py``; ``integrate``'s default solver,
+    core/model.py:75) with the BC applied before each RHS."""
+    from . import rk
+
+    return rk.explicit_step(lambda y: rhs(s, *apply_bc(s, *y)), (h, u, v), dt)
+
+
 def run(s: SWMSetup, h, u, v, dt, nsteps):
     y = (h, u, v)
     for _ in range(nsteps):

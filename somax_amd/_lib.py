@@ -112,6 +112,11 @@ _SIGNATURES = {
         ctypes.c_int,
         [_vp, _vp, _vp, ctypes.c_int, ctypes.c_double, ctypes.POINTER(QGParams), _vp, _vp],
     ),
+    "somaxhip_qg_tsit5_workspace_bytes": (ctypes.c_size_t, [_vp]),
+    "somaxhip_qg_step_tsit5": (
+        ctypes.c_int,
+        [_vp, _vp, ctypes.c_int, ctypes.c_double, ctypes.POINTER(QGParams), _vp, _vp, _vp],
+    ),
     "somaxhip_zero_ghost": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _vp]),
     "somaxhip_qg_diagnose": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "somaxhip_qg_step_profile": (
@@ -126,6 +131,11 @@ _SIGNATURES = {
     "somaxhip_swm_apply_bc": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "somaxhip_swm_diagnose": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "somaxhip_swm_step": (
+        ctypes.c_int,
+        [_vp, _vp, _vp, _vp, ctypes.c_int, ctypes.c_double, ctypes.POINTER(SWMParams), _vp, _vp],
+    ),
+    "somaxhip_swm_tsit5_workspace_bytes": (ctypes.c_size_t, [_vp]),
+    "somaxhip_swm_step_tsit5": (
         ctypes.c_int,
         [_vp, _vp, _vp, _vp, ctypes.c_int, ctypes.c_double, ctypes.POINTER(SWMParams), _vp, _vp],
     ),

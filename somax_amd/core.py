@@ -213,6 +213,18 @@ class Solution:
     stats: dict = field(default_factory=dict)
 
 
+def solver_name(solver) -> str:
+    """``"tsit5"`` or ``"ssprk3"`` from a solver name or a solver object
+    (``diffrax.Tsit5()`` -> ``"tsit5"``): the schemes the HIP path runs."""
+    name = solver if isinstance(solver, str) else type(solver).__name__
+    key = name.lower().replace("-", "").replace("_", "")
+    if key == "tsit5":
+        return "tsit5"
+    if key in ("ssprk3", "ssprk33"):
+        return "ssprk3"
+    raise NotImplementedError(f"solver {name!r}: the HIP path implements Tsit5 and SSP-RK3")
+
+
 class SomaxModel(abc.ABC):
     """Abstract model contract (model.py:12-95)."""
 

@@ -863,6 +863,41 @@ void rk3(const Batch& p, float* q, float* psi_last, int nsteps, double dt, const
   }
 }
 
+// Tsit5 (FSAL): k0 = f(y) once, then per step six more evaluations
+// k_i = f(y + dt sum_j a_ij k_j), i = 1..6, the last at the new state
+// y + dt sum_j b_j k_j, which becomes k0 of the next step.  f = the RHS with
+// the boundary condition applied first (SomaxModel.build_terms).
+// tws: Y (stage state) then K0..K6, each P planes.
+void tsit5(const Batch& p, float* q, int nsteps, double dt, const CallParams& cp, void* ws, void* tws,
+           hipStream_t st) {
+  const long n = (long)p.P * p.splane;
+  float* psi = wsf(ws, p.off_PSI);
+  float* Y = static_cast<float*>(tws);
+  float* K[7];
+  for (int i = 0; i < 7; ++i) K[i] = Y + (i + 1) * n;
+  auto f = [&](const float* y, float* k) {
+    invert(p, y, psi, ws, st);
+    stage(p, y, psi, nullptr, k, cp, 0.0f, 1.0f, true, ws, st);
+  };
+  if (nsteps > 0) f(q, K[0]);
+  for (int s = 0; s < nsteps; ++s) {
+    for (int i = 1; i <= 6; ++i) {
+      LinComb c{};
+      c.y = q;
+      c.nk = i;
+      for (int j = 0; j < i; ++j) {
+        c.k[j] = K[j];
+        c.w[j] = dt * TSIT5_A[i][j];
+      }
+      c.n = n;
+      c.out = i < 6 ? Y : q;  // row 6 = b: the new state, in place
+      launch_lincomb(c, st);
+      f(c.out, K[i]);
+    }
+    std::swap(K[0], K[6]);  // first same as last
+  }
+}
+
 CallParams uniform_params(const somaxhip_qg_params* prm) {
   return CallParams{nullptr, to_member_params(*prm), prm->nu4 != 0.0 ? 1 : 0};
 }
@@ -942,6 +977,24 @@ int somaxhip_qg_step(somaxhip_qg_plan p, float* q, float* psi_out, int nsteps, d
   }
   hipStream_t st = reinterpret_cast<hipStream_t>(stream);
   rk3(p->b, q, psi_out, nsteps, dt, uniform_params(prm), ws, st);
+  SOMAX_HIP_CHECK(hipGetLastError());
+  return SOMAXHIP_OK;
+}
+
+size_t somaxhip_qg_tsit5_workspace_bytes(somaxhip_qg_plan p) {
+  return p ? (size_t)8 * p->b.P * p->b.splane * sizeof(float) : 0;
+}
+
+int somaxhip_qg_step_tsit5(somaxhip_qg_plan p, float* q, int nsteps, double dt, const somaxhip_qg_params* prm,
+                           void* ws, void* tws, somaxhip_stream_t stream) {
+  int rc = check_call(p, q, q, ws, "somaxhip_qg_step_tsit5");
+  if (rc) return rc;
+  if (!prm || !tws || nsteps < 0) {
+    set_error("somaxhip_qg_step_tsit5: null params or Tsit5 workspace, or negative nsteps");
+    return SOMAXHIP_ERR_ARG;
+  }
+  hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+  tsit5(p->b, q, nsteps, dt, uniform_params(prm), ws, tws, st);
   SOMAX_HIP_CHECK(hipGetLastError());
   return SOMAXHIP_OK;
 }

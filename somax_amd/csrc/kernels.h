@@ -213,6 +213,34 @@ void launch_qg_stage(const StageArgs& a, hipStream_t st);
 // Ghost ring of nl planes of Ny x Nx set to 0 (finitevolx.zero_boundaries).
 void launch_zero_ghost(float* f, int nl, int Ny, int Nx, hipStream_t st);
 
+// Runge-Kutta stage state out = y + sum_{j < nk} w[j] * k[j], elementwise over
+// n floats, accumulated in FP64 (out may alias y).
+constexpr int LINCOMB_MAX = 7;
+struct LinComb {
+  const float* y;
+  const float* k[LINCOMB_MAX];
+  double w[LINCOMB_MAX];
+  int nk;
+  float* out;
+  long n;
+};
+void launch_lincomb(const LinComb& a, hipStream_t st);
+
+// Tsitouras' 5(4) pair (Ch. Tsitouras, "Runge-Kutta pairs of order 5(4)
+// satisfying only the first column simplifying assumption", Computers &
+// Mathematics with Applications 62 (2011) 770-775), the default solver of
+// SomaxModel.integrate (diffrax.Tsit5).  Row i of TSIT5_A holds a[i][0..i-1]
+// of stage i (stage 0 is the step start); row 6 is the solution weights b
+// (first same as last: stage 6 evaluates the new state).
+constexpr double TSIT5_A[7][6] = {
+    {0, 0, 0, 0, 0, 0},
+    {0.161, 0, 0, 0, 0, 0},
+    {-0.008480655492356989, 0.335480655492357, 0, 0, 0, 0},
+    {2.897153057105493, -6.359448489975075, 4.3622954328695815, 0, 0, 0},
+    {5.325864828439257, -11.748883564062828, 7.4955393428898365, -0.09249506636175525, 0, 0},
+    {5.86145544294642, -12.92096931784711, 8.159367898576159, -0.071584973281401, -0.028269050394068383, 0},
+    {0.09646076681806523, 0.01, 0.4798896504144996, 1.379008574103742, -3.290069515436081, 2.324710524099774}};
+
 }  // namespace somax
 
 namespace somax {

@@ -815,3 +815,27 @@ void launch_mass_alpha(const double* sums, const double* hmean, Tables t, int P,
 }
 
 }  // namespace somax
+
+namespace somax {
+
+// ---------------------------------------------------------------------------
+// Runge-Kutta stage combination (Tsit5): out = y + sum_j w_j k_j in FP64
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void lincomb_kernel(LinComb a) {
+  const long stride = (long)gridDim.x * blockDim.x;
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < a.n; i += stride) {
+    double v = (double)a.y[i];
+#pragma unroll
+    for (int j = 0; j < LINCOMB_MAX; ++j)
+      if (j < a.nk) v += a.w[j] * (double)a.k[j][i];
+    a.out[i] = (float)v;
+  }
+}
+
+void launch_lincomb(const LinComb& a, hipStream_t st) {
+  long blocks = (a.n + 255) / 256;
+  if (blocks > 8192) blocks = 8192;
+  hipLaunchKernelGGL(lincomb_kernel, dim3((unsigned)blocks), dim3(256), 0, st, a);
+}
+
+}  // namespace somax

@@ -21,6 +21,7 @@
 #include <cstring>
 #include <new>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "../../include/somaxhip.h"
@@ -593,6 +594,63 @@ int somaxhip_swm_step(somaxhip_swm_plan p, float* h, float* u, float* v, int nst
   }
   hipLaunchKernelGGL(swm_shift_kernel, sg, dim3(256), 0, st, e0, p->plane, sh, 1.0f);
   SOMAX_HIP_CHECK(hipMemcpyAsync(h, e0, n * sizeof(float), hipMemcpyDeviceToDevice, st));
+  SOMAX_HIP_CHECK(hipGetLastError());
+  return SOMAXHIP_OK;
+}
+
+size_t somaxhip_swm_tsit5_workspace_bytes(somaxhip_swm_plan p) {
+  return p ? (size_t)9 * 3 * p->nl * p->plane * sizeof(float) : 0;
+}
+
+// Tsit5 (FSAL) on (h, u, v), as somaxhip_qg_step_tsit5; thickness carried as
+// its deviation from the rest thickness.  tws: Y0 (step start), Y (stage
+// state), K0..K6, each the three fields back to back.
+int somaxhip_swm_step_tsit5(somaxhip_swm_plan p, float* h, float* u, float* v, int nsteps, double dt,
+                            const somaxhip_swm_params* prm, void* tws, somaxhip_stream_t stream) {
+  int rc = check_plan(p, "somaxhip_swm_step_tsit5");
+  if (rc) return rc;
+  if (!h || !u || !v || !prm || !tws || nsteps < 0) {
+    set_error("somaxhip_swm_step_tsit5: null field, params or Tsit5 workspace, or negative nsteps");
+    return SOMAXHIP_ERR_ARG;
+  }
+  if (nsteps == 0) return SOMAXHIP_OK;
+  hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+  const long n = (long)p->nl * p->plane;
+  float* Y0 = static_cast<float*>(tws);
+  float* Y = Y0 + 3 * n;
+  float* K[7];
+  for (int i = 0; i < 7; ++i) K[i] = Y + (i + 1) * 3 * n;
+  SOMAX_HIP_CHECK(hipMemcpyAsync(Y0, h, n * sizeof(float), hipMemcpyDeviceToDevice, st));
+  SOMAX_HIP_CHECK(hipMemcpyAsync(Y0 + n, u, n * sizeof(float), hipMemcpyDeviceToDevice, st));
+  SOMAX_HIP_CHECK(hipMemcpyAsync(Y0 + 2 * n, v, n * sizeof(float), hipMemcpyDeviceToDevice, st));
+  SwmArgs sh{};
+  for (int k = 0; k < p->nl; ++k) sh.hoff[k] = p->Hk[k];
+  const dim3 sg((unsigned)((p->plane + 255) / 256), p->nl, 1);
+  hipLaunchKernelGGL(swm_shift_kernel, sg, dim3(256), 0, st, Y0, p->plane, sh, -1.0f);
+  auto f = [&](const float* y, float* k) {
+    swm_stage(p, y, y + n, y + 2 * n, nullptr, nullptr, nullptr, k, k + n, k + 2 * n, prm, 0.0f, 1.0f, 1, true, st);
+  };
+  f(Y0, K[0]);
+  for (int s = 0; s < nsteps; ++s) {
+    for (int i = 1; i <= 6; ++i) {
+      LinComb c{};
+      c.y = Y0;
+      c.nk = i;
+      for (int j = 0; j < i; ++j) {
+        c.k[j] = K[j];
+        c.w[j] = dt * TSIT5_A[i][j];
+      }
+      c.n = 3 * n;
+      c.out = i < 6 ? Y : Y0;
+      launch_lincomb(c, st);
+      f(c.out, K[i]);
+    }
+    std::swap(K[0], K[6]);
+  }
+  hipLaunchKernelGGL(swm_shift_kernel, sg, dim3(256), 0, st, Y0, p->plane, sh, 1.0f);
+  SOMAX_HIP_CHECK(hipMemcpyAsync(h, Y0, n * sizeof(float), hipMemcpyDeviceToDevice, st));
+  SOMAX_HIP_CHECK(hipMemcpyAsync(u, Y0 + n, n * sizeof(float), hipMemcpyDeviceToDevice, st));
+  SOMAX_HIP_CHECK(hipMemcpyAsync(v, Y0 + 2 * n, n * sizeof(float), hipMemcpyDeviceToDevice, st));
   SOMAX_HIP_CHECK(hipGetLastError());
   return SOMAXHIP_OK;
 }

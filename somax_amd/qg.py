@@ -31,6 +31,7 @@ from .core import (
     SomaxModel,
     State,
     StratificationProfile,
+    solver_name,
 )
 
 _ADV = {"arakawa": _lib.ADV_ARAKAWA, "weno": _lib.ADV_WENO}
@@ -174,6 +175,7 @@ class _QGBase(SomaxModel):
         self.device = device
         self._plan = None
         self._ws = None
+        self._tws = None
         self._homsol = None
         self._homsol_mean = None
         if self.mass_correction:
@@ -286,6 +288,18 @@ class _QGBase(SomaxModel):
             "somaxhip_qg_step",
         )
 
+    def _step_tsit5_inplace(self, q, dt: float, nsteps: int):
+        plan = self._ensure_plan()
+        if self._tws is None:
+            nbytes = _lib.load().somaxhip_qg_tsit5_workspace_bytes(plan)
+            self._tws = _torch().empty(max(int(nbytes), 256), dtype=_torch().uint8, device=self.device)
+        prm = self._c_params()
+        _lib.check(
+            _lib.load().somaxhip_qg_step_tsit5(plan, q.data_ptr(), int(nsteps), float(dt), ctypes.byref(prm),
+                                              self._ws.data_ptr(), self._tws.data_ptr(), self._stream()),
+            "somaxhip_qg_step_tsit5",
+        )
+
     def _zero_ghost(self, q):
         q = self._field(q).clone()
         _lib.check(
@@ -295,10 +309,14 @@ class _QGBase(SomaxModel):
         return q
 
     # -- time stepping (SomaxModel.integrate, model.py:53-88) ----------------
-    def step(self, state, dt: float, nsteps: int = 1):
-        """``nsteps`` SSP-RK3 steps of ``dt`` (BC applied before each RHS); returns a new state."""
+    def step(self, state, dt: float, nsteps: int = 1, solver="ssprk3"):
+        """``nsteps`` steps of ``dt`` (BC applied before each RHS) with SSP-RK3
+        (the benchmark scheme) or Tsit5; returns a new state."""
         q = self._field(state.q).clone()
-        self._step_inplace(q, dt, nsteps)
+        if solver_name(solver) == "tsit5":
+            self._step_tsit5_inplace(q, dt, nsteps)
+        else:
+            self._step_inplace(q, dt, nsteps)
         return self._state_cls(q=q.reshape(state.q.shape))
 
     def integrate(self, state0, t0: float, t1: float, dt: float, **kw) -> Solution:
@@ -306,14 +324,12 @@ class _QGBase(SomaxModel):
 
         Like the reference (model.py:53-88) the boundary condition is applied
         to ``state0`` first and before every RHS; the last step is shortened
-        to land on ``t1``.  ``solver`` selects the scheme: ``"ssprk3"`` (the
-        scheme this library implements natively).  ``saveat`` may be ``None``
-        (save ``t1`` only) or a sequence of save times on step boundaries.
+        to land on ``t1``.  ``solver``: Tsit5 by default, as the reference
+        (``"tsit5"`` or a ``Tsit5`` object), or ``"ssprk3"``.  ``saveat`` may
+        be ``None`` (save ``t1`` only) or a sequence of save times.
         """
         torch = _torch()
-        solver = kw.pop("solver", "ssprk3")
-        if not (isinstance(solver, str) and solver.lower() in ("ssprk3", "ssp-rk3", "ssprk33")):
-            raise NotImplementedError(f"solver {solver!r}: the HIP path implements SSP-RK3 only")
+        advance = self._step_tsit5_inplace if solver_name(kw.pop("solver", "tsit5")) == "tsit5" else self._step_inplace
         saveat = kw.pop("saveat", None)
         if kw:
             raise TypeError(f"unexpected keyword arguments {sorted(kw)}")
@@ -325,11 +341,11 @@ class _QGBase(SomaxModel):
             span = ts_k - t
             n = int(math.floor(span / dt + 1e-9))
             if n > 0:
-                self._step_inplace(q, dt, n)
+                advance(q, dt, n)
                 t += n * dt
             rem = ts_k - t
             if rem > 1e-12 * max(1.0, abs(ts_k)):
-                self._step_inplace(q, rem, 1)
+                advance(q, rem, 1)
                 t = ts_k
             out.append(q.clone().reshape(state0.q.shape))
         ys = self._state_cls(q=torch.stack(out))

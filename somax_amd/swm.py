@@ -26,6 +26,7 @@ from .core import (
     SomaxModel,
     State,
     StratificationProfile,
+    solver_name,
 )
 
 _BC = {"periodic": _lib.BC_PERIODIC, "wall": _lib.BC_WALL, "channel": _lib.BC_CHANNEL}
@@ -100,6 +101,7 @@ class MultilayerShallowWater2D(SomaxModel):
         self._nl = strat.nl
         self._plan = None
         self._ws = None
+        self._tws = None
 
     # -- reference fields (host arrays) ---------------------------------------
     @property
@@ -190,6 +192,18 @@ class MultilayerShallowWater2D(SomaxModel):
             "somaxhip_swm_step",
         )
 
+    def _step_tsit5_inplace(self, h, u, v, dt: float, nsteps: int):
+        plan = self._ensure_plan()
+        if self._tws is None:
+            nbytes = _lib.load().somaxhip_swm_tsit5_workspace_bytes(plan)
+            self._tws = _torch().empty(max(int(nbytes), 256), dtype=_torch().uint8, device=self.device)
+        prm = self._c_params()
+        _lib.check(
+            _lib.load().somaxhip_swm_step_tsit5(plan, h.data_ptr(), u.data_ptr(), v.data_ptr(), int(nsteps),
+                                               float(dt), ctypes.byref(prm), self._tws.data_ptr(), self._stream()),
+            "somaxhip_swm_step_tsit5",
+        )
+
     # -- SomaxModel -----------------------------------------------------------
     def vector_field(self, t: float, state, args=None):
         """Tendencies on the state as given (multilayer.py:150-201)."""
@@ -214,19 +228,22 @@ class MultilayerShallowWater2D(SomaxModel):
         )
         return MultilayerSW2DState(h=ho, u=uo, v=vo)
 
-    def step(self, state, dt: float, nsteps: int = 1):
-        """``nsteps`` SSP-RK3 steps (BC before every RHS); returns a new state."""
+    def step(self, state, dt: float, nsteps: int = 1, solver="ssprk3"):
+        """``nsteps`` steps (BC before every RHS) with SSP-RK3 (the benchmark
+        scheme) or Tsit5; returns a new state."""
         h, u, v = (f.clone() for f in self._fields(state))
-        self._step_inplace(h, u, v, dt, nsteps)
+        if solver_name(solver) == "tsit5":
+            self._step_tsit5_inplace(h, u, v, dt, nsteps)
+        else:
+            self._step_inplace(h, u, v, dt, nsteps)
         return MultilayerSW2DState(h=h, u=u, v=v)
 
     def integrate(self, state0, t0: float, t1: float, dt: float, **kw) -> Solution:
-        """Constant-step SSP-RK3 integration from ``t0`` to ``t1`` (the last
-        step shortened to land on ``t1``); ``saveat`` as for the QG models."""
+        """Constant-step integration from ``t0`` to ``t1`` (the last step
+        shortened to land on ``t1``); ``solver`` (Tsit5 by default, as the
+        reference, or ``"ssprk3"``) and ``saveat`` as for the QG models."""
         torch = _torch()
-        solver = kw.pop("solver", "ssprk3")
-        if not (isinstance(solver, str) and solver.lower() in ("ssprk3", "ssp-rk3", "ssprk33")):
-            raise NotImplementedError(f"solver {solver!r}: the HIP path implements SSP-RK3 only")
+        advance = self._step_tsit5_inplace if solver_name(kw.pop("solver", "tsit5")) == "tsit5" else self._step_inplace
         saveat = kw.pop("saveat", None)
         if kw:
             raise TypeError(f"unexpected keyword arguments {sorted(kw)}")
@@ -237,11 +254,11 @@ class MultilayerShallowWater2D(SomaxModel):
         for ts_k in ts:
             n = int(math.floor((ts_k - t) / dt + 1e-9))
             if n > 0:
-                self._step_inplace(h, u, v, dt, n)
+                advance(h, u, v, dt, n)
                 t += n * dt
             rem = ts_k - t
             if rem > 1e-12 * max(1.0, abs(ts_k)):
-                self._step_inplace(h, u, v, rem, 1)
+                advance(h, u, v, rem, 1)
                 t = ts_k
             out.append((h.clone(), u.clone(), v.clone()))
         ys = MultilayerSW2DState(h=torch.stack([o[0] for o in out]), u=torch.stack([o[1] for o in out]),

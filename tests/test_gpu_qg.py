@@ -151,19 +151,48 @@ def test_hundred_steps_match_oracle(gpu, precision):
     assert rq < 1e-4 and rp < 1e-4
 
 
-def test_integrate_matches_steps(gpu):
-    """integrate() applies the BC to state0, steps, and shortens the last step."""
-    _log("[integrate] t1 not a multiple of dt")
+@pytest.mark.parametrize("solver", [None, "ssprk3"])
+def test_integrate_matches_steps(gpu, solver):
+    """integrate() applies the BC to state0, steps, and shortens the last
+    step; its default solver is Tsit5, as the reference's (model.py:75)."""
+    _log(f"[integrate] t1 not a multiple of dt, solver={solver or 'default'}")
     from somax_amd import BaroclinicQGState
 
     model, setup = _pair(64)
     q, qd = _state(model, seed=8, ghost_noise=True)
-    sol = model.integrate(BaroclinicQGState(q=qd), 0.0, 2.5 * 1200.0, 1200.0)
+    kw = {} if solver is None else {"solver": solver}
+    sol = model.integrate(BaroclinicQGState(q=qd), 0.0, 2.5 * 1200.0, 1200.0, **kw)
+    one = oqg.ssprk3_step if solver == "ssprk3" else oqg.tsit5_step
     ref = oqg.apply_bc(q.astype(np.float64))
-    ref = oqg.run(setup, ref, 1200.0, 2)
-    ref = oqg.ssprk3_step(setup, ref, 600.0)
+    for dt in (1200.0, 1200.0, 600.0):
+        ref = one(setup, ref, dt)
     assert sol.ys.q.shape == (1,) + q.shape
-    assert rel(sol.ys.q[0].cpu().numpy(), ref) < 1e-6
+    r = rel(sol.ys.q[0].cpu().numpy(), ref)
+    _log(f"  rel-L2 q = {r:.3e}")
+    assert r < 1e-6
+
+
+@pytest.mark.parametrize("precision", ["fp32", "mixed"])
+def test_tsit5_matches_oracle(gpu, precision):
+    """Tsit5 (the reference's default solver): one step to 1e-6 and 100
+    steps to 1e-4 relative L2 against the FP64 oracle, 3-layer 128^2."""
+    import torch
+
+    _log(f"[tsit5] n=128 precision={precision}")
+    model, setup = _pair(128, precision=precision)
+    q, qd = _state(model, seed=71)
+    q1 = qd.clone()
+    model._step_tsit5_inplace(q1, 1200.0, 1)
+    r1 = rel(q1.cpu().numpy(), oqg.tsit5_step(setup, q.astype(np.float64), 1200.0))
+    q100 = qd.clone()
+    model._step_tsit5_inplace(q100, 1200.0, 100)
+    torch.cuda.synchronize()
+    ref = q.astype(np.float64)
+    for _ in range(100):
+        ref = oqg.tsit5_step(setup, ref, 1200.0)
+    r100 = rel(q100.cpu().numpy(), ref)
+    _log(f"  one step {r1:.3e}, 100 steps {r100:.3e}")
+    assert r1 < 1e-6 and r100 < 1e-4
 
 
 @pytest.mark.parametrize("n,precision", [(64, "fp32"), (128, "mixed")])
@@ -250,7 +279,7 @@ def test_boundary_conditions(gpu):
 
 
 def test_wind_develops_flow(gpu):
-    """test_qg_baroclinic.py:202-223 with the HIP SSP-RK3 integrator."""
+    """test_qg_baroclinic.py:202-223 with integrate's default solver (Tsit5)."""
     import torch
 
     from somax_amd import BaroclinicQG, BaroclinicQGState

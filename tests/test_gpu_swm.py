@@ -95,6 +95,21 @@ def test_one_step_matches_oracle(gpu, n, bc):
     assert max(rs) < 1e-6
 
 
+@pytest.mark.parametrize("bc", ["channel", "wall", "periodic"])
+def test_tsit5_step_matches_oracle(gpu, bc):
+    """Tsit5 (integrate's default solver, core/model.py:75): one step to 1e-6."""
+    from somax_amd import testcases as tc
+
+    _log(f"[swm] Tsit5 one step n=128 {bc}")
+    model, s, f32 = _jet(128, bc, **KW)
+    dt = tc.jet_dt(model)
+    out = model.step(_state(f32), dt, 1, solver="tsit5")
+    ref = oswm.tsit5_step(s, *[a.astype(np.float64) for a in f32], dt)
+    rs = [rel(a.cpu().numpy(), b) for a, b in zip((out.h, out.u, out.v), ref)]
+    _log(f"  rel-L2 h {rs[0]:.3e} u {rs[1]:.3e} v {rs[2]:.3e}")
+    assert max(rs) < 1e-6
+
+
 def test_hundred_steps_match_oracle(gpu):
     from somax_amd import testcases as tc
 

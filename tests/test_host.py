@@ -113,3 +113,16 @@ def test_baroclinic_jet_balance():
     np.testing.assert_allclose(tc.JET_F0 * u[0, :, 5], -9.81 * dh0, atol=1e-3 * tc.JET_F0 * 0.5)
     np.testing.assert_allclose(9.81 * dh0 + 0.02 * dh1, 0.0, atol=1e-12)
     assert h.min() > 0 and 20.0 < tc.jet_dt(model) < 500.0
+
+
+def test_solver_names():
+    """integrate's solver argument: names or solver objects (by class name)."""
+    from somax_amd.core import solver_name
+
+    class Tsit5:  # stands for diffrax.Tsit5, which is not installed here
+        pass
+
+    assert solver_name("tsit5") == "tsit5" and solver_name(Tsit5()) == "tsit5"
+    assert solver_name("SSP-RK3") == "ssprk3" and solver_name("ssprk3") == "ssprk3"
+    with pytest.raises(NotImplementedError):
+        solver_name("dopri5")

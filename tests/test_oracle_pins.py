@@ -380,3 +380,45 @@ def test_qg_diagnose_closed_form():
     # reflection symmetry of the summed energy
     d2 = oqg.diagnose(s, q[::-1, ::-1].copy())
     np.testing.assert_allclose(d2["kinetic_energy"], d["kinetic_energy"], rtol=1e-9)
+
+
+# ---------------------------------------------------------------------------
+# Tsit5 (oracle/rk.py): order conditions and convergence
+# ---------------------------------------------------------------------------
+def test_tsit5_order_conditions():
+    """The 17 Butcher order conditions of orders 1-5 (one per rooted tree) and
+    c_i = sum_j a_ij: a wrong digit in any coefficient breaks one of them."""
+    from oracle.rk import TSIT5_A as A, TSIT5_B as b, TSIT5_C as c
+
+    np.testing.assert_allclose(A.sum(axis=1), c, atol=1e-14)
+    Ac = A @ c
+    Ac2, Ac3 = A @ c**2, A @ c**3
+    AAc = A @ Ac
+    conds = [
+        (b.sum(), 1.0),
+        (b @ c, 1 / 2),
+        (b @ c**2, 1 / 3), (b @ Ac, 1 / 6),
+        (b @ c**3, 1 / 4), (b @ (c * Ac), 1 / 8), (b @ Ac2, 1 / 12), (b @ AAc, 1 / 24),
+        (b @ c**4, 1 / 5), (b @ (c**2 * Ac), 1 / 10), (b @ (c * Ac2), 1 / 15), (b @ (c * AAc), 1 / 30),
+        (b @ Ac**2, 1 / 20), (b @ Ac3, 1 / 20), (b @ (A @ (c * Ac)), 1 / 40), (b @ (A @ Ac2), 1 / 60),
+        (b @ (A @ AAc), 1 / 120),
+    ]
+    assert len(conds) == 17
+    for k, (got, want) in enumerate(conds):
+        assert abs(got - want) < 1e-12, (k, got, want)
+
+
+def test_tsit5_fifth_order_convergence():
+    """Global error at t = 1 of y' = lambda y (exact exp(lambda t)) falls by
+    ~2^5 per halving of dt."""
+    from oracle import rk
+
+    lam = -1.3 + 0.7j
+    errs = []
+    for n in (10, 20, 40):
+        y = (np.array([1.0 + 0j]),)
+        for _ in range(n):
+            y = rk.explicit_step(lambda v: (lam * v[0],), y, 1.0 / n)
+        errs.append(abs(y[0][0] - np.exp(lam)))
+    rates = [np.log2(errs[i] / errs[i + 1]) for i in range(2)]
+    assert all(4.7 < r < 5.6 for r in rates), rates
