@@ -18,9 +18,10 @@
 // layouts.  With acc64 the FP32 MFMA partial of every few k-tiles is added to
 // an FP64 running sum, so the FP32 rounding chain per output stays short.
 //
-// Block: 256 threads = 4 waves (2x2), block tile 128x128, wave tile 64x64 =
-// 2x2 MFMA tiles of 32x32.  Grids are 1-D and remapped so that each XCD
-// works on a contiguous run of tiles (shared operand rows in its L2).
+// Block tile 128x128 of 32x32 MFMA tiles: 8 waves (2x4) of 64x32 for the x1
+// and x3 instances, 4 waves (2x2) of 64x64 for x6 and FP32.  Grids are 1-D
+// and remapped so that each XCD works on a contiguous run of tiles (shared
+// operand rows in its L2).
 
 #include <type_traits>
 
@@ -37,11 +38,14 @@ constexpr int BK32 = 16;          // fp32 k-tile
 constexpr int LD32 = BK32 + 4;    // padded LDS row (80 bytes)
 constexpr int TILE32 = BM * LD32;
 
-__device__ inline void zero_acc(floatx16 (&acc)[2][2], double (&sum)[2][2][16]) {
+// TN: 32-column MFMA tiles per wave along N.  TN = 2: 4 waves of 64 x 64
+// (2 x 2 waves); TN = 1: 8 waves of 64 x 32 (2 x 4 waves, two per SIMD).
+template <int TN>
+__device__ inline void zero_acc(floatx16 (&acc)[2][TN], double (&sum)[2][TN][16]) {
 #pragma unroll
   for (int i = 0; i < 2; ++i)
 #pragma unroll
-    for (int j = 0; j < 2; ++j)
+    for (int j = 0; j < TN; ++j)
 #pragma unroll
       for (int r = 0; r < 16; ++r) {
         acc[i][j][r] = 0.0f;
@@ -77,13 +81,14 @@ struct Spectral {
   float lamz;
 };
 
-template <bool SUM64>
-__device__ inline void store_tile(const GemmArgs& a, long z, int a0, int b0, const floatx16 (&acc)[2][2],
-                                  const double (&sum)[2][2][16], int opieces, float* stage, const Spectral& sp) {
+template <bool SUM64, int TN>
+__device__ inline void store_tile(const GemmArgs& a, long z, int a0, int b0, const floatx16 (&acc)[2][TN],
+                                  const double (&sum)[2][TN][16], int opieces, float* stage, const Spectral& sp) {
+  constexpr int WN = 4 / TN, NT = 128 * WN;  // waves along N, threads
   const int tid = threadIdx.x;
   const int lane = tid & 63;
   const int wave = tid >> 6;
-  const int wm = wave >> 1, wn = wave & 1;
+  const int wm = wave / WN, wn = wave % WN;
   const float lamz = sp.lamz;
 #pragma unroll
   for (int h = 0; h < 2; ++h) {
@@ -92,11 +97,11 @@ __device__ inline void store_tile(const GemmArgs& a, long z, int a0, int b0, con
 #pragma unroll
       for (int i = 0; i < 2; ++i)
 #pragma unroll
-        for (int j = 0; j < 2; ++j)
+        for (int j = 0; j < TN; ++j)
 #pragma unroll
           for (int r = 0; r < 16; ++r) {
             const int lr = i * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
-            const int lc = wn * 64 + j * 32 + (lane & 31);
+            const int lc = wn * 32 * TN + j * 32 + (lane & 31);
             stage[lr * EPI_PITCH + lc] = SUM64 ? (float)(sum[i][j][r] + (double)acc[i][j][r]) : acc[i][j][r];
           }
     }
@@ -104,7 +109,7 @@ __device__ inline void store_tile(const GemmArgs& a, long z, int a0, int b0, con
     const int r0 = a0 + h * 64;
     if (opieces == 0) {
       // FP32 (any pitch): 64 lanes on 64 consecutive columns
-      for (int e = tid; e < 64 * 128; e += 256) {
+      for (int e = tid; e < 64 * 128; e += NT) {
         const int lr = e >> 7, lc = e & 127;
         const int ro = r0 + lr, co = b0 + lc;
         if (ro >= a.RA || co >= a.RBpad) continue;
@@ -113,13 +118,13 @@ __device__ inline void store_tile(const GemmArgs& a, long z, int a0, int b0, con
         a.Of[z * a.o_batch + (long)ro * a.ldo + co] = co < a.RB ? v : 0.0f;
       }
     } else {
-      // bf16 pieces: 8 consecutive columns per thread, 16 rows per pass.
+      // bf16 pieces: 8 consecutive columns per thread, NT / 16 rows per pass.
       // RBpad is a multiple of 32 and b0 of 128, so a vector is all in or out.
       const int lc = (tid & 15) * 8;
       const int co = b0 + lc;
 #pragma unroll
-      for (int k = 0; k < 4; ++k) {
-        const int lr = (tid >> 4) + 16 * k;
+      for (int k = 0; k < 64 / (NT / 16); ++k) {
+        const int lr = (tid >> 4) + (NT / 16) * k;
         const int ro = r0 + lr;
         if (ro >= a.RA || co >= a.RBpad) continue;
         const floatx4 va = *reinterpret_cast<const floatx4*>(stage + lr * EPI_PITCH + lc);
@@ -216,9 +221,10 @@ __device__ inline void store_one(const GemmArgs& a, long z, int r, int c, float 
 // per lane group are in flight, and the loads of KB 64-element k-chunks are
 // issued before any is used.  Accumulation is FP64 for FP32 operands and
 // six-product planes, FP32 otherwise (the precision of those products).
-template <bool F32, int NPC>
+template <bool F32, int NPC, int NT>
 __device__ void gemm_fringe(const GemmArgs& a, long z, int part, const GemmCut& cut, const PlaneSel& ps, int opieces,
                             void* lds) {
+  constexpr int G = NT / 8;  // lane groups
   using acc_t = typename std::conditional<(F32 || NPC == 3), double, float>::type;
   constexpr int KB = 9;
   constexpr int NV = F32 ? 2 : NPC;  // 16-byte loads per variable-row chunk
@@ -276,7 +282,7 @@ __device__ void gemm_fringe(const GemmArgs& a, long z, int part, const GemmCut& 
     if (padw > 0 && nzr > 0) {
       const int zper = (nzr + fpw - 1) / fpw;
       const int r0 = part * zper, r1 = r0 + zper < nzr ? r0 + zper : nzr;
-      for (int e = r0 * 32 + tid; e < r1 * 32; e += 256) {
+      for (int e = r0 * 32 + tid; e < r1 * 32; e += NT) {
         const int rq = e >> 5;
         const int r = fb > 0 ? rq : cut.am + rq;
         for (int cq = e & 31; cq < padw; cq += 32) {
@@ -294,7 +300,7 @@ __device__ void gemm_fringe(const GemmArgs& a, long z, int part, const GemmCut& 
   const int K8 = (a.K + 7) & ~7;
   acc_t* fix = static_cast<acc_t*>(lds);
   const int nfix = fa + fb;
-  for (int e = tid; e < nfix * (K8 / 8); e += 256) {
+  for (int e = tid; e < nfix * (K8 / 8); e += NT) {
     const int i = e / (K8 / 8), k0 = (e - i * (K8 / 8)) * 8;
     long piece;
     const char* row = i < fa ? row_of(true, cut.am + i, piece) : row_of(false, cut.bm + i - fa, piece);
@@ -308,14 +314,14 @@ __device__ void gemm_fringe(const GemmArgs& a, long z, int part, const GemmCut& 
   const int per = (dots + fpw - 1) / fpw;
   const int d0 = part * per;
   const int d1 = d0 + per < dots ? d0 + per : dots;
-  for (int base = d0; base < d1; base += 64) {
+  for (int base = d0; base < d1; base += 2 * G) {
     const char* vrow[2];
     long vpiece[2];
     int fixi[2], rr[2], cc[2];
     bool live[2];
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
-      const int d = base + 32 * h + g;
+      const int d = base + G * h + g;
       live[h] = d < d1;
       int r = 0, c = 0, f = 0;
       bool var_is_b = true;
@@ -395,12 +401,12 @@ __device__ void gemm_fringe(const GemmArgs& a, long z, int part, const GemmCut& 
   }
 }
 
-template <int NPROD>
-__device__ inline void mfma_products(const bf16x8 (&fa)[3][2], const bf16x8 (&fb)[3][2], floatx16 (&acc)[2][2]) {
+template <int NPROD, int TN>
+__device__ inline void mfma_products(const bf16x8 (&fa)[3][2], const bf16x8 (&fb)[3][TN], floatx16 (&acc)[2][TN]) {
 #pragma unroll
   for (int i = 0; i < 2; ++i)
 #pragma unroll
-    for (int j = 0; j < 2; ++j) {
+    for (int j = 0; j < TN; ++j) {
       acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fa[0][i], fb[0][j], acc[i][j], 0, 0, 0);
       if (NPROD >= 3) {
         acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fa[0][i], fb[1][j], acc[i][j], 0, 0, 0);
@@ -426,13 +432,14 @@ __device__ inline void mfma_products(const bf16x8 (&fa)[3][2], const bf16x8 (&fb
 // ---------------------------------------------------------------------------
 __device__ inline int swz(int r) { return ((r >> 1) ^ (r >> 2)) & 3; }
 
-template <int NPROD>
+template <int NPROD, int TN>
 struct Bf16Cfg {
   static constexpr int NPC = NPROD == 1 ? 1 : (NPROD == 3 ? 2 : 3);
   static constexpr int NSTAGE = NPC == 3 ? 3 : 4;
   static constexpr int TILE = 128 * 32;         // bf16 elements per operand piece
   static constexpr int STAGE = 2 * NPC * TILE;  // A pieces then B pieces
-  static constexpr int LPT = 2 * NPC * 2;       // global_load_lds per thread per k-tile
+  static constexpr int RPT = TN;                // 16-row blocks each thread fills per operand piece
+  static constexpr int LPT = 2 * NPC * RPT;     // global_load_lds per thread per k-tile
   static constexpr int FLUSH = 2;               // k-tiles between FP64 flushes
 };
 
@@ -442,28 +449,30 @@ __device__ inline void wait_vmcnt() {
   __builtin_amdgcn_s_waitcnt((N & 0xF) | (0x7 << 4) | (0xF << 8) | ((N >> 4) << 14));
 }
 
-template <int NPROD, bool ACC64>
+template <int NPROD, bool ACC64, int TN>
 __device__ inline void bf16_mainloop(const GemmArgs& a, const uint16_t* S, uint16_t* lds, long z, int a0, int b0,
-                                     bool x_in_a, floatx16 (&acc)[2][2], double (&sum)[2][2][16]) {
-  using C = Bf16Cfg<NPROD>;
+                                     bool x_in_a, floatx16 (&acc)[2][TN], double (&sum)[2][TN][16]) {
+  using C = Bf16Cfg<NPROD, TN>;
   constexpr int NPC = C::NPC;
+  constexpr int RPT = C::RPT;
+  constexpr int WN = 4 / TN;
   const int tid = threadIdx.x;
   const int lane = tid & 63;
   const int wave = tid >> 6;
-  const int wm = wave >> 1, wn = wave & 1;
+  const int wm = wave / WN, wn = wave % WN;
   const uint16_t* X = a.Xb + z * a.x_batch;
   const int x0 = x_in_a ? a0 : b0;
   const int s0 = x_in_a ? b0 : a0;
   const int xslot = x_in_a ? 0 : NPC;  // piece-tile index of X inside a stage
   const int sslot = x_in_a ? NPC : 0;
 
-  // this lane's two rows of each 128-row tile and the swizzled source chunk
-  const uint16_t* xsrc[2];
-  const uint16_t* ssrc[2];
-  int ldst[2];
+  // this lane's RPT rows of each 128-row tile and the swizzled source chunk
+  const uint16_t* xsrc[RPT];
+  const uint16_t* ssrc[RPT];
+  int ldst[RPT];
 #pragma unroll
-  for (int j = 0; j < 2; ++j) {
-    const int r0 = (wave * 2 + j) * 16;  // first row filled by this wave instruction
+  for (int j = 0; j < RPT; ++j) {
+    const int r0 = (wave * RPT + j) * 16;  // first row filled by this wave instruction
     const int row = r0 + (lane >> 2);
     const int chunk = (lane & 3) ^ swz(row);
     const int xr = x0 + row < a.xrows ? x0 + row : a.xrows - 1;  // rows past R are never stored
@@ -477,7 +486,7 @@ __device__ inline void bf16_mainloop(const GemmArgs& a, const uint16_t* S, uint1
     uint16_t* sb_ = lds + (STG) * C::STAGE;                                                          \
     const int k0_ = (KT) * 32;                                                                       \
     _Pragma("unroll") for (int p = 0; p < NPC; ++p)                                                 \
-      _Pragma("unroll") for (int j = 0; j < 2; ++j) {                                                \
+      _Pragma("unroll") for (int j = 0; j < RPT; ++j) {                                              \
         __builtin_amdgcn_global_load_lds(xsrc[j] + p * a.x_piece + k0_, sb_ + (xslot + p) * C::TILE + ldst[j], 16, \
                                          0, 0);                                                      \
         __builtin_amdgcn_global_load_lds(ssrc[j] + p * a.s_piece + k0_, sb_ + (sslot + p) * C::TILE + ldst[j], 16, \
@@ -490,22 +499,23 @@ __device__ inline void bf16_mainloop(const GemmArgs& a, const uint16_t* S, uint1
   for (int s = 0; s < C::NSTAGE - 1; ++s)
     if (s < nk) SOMAX_ISSUE(s, s)
 
-  int ra[2], rb[2];
+  int ra[2], rb[TN];
 #pragma unroll
-  for (int t = 0; t < 2; ++t) {
-    ra[t] = wm * 64 + t * 32 + (lane & 31);
-    rb[t] = wn * 64 + t * 32 + (lane & 31);
-  }
+  for (int t = 0; t < 2; ++t) ra[t] = wm * 64 + t * 32 + (lane & 31);
+#pragma unroll
+  for (int t = 0; t < TN; ++t) rb[t] = wn * 32 * TN + t * 32 + (lane & 31);
   // fragments of k16 step kk of the tile in stage buffer sb
-  auto read_frags = [&](const uint16_t* sb, int kk, bf16x8(&fa)[3][2], bf16x8(&fb)[3][2]) {
+  auto read_frags = [&](const uint16_t* sb, int kk, bf16x8(&fa)[3][2], bf16x8(&fb)[3][TN]) {
     const int c = kk * 2 + (lane >> 5);  // logical 16-byte chunk of this lane's k-slice
 #pragma unroll
-    for (int p = 0; p < NPC; ++p)
+    for (int p = 0; p < NPC; ++p) {
 #pragma unroll
-      for (int t = 0; t < 2; ++t) {
+      for (int t = 0; t < 2; ++t)
         fa[p][t] = *reinterpret_cast<const bf16x8*>(sb + p * C::TILE + ra[t] * 32 + ((c ^ swz(ra[t])) << 3));
+#pragma unroll
+      for (int t = 0; t < TN; ++t)
         fb[p][t] = *reinterpret_cast<const bf16x8*>(sb + (NPC + p) * C::TILE + rb[t] * 32 + ((c ^ swz(rb[t])) << 3));
-      }
+    }
   };
   // Make tile kt current: its loads (this wave's) have landed once at most
   // min(NSTAGE-2, nk-1-kt) later tiles are outstanding; after the barrier
@@ -533,7 +543,7 @@ __device__ inline void bf16_mainloop(const GemmArgs& a, const uint16_t* S, uint1
   // consecutive tiles, so a read never targets registers that a queued MFMA
   // still has to consume (that write-after-read would hold the read back
   // until the MFMA starts and serialise LDS latency with MFMA execution).
-  bf16x8 fa0[3][2], fb0[3][2], fa1[3][2], fb1[3][2], fa2[3][2], fb2[3][2], fa3[3][2], fb3[3][2];
+  bf16x8 fa0[3][2], fb0[3][TN], fa1[3][2], fb1[3][TN], fa2[3][2], fb2[3][TN], fa3[3][2], fb3[3][TN];
   auto stage_of = [&](int kt) { return lds + (kt % C::NSTAGE) * C::STAGE; };
   auto lgkm0 = [] { __builtin_amdgcn_s_waitcnt((0xF) | (0x7 << 4) | (0x0 << 8) | (0x3 << 14)); };
   enter_tile(0);
@@ -546,16 +556,16 @@ __device__ inline void bf16_mainloop(const GemmArgs& a, const uint16_t* S, uint1
     lgkm0();                                                \
     read_frags(stage_of(KT), 1, YA, YB);                    \
     __builtin_amdgcn_sched_barrier(0);                      \
-    mfma_products<NPROD>(XA, XB, W);                        \
+    mfma_products<NPROD, TN>(XA, XB, W);                        \
     __builtin_amdgcn_sched_barrier(0);                      \
     enter_tile((KT) + 1);                                   \
     read_frags(stage_of((KT) + 1), 0, ZA, ZB);              \
     __builtin_amdgcn_sched_barrier(0);                      \
-    mfma_products<NPROD>(YA, YB, W);                        \
+    mfma_products<NPROD, TN>(YA, YB, W);                        \
     __builtin_amdgcn_sched_barrier(0);                      \
   }
   // two tiles, leaving the fragments of tile kt+2 step 0 in set 0
-  auto pair = [&](int kt, floatx16(&w)[2][2]) {
+  auto pair = [&](int kt, floatx16(&w)[2][TN]) {
     SOMAX_TILE_MID(kt, w, fa0, fb0, fa1, fb1, fa2, fb2)
     SOMAX_TILE_MID(kt + 1, w, fa2, fb2, fa3, fb3, fa0, fb0)
   };
@@ -566,16 +576,16 @@ __device__ inline void bf16_mainloop(const GemmArgs& a, const uint16_t* S, uint1
     // accumulator is carried around the loop (it stays in AGPRs).
     static_assert(C::FLUSH == 2, "one window per tile pair");
     for (; kt + 2 <= nk - 1; kt += 2) {
-      floatx16 w[2][2];
+      floatx16 w[2][TN];
 #pragma unroll
       for (int i = 0; i < 2; ++i)
 #pragma unroll
-        for (int j = 0; j < 2; ++j) w[i][j] = floatx16{};
+        for (int j = 0; j < TN; ++j) w[i][j] = floatx16{};
       pair(kt, w);
 #pragma unroll
       for (int i = 0; i < 2; ++i)
 #pragma unroll
-        for (int j = 0; j < 2; ++j)
+        for (int j = 0; j < TN; ++j)
 #pragma unroll
           for (int r = 0; r < 16; ++r) sum[i][j][r] += (double)w[i][j][r];
     }
@@ -587,13 +597,13 @@ __device__ inline void bf16_mainloop(const GemmArgs& a, const uint16_t* S, uint1
     SOMAX_TILE_MID(kt, acc, fa0, fb0, fa1, fb1, fa2, fb2)
     lgkm0();
     read_frags(stage_of(nk - 1), 1, fa3, fb3);
-    mfma_products<NPROD>(fa2, fb2, acc);
-    mfma_products<NPROD>(fa3, fb3, acc);
+    mfma_products<NPROD, TN>(fa2, fb2, acc);
+    mfma_products<NPROD, TN>(fa3, fb3, acc);
   } else {
     lgkm0();
     read_frags(stage_of(nk - 1), 1, fa1, fb1);
-    mfma_products<NPROD>(fa0, fb0, acc);
-    mfma_products<NPROD>(fa1, fb1, acc);
+    mfma_products<NPROD, TN>(fa0, fb0, acc);
+    mfma_products<NPROD, TN>(fa1, fb1, acc);
   }
 #undef SOMAX_TILE_MID
 #undef SOMAX_ISSUE
@@ -607,10 +617,23 @@ size_t dst_gemm_lds_bytes(int max_np) {
   return (size_t)nstage * 2 * npc * 128 * 32 * sizeof(uint16_t);
 }
 
+// Workgroup shape per instance: the x1 / x3 instances run 8 waves of 64 x 32
+// (TN = 1, two waves per SIMD, so one wave's MFMAs overlap the other's LDS
+// traffic: 23-33% faster than 4 waves of 64 x 64 at the same tile); the x6
+// instance keeps 4 waves (its three-piece fragments need the registers).
+template <int NPMAX>
+struct KernelShape {
+  static constexpr int TN = NPMAX == 6 ? 2 : 1;
+  static constexpr int NT = 128 * (4 / TN);
+};
+
 // One instantiation per largest product count of a launch (x1, x3, x6), so
 // each is a kernel symbol of its own; planes may use fewer products.
 template <int NPMAX>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) void dst_gemm_bf16_kernel(GemmArgs a) {
+__global__ __launch_bounds__(KernelShape<NPMAX>::NT)
+__attribute__((amdgpu_waves_per_eu(KernelShape<NPMAX>::TN == 1 ? 2 : 1, KernelShape<NPMAX>::TN == 1 ? 2 : 1)))
+void dst_gemm_bf16_kernel(GemmArgs a) {
+  constexpr int TN = KernelShape<NPMAX>::TN, NT = KernelShape<NPMAX>::NT;
   extern __shared__ __attribute__((aligned(16))) uint16_t lds16[];
   const GemmCut cut = gemm_cut(a.RA, a.RB, a.K);
   const int GA = (cut.am + BM - 1) / BM, GB = (cut.bm + BN - 1) / BN;
@@ -629,9 +652,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
   const int np = lo ? a.np_lo : a.np_hi;
   const int opieces = lo ? a.op_lo : a.op_hi;
   if (fringe) {
-    if (np == 1) gemm_fringe<false, 1>(a, z, part, cut, ps, opieces, lds16);
-    else if (NPMAX == 3 || np == 3) gemm_fringe<false, 2>(a, z, part, cut, ps, opieces, lds16);
-    else gemm_fringe<false, 3>(a, z, part, cut, ps, opieces, lds16);
+    if (np == 1) gemm_fringe<false, 1, NT>(a, z, part, cut, ps, opieces, lds16);
+    else if (NPMAX == 3 || np == 3) gemm_fringe<false, 2, NT>(a, z, part, cut, ps, opieces, lds16);
+    else gemm_fringe<false, 3, NT>(a, z, part, cut, ps, opieces, lds16);
     return;
   }
   const int a0 = ta * BM;
@@ -639,18 +662,18 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
   const bool x_in_a = !a.trans;
   const uint16_t* S = a.Sb + ps.s_off;
 
-  floatx16 acc[2][2];
-  double sum[2][2][16];
-  zero_acc(acc, sum);
+  floatx16 acc[2][TN];
+  double sum[2][TN][16];
+  zero_acc<TN>(acc, sum);
   // FP32 accumulation over K ~ 1e3 costs ~2e-7 relative, far below the
   // 3-product split error (~1e-5): only 6-product planes accumulate in FP64.
-  if (NPMAX == 1 || np == 1) bf16_mainloop<1, false>(a, S, lds16, z, a0, b0, x_in_a, acc, sum);
-  else if (NPMAX == 3 || np == 3) bf16_mainloop<3, false>(a, S, lds16, z, a0, b0, x_in_a, acc, sum);
-  else if (a.acc64) bf16_mainloop<6, true>(a, S, lds16, z, a0, b0, x_in_a, acc, sum);
-  else bf16_mainloop<6, false>(a, S, lds16, z, a0, b0, x_in_a, acc, sum);
+  if (NPMAX == 1 || np == 1) bf16_mainloop<1, false, TN>(a, S, lds16, z, a0, b0, x_in_a, acc, sum);
+  else if (NPMAX == 3 || np == 3) bf16_mainloop<3, false, TN>(a, S, lds16, z, a0, b0, x_in_a, acc, sum);
+  else if (a.acc64) bf16_mainloop<6, true, TN>(a, S, lds16, z, a0, b0, x_in_a, acc, sum);
+  else bf16_mainloop<6, false, TN>(a, S, lds16, z, a0, b0, x_in_a, acc, sum);
   float* stage = reinterpret_cast<float*>(lds16);
-  if (NPMAX == 6 && np == 6 && a.acc64) store_tile<true>(a, z, a0, b0, acc, sum, opieces, stage, ps.sp);
-  else store_tile<false>(a, z, a0, b0, acc, sum, opieces, stage, ps.sp);
+  if (NPMAX == 6 && np == 6 && a.acc64) store_tile<true, TN>(a, z, a0, b0, acc, sum, opieces, stage, ps.sp);
+  else store_tile<false, TN>(a, z, a0, b0, acc, sum, opieces, stage, ps.sp);
 }
 template __global__ void dst_gemm_bf16_kernel<1>(GemmArgs);
 template __global__ void dst_gemm_bf16_kernel<3>(GemmArgs);
@@ -667,7 +690,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 2))) voi
   const int T = GA * GB * a.nplanes;
   if ((int)blockIdx.x >= T) {
     const int zf = ((int)blockIdx.x - T) / cut.fpw;
-    gemm_fringe<true, 1>(a, zf, ((int)blockIdx.x - T) - zf * cut.fpw, cut, select_plane(a, zf), 0, lds);
+    gemm_fringe<true, 1, 256>(a, zf, ((int)blockIdx.x - T) - zf * cut.fpw, cut, select_plane(a, zf), 0, lds);
     return;
   }
   int zi, ta, tb;
@@ -699,7 +722,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 2))) voi
   float4 rx0, rx1, rs0, rs1;
   floatx16 acc[2][2];
   double sum[2][2][16];
-  zero_acc(acc, sum);
+  zero_acc<2>(acc, sum);
   const bool acc64 = a.acc64 != 0;
   const int nk = (a.K + BK32 - 1) / BK32;
   rx0 = *reinterpret_cast<const float4*>(xg0);
@@ -770,8 +793,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 2))) voi
     }
     for (; kt < nk; ++kt) ktile(kt, acc);
   }
-  if (acc64) store_tile<true>(a, z, a0, b0, acc, sum, 0, lds, ps.sp);
-  else store_tile<false>(a, z, a0, b0, acc, sum, 0, lds, ps.sp);
+  if (acc64) store_tile<true, 2>(a, z, a0, b0, acc, sum, 0, lds, ps.sp);
+  else store_tile<false, 2>(a, z, a0, b0, acc, sum, 0, lds, ps.sp);
 }
 
 hipError_t prepare_dst_gemm() {
@@ -803,9 +826,12 @@ void launch_dst_gemm(const GemmArgs& args, int batch, hipStream_t st) {
   }
   const int max_np = a.np_lo > a.np_hi ? a.np_lo : a.np_hi;
   const size_t lds = dst_gemm_lds_bytes(max_np);
-  if (max_np == 1) hipLaunchKernelGGL(dst_gemm_bf16_kernel<1>, dim3(blocks), dim3(256), lds, st, a);
-  else if (max_np == 3) hipLaunchKernelGGL(dst_gemm_bf16_kernel<3>, dim3(blocks), dim3(256), lds, st, a);
-  else hipLaunchKernelGGL(dst_gemm_bf16_kernel<6>, dim3(blocks), dim3(256), lds, st, a);
+  if (max_np == 1)
+    hipLaunchKernelGGL(dst_gemm_bf16_kernel<1>, dim3(blocks), dim3(KernelShape<1>::NT), lds, st, a);
+  else if (max_np == 3)
+    hipLaunchKernelGGL(dst_gemm_bf16_kernel<3>, dim3(blocks), dim3(KernelShape<3>::NT), lds, st, a);
+  else
+    hipLaunchKernelGGL(dst_gemm_bf16_kernel<6>, dim3(blocks), dim3(KernelShape<6>::NT), lds, st, a);
 }
 
 }  // namespace somax
