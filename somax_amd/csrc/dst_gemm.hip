@@ -18,8 +18,8 @@
 // layouts.  With acc64 the FP32 MFMA partial of every few k-tiles is added to
 // an FP64 running sum, so the FP32 rounding chain per output stays short.
 //
-// Block tile 128x128 of 32x32 MFMA tiles: 8 waves (2x4) of 64x32 for the x1
-// and x3 instances, 4 waves (2x2) of 64x64 for x6 and FP32.  Grids are 1-D
+// Block tile 128x128 of 32x32 MFMA tiles: 8 waves (2x4) of 64x32 for the
+// bf16 instances, 4 waves (2x2) of 64x64 for FP32.  Grids are 1-D
 // and remapped so that each XCD works on a contiguous run of tiles (shared
 // operand rows in its L2).
 
@@ -226,7 +226,7 @@ __device__ void gemm_fringe(const GemmArgs& a, long z, int part, const GemmCut& 
                             void* lds) {
   constexpr int G = NT / 8;  // lane groups
   using acc_t = typename std::conditional<(F32 || NPC == 3), double, float>::type;
-  constexpr int KB = 9;
+  constexpr int KB = NPC == 3 ? 4 : 9;  // k-chunks per batch of loads (register bound)
   constexpr int NV = F32 ? 2 : NPC;  // 16-byte loads per variable-row chunk
   const int fpw = cut.fpw;
   const int fa = a.RA - cut.am, fb = a.RB - cut.bm;
@@ -543,7 +543,18 @@ __device__ inline void bf16_mainloop(const GemmArgs& a, const uint16_t* S, uint1
   // consecutive tiles, so a read never targets registers that a queued MFMA
   // still has to consume (that write-after-read would hold the read back
   // until the MFMA starts and serialise LDS latency with MFMA execution).
-  bf16x8 fa0[3][2], fb0[3][TN], fa1[3][2], fb1[3][TN], fa2[3][2], fb2[3][TN], fa3[3][2], fb3[3][TN];
+  // (Two sets when two waves share a SIMD and three-piece fragments would not
+  // fit: the other wave covers the read-after-MFMA wait.)
+  constexpr int NS = (TN == 1 && NPC == 3) ? 2 : 4;
+  bf16x8 fsa[NS][3][2], fsb[NS][3][TN];
+  auto& fa0 = fsa[0];
+  auto& fb0 = fsb[0];
+  auto& fa1 = fsa[1];
+  auto& fb1 = fsb[1];
+  auto& fa2 = fsa[2 % NS];
+  auto& fb2 = fsb[2 % NS];
+  auto& fa3 = fsa[3 % NS];
+  auto& fb3 = fsb[3 % NS];
   auto stage_of = [&](int kt) { return lds + (kt % C::NSTAGE) * C::STAGE; };
   auto lgkm0 = [] { __builtin_amdgcn_s_waitcnt((0xF) | (0x7 << 4) | (0x0 << 8) | (0x3 << 14)); };
   enter_tile(0);
@@ -617,13 +628,12 @@ size_t dst_gemm_lds_bytes(int max_np) {
   return (size_t)nstage * 2 * npc * 128 * 32 * sizeof(uint16_t);
 }
 
-// Workgroup shape per instance: the x1 / x3 instances run 8 waves of 64 x 32
-// (TN = 1, two waves per SIMD, so one wave's MFMAs overlap the other's LDS
-// traffic: 23-33% faster than 4 waves of 64 x 64 at the same tile); the x6
-// instance keeps 4 waves (its three-piece fragments need the registers).
+// Workgroup shape per instance: 8 waves of 64 x 32 (TN = 1, two waves per
+// SIMD, so one wave's MFMAs overlap the other's LDS traffic: 23-33% faster
+// than 4 waves of 64 x 64 at the same tile).
 template <int NPMAX>
 struct KernelShape {
-  static constexpr int TN = NPMAX == 6 ? 2 : 1;
+  static constexpr int TN = 1;
   static constexpr int NT = 128 * (4 / TN);
 };
 
@@ -652,7 +662,7 @@ void dst_gemm_bf16_kernel(GemmArgs a) {
   const int np = lo ? a.np_lo : a.np_hi;
   const int opieces = lo ? a.op_lo : a.op_hi;
   if (fringe) {
-    if (np == 1) gemm_fringe<false, 1, NT>(a, z, part, cut, ps, opieces, lds16);
+    if (NPMAX == 1 || np == 1) gemm_fringe<false, 1, NT>(a, z, part, cut, ps, opieces, lds16);
     else if (NPMAX == 3 || np == 3) gemm_fringe<false, 2, NT>(a, z, part, cut, ps, opieces, lds16);
     else gemm_fringe<false, 3, NT>(a, z, part, cut, ps, opieces, lds16);
     return;
