@@ -427,11 +427,11 @@ __global__ __launch_bounds__(256) void residual_kernel(ResidualArgs a) {
 // zero exterior and the mirror rule at the fold line (the neighbour past the
 // last physical index h-1 is +-the value itself: even / odd parity).
 __global__ __launch_bounds__(256) void residual_fold_kernel(ResidualArgs a) {
-  const int kx = blockIdx.x * blockDim.x + threadIdx.x;  // up to op_kpad
-  const int ky = blockIdx.y;
-  const int z = blockIdx.z;
-  if (kx >= a.op_kpad) return;
   const int hx = a.Nx / 2, hy = a.Ny / 2;
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;  // row-major over hy x op_kpad
+  const int z = blockIdx.y;
+  if (idx >= hy * a.op_kpad) return;
+  const int ky = idx / a.op_kpad, kx = idx - ky * a.op_kpad;
   uint16_t* o = a.r + z * a.op_batch + (long)ky * a.op_ld + kx;
   if (kx >= hx) {
     store_pieces(o, a.op_piece, 2, 0.0f);
@@ -455,7 +455,7 @@ __global__ __launch_bounds__(256) void residual_fold_kernel(ResidualArgs a) {
 
 void launch_residual(const ResidualArgs& a, hipStream_t st) {
   if (a.fold) {
-    dim3 grid((a.op_kpad + 255) / 256, a.Ny / 2, 4 * a.P);
+    dim3 grid((unsigned)(((long)a.op_kpad * (a.Ny / 2) + 255) / 256), 4 * a.P);
     hipLaunchKernelGGL(residual_fold_kernel, grid, dim3(256), 0, st, a);
     return;
   }
@@ -509,10 +509,12 @@ __global__ __launch_bounds__(256) void to_modal_kernel(ModalArgs a) {
 // the four mirror points of every layer, projects them to modes in FP64 and
 // writes the four parity fold sums of each mode.
 __global__ __launch_bounds__(256) void to_modal_fold_kernel(ModalArgs a) {
-  const int kx = blockIdx.x * blockDim.x + threadIdx.x;
-  const int ky = blockIdx.y;
-  const int mem = blockIdx.z;
   const int hx = a.Nx / 2, hy = a.Ny / 2;
+  const int width = hx > a.op_kpad ? hx : a.op_kpad;
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;  // row-major over hy x width
+  const int mem = blockIdx.y;
+  if (e >= hy * width) return;
+  const int ky = e / width, kx = e - ky * width;
   const int K0 = a.apply_bc ? hx - 1 : hx;  // operand columns carrying data
   const int R0 = a.apply_bc ? hy - 1 : hy;  // operand rows
   const bool inside = kx < hx;
@@ -563,7 +565,7 @@ void launch_to_modal(const ModalArgs& a, hipStream_t st) {
   if (a.fold) {
     const int hx = a.Nx / 2;
     const int width = hx > a.op_kpad ? hx : a.op_kpad;
-    dim3 grid((width + 255) / 256, a.Ny / 2, a.M);
+    dim3 grid((unsigned)(((long)width * (a.Ny / 2) + 255) / 256), a.M);
     hipLaunchKernelGGL(to_modal_fold_kernel, grid, dim3(256), 0, st, a);
     return;
   }
@@ -610,11 +612,11 @@ __global__ __launch_bounds__(256) void to_layer_kernel(LayerArgs a) {
 // parity components of every mode at the four mirror points, adds the mass
 // correction and mixes modes into layers.
 __global__ __launch_bounds__(256) void to_layer_fold_kernel(LayerArgs a) {
-  const int kx = blockIdx.x * blockDim.x + threadIdx.x;
-  const int ky = blockIdx.y;
-  const int mem = blockIdx.z;
   const int hx = a.Nx / 2, hy = a.Ny / 2;
-  if (kx >= hx) return;
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;  // row-major over hy x hx
+  const int mem = blockIdx.y;
+  if (e >= hy * hx) return;
+  const int ky = e / hx, kx = e - ky * hx;
   const int first = a.t.mfirst[mem], nl = a.t.mnl[mem];
   const float* cm2l = a.t.cm2l + mem * MAX_LAYERS * MAX_LAYERS;
   const int y0 = fold_phys(ky, hy), y1 = a.Ny - 1 - y0;
@@ -662,7 +664,7 @@ __global__ __launch_bounds__(256) void to_layer_fold_kernel(LayerArgs a) {
 
 void launch_to_layer(const LayerArgs& a, hipStream_t st) {
   if (a.fold) {
-    dim3 grid((a.Nx / 2 + 255) / 256, a.Ny / 2, a.M);
+    dim3 grid((unsigned)(((long)(a.Nx / 2) * (a.Ny / 2) + 255) / 256), a.M);
     hipLaunchKernelGGL(to_layer_fold_kernel, grid, dim3(256), 0, st, a);
     return;
   }

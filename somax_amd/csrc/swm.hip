@@ -59,7 +59,7 @@ struct SwmArgs {
 namespace {
 
 constexpr int STX = 64;      // tile width
-constexpr int SRW = 2;       // rows per thread
+constexpr int SRW = 1;       // rows per thread
 constexpr int STY = 4 * SRW; // tile height (256 threads = 64 x 4)
 constexpr int SPW = STX + 2; // LDS pitch (halo 1)
 constexpr int SPH = STY + 2;
