@@ -234,6 +234,39 @@ int somaxhip_swm_step_tsit5(somaxhip_swm_plan plan, float* h, float* u, float* v
                             const somaxhip_swm_params* params, void* tsit5_workspace, somaxhip_stream_t stream);
 
 /* ------------------------------------------------------------------------ */
+/* Reparameterized QG (QG = shallow water + geostrophic projection)          */
+/* ------------------------------------------------------------------------ */
+
+/* ReparameterizedQG.project (qg/reparameterized.py:142-178): P = G (QG)^-1 Q
+ * of a shallow-water state (h, u, v), each float32[nl][ny+2][nx+2]:
+ *   q = zeta(u, v) - f0 (h - H) / H   (zeta at X points, forward differences)
+ *   psi = the Helmholtz inversion of `plan`, a QG plan of the same grid and
+ *         layers with zero_psi_ghost = 1 and lambdas = f0^2 * eigenvalues
+ *   (u, v) = grad_perp(psi) = (-dpsi/dy, dpsi/dx) (backward differences)
+ *   h = H (1 + f0 M psi),  M = Cm2l diag(eigenvalues) Cl2m (nl x nl, row-major)
+ * h_deviation != 0: h and ho hold h - H instead.  psi_out nullable.  Uses the
+ * plan's workspace. */
+int somaxhip_qg_project(somaxhip_qg_plan plan, double f0, const double* H, const double* M, const float* h,
+                        const float* u, const float* v, int h_deviation, float* ho, float* uo, float* vo,
+                        float* psi_out, void* workspace, somaxhip_stream_t stream);
+
+enum {
+  SOMAXHIP_SOLVER_SSPRK3 = 0,
+  SOMAXHIP_SOLVER_TSIT5 = 1
+};
+
+/* Advances a ReparameterizedQG state (h, u, v) in place by nsteps steps of
+ * dt of y' = RHS(P(BC(y))): the shallow-water RHS of `swm` after its boundary
+ * condition and the projection of `qg` (ReparameterizedQG.build_terms with
+ * apply_boundary_conditions = project o swm BC, reparameterized.py:186-189).
+ * qg_workspace: the QG plan's; tsit5_workspace of somaxhip_rqg_workspace_bytes. */
+size_t somaxhip_rqg_workspace_bytes(somaxhip_swm_plan swm);
+int somaxhip_rqg_step(somaxhip_swm_plan swm, somaxhip_qg_plan qg, int solver, double f0, const double* H,
+                      const double* M, float* h, float* u, float* v, int nsteps, double dt,
+                      const somaxhip_swm_params* params, void* qg_workspace, void* workspace,
+                      somaxhip_stream_t stream);
+
+/* ------------------------------------------------------------------------ */
 /* Ensembles                                                                */
 /* ------------------------------------------------------------------------ */
 

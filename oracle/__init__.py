@@ -33,4 +33,8 @@ Parity pins (each function's docstring names its own):
                    equation, C-grid dispersion relation.
 * ``rk``        -- Tsit5 tableau pinned by the 17 order conditions of orders
                    1-5 and by its dt^5 convergence on y' = lambda y.
+* ``reparam``   -- ReparameterizedQG: Q(G(psi)) = (lap - f0^2 A) psi where the
+                   stencils are inside the array, divergence-free geostrophic
+                   velocity, and the reference's qualitative tests (rest state
+                   fixed point, approximate idempotence).
 """

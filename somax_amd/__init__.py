@@ -36,7 +36,8 @@ from .swm import (
     MultilayerSW2DPhysConsts,
     MultilayerSW2DState,
 )
-from .testcases import doublegyre_baroclinic_qg, doublegyre_qg
+from .reparam import ReparameterizedQG, ReparamQGDiagnostics
+from .testcases import doublegyre_baroclinic_qg, doublegyre_qg, doublegyre_reparameterized_qg
 
 __all__ = [
     "ArakawaCGrid2D", "Diagnostics", "ModalTransform", "Params", "PhysConsts", "Solution", "SomaxModel",

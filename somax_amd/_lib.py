@@ -20,6 +20,8 @@ PREC_FP32 = 0
 PREC_MIXED = 1
 ENS_CONCURRENT = 1
 ENS_GRAPH = 2
+SOLVER_SSPRK3 = 0
+SOLVER_TSIT5 = 1
 BC_PERIODIC = 0
 BC_WALL = 1
 BC_CHANNEL = 2
@@ -138,6 +140,16 @@ _SIGNATURES = {
     "somaxhip_swm_step_tsit5": (
         ctypes.c_int,
         [_vp, _vp, _vp, _vp, ctypes.c_int, ctypes.c_double, ctypes.POINTER(SWMParams), _vp, _vp],
+    ),
+    "somaxhip_qg_project": (
+        ctypes.c_int,
+        [_vp, ctypes.c_double, c_double_p, c_double_p, _vp, _vp, _vp, ctypes.c_int, _vp, _vp, _vp, _vp, _vp, _vp],
+    ),
+    "somaxhip_rqg_workspace_bytes": (ctypes.c_size_t, [_vp]),
+    "somaxhip_rqg_step": (
+        ctypes.c_int,
+        [_vp, _vp, ctypes.c_int, ctypes.c_double, c_double_p, c_double_p, _vp, _vp, _vp, ctypes.c_int,
+         ctypes.c_double, ctypes.POINTER(SWMParams), _vp, _vp, _vp],
     ),
     "somaxhip_swm_step_profile": (
         ctypes.c_int,

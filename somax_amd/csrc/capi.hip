@@ -981,6 +981,49 @@ int somaxhip_qg_step(somaxhip_qg_plan p, float* q, float* psi_out, int nsteps, d
   return SOMAXHIP_OK;
 }
 
+int somaxhip_qg_project(somaxhip_qg_plan p, double f0, const double* H, const double* M, const float* h,
+                        const float* u, const float* v, int h_deviation, float* ho, float* uo, float* vo,
+                        float* psi_out, void* ws, somaxhip_stream_t stream) {
+  int rc = check_call(p, h, ho, ws, "somaxhip_qg_project");
+  if (rc) return rc;
+  if (!H || !M || !u || !v || !uo || !vo) {
+    set_error("somaxhip_qg_project: null H, M or field");
+    return SOMAXHIP_ERR_ARG;
+  }
+  const Batch& b = p->b;
+  if (b.M != 1) {
+    set_error("somaxhip_qg_project: needs a single-model plan");
+    return SOMAXHIP_ERR_ARG;
+  }
+  hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+  ProjArgs a{};
+  a.h = h;
+  a.u = u;
+  a.v = v;
+  a.q = wsf(ws, b.off_Q1);
+  a.psi = psi_out ? psi_out : wsf(ws, b.off_PSI);
+  a.ho = ho;
+  a.uo = uo;
+  a.vo = vo;
+  a.Nx = b.Nx;
+  a.Ny = b.Ny;
+  a.nl = b.P;
+  a.plane = b.splane;
+  a.idx = (float)(1.0 / b.dx);
+  a.idy = (float)(1.0 / b.dy);
+  a.f0 = (float)f0;
+  for (int k = 0; k < b.P; ++k) {
+    a.H[k] = (float)H[k];
+    for (int l = 0; l < b.P; ++l) a.M[k * MAX_LAYERS + l] = (float)M[k * b.P + l];
+  }
+  a.hdev = h_deviation ? 1 : 0;
+  launch_project_pv(a, st);
+  invert(b, a.q, const_cast<float*>(a.psi), ws, st, false);
+  launch_project_geo(a, st);
+  SOMAX_HIP_CHECK(hipGetLastError());
+  return SOMAXHIP_OK;
+}
+
 size_t somaxhip_qg_tsit5_workspace_bytes(somaxhip_qg_plan p) {
   return p ? (size_t)8 * p->b.P * p->b.splane * sizeof(float) : 0;
 }

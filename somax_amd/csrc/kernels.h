@@ -213,6 +213,33 @@ void launch_qg_stage(const StageArgs& a, hipStream_t st);
 // Ghost ring of nl planes of Ny x Nx set to 0 (finitevolx.zero_boundaries).
 void launch_zero_ghost(float* f, int nl, int Ny, int Nx, hipStream_t st);
 
+// Geostrophic projection of a shallow-water state (ReparameterizedQG.project):
+//   q_k   = zeta(u_k, v_k) - f0 / H_k * eta_k              (pv kernel)
+//   psi   = Helmholtz inversion of q (the QG solve)
+//   u_k   = -(psi_k[j] - psi_k[j-1]) / dy, v_k = (psi_k[i] - psi_k[i-1]) / dx
+//   eta_k = f0 H_k sum_l M[k][l] psi_l,  M = Cm2l diag(eigenvalues) Cl2m
+// with zeta at the X point [j, i] (north-east corner of cell [j, i]) by forward
+// differences, 0 on the last row and column; psi read as 0 outside the array.
+// h is the total thickness, or the deviation eta when hdev (then ho is too).
+struct ProjArgs {
+  const float* h;
+  const float* u;
+  const float* v;
+  float* q;
+  const float* psi;
+  float* ho;
+  float* uo;
+  float* vo;
+  int Nx, Ny, nl;
+  long plane;
+  float idx, idy, f0;
+  float H[MAX_LAYERS];
+  float M[MAX_LAYERS * MAX_LAYERS];
+  int hdev;
+};
+void launch_project_pv(const ProjArgs& a, hipStream_t st);
+void launch_project_geo(const ProjArgs& a, hipStream_t st);
+
 // Runge-Kutta stage state out = y + sum_{j < nk} w[j] * k[j], elementwise over
 // n floats, accumulated in FP64 (out may alias y).
 constexpr int LINCOMB_MAX = 7;

@@ -841,3 +841,56 @@ void launch_lincomb(const LinComb& a, hipStream_t st) {
 }
 
 }  // namespace somax
+
+namespace somax {
+
+// ---------------------------------------------------------------------------
+// geostrophic projection (ReparameterizedQG.project, reparameterized.py:142-178)
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void project_pv_kernel(ProjArgs a) {
+  const int x = blockIdx.x * blockDim.x + threadIdx.x;
+  const int y = blockIdx.y;
+  const int k = blockIdx.z;
+  if (x >= a.Nx) return;
+  const long s = (long)k * a.plane + (long)y * a.Nx + x;
+  float zeta = 0.0f;
+  if (y + 1 < a.Ny && x + 1 < a.Nx)
+    zeta = (a.v[s + 1] - a.v[s]) * a.idx - (a.u[s + a.Nx] - a.u[s]) * a.idy;
+  const float eta = a.hdev ? a.h[s] : a.h[s] - a.H[k];
+  a.q[s] = zeta - a.f0 / a.H[k] * eta;
+}
+
+__global__ __launch_bounds__(256) void project_geo_kernel(ProjArgs a) {
+  const int x = blockIdx.x * blockDim.x + threadIdx.x;
+  const int y = blockIdx.y;
+  if (x >= a.Nx) return;
+  const long c = (long)y * a.Nx + x;
+  float ps[MAX_LAYERS];
+#pragma unroll
+  for (int k = 0; k < MAX_LAYERS; ++k) ps[k] = k < a.nl ? a.psi[k * a.plane + c] : 0.0f;
+#pragma unroll
+  for (int k = 0; k < MAX_LAYERS; ++k) {
+    if (k >= a.nl) break;
+    const float* p = a.psi + k * a.plane;
+    const float s_ = y > 0 ? p[c - a.Nx] : 0.0f;
+    const float w = x > 0 ? p[c - 1] : 0.0f;
+    a.uo[k * a.plane + c] = -(ps[k] - s_) * a.idy;
+    a.vo[k * a.plane + c] = (ps[k] - w) * a.idx;
+    float m = 0.0f;
+#pragma unroll
+    for (int l = 0; l < MAX_LAYERS; ++l)
+      if (l < a.nl) m += a.M[k * MAX_LAYERS + l] * ps[l];
+    const float eta = a.f0 * a.H[k] * m;
+    a.ho[k * a.plane + c] = a.hdev ? eta : a.H[k] + eta;
+  }
+}
+
+void launch_project_pv(const ProjArgs& a, hipStream_t st) {
+  hipLaunchKernelGGL(project_pv_kernel, dim3((a.Nx + 255) / 256, a.Ny, a.nl), dim3(256), 0, st, a);
+}
+
+void launch_project_geo(const ProjArgs& a, hipStream_t st) {
+  hipLaunchKernelGGL(project_geo_kernel, dim3((a.Nx + 255) / 256, a.Ny), dim3(256), 0, st, a);
+}
+
+}  // namespace somax

@@ -655,6 +655,92 @@ int somaxhip_swm_step_tsit5(somaxhip_swm_plan p, float* h, float* u, float* v, i
   return SOMAXHIP_OK;
 }
 
+size_t somaxhip_rqg_workspace_bytes(somaxhip_swm_plan p) {
+  return p ? (size_t)11 * 3 * p->nl * p->plane * sizeof(float) : 0;
+}
+
+// ReparameterizedQG time stepping: y' = f(y) with f = RHS(P(BC(y))) (the
+// projection inside apply_boundary_conditions, reparameterized.py:186-189),
+// Tsit5 (FSAL) or SSP-RK3 in Butcher form, thickness carried as the
+// deviation eta.  tws: Y0, Y, K0..K6, B (after the BC), Pj (projected).
+int somaxhip_rqg_step(somaxhip_swm_plan p, somaxhip_qg_plan qp, int solver, double f0, const double* H,
+                      const double* M, float* h, float* u, float* v, int nsteps, double dt,
+                      const somaxhip_swm_params* prm, void* qg_ws, void* tws, somaxhip_stream_t stream) {
+  int rc = check_plan(p, "somaxhip_rqg_step");
+  if (rc) return rc;
+  if (!qp || !H || !M || !h || !u || !v || !prm || !qg_ws || !tws || nsteps < 0 ||
+      (solver != SOMAXHIP_SOLVER_SSPRK3 && solver != SOMAXHIP_SOLVER_TSIT5)) {
+    set_error("somaxhip_rqg_step: null argument, negative nsteps or unknown solver");
+    return SOMAXHIP_ERR_ARG;
+  }
+  if (nsteps == 0) return SOMAXHIP_OK;
+  hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+  const long n = (long)p->nl * p->plane;
+  float* Y0 = static_cast<float*>(tws);
+  float* Y = Y0 + 3 * n;
+  float* K[7];
+  for (int i = 0; i < 7; ++i) K[i] = Y + (i + 1) * 3 * n;
+  float* B = Y + 8 * 3 * n;
+  float* Pj = B + 3 * n;
+  SOMAX_HIP_CHECK(hipMemcpyAsync(Y0, h, n * sizeof(float), hipMemcpyDeviceToDevice, st));
+  SOMAX_HIP_CHECK(hipMemcpyAsync(Y0 + n, u, n * sizeof(float), hipMemcpyDeviceToDevice, st));
+  SOMAX_HIP_CHECK(hipMemcpyAsync(Y0 + 2 * n, v, n * sizeof(float), hipMemcpyDeviceToDevice, st));
+  SwmArgs sh{};
+  for (int k = 0; k < p->nl; ++k) sh.hoff[k] = p->Hk[k];
+  const dim3 sg((unsigned)((p->plane + 255) / 256), p->nl, 1);
+  hipLaunchKernelGGL(swm_shift_kernel, sg, dim3(256), 0, st, Y0, p->plane, sh, -1.0f);
+  int err = SOMAXHIP_OK;
+  auto f = [&](const float* y, float* k) {
+    hipLaunchKernelGGL(swm_bc_kernel, dim3((p->Nx + 255) / 256, p->Ny, p->nl), dim3(256), 0, st, y, y + n,
+                       y + 2 * n, B, B + n, B + 2 * n, p->Ny, p->Nx, p->plane, p->bc);
+    const int r = somaxhip_qg_project(qp, f0, H, M, B, B + n, B + 2 * n, 1, Pj, Pj + n, Pj + 2 * n, nullptr, qg_ws,
+                                      stream);
+    if (r && !err) err = r;
+    swm_stage(p, Pj, Pj + n, Pj + 2 * n, nullptr, nullptr, nullptr, k, k + n, k + 2 * n, prm, 0.0f, 1.0f, 0, true,
+              st);
+  };
+  auto comb = [&](const double* w, int nk, float* out) {
+    LinComb c{};
+    c.y = Y0;
+    c.nk = nk;
+    for (int j = 0; j < nk; ++j) {
+      c.k[j] = K[j];
+      c.w[j] = dt * w[j];
+    }
+    c.n = 3 * n;
+    c.out = out;
+    launch_lincomb(c, st);
+  };
+  if (solver == SOMAXHIP_SOLVER_TSIT5) {
+    f(Y0, K[0]);
+    for (int s = 0; s < nsteps && !err; ++s) {
+      for (int i = 1; i <= 6; ++i) {
+        comb(TSIT5_A[i], i, i < 6 ? Y : Y0);  // row 6: the new state, in place
+        f(i < 6 ? Y : Y0, K[i]);
+      }
+      std::swap(K[0], K[6]);  // first same as last
+    }
+  } else {
+    // SSP-RK3 in Butcher form: c = (0, 1, 1/2), b = (1/6, 1/6, 2/3)
+    static const double a1[1] = {1.0}, a2[2] = {0.25, 0.25}, b3[3] = {1.0 / 6.0, 1.0 / 6.0, 2.0 / 3.0};
+    for (int s = 0; s < nsteps && !err; ++s) {
+      f(Y0, K[0]);
+      comb(a1, 1, Y);
+      f(Y, K[1]);
+      comb(a2, 2, Y);
+      f(Y, K[2]);
+      comb(b3, 3, Y0);
+    }
+  }
+  if (err) return err;
+  hipLaunchKernelGGL(swm_shift_kernel, sg, dim3(256), 0, st, Y0, p->plane, sh, 1.0f);
+  SOMAX_HIP_CHECK(hipMemcpyAsync(h, Y0, n * sizeof(float), hipMemcpyDeviceToDevice, st));
+  SOMAX_HIP_CHECK(hipMemcpyAsync(u, Y0 + n, n * sizeof(float), hipMemcpyDeviceToDevice, st));
+  SOMAX_HIP_CHECK(hipMemcpyAsync(v, Y0 + 2 * n, n * sizeof(float), hipMemcpyDeviceToDevice, st));
+  SOMAX_HIP_CHECK(hipGetLastError());
+  return SOMAXHIP_OK;
+}
+
 int somaxhip_swm_step_profile(somaxhip_swm_plan p, float* h, float* u, float* v, int nsteps, double dt,
                               const somaxhip_swm_params* prm, void* ws, somaxhip_stream_t stream,
                               somaxhip_kernel_stat* stats, int max_stats, int* nstats) {

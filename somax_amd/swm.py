@@ -248,7 +248,7 @@ class MultilayerShallowWater2D(SomaxModel):
         if kw:
             raise TypeError(f"unexpected keyword arguments {sorted(kw)}")
         ts = [float(t1)] if saveat is None else [float(t) for t in saveat]
-        h, u, v = (f.clone() for f in self._fields(state0))
+        h, u, v = self._fields(self.apply_boundary_conditions(state0))  # model.py:71
         t = float(t0)
         out = []
         for ts_k in ts:

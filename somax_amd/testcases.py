@@ -273,3 +273,26 @@ def jet_dt(model, cfl: float = 0.5) -> float:
     M = np.tril(np.ones((H.size, H.size))) * gp[None, :]
     c = float(np.sqrt(np.max(np.linalg.eigvals(np.diag(H) @ M).real)))
     return cfl * min(model.grid.dx, model.grid.dy) / (c + 0.5)
+
+
+def doublegyre_reparameterized_qg(nx: int = 128, ny: int = 128, Lx: float = 4e6, Ly: float = 4e6,
+                                  f0: float = 9.375e-5, beta: float = 1.754e-11, n_layers: int = 3,
+                                  H=(400.0, 1100.0, 2600.0), g_prime=(9.81, 0.025, 0.0125),
+                                  lateral_viscosity: float = 15.0, bottom_drag: float = 3.6e-8,
+                                  wind_amplitude: float = 8e-5, **kw):
+    """``doublegyre_reparameterized_qg`` (gfd_testcases.py:365-430): the wind-driven
+    double gyre in (u, v, h) with the geostrophic projection, from rest."""
+    import torch
+
+    from .reparam import ReparameterizedQG
+    from .swm import MultilayerSW2DState
+
+    model = ReparameterizedQG.create(nx=nx, ny=ny, Lx=Lx, Ly=Ly, f0=f0, beta=beta, n_layers=n_layers, H=H,
+                                     g_prime=g_prime, lateral_viscosity=lateral_viscosity, bottom_drag=bottom_drag,
+                                     wind_amplitude=wind_amplitude, wind_profile="doublegyre", bc="wall", **kw)
+    nl, Ny, Nx = model.consts.n_layers, model.grid.Ny, model.grid.Nx
+    dev = model.swm.device
+    h0 = torch.ones((nl, Ny, Nx), device=dev) * torch.as_tensor(np.asarray(model.strat.H), dtype=torch.float32,
+                                                               device=dev)[:, None, None]
+    z = torch.zeros((nl, Ny, Nx), device=dev)
+    return model, MultilayerSW2DState(h=h0, u=z, v=z.clone())

@@ -422,3 +422,66 @@ def test_tsit5_fifth_order_convergence():
         errs.append(abs(y[0][0] - np.exp(lam)))
     rates = [np.log2(errs[i] / errs[i + 1]) for i in range(2)]
     assert all(4.7 < r < 5.6 for r in rates), rates
+
+
+# ---------------------------------------------------------------------------
+# ReparameterizedQG (oracle/reparam.py)
+# ---------------------------------------------------------------------------
+def _rqg(n=16, **kw):
+    from oracle import reparam
+
+    args = dict(nx=n, ny=n, H=(500.0, 500.0), g_prime=(9.81, 0.02))
+    args.update(kw)
+    return reparam.setup(**args)
+
+
+def test_reparam_q_of_g_is_the_helmholtz_operator():
+    """Q(G(psi)) = lap(psi) - f0^2 M psi wherever both stencils lie in the
+    array (all but the last row and column): pins the vorticity and grad_perp
+    index senses against each other and the f0, H factors of Q and G."""
+    from oracle import reparam
+
+    r = _rqg(12, H=(400.0, 1100.0, 2600.0), g_prime=(9.81, 0.025, 0.0125), Lx=1.2e6, Ly=0.9e6)
+    rng = np.random.default_rng(5)
+    psi = rng.standard_normal((3, r.swm.Ny, r.swm.Nx)) * 1e4
+    psi[:, 0, :] = psi[:, -1, :] = psi[:, :, 0] = psi[:, :, -1] = 0.0
+    q = reparam.pv(r, *reparam.geostrophic(r, psi))
+    pad = np.pad(psi, ((0, 0), (1, 1), (1, 1)))
+    lap = (pad[:, 1:-1, 2:] - 2 * psi + pad[:, 1:-1, :-2]) / r.swm.dx**2 + (
+        pad[:, 2:, 1:-1] - 2 * psi + pad[:, :-2, 1:-1]) / r.swm.dy**2
+    want = lap - r.f0**2 * np.einsum("kl,lyx->kyx", r.M, psi)
+    np.testing.assert_allclose(q[:, :-1, :-1], want[:, :-1, :-1], rtol=0, atol=1e-12 * np.abs(want).max())
+
+
+def test_reparam_geostrophic_velocity_divergence_free():
+    from oracle import reparam
+
+    r = _rqg(10)
+    rng = np.random.default_rng(6)
+    psi = rng.standard_normal((2, r.swm.Ny, r.swm.Nx))
+    _, u, v = reparam.geostrophic(r, psi)
+    div = (u[:, 1:, 1:] - u[:, 1:, :-1]) / r.swm.dx + (v[:, 1:, 1:] - v[:, :-1, 1:]) / r.swm.dy
+    assert np.abs(div).max() < 1e-12 * np.abs(psi).max() / min(r.swm.dx, r.swm.dy) ** 2
+
+
+def test_reparam_reference_behaviours():
+    """tests/models/test_qg_reparameterized.py:60-87, 117-125: rest state is
+    a fixed point, the projection is idempotent to O(dx^2) (atol 0.1 for a 1 m
+    bump), and the rest state has zero tendency."""
+    from oracle import reparam
+
+    r = _rqg(16)
+    H = r.swm.H[:, None, None]
+    h0 = np.ones((2, r.swm.Ny, r.swm.Nx)) * H
+    z = np.zeros_like(h0)
+    hp, up, vp = reparam.project(r, h0, z, z)
+    assert np.abs(hp - h0).max() < 1e-6 and np.abs(up).max() < 1e-10 and np.abs(vp).max() < 1e-10
+    h1 = h0.copy()
+    h1[0, r.swm.Ny // 2, r.swm.Nx // 2] += 1.0
+    p1 = reparam.project(r, h1, z, z)
+    p2 = reparam.project(r, *p1)
+    for a, b in zip(p2, p1):
+        assert np.abs(a - b).max() < 0.1
+    assert np.abs(p1[1]).max() > 0 and np.abs(p1[2]).max() > 0
+    t = reparam.rhs_of_bc(r, (h0, z, z))
+    assert max(np.abs(x).max() for x in t) < 1e-10
