@@ -302,3 +302,30 @@ def diagnose(s: SWMSetup, h, u, v):
     return {"energy": energy, "total_energy": energy.sum(), "enstrophy": enstrophy,
             "total_enstrophy": enstrophy.sum(), "potential_vorticity": q,
             "relative_vorticity": relative_vorticity(s, u, v), "kinetic_energy_field": ke}
+
+
+# ---------------------------------------------------------------------------
+# single-layer NonlinearShallowWater2D (swm/nonlinear_2d.py)
+# ---------------------------------------------------------------------------
+def nonlinear_setup(nx=64, ny=64, Lx=1e6, Ly=1e6, g=9.81, f0=1e-4, beta=0.0, H0=100.0, lateral_viscosity=0.0,
+                    bottom_drag=0.0, wind_amplitude=0.0, wind_profile="doublegyre", bc="periodic") -> SWMSetup:
+    """``NonlinearShallowWater2D.create`` (nonlinear_2d.py:243-318): the
+    vector-invariant equations of ``vector_field`` (:132-194) are the
+    multilayer ones at one layer with g' = g (P = KE + g h, drag on the only
+    layer), except that the wind stress is *not* divided by the depth
+    (:180-182 against multilayer.py:190-191).  The setup therefore holds the
+    one-layer model with the wind amplitude multiplied by H0, which the
+    multilayer RHS divides out again."""
+    if bc not in ("periodic", "wall"):
+        raise ValueError(f"unknown bc {bc!r}")
+    return setup(nx=nx, ny=ny, Lx=Lx, Ly=Ly, f0=f0, beta=beta, H=(H0,), g_prime=(g,),
+                 lateral_viscosity=lateral_viscosity, bottom_drag=bottom_drag, wind_amplitude=wind_amplitude * H0,
+                 wind_profile=wind_profile, bc=bc)
+
+
+def nonlinear_diagnose(s: SWMSetup, h, u, v):
+    """``diagnose`` (nonlinear_2d.py:219-241): scalar energy and enstrophy."""
+    d = diagnose(s, *(np.asarray(a, dtype=np.float64)[None] for a in (h, u, v)))
+    return {"energy": d["energy"][0], "enstrophy": d["enstrophy"][0],
+            "potential_vorticity": d["potential_vorticity"][0], "relative_vorticity": d["relative_vorticity"][0],
+            "kinetic_energy_field": d["kinetic_energy_field"][0]}

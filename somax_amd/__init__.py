@@ -35,6 +35,11 @@ from .swm import (
     MultilayerSW2DParams,
     MultilayerSW2DPhysConsts,
     MultilayerSW2DState,
+    NonlinearShallowWater2D,
+    NonlinearSW2DDiagnostics,
+    NonlinearSW2DParams,
+    NonlinearSW2DPhysConsts,
+    NonlinearSW2DState,
 )
 from .reparam import ReparameterizedQG, ReparamQGDiagnostics
 from .testcases import doublegyre_baroclinic_qg, doublegyre_qg, doublegyre_reparameterized_qg

@@ -314,3 +314,142 @@ class MultilayerShallowWater2D(SomaxModel):
             wind = -np.cos(2.0 * np.pi * y / Ly)
         return MultilayerShallowWater2D(params, consts, grid, strat, modal, f_row, wind, bc_type=bc, method=method,
                                         device=device)
+
+
+# ---------------------------------------------------------------------------
+# single-layer NonlinearShallowWater2D (swm/nonlinear_2d.py)
+# ---------------------------------------------------------------------------
+@dataclass
+class NonlinearSW2DState(State):
+    """``h``, ``u``, ``v``, each ``(Ny, Nx)`` (nonlinear_2d.py:25-36)."""
+
+    h: object
+    u: object
+    v: object
+
+
+@dataclass
+class NonlinearSW2DParams(Params):
+    """nonlinear_2d.py:39-50."""
+
+    lateral_viscosity: float = 0.0
+    bottom_drag: float = 0.0
+    wind_amplitude: float = 0.0
+
+
+@dataclass(frozen=True)
+class NonlinearSW2DPhysConsts(PhysConsts):
+    """nonlinear_2d.py:53-66."""
+
+    gravity: float = 9.81
+    f0: float = 1e-4
+    beta: float = 0.0
+    H0: float = 100.0
+
+
+@dataclass(frozen=True)
+class NonlinearSW2DDiagnostics(Diagnostics):
+    """nonlinear_2d.py:69-85."""
+
+    energy: object
+    enstrophy: object
+    potential_vorticity: object
+    relative_vorticity: object
+    kinetic_energy_field: object
+
+
+class NonlinearShallowWater2D(SomaxModel):
+    """2-D nonlinear (vector-invariant) shallow water, one layer
+    (nonlinear_2d.py:88-318).  Its equations are the multilayer model's at one
+    layer with g' = g, except that the wind stress is not divided by the depth
+    (nonlinear_2d.py:180-182); it runs on a one-layer multilayer plan whose
+    wind amplitude is scaled by H0."""
+
+    def __init__(self, params: NonlinearSW2DParams, consts: NonlinearSW2DPhysConsts, ml: MultilayerShallowWater2D):
+        self.params = params
+        self.consts = consts
+        self._ml = ml
+
+    @property
+    def grid(self):
+        return self._ml.grid
+
+    @property
+    def bc_type(self):
+        return self._ml.bc_type
+
+    @property
+    def method(self):
+        return self._ml.method
+
+    @property
+    def f_field(self):
+        return self._ml.f_field
+
+    @property
+    def wind_stress_x(self):
+        return self._ml.wind_stress_x
+
+    @property
+    def wind_stress_y(self):
+        return self._ml.wind_stress_y
+
+    def _sync(self) -> MultilayerShallowWater2D:
+        p = self.params
+        self._ml.params = MultilayerSW2DParams(lateral_viscosity=float(p.lateral_viscosity),
+                                               bottom_drag=float(p.bottom_drag),
+                                               wind_amplitude=float(p.wind_amplitude) * self.consts.H0)
+        return self._ml
+
+    def _to2d(self, st) -> NonlinearSW2DState:
+        shape = (self.grid.Ny, self.grid.Nx)
+        return NonlinearSW2DState(h=st.h.reshape(shape), u=st.u.reshape(shape), v=st.v.reshape(shape))
+
+    def vector_field(self, t: float, state, args=None) -> NonlinearSW2DState:
+        """nonlinear_2d.py:132-194 on the state as given."""
+        return self._to2d(self._sync().vector_field(t, state, args))
+
+    def apply_boundary_conditions(self, state) -> NonlinearSW2DState:
+        """Periodic or free-slip wall (nonlinear_2d.py:196-217)."""
+        return self._to2d(self._sync().apply_boundary_conditions(state))
+
+    def build_terms(self):
+        def _rhs(t, state, args=None):
+            return self._to2d(self._sync()._rhs(state, apply_bc=True))
+
+        return _rhs
+
+    def step(self, state, dt: float, nsteps: int = 1, solver="ssprk3") -> NonlinearSW2DState:
+        return self._to2d(self._sync().step(state, dt, nsteps, solver=solver))
+
+    def integrate(self, state0, t0: float, t1: float, dt: float, **kw) -> Solution:
+        """As the multilayer model's (Tsit5 by default, BC applied to state0)."""
+        sol = self._sync().integrate(state0, t0, t1, dt, **kw)
+        shape = (-1, self.grid.Ny, self.grid.Nx)
+        ys = NonlinearSW2DState(h=sol.ys.h.reshape(shape), u=sol.ys.u.reshape(shape), v=sol.ys.v.reshape(shape))
+        return Solution(ts=sol.ts, ys=ys)
+
+    def diagnose(self, state) -> NonlinearSW2DDiagnostics:
+        """Scalar energy and potential enstrophy, PV, vorticity, KE fields
+        (nonlinear_2d.py:219-241)."""
+        d = self._sync().diagnose(state)
+        shape = (self.grid.Ny, self.grid.Nx)
+        return NonlinearSW2DDiagnostics(energy=d.energy[0], enstrophy=d.enstrophy[0],
+                                        potential_vorticity=d.potential_vorticity.reshape(shape),
+                                        relative_vorticity=d.relative_vorticity.reshape(shape),
+                                        kinetic_energy_field=d.kinetic_energy_field.reshape(shape))
+
+    @staticmethod
+    def create(nx: int = 64, ny: int = 64, Lx: float = 1e6, Ly: float = 1e6, g: float = 9.81, f0: float = 1e-4,
+               beta: float = 0.0, H0: float = 100.0, lateral_viscosity: float = 0.0, bottom_drag: float = 0.0,
+               wind_amplitude: float = 0.0, wind_profile: str = "doublegyre", bc: str = "periodic",
+               method: str = "upwind1", *, device="cuda") -> "NonlinearShallowWater2D":
+        """``NonlinearShallowWater2D.create`` (nonlinear_2d.py:243-318), same arguments."""
+        if bc not in ("periodic", "wall"):
+            raise ValueError(f"bc must be 'periodic' or 'wall', got {bc!r}")
+        ml = MultilayerShallowWater2D.create(nx=nx, ny=ny, Lx=Lx, Ly=Ly, g=g, f0=f0, beta=beta, n_layers=1,
+                                             H=(H0,), g_prime=(g,), wind_profile=wind_profile, bc=bc, method=method,
+                                             device=device)
+        params = NonlinearSW2DParams(lateral_viscosity=float(lateral_viscosity), bottom_drag=float(bottom_drag),
+                                     wind_amplitude=float(wind_amplitude))
+        return NonlinearShallowWater2D(params, NonlinearSW2DPhysConsts(gravity=g, f0=f0, beta=beta, H0=H0), ml)

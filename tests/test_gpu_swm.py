@@ -189,3 +189,70 @@ def test_diagnose_matches_oracle(gpu, bc):
     assert rs["energy"] < 1e-6 and rs["enstrophy"] < 1e-6
     assert rs["potential_vorticity"] < 1e-5 and rs["relative_vorticity"] < 1e-5 and rs["kinetic_energy_field"] < 1e-5
     assert np.all(d.energy.cpu().numpy() >= 0)
+
+
+# ---------------------------------------------------------------------------
+# single-layer NonlinearShallowWater2D (nonlinear_2d.py)
+# ---------------------------------------------------------------------------
+def _nonlinear(n, bc, seed=4):
+    from somax_amd import NonlinearShallowWater2D
+
+    kw = dict(nx=n, ny=n, H0=100.0, lateral_viscosity=20.0, bottom_drag=1e-6, wind_amplitude=2e-6, bc=bc)
+    model = NonlinearShallowWater2D.create(**kw)
+    s = oswm.nonlinear_setup(**kw)
+    rng = np.random.default_rng(seed)
+    shape = (s.Ny, s.Nx)
+    h = 100.0 + 0.5 * rng.standard_normal(shape)
+    u = 0.05 * rng.standard_normal(shape)
+    v = 0.05 * rng.standard_normal(shape)
+    return model, s, [np.asarray(a, dtype=np.float32) for a in (h, u, v)]
+
+
+def _nl_state(f32):
+    import torch
+
+    from somax_amd import NonlinearSW2DState
+
+    h, u, v = (torch.from_numpy(a.copy()).cuda() for a in f32)
+    return NonlinearSW2DState(h=h, u=u, v=v)
+
+
+@pytest.mark.parametrize("bc", ["periodic", "wall"])
+def test_nonlinear_rhs_and_step_match_oracle(gpu, bc):
+    _log(f"[swm1] single layer n=64 {bc}")
+    model, s, f32 = _nonlinear(64, bc)
+    x = [a.astype(np.float64)[None] for a in f32]
+    d = model.build_terms()(0.0, _nl_state(f32))
+    ref = oswm.rhs(s, *oswm.apply_bc(s, *x))
+    rs = [rel(a.cpu().numpy(), b[0]) for a, b in zip((d.h, d.u, d.v), ref)]
+    _log(f"  rhs rel-L2 dh {rs[0]:.3e} du {rs[1]:.3e} dv {rs[2]:.3e}")
+    assert d.h.shape == f32[0].shape and max(rs) < 1e-5
+    dt = 20.0
+    for solver, one in (("ssprk3", oswm.ssprk3_step), ("tsit5", oswm.tsit5_step)):
+        out = model.step(_nl_state(f32), dt, 1, solver=solver)
+        ref = one(s, *x, dt)
+        rs = [rel(a.cpu().numpy(), b[0]) for a, b in zip((out.h, out.u, out.v), ref)]
+        _log(f"  one {solver} step rel-L2 h {rs[0]:.3e} u {rs[1]:.3e} v {rs[2]:.3e}")
+        assert max(rs) < 1e-6
+
+
+def test_nonlinear_reference_behaviours(gpu):
+    """tests/models/test_swm_2d.py:137-159."""
+    import torch
+
+    from somax_amd import NonlinearShallowWater2D, NonlinearSW2DDiagnostics, NonlinearSW2DState
+
+    model = NonlinearShallowWater2D.create(nx=16, ny=16, H0=100.0, lateral_viscosity=100.0)
+    g = model.grid
+    x = torch.arange(g.Nx, dtype=torch.float32, device="cuda") * g.dx
+    y = torch.arange(g.Ny, dtype=torch.float32, device="cuda") * g.dy
+    Y, X = torch.meshgrid(y, x, indexing="ij")
+    h0 = 100.0 + 0.1 * torch.exp(-((X - g.Lx / 2) ** 2 + (Y - g.Ly / 2) ** 2) / (2 * (g.Lx / 5) ** 2))
+    state0 = NonlinearSW2DState(h=h0, u=torch.zeros_like(h0), v=torch.zeros_like(h0))
+    sol = model.integrate(state0, t0=0.0, t1=10.0, dt=0.1)
+    assert bool(torch.isfinite(sol.ys.h).all()) and sol.ys.h.shape == (1, g.Ny, g.Nx)
+    rest = NonlinearSW2DState(h=torch.full_like(h0, 100.0), u=torch.zeros_like(h0), v=torch.zeros_like(h0))
+    d = model.diagnose(rest)
+    assert isinstance(d, NonlinearSW2DDiagnostics)
+    assert d.potential_vorticity.shape == h0.shape and d.relative_vorticity.shape == h0.shape
+    assert d.energy.ndim == 0 and float(d.energy) > 0

@@ -141,3 +141,16 @@ def test_diagnose_rest_energy():
     expect = 0.5 * s.g_prime * s.H**2 * s.nx * s.ny * s.dx * s.dy
     np.testing.assert_allclose(d["energy"], expect, rtol=1e-12)
     assert np.all(d["energy"] >= 0)
+
+
+def test_nonlinear_single_layer_wind_not_divided():
+    """nonlinear_2d.py:180-182: du/dt gains tau0 * tau_x (no 1/H0), so on a
+    resting layer du = tau0 * (-cos(2 pi y / Ly)) in closed form."""
+    s = swm.nonlinear_setup(nx=12, ny=10, H0=250.0, wind_amplitude=3e-6)
+    h = np.full((1, s.Ny, s.Nx), 250.0)
+    z = np.zeros_like(h)
+    dh, du, dv = swm.rhs(s, *swm.apply_bc(s, h, z, z))
+    y = np.arange(s.Ny) * s.dy
+    want = 3e-6 * -np.cos(2 * np.pi * y / s.Ly)
+    np.testing.assert_allclose(du[0, 1:-1, 1:-1], np.broadcast_to(want[1:-1, None], (s.ny, s.nx)), rtol=1e-12)
+    assert np.abs(dh).max() < 1e-12 and np.abs(dv).max() < 1e-18
