@@ -18,8 +18,9 @@
 // layouts.  With acc64 the FP32 MFMA partial of every few k-tiles is added to
 // an FP64 running sum, so the FP32 rounding chain per output stays short.
 //
-// Block tile 128x128 of 32x32 MFMA tiles: 8 waves (2x4) of 64x32 for the
-// bf16 instances, 4 waves (2x2) of 64x64 for FP32.  Grids are 1-D
+// Block tile 128x128 of 32x32 MFMA tiles: 16 waves (4x4) of 32x32 for the
+// x1 / x3 instances, 8 waves (2x4) of 64x32 for x6, 4 waves (2x2) of 64x64
+// for FP32.  Grids are 1-D
 // and remapped so that each XCD works on a contiguous run of tiles (shared
 // operand rows in its L2).
 
@@ -38,12 +39,13 @@ constexpr int BK32 = 16;          // fp32 k-tile
 constexpr int LD32 = BK32 + 4;    // padded LDS row (80 bytes)
 constexpr int TILE32 = BM * LD32;
 
-// TN: 32-column MFMA tiles per wave along N.  TN = 2: 4 waves of 64 x 64
-// (2 x 2 waves); TN = 1: 8 waves of 64 x 32 (2 x 4 waves, two per SIMD).
-template <int TN>
-__device__ inline void zero_acc(floatx16 (&acc)[2][TN], double (&sum)[2][TN][16]) {
+// TM x TN: 32 x 32 MFMA tiles per wave (along M, along N); the 128 x 128
+// block has (4 / TM) x (4 / TN) waves: 4 of 64 x 64 (TM = TN = 2), 8 of
+// 64 x 32 (TM = 2, TN = 1) or 16 of 32 x 32 (TM = TN = 1).
+template <int TM, int TN>
+__device__ inline void zero_acc(floatx16 (&acc)[TM][TN], double (&sum)[TM][TN][16]) {
 #pragma unroll
-  for (int i = 0; i < 2; ++i)
+  for (int i = 0; i < TM; ++i)
 #pragma unroll
     for (int j = 0; j < TN; ++j)
 #pragma unroll
@@ -81,10 +83,10 @@ struct Spectral {
   float lamz;
 };
 
-template <bool SUM64, int TN>
-__device__ inline void store_tile(const GemmArgs& a, long z, int a0, int b0, const floatx16 (&acc)[2][TN],
-                                  const double (&sum)[2][TN][16], int opieces, float* stage, const Spectral& sp) {
-  constexpr int WN = 4 / TN, NT = 128 * WN;  // waves along N, threads
+template <bool SUM64, int TM, int TN>
+__device__ inline void store_tile(const GemmArgs& a, long z, int a0, int b0, const floatx16 (&acc)[TM][TN],
+                                  const double (&sum)[TM][TN][16], int opieces, float* stage, const Spectral& sp) {
+  constexpr int WN = 4 / TN, WM = 4 / TM, NT = 64 * WM * WN;  // waves along N and M, threads
   const int tid = threadIdx.x;
   const int lane = tid & 63;
   const int wave = tid >> 6;
@@ -93,14 +95,15 @@ __device__ inline void store_tile(const GemmArgs& a, long z, int a0, int b0, con
 #pragma unroll
   for (int h = 0; h < 2; ++h) {
     __syncthreads();  // the LDS is free (pipeline drained / previous half stored)
-    if (wm == h) {
+    if ((wm * 32 * TM) / 64 == h) {  // this wave's rows lie in half h
+      const int lr0 = (wm * 32 * TM) & 63;
 #pragma unroll
-      for (int i = 0; i < 2; ++i)
+      for (int i = 0; i < TM; ++i)
 #pragma unroll
         for (int j = 0; j < TN; ++j)
 #pragma unroll
           for (int r = 0; r < 16; ++r) {
-            const int lr = i * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+            const int lr = lr0 + i * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
             const int lc = wn * 32 * TN + j * 32 + (lane & 31);
             stage[lr * EPI_PITCH + lc] = SUM64 ? (float)(sum[i][j][r] + (double)acc[i][j][r]) : acc[i][j][r];
           }
@@ -226,7 +229,7 @@ __device__ void gemm_fringe(const GemmArgs& a, long z, int part, const GemmCut& 
                             void* lds) {
   constexpr int G = NT / 8;  // lane groups
   using acc_t = typename std::conditional<(F32 || NPC == 3), double, float>::type;
-  constexpr int KB = NPC == 3 ? 4 : 9;  // k-chunks per batch of loads (register bound)
+  constexpr int KB = (NPC == 3 || NT >= 1024) ? 4 : 9;  // k-chunks per batch of loads (register bound)
   constexpr int NV = F32 ? 2 : NPC;  // 16-byte loads per variable-row chunk
   const int fpw = cut.fpw;
   const int fa = a.RA - cut.am, fb = a.RB - cut.bm;
@@ -401,10 +404,10 @@ __device__ void gemm_fringe(const GemmArgs& a, long z, int part, const GemmCut& 
   }
 }
 
-template <int NPROD, int TN>
-__device__ inline void mfma_products(const bf16x8 (&fa)[3][2], const bf16x8 (&fb)[3][TN], floatx16 (&acc)[2][TN]) {
+template <int NPROD, int TM, int TN>
+__device__ inline void mfma_products(const bf16x8 (&fa)[3][TM], const bf16x8 (&fb)[3][TN], floatx16 (&acc)[TM][TN]) {
 #pragma unroll
-  for (int i = 0; i < 2; ++i)
+  for (int i = 0; i < TM; ++i)
 #pragma unroll
     for (int j = 0; j < TN; ++j) {
       acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fa[0][i], fb[0][j], acc[i][j], 0, 0, 0);
@@ -432,14 +435,16 @@ __device__ inline void mfma_products(const bf16x8 (&fa)[3][2], const bf16x8 (&fb
 // ---------------------------------------------------------------------------
 __device__ inline int swz(int r) { return ((r >> 1) ^ (r >> 2)) & 3; }
 
-template <int NPROD, int TN>
+template <int NPROD, int TM, int TN>
 struct Bf16Cfg {
   static constexpr int NPC = NPROD == 1 ? 1 : (NPROD == 3 ? 2 : 3);
   static constexpr int NSTAGE = NPC == 3 ? 3 : 4;
   static constexpr int TILE = 128 * 32;         // bf16 elements per operand piece
   static constexpr int STAGE = 2 * NPC * TILE;  // A pieces then B pieces
-  static constexpr int RPT = TN;                // 16-row blocks each thread fills per operand piece
-  static constexpr int LPT = 2 * NPC * RPT;     // global_load_lds per thread per k-tile
+  static constexpr int NWAVES = (4 / TM) * (4 / TN);
+  static constexpr int OPW = NWAVES >= 16 ? 1 : 2;           // operands each wave loads
+  static constexpr int RPT = 16 / (NWAVES * OPW);            // 16-row blocks per operand piece
+  static constexpr int LPT = OPW * NPC * RPT;                // global_load_lds per thread per k-tile
   static constexpr int FLUSH = 2;               // k-tiles between FP64 flushes
 };
 
@@ -449,12 +454,13 @@ __device__ inline void wait_vmcnt() {
   __builtin_amdgcn_s_waitcnt((N & 0xF) | (0x7 << 4) | (0xF << 8) | ((N >> 4) << 14));
 }
 
-template <int NPROD, bool ACC64, int TN>
+template <int NPROD, bool ACC64, int TM, int TN>
 __device__ inline void bf16_mainloop(const GemmArgs& a, const uint16_t* S, uint16_t* lds, long z, int a0, int b0,
-                                     bool x_in_a, floatx16 (&acc)[2][TN], double (&sum)[2][TN][16]) {
-  using C = Bf16Cfg<NPROD, TN>;
+                                     bool x_in_a, floatx16 (&acc)[TM][TN], double (&sum)[TM][TN][16]) {
+  using C = Bf16Cfg<NPROD, TM, TN>;
   constexpr int NPC = C::NPC;
   constexpr int RPT = C::RPT;
+  constexpr int OPW = C::OPW;
   constexpr int WN = 4 / TN;
   const int tid = threadIdx.x;
   const int lane = tid & 63;
@@ -472,7 +478,9 @@ __device__ inline void bf16_mainloop(const GemmArgs& a, const uint16_t* S, uint1
   int ldst[RPT];
 #pragma unroll
   for (int j = 0; j < RPT; ++j) {
-    const int r0 = (wave * RPT + j) * 16;  // first row filled by this wave instruction
+    // first row filled by this wave instruction (with one operand per wave,
+    // waves 0-7 load X and 8-15 load S)
+    const int r0 = ((OPW == 2 ? wave : (wave & 7)) * RPT + j) * 16;
     const int row = r0 + (lane >> 2);
     const int chunk = (lane & 3) ^ swz(row);
     const int xr = x0 + row < a.xrows ? x0 + row : a.xrows - 1;  // rows past R are never stored
@@ -481,16 +489,19 @@ __device__ inline void bf16_mainloop(const GemmArgs& a, const uint16_t* S, uint1
     ldst[j] = r0 * 32;  // wave-uniform LDS element offset of the 16-row block
   }
 
+  const bool load_x = OPW == 2 || wave < 8, load_s = OPW == 2 || wave >= 8;
 #define SOMAX_ISSUE(KT, STG)                                                                         \
   {                                                                                                  \
     uint16_t* sb_ = lds + (STG) * C::STAGE;                                                          \
     const int k0_ = (KT) * 32;                                                                       \
     _Pragma("unroll") for (int p = 0; p < NPC; ++p)                                                 \
       _Pragma("unroll") for (int j = 0; j < RPT; ++j) {                                              \
-        __builtin_amdgcn_global_load_lds(xsrc[j] + p * a.x_piece + k0_, sb_ + (xslot + p) * C::TILE + ldst[j], 16, \
-                                         0, 0);                                                      \
-        __builtin_amdgcn_global_load_lds(ssrc[j] + p * a.s_piece + k0_, sb_ + (sslot + p) * C::TILE + ldst[j], 16, \
-                                         0, 0);                                                      \
+        if (load_x)                                                                                  \
+          __builtin_amdgcn_global_load_lds(xsrc[j] + p * a.x_piece + k0_, sb_ + (xslot + p) * C::TILE + ldst[j], \
+                                           16, 0, 0);                                                \
+        if (load_s)                                                                                  \
+          __builtin_amdgcn_global_load_lds(ssrc[j] + p * a.s_piece + k0_, sb_ + (sslot + p) * C::TILE + ldst[j], \
+                                           16, 0, 0);                                                \
       }                                                                                              \
   }
 
@@ -499,18 +510,18 @@ __device__ inline void bf16_mainloop(const GemmArgs& a, const uint16_t* S, uint1
   for (int s = 0; s < C::NSTAGE - 1; ++s)
     if (s < nk) SOMAX_ISSUE(s, s)
 
-  int ra[2], rb[TN];
+  int ra[TM], rb[TN];
 #pragma unroll
-  for (int t = 0; t < 2; ++t) ra[t] = wm * 64 + t * 32 + (lane & 31);
+  for (int t = 0; t < TM; ++t) ra[t] = wm * 32 * TM + t * 32 + (lane & 31);
 #pragma unroll
   for (int t = 0; t < TN; ++t) rb[t] = wn * 32 * TN + t * 32 + (lane & 31);
   // fragments of k16 step kk of the tile in stage buffer sb
-  auto read_frags = [&](const uint16_t* sb, int kk, bf16x8(&fa)[3][2], bf16x8(&fb)[3][TN]) {
+  auto read_frags = [&](const uint16_t* sb, int kk, bf16x8(&fa)[3][TM], bf16x8(&fb)[3][TN]) {
     const int c = kk * 2 + (lane >> 5);  // logical 16-byte chunk of this lane's k-slice
 #pragma unroll
     for (int p = 0; p < NPC; ++p) {
 #pragma unroll
-      for (int t = 0; t < 2; ++t)
+      for (int t = 0; t < TM; ++t)
         fa[p][t] = *reinterpret_cast<const bf16x8*>(sb + p * C::TILE + ra[t] * 32 + ((c ^ swz(ra[t])) << 3));
 #pragma unroll
       for (int t = 0; t < TN; ++t)
@@ -546,7 +557,7 @@ __device__ inline void bf16_mainloop(const GemmArgs& a, const uint16_t* S, uint1
   // (Two sets when two waves share a SIMD and three-piece fragments would not
   // fit: the other wave covers the read-after-MFMA wait.)
   constexpr int NS = (TN == 1 && NPC == 3) ? 2 : 4;
-  bf16x8 fsa[NS][3][2], fsb[NS][3][TN];
+  bf16x8 fsa[NS][3][TM], fsb[NS][3][TN];
   auto& fa0 = fsa[0];
   auto& fb0 = fsb[0];
   auto& fa1 = fsa[1];
@@ -567,16 +578,16 @@ __device__ inline void bf16_mainloop(const GemmArgs& a, const uint16_t* S, uint1
     lgkm0();                                                \
     read_frags(stage_of(KT), 1, YA, YB);                    \
     __builtin_amdgcn_sched_barrier(0);                      \
-    mfma_products<NPROD, TN>(XA, XB, W);                        \
+    mfma_products<NPROD, TM, TN>(XA, XB, W);                        \
     __builtin_amdgcn_sched_barrier(0);                      \
     enter_tile((KT) + 1);                                   \
     read_frags(stage_of((KT) + 1), 0, ZA, ZB);              \
     __builtin_amdgcn_sched_barrier(0);                      \
-    mfma_products<NPROD, TN>(YA, YB, W);                        \
+    mfma_products<NPROD, TM, TN>(YA, YB, W);                        \
     __builtin_amdgcn_sched_barrier(0);                      \
   }
   // two tiles, leaving the fragments of tile kt+2 step 0 in set 0
-  auto pair = [&](int kt, floatx16(&w)[2][TN]) {
+  auto pair = [&](int kt, floatx16(&w)[TM][TN]) {
     SOMAX_TILE_MID(kt, w, fa0, fb0, fa1, fb1, fa2, fb2)
     SOMAX_TILE_MID(kt + 1, w, fa2, fb2, fa3, fb3, fa0, fb0)
   };
@@ -587,14 +598,14 @@ __device__ inline void bf16_mainloop(const GemmArgs& a, const uint16_t* S, uint1
     // accumulator is carried around the loop (it stays in AGPRs).
     static_assert(C::FLUSH == 2, "one window per tile pair");
     for (; kt + 2 <= nk - 1; kt += 2) {
-      floatx16 w[2][TN];
+      floatx16 w[TM][TN];
 #pragma unroll
-      for (int i = 0; i < 2; ++i)
+      for (int i = 0; i < TM; ++i)
 #pragma unroll
         for (int j = 0; j < TN; ++j) w[i][j] = floatx16{};
       pair(kt, w);
 #pragma unroll
-      for (int i = 0; i < 2; ++i)
+      for (int i = 0; i < TM; ++i)
 #pragma unroll
         for (int j = 0; j < TN; ++j)
 #pragma unroll
@@ -608,13 +619,13 @@ __device__ inline void bf16_mainloop(const GemmArgs& a, const uint16_t* S, uint1
     SOMAX_TILE_MID(kt, acc, fa0, fb0, fa1, fb1, fa2, fb2)
     lgkm0();
     read_frags(stage_of(nk - 1), 1, fa3, fb3);
-    mfma_products<NPROD, TN>(fa2, fb2, acc);
-    mfma_products<NPROD, TN>(fa3, fb3, acc);
+    mfma_products<NPROD, TM, TN>(fa2, fb2, acc);
+    mfma_products<NPROD, TM, TN>(fa3, fb3, acc);
   } else {
     lgkm0();
     read_frags(stage_of(nk - 1), 1, fa1, fb1);
-    mfma_products<NPROD, TN>(fa0, fb0, acc);
-    mfma_products<NPROD, TN>(fa1, fb1, acc);
+    mfma_products<NPROD, TM, TN>(fa0, fb0, acc);
+    mfma_products<NPROD, TM, TN>(fa1, fb1, acc);
   }
 #undef SOMAX_TILE_MID
 #undef SOMAX_ISSUE
@@ -628,22 +639,26 @@ size_t dst_gemm_lds_bytes(int max_np) {
   return (size_t)nstage * 2 * npc * 128 * 32 * sizeof(uint16_t);
 }
 
-// Workgroup shape per instance: 8 waves of 64 x 32 (TN = 1, two waves per
-// SIMD, so one wave's MFMAs overlap the other's LDS traffic: 23-33% faster
-// than 4 waves of 64 x 64 at the same tile).
+// Workgroup shape per instance: x1 / x3 run 16 waves of 32 x 32 (four per
+// SIMD), x6 8 waves of 64 x 32 (its three-piece fragments need the
+// registers).  Several waves per SIMD let one wave's MFMAs overlap another's
+// LDS traffic: 8 waves are 23-33% faster than 4 at the same tile, and 16 a
+// further 12-20% on the many small tiles of ensemble grids.
 template <int NPMAX>
 struct KernelShape {
+  static constexpr int TM = NPMAX == 6 ? 2 : 1;
   static constexpr int TN = 1;
-  static constexpr int NT = 128 * (4 / TN);
+  static constexpr int NT = 64 * (4 / TM) * (4 / TN);
+  static constexpr int WPE = NT / 256;  // waves per SIMD
 };
 
 // One instantiation per largest product count of a launch (x1, x3, x6), so
 // each is a kernel symbol of its own; planes may use fewer products.
 template <int NPMAX>
 __global__ __launch_bounds__(KernelShape<NPMAX>::NT)
-__attribute__((amdgpu_waves_per_eu(KernelShape<NPMAX>::TN == 1 ? 2 : 1, KernelShape<NPMAX>::TN == 1 ? 2 : 1)))
+__attribute__((amdgpu_waves_per_eu(KernelShape<NPMAX>::WPE, KernelShape<NPMAX>::WPE)))
 void dst_gemm_bf16_kernel(GemmArgs a) {
-  constexpr int TN = KernelShape<NPMAX>::TN, NT = KernelShape<NPMAX>::NT;
+  constexpr int TM = KernelShape<NPMAX>::TM, TN = KernelShape<NPMAX>::TN, NT = KernelShape<NPMAX>::NT;
   extern __shared__ __attribute__((aligned(16))) uint16_t lds16[];
   const GemmCut cut = gemm_cut(a.RA, a.RB, a.K);
   const int GA = (cut.am + BM - 1) / BM, GB = (cut.bm + BN - 1) / BN;
@@ -672,18 +687,18 @@ void dst_gemm_bf16_kernel(GemmArgs a) {
   const bool x_in_a = !a.trans;
   const uint16_t* S = a.Sb + ps.s_off;
 
-  floatx16 acc[2][TN];
-  double sum[2][TN][16];
-  zero_acc<TN>(acc, sum);
+  floatx16 acc[TM][TN];
+  double sum[TM][TN][16];
+  zero_acc<TM, TN>(acc, sum);
   // FP32 accumulation over K ~ 1e3 costs ~2e-7 relative, far below the
   // 3-product split error (~1e-5): only 6-product planes accumulate in FP64.
-  if (NPMAX == 1 || np == 1) bf16_mainloop<1, false, TN>(a, S, lds16, z, a0, b0, x_in_a, acc, sum);
-  else if (NPMAX == 3 || np == 3) bf16_mainloop<3, false, TN>(a, S, lds16, z, a0, b0, x_in_a, acc, sum);
-  else if (a.acc64) bf16_mainloop<6, true, TN>(a, S, lds16, z, a0, b0, x_in_a, acc, sum);
-  else bf16_mainloop<6, false, TN>(a, S, lds16, z, a0, b0, x_in_a, acc, sum);
+  if (NPMAX == 1 || np == 1) bf16_mainloop<1, false, TM, TN>(a, S, lds16, z, a0, b0, x_in_a, acc, sum);
+  else if (NPMAX == 3 || np == 3) bf16_mainloop<3, false, TM, TN>(a, S, lds16, z, a0, b0, x_in_a, acc, sum);
+  else if (a.acc64) bf16_mainloop<6, true, TM, TN>(a, S, lds16, z, a0, b0, x_in_a, acc, sum);
+  else bf16_mainloop<6, false, TM, TN>(a, S, lds16, z, a0, b0, x_in_a, acc, sum);
   float* stage = reinterpret_cast<float*>(lds16);
-  if (NPMAX == 6 && np == 6 && a.acc64) store_tile<true, TN>(a, z, a0, b0, acc, sum, opieces, stage, ps.sp);
-  else store_tile<false, TN>(a, z, a0, b0, acc, sum, opieces, stage, ps.sp);
+  if (NPMAX == 6 && np == 6 && a.acc64) store_tile<true, TM, TN>(a, z, a0, b0, acc, sum, opieces, stage, ps.sp);
+  else store_tile<false, TM, TN>(a, z, a0, b0, acc, sum, opieces, stage, ps.sp);
 }
 template __global__ void dst_gemm_bf16_kernel<1>(GemmArgs);
 template __global__ void dst_gemm_bf16_kernel<3>(GemmArgs);
@@ -732,7 +747,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 2))) voi
   float4 rx0, rx1, rs0, rs1;
   floatx16 acc[2][2];
   double sum[2][2][16];
-  zero_acc<2>(acc, sum);
+  zero_acc<2, 2>(acc, sum);
   const bool acc64 = a.acc64 != 0;
   const int nk = (a.K + BK32 - 1) / BK32;
   rx0 = *reinterpret_cast<const float4*>(xg0);
@@ -803,8 +818,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 2))) voi
     }
     for (; kt < nk; ++kt) ktile(kt, acc);
   }
-  if (acc64) store_tile<true, 2>(a, z, a0, b0, acc, sum, 0, lds, ps.sp);
-  else store_tile<false, 2>(a, z, a0, b0, acc, sum, 0, lds, ps.sp);
+  if (acc64) store_tile<true, 2, 2>(a, z, a0, b0, acc, sum, 0, lds, ps.sp);
+  else store_tile<false, 2, 2>(a, z, a0, b0, acc, sum, 0, lds, ps.sp);
 }
 
 hipError_t prepare_dst_gemm() {
