@@ -114,16 +114,20 @@ __global__ __launch_bounds__(256) void qg_stage_arakawa_kernel(StageArgs a) {
     }
     rhs += wc * wind_row[gy];
     const long idx = off + (long)gy * Nx + gx;
+    // raw q_s: the tile's copy, except on the ghost ring when the boundary
+    // condition zeroed it there
+    const bool gring = gy == 0 || gx == 0 || gy == Ny - 1 || gx == Nx - 1;
+    auto qraw = [&]() { return (gring && a.apply_bc) ? a.qs[idx] : sq[(ly + 2) * QW + tx + 2]; };
     // Stage combination qn + c1 (qs - qn) + c2 rhs: the FP32 weights of the
     // Shu-Osher form (1/3 + 2/3 != 1 in FP32) would otherwise scale q by
     // 1 + 3e-8 per step.  Without qn: c1 qs + c2 rhs.
     float o;
     if (a.qn) {
       const float qn = a.qn[idx];
-      o = qn + a.c1 * (a.qs[idx] - qn) + a.c2 * rhs;
+      o = qn + a.c1 * (qraw() - qn) + a.c2 * rhs;
     } else {
       o = a.c2 * rhs;
-      if (a.c1 != 0.0f) o += a.c1 * a.qs[idx];
+      if (a.c1 != 0.0f) o += a.c1 * qraw();
     }
     a.out[idx] = o;
   }
@@ -337,13 +341,15 @@ __global__ __launch_bounds__(256) void qg_stage_weno_kernel(StageArgs a) {
     }
     rhs += wc * wind_row[gy];
     const long idx = off + (long)gy * Nx + gx;
+    const bool gring = gy == 0 || gx == 0 || gy == Ny - 1 || gx == Nx - 1;
+    auto qraw = [&]() { return (gring && a.apply_bc) ? a.qs[idx] : sq[(ly + 3) * WQ + tx + 3]; };
     float o;
     if (a.qn) {
       const float qn = a.qn[idx];
-      o = qn + a.c1 * (a.qs[idx] - qn) + a.c2 * rhs;
+      o = qn + a.c1 * (qraw() - qn) + a.c2 * rhs;
     } else {
       o = a.c2 * rhs;
-      if (a.c1 != 0.0f) o += a.c1 * a.qs[idx];
+      if (a.c1 != 0.0f) o += a.c1 * qraw();
     }
     a.out[idx] = o;
   }
