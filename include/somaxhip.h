@@ -267,6 +267,28 @@ int somaxhip_rqg_step(somaxhip_swm_plan swm, somaxhip_qg_plan qg, int solver, do
                       somaxhip_stream_t stream);
 
 /* ------------------------------------------------------------------------ */
+/* Run diagnostics (NaN flags and field ranges between integration chunks)  */
+/* ------------------------------------------------------------------------ */
+
+#define SOMAXHIP_STATS_MAX_FIELDS 64
+
+/* The per-field reduction of the runner's chunk protocol: _state_diagnostics
+ * (cli/_run.py:408-439, min / mean / max / NaN count of each State field),
+ * _has_non_finite (:499-501) and the non-finite counts of _assert_finite_state
+ * (:703-753), computed on the device instead of after a full-field host copy.
+ * fields[f]: device float32 pointer (host array of nfields pointers, any
+ * alignment), counts[f] elements.  stats: device double[5 * nfields], per
+ * field { min, max, sum, n_nan, n_nonfinite }, where min, max and sum skip
+ * NaN only (numpy's nanmin / nanmax / nansum: +-Inf take part) and
+ * n_nonfinite counts NaN and +-Inf (numpy's ~isfinite).  An empty field gives
+ * { +Inf, -Inf, 0, 0, 0 }.  Deterministic; asynchronous on stream.
+ * workspace: somaxhip_stats_workspace_bytes(nfields) bytes (0 if nfields is
+ * out of range). */
+size_t somaxhip_stats_workspace_bytes(int nfields);
+int somaxhip_field_stats(const float* const* fields, const size_t* counts, int nfields, double* stats,
+                         void* workspace, somaxhip_stream_t stream);
+
+/* ------------------------------------------------------------------------ */
 /* Ensembles                                                                */
 /* ------------------------------------------------------------------------ */
 

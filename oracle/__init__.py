@@ -37,4 +37,7 @@ Parity pins (each function's docstring names its own):
                    stencils are inside the array, divergence-free geostrophic
                    velocity, and the reference's qualitative tests (rest state
                    fixed point, approximate idempotence).
+* ``run``       -- the runner's per-field reduction (NaN-skipping min / mean /
+                   max, non-finite count): the reference's own cases and a
+                   brute-force loop over IEEE-754 bit classes.
 """

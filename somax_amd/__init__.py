@@ -43,6 +43,7 @@ from .swm import (
 )
 from .reparam import ReparameterizedQG, ReparamQGDiagnostics
 from .testcases import doublegyre_baroclinic_qg, doublegyre_qg, doublegyre_reparameterized_qg
+from .run import IntegrationDivergedError, assert_finite_state, chunked_integrate, state_diagnostics
 
 __all__ = [
     "ArakawaCGrid2D", "Diagnostics", "ModalTransform", "Params", "PhysConsts", "Solution", "SomaxModel",
@@ -50,5 +51,8 @@ __all__ = [
     "BaroclinicQGPhysConsts", "BaroclinicQGState", "BarotropicQG", "BarotropicQGDiagnostics",
     "BarotropicQGParams", "BarotropicQGPhysConsts", "BarotropicQGState", "doublegyre_baroclinic_qg",
     "doublegyre_qg", "QGEnsemble", "MultilayerShallowWater2D", "MultilayerSW2DDiagnostics", "MultilayerSW2DParams",
-    "MultilayerSW2DPhysConsts", "MultilayerSW2DState",
+    "MultilayerSW2DPhysConsts", "MultilayerSW2DState", "NonlinearShallowWater2D", "NonlinearSW2DDiagnostics",
+    "NonlinearSW2DParams", "NonlinearSW2DPhysConsts", "NonlinearSW2DState", "ReparameterizedQG",
+    "ReparamQGDiagnostics", "doublegyre_reparameterized_qg", "IntegrationDivergedError", "assert_finite_state",
+    "chunked_integrate", "state_diagnostics",
 ]

@@ -25,6 +25,7 @@ SOLVER_TSIT5 = 1
 BC_PERIODIC = 0
 BC_WALL = 1
 BC_CHANNEL = 2
+STATS_MAX_FIELDS = 64
 
 
 class SomaxHipError(RuntimeError):
@@ -155,6 +156,11 @@ _SIGNATURES = {
         ctypes.c_int,
         [_vp, _vp, _vp, _vp, ctypes.c_int, ctypes.c_double, ctypes.POINTER(SWMParams), _vp, _vp,
          ctypes.POINTER(KernelStat), ctypes.c_int, ctypes.POINTER(ctypes.c_int)],
+    ),
+    "somaxhip_stats_workspace_bytes": (ctypes.c_size_t, [ctypes.c_int]),
+    "somaxhip_field_stats": (
+        ctypes.c_int,
+        [ctypes.POINTER(_vp), ctypes.POINTER(ctypes.c_size_t), ctypes.c_int, _vp, _vp, _vp],
     ),
     "somaxhip_ens_create": (
         ctypes.c_int,

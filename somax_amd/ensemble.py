@@ -119,6 +119,18 @@ class QGEnsemble:
             "somaxhip_ens_step",
         )
 
+    def member_diagnostics(self, packed) -> list[dict]:
+        """Per member ``{"min", "mean", "max", "nan"}`` of its PV field, as
+        the runner's state diagnostics (NaN-skipping statistics, count of
+        non-finite values), reduced on the device in one call per 64
+        members: the ensemble's NaN flags."""
+        from .run import _summary, field_stats
+
+        self._check_packed(packed)
+        views = self.unpack(packed)
+        rows = field_stats(views)
+        return [_summary(r, v.numel()) for r, v in zip(rows, views)]
+
     def set_params(self, params) -> None:
         """Replace the members' parameters (one params object per member)."""
         if len(params) != len(self.models):
