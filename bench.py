@@ -122,11 +122,14 @@ def cpu_baseline(n: int, q64, dt: float, steps: int) -> dict:
 
     setup = oqg.baroclinic_setup(nx=n, ny=n, Lx=tc.BASIN, Ly=tc.BASIN, f0=tc.F0, beta=tc.BETA, bottom_drag=tc.DRAG,
                                  wind_amplitude=tc.wind_amplitude(), biharmonic_viscosity=tc.NU4_128 * (128 / n) ** 4)
+    from threadpoolctl import threadpool_limits
+
     q = q64.copy()
-    t0 = time.perf_counter()
-    for _ in range(steps):
-        q = oqg.ssprk3_step(setup, q, dt)
-    el = time.perf_counter() - t0
+    with threadpool_limits(1):
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            q = oqg.ssprk3_step(setup, q, dt)
+        el = time.perf_counter() - t0
     nl = q.shape[0]
     return {"value": round(nl * n * n * steps / el, 1), "unit": "layer-cell-steps/s", "cores": 1, "kind": "port",
             "sample": f"{steps} SSP-RK3 steps of the same {nl}x{n}x{n} workload, FP64 NumPy/SciPy oracle "
@@ -239,6 +242,62 @@ def bench_swm(steps: int, warmup: int) -> dict:
             "value": round(cells / (ms * 1e-3), 1), "unit": "layer-cell-steps/s", "roofline": roof}
 
 
+def bench_cmm(solves: int, cpu_solves: int) -> dict:
+    """The masked-domain Helmholtz solve of mqg_demo_octagonal.ipynb: the
+    octagon of 256 x 256 nodes, three rigid-lid modes (H = 400/1100/2600 m),
+    mixed-precision inversions; one capacitance-matrix solve = two rectangular
+    DST-I inversions + the FP64 boundary correction.  CPU beside it: the FP64
+    NumPy oracle of the same solve."""
+    import time
+
+    import numpy as np
+    import torch
+
+    from somax_amd.core import coupling_matrix
+    from somax_amd.elliptic import MaskedHelmholtz, octagon_node_mask
+
+    N, L, f0 = 256, 5120e3, 9.375e-5
+    A = coupling_matrix([400.0, 1100.0, 2600.0], [9.81, 0.025, 0.0125])
+    A[0, 0] -= 1.0 / (400.0 * 9.81)  # rigid lid
+    lams = f0**2 * np.sort(np.linalg.eigvals(A).real)
+    mask = octagon_node_mask(N, N)
+    d = L / (N - 1)
+    t = time.perf_counter()
+    solver = MaskedHelmholtz(mask, d, d, lams, precision="mixed")
+    setup_s = time.perf_counter() - t
+    rhs64 = np.random.default_rng(11).standard_normal((3, N, N)) * 1e-10
+    rhs = torch.from_numpy(rhs64.astype(np.float32)).cuda()
+    psi = solver.solve(rhs)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(solves):
+        solver.solve(rhs)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / solves
+    pts = int(mask.sum()) * 3
+    cpu, err = None, None
+    if cpu_solves > 0:  # the CPU leg: the oracle timed on the same solve, its result checks the GPU's
+        from oracle import cmm as ocmm
+
+        from threadpoolctl import threadpool_limits
+
+        r = rhs.double().cpu().numpy()
+        with threadpool_limits(1):
+            t = time.perf_counter()
+            for _ in range(cpu_solves):
+                want = ocmm.solve(r, mask, d, d, lams, solver.cap, solver.yids, solver.xids)
+            cs = (time.perf_counter() - t) / cpu_solves
+        got = psi.double().cpu().numpy()
+        err = float(f"{np.linalg.norm(got - want) / np.linalg.norm(want):.3e}")
+        cpu = {"value": round(pts / cs, 1), "unit": "masked-point-solves/s", "cores": 1, "kind": "port",
+               "sample": f"{cpu_solves} solves of the same 3x256x256 octagon, FP64 NumPy oracle (oracle/cmm.py)"}
+    return {"workload": "cmm_octagon3x256x256_mixed", "boundary_points": solver.num_boundary_points,
+            "ms_per_solve": round(ms, 4), "value": round(pts / (ms * 1e-3), 1), "unit": "masked-point-solves/s",
+            "rel_l2_vs_oracle": err, "setup_s": round(setup_s, 2), "cpu_baseline": cpu}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -251,6 +310,7 @@ def main() -> int:
     ap.add_argument("--cpu-steps", type=int, default=15, help="oracle steps timed for cpu_baseline (0 = skip)")
     ap.add_argument("--ensemble-steps", type=int, default=10, help="steps timed for the 256-member ensemble (0 = skip)")
     ap.add_argument("--swm-steps", type=int, default=50, help="steps timed for the shallow-water jet (0 = skip)")
+    ap.add_argument("--cmm-solves", type=int, default=20, help="masked-domain solves timed (0 = skip)")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -301,6 +361,12 @@ def main() -> int:
         swm = bench_swm(args.swm_steps, 2)
         log(f"[bench] swm {swm['ms_per_step']:.3f} ms/step")
 
+    cmm = None
+    if args.cmm_solves > 0:
+        log("[bench] masked-domain Helmholtz (octagon 3x256x256)")
+        cmm = bench_cmm(args.cmm_solves, 3 if args.cpu_steps > 0 else 0)
+        log(f"[bench] cmm {cmm['ms_per_solve']:.3f} ms/solve")
+
     cpu = None
     if args.cpu_steps > 0:
         log("[bench] cpu baseline (oracle)")
@@ -331,6 +397,7 @@ def main() -> int:
                                  for k, v in prof_mixed.items()},
         "ensemble": ens,
         "swm": swm,
+        "cmm": cmm,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

@@ -267,6 +267,32 @@ int somaxhip_rqg_step(somaxhip_swm_plan swm, somaxhip_qg_plan qg, int solver, do
                       somaxhip_stream_t stream);
 
 /* ------------------------------------------------------------------------ */
+/* Helmholtz solve on a masked domain (capacitance-matrix method)           */
+/* ------------------------------------------------------------------------ */
+
+/* Replaces inverse_elliptic_dst_cmm / compute_capacitance_matrices of the
+ * former somax._src.operators.dst, used in
+ * notebooks/dev/QG_Multilayer/helmholtz_decomp.ipynb (cells 19-33) and
+ * mqg_demo_octagonal.ipynb (cell 12); the module is no longer in the tree.
+ * Solves (lap - lam_p) psi_p = rhs_p on the points where mask = 1, with
+ * psi = 0 elsewhere, for every plane p of float32[P][Ny][Nx] fields.
+ * plan: a QG plan whose inversion is the rectangular solve G: identity
+ * modal matrices (cl2m = cm2l = I), lambdas = lam_p, zero_psi_ghost = 0,
+ * no mass correction; it must outlive the solver.
+ * mask: host float[Ny*Nx] of 0 / 1.  bidx: host int[nb], the flat indices
+ * (row * Nx + column) of the points outside the mask that touch it in a 3x3
+ * window.  cap: host double[P][nb][nb], per plane the inverse of the matrix
+ * G[i][j] = (G e_{b_j})[b_i].  A solve is
+ *   r0 = rhs * mask;  psi1 = G r0;  r0[b] = -cap psi1[b];  psi = (G r0) * mask. */
+typedef struct somaxhip_cmm_s* somaxhip_cmm;
+int somaxhip_cmm_create(somaxhip_qg_plan plan, const float* mask, int nb, const int* bidx, const double* cap,
+                        somaxhip_cmm* cmm);
+int somaxhip_cmm_destroy(somaxhip_cmm cmm);
+/* Workspace of a solve: the plan's workspace followed by two fields. */
+size_t somaxhip_cmm_workspace_bytes(somaxhip_cmm cmm);
+int somaxhip_cmm_solve(somaxhip_cmm cmm, const float* rhs, float* psi, void* workspace, somaxhip_stream_t stream);
+
+/* ------------------------------------------------------------------------ */
 /* Run diagnostics (NaN flags and field ranges between integration chunks)  */
 /* ------------------------------------------------------------------------ */
 

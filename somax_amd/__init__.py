@@ -43,6 +43,7 @@ from .swm import (
 )
 from .reparam import ReparameterizedQG, ReparamQGDiagnostics
 from .testcases import doublegyre_baroclinic_qg, doublegyre_qg, doublegyre_reparameterized_qg
+from .elliptic import MaskedHelmholtz, compute_capacitance_matrices, irregular_boundary
 from .run import IntegrationDivergedError, assert_finite_state, chunked_integrate, state_diagnostics
 
 __all__ = [
@@ -54,5 +55,6 @@ __all__ = [
     "MultilayerSW2DPhysConsts", "MultilayerSW2DState", "NonlinearShallowWater2D", "NonlinearSW2DDiagnostics",
     "NonlinearSW2DParams", "NonlinearSW2DPhysConsts", "NonlinearSW2DState", "ReparameterizedQG",
     "ReparamQGDiagnostics", "doublegyre_reparameterized_qg", "IntegrationDivergedError", "assert_finite_state",
-    "chunked_integrate", "state_diagnostics",
+    "chunked_integrate", "state_diagnostics", "MaskedHelmholtz", "compute_capacitance_matrices",
+    "irregular_boundary",
 ]

@@ -157,6 +157,13 @@ _SIGNATURES = {
         [_vp, _vp, _vp, _vp, ctypes.c_int, ctypes.c_double, ctypes.POINTER(SWMParams), _vp, _vp,
          ctypes.POINTER(KernelStat), ctypes.c_int, ctypes.POINTER(ctypes.c_int)],
     ),
+    "somaxhip_cmm_create": (
+        ctypes.c_int,
+        [_vp, c_float_p, ctypes.c_int, ctypes.POINTER(ctypes.c_int), c_double_p, ctypes.POINTER(_vp)],
+    ),
+    "somaxhip_cmm_destroy": (ctypes.c_int, [_vp]),
+    "somaxhip_cmm_workspace_bytes": (ctypes.c_size_t, [_vp]),
+    "somaxhip_cmm_solve": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp]),
     "somaxhip_stats_workspace_bytes": (ctypes.c_size_t, [ctypes.c_int]),
     "somaxhip_field_stats": (
         ctypes.c_int,

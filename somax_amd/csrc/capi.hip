@@ -345,6 +345,16 @@ struct somaxhip_qg_plan_s {
   Batch b;
 };
 
+namespace somax {
+int qg_plan_shape(somaxhip_qg_plan p, int* planes, int* Ny, int* Nx) {
+  if (!p) return SOMAXHIP_ERR_ARG;
+  *planes = p->b.P;
+  *Ny = p->b.Ny;
+  *Nx = p->b.Nx;
+  return SOMAXHIP_OK;
+}
+}  // namespace somax
+
 struct somaxhip_ens_s {
   std::vector<std::unique_ptr<Batch>> cls;  // grid classes, largest first
   std::vector<int> mclass;                  // input member -> class

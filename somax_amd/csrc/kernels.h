@@ -8,9 +8,14 @@
 
 #include "common.h"
 
+struct somaxhip_qg_plan_s;
+
 namespace somax {
 
 constexpr int MAX_LAYERS = 8;
+
+// Planes, rows and columns (ghosts included) of a QG plan's fields.
+int qg_plan_shape(somaxhip_qg_plan_s* plan, int* planes, int* Ny, int* Nx);
 
 // Member and plane tables of a batch (device arrays).
 struct Tables {

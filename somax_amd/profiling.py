@@ -127,7 +127,7 @@ def main(argv=None) -> int:
                 json.dump(per, f, indent=1)
         return 0
     ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
-    ap.add_argument("--workload", default="qg1024-mixed", choices=["qg1024-mixed", "qg1024-fp32", "ensemble", "swm512"])
+    ap.add_argument("--workload", default="qg1024-mixed", choices=["qg1024-mixed", "qg1024-fp32", "ensemble", "swm512", "cmm256"])
     ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--copy-mib", type=int, default=1024)
     ap.add_argument("--dt", type=float, default=1200.0)
@@ -173,6 +173,30 @@ def main(argv=None) -> int:
         st = model.step(st, dt, args.steps)
         torch.cuda.synchronize()
         print(f"swm512: {args.steps} steps", flush=True)
+        return 0
+
+    if args.workload == "cmm256":
+        # The octagon of mqg_demo_octagonal.ipynb: 256 nodes, L = 5120 km,
+        # three rigid-lid modes; one masked Helmholtz solve per "step".
+        from .core import coupling_matrix
+        from .elliptic import MaskedHelmholtz, octagon_node_mask
+
+        N, L, f0 = 256, 5120e3, 9.375e-5
+        A = coupling_matrix([400.0, 1100.0, 2600.0], [9.81, 0.025, 0.0125])
+        A[0, 0] -= 1.0 / (400.0 * 9.81)  # rigid lid, as the notebook's LayerDomain
+        lams = f0**2 * np.sort(np.linalg.eigvals(A).real)
+        solver = MaskedHelmholtz(octagon_node_mask(N, N), L / (N - 1), L / (N - 1), lams, precision="mixed")
+        rhs = torch.randn((3, N, N), device="cuda") * 1e-10
+        solver.solve(rhs)
+        torch.cuda.synchronize()
+        t0, t1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        t0.record()
+        for _ in range(args.steps):
+            solver.solve(rhs)
+        t1.record()
+        torch.cuda.synchronize()
+        print(f"cmm256: {solver.num_boundary_points} boundary points, 3 planes, {args.steps} solves, "
+              f"{t0.elapsed_time(t1) / max(args.steps, 1):.4f} ms per solve", flush=True)
         return 0
 
     precision = "mixed" if args.workload.endswith("mixed") else "fp32"
