@@ -376,8 +376,9 @@ int somaxhip_qg_step_profile(somaxhip_qg_plan plan, float* q, int nsteps, double
                              somaxhip_stream_t stream, somaxhip_kernel_stat* stats,
                              int max_stats, int* nstats);
 
-/* somaxhip_swm_step with every stage launch timed (HIP events, one
- * synchronisation per launch). */
+/* somaxhip_swm_step with every stage launch bracketed by HIP events, read
+ * after the step (one synchronisation at the end, none between launches);
+ * stats[0] = "swm_stage". */
 int somaxhip_swm_step_profile(somaxhip_swm_plan plan, float* h, float* u, float* v, int nsteps, double dt,
                               const somaxhip_swm_params* params, void* workspace, somaxhip_stream_t stream,
                               somaxhip_kernel_stat* stats, int max_stats, int* nstats);

@@ -14,6 +14,21 @@ namespace somax {
 
 constexpr int MAX_LAYERS = 8;
 
+// Workgroups are dispatched to the 8 XCDs round-robin by linear ID (x
+// fastest, then y, then z).  xcd_tile3 remaps the linear ID of a 3-D grid so
+// that each XCD takes one contiguous run of tiles in (x, y, z) order: tiles
+// that share halo rows then share that XCD's L2.
+__device__ inline void xcd_tile3(int& bx, int& by, int& bz) {
+  const unsigned nx = gridDim.x, ny = gridDim.y;
+  const unsigned T = nx * ny * gridDim.z;
+  const unsigned b = blockIdx.x + nx * (blockIdx.y + ny * blockIdx.z);
+  const unsigned xcd = b & 7u, slot = b >> 3, per = T >> 3, extra = T & 7u;
+  const unsigned t = xcd * per + (xcd < extra ? xcd : extra) + slot;
+  bx = (int)(t % nx);
+  by = (int)((t / nx) % ny);
+  bz = (int)(t / (nx * ny));
+}
+
 // Planes, rows and columns (ghosts included) of a QG plan's fields.
 int qg_plan_shape(somaxhip_qg_plan_s* plan, int* planes, int* Ny, int* Nx);
 

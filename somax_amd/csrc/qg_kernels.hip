@@ -27,9 +27,10 @@ __global__ __launch_bounds__(256) void qg_stage_arakawa_kernel(StageArgs a) {
   __shared__ float sdb[TY + 4];
 
   const int tid = threadIdx.x;
-  const int z = blockIdx.z;
-  const int x0 = blockIdx.x * TX;
-  const int y0 = blockIdx.y * TY;
+  int bx, by, z;
+  xcd_tile3(bx, by, z);
+  const int x0 = bx * TX;
+  const int y0 = by * TY;
   const int Nx = a.Nx, Ny = a.Ny;
   const long off = (long)z * a.plane;
   const float* qs = a.qs + off;
@@ -206,9 +207,10 @@ __global__ __launch_bounds__(256) void qg_stage_weno_kernel(StageArgs a) {
   __shared__ float sdb[WY + 6];                 // b[j+1] - b[j] for rows y0-3 ..
 
   const int tid = threadIdx.x;
-  const int z = blockIdx.z;
-  const int x0 = blockIdx.x * WX;
-  const int y0 = blockIdx.y * WY;
+  int bx, by, z;
+  xcd_tile3(bx, by, z);
+  const int x0 = bx * WX;
+  const int y0 = by * WY;
   const int Nx = a.Nx, Ny = a.Ny;
   const int nxi = Nx - 2, nyi = Ny - 2;
   const long off = (long)z * a.plane;
@@ -428,6 +430,8 @@ __global__ __launch_bounds__(256) void residual_kernel(ResidualArgs a) {
   store_pieces(o, a.op_piece, 2, r);
 }
 
+// (Remapping these workgroups XCD by XCD, as the stage kernels are, halves
+// the residual's HBM traffic but makes it slower: 16 -> 20 us at 1024^2.)
 // Folded residual on the parity components (block b = 2p + q of plane m at
 // b * P + m): the 5-point Laplacian of a component on the half array, with
 // zero exterior and the mirror rule at the fold line (the neighbour past the

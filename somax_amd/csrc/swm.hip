@@ -355,11 +355,15 @@ struct somaxhip_swm_plan_s {
 
 namespace {
 
+// Stage launches bracketed by event pairs; the pairs are read after the
+// whole step so that no host synchronisation sits between launches.
 struct SwmProf {
   somaxhip_kernel_stat* stats;
   int max_stats;
   int n;
   hipError_t err;
+  std::vector<hipEvent_t> ev;  // e0, e1 per launch
+  std::vector<double> bytes;   // algorithmic bytes per launch
 };
 thread_local SwmProf* g_swm_prof = nullptr;
 
@@ -408,25 +412,12 @@ void swm_stage(somaxhip_swm_plan p, const float* hs, const float* us, const floa
   if (g_swm_prof) {
     (void)hipEventRecord(e1, st);
     SwmProf& pr = *g_swm_prof;
-    float ms = 0.0f;
-    hipError_t e = hipEventSynchronize(e1);
-    if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
-    if (e != hipSuccess && pr.err == hipSuccess) pr.err = e;
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
-    if (pr.n == 0 && pr.max_stats > 0) {
-      std::memset(&pr.stats[0], 0, sizeof(pr.stats[0]));
-      std::strncpy(pr.stats[0].name, "swm_stage", sizeof(pr.stats[0].name) - 1);
-      pr.n = 1;
-    }
-    if (pr.n > 0) {
-      // algorithmic bytes: read the stage state (3 fields), the step start
-      // state when combined, write 3 fields; interior cells
-      const double cells = (double)p->nl * p->nx * p->ny;
-      pr.stats[0].launches += 1;
-      pr.stats[0].total_ms += ms;
-      pr.stats[0].bytes += cells * 4.0 * 3.0 * (hn ? 3.0 : 2.0);
-    }
+    pr.ev.push_back(e0);
+    pr.ev.push_back(e1);
+    // algorithmic bytes: read the stage state (3 fields), the step start
+    // state when combined, write 3 fields; interior cells
+    const double cells = (double)p->nl * p->nx * p->ny;
+    pr.bytes.push_back(cells * 4.0 * 3.0 * (hn ? 3.0 : 2.0));
   }
 }
 
@@ -748,13 +739,24 @@ int somaxhip_swm_step_profile(somaxhip_swm_plan p, float* h, float* u, float* v,
     set_error("somaxhip_swm_step_profile: null stats");
     return SOMAXHIP_ERR_ARG;
   }
-  SwmProf prof{stats, max_stats, 0, hipSuccess};
+  SwmProf prof{stats, max_stats, 0, hipSuccess, {}, {}};
   g_swm_prof = &prof;
   const int rc = somaxhip_swm_step(p, h, u, v, nsteps, dt, prm, ws, stream);
   g_swm_prof = nullptr;
-  *nstats = prof.n;
+  hipError_t e = hipStreamSynchronize(reinterpret_cast<hipStream_t>(stream));
+  std::memset(&stats[0], 0, sizeof(stats[0]));
+  std::strncpy(stats[0].name, "swm_stage", sizeof(stats[0].name) - 1);
+  for (size_t i = 0; i < prof.bytes.size(); ++i) {
+    float ms = 0.0f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, prof.ev[2 * i], prof.ev[2 * i + 1]);
+    stats[0].launches += 1;
+    stats[0].total_ms += ms;
+    stats[0].bytes += prof.bytes[i];
+  }
+  for (hipEvent_t ev : prof.ev) (void)hipEventDestroy(ev);
+  *nstats = 1;
   if (rc) return rc;
-  SOMAX_HIP_CHECK(prof.err);
+  SOMAX_HIP_CHECK(e);
   return SOMAXHIP_OK;
 }
 
