@@ -230,3 +230,31 @@ def test_qg_chunked_and_ensemble_flags(gpu):
     flags = ens.member_diagnostics(packed)
     assert [f["nan"] for f in flags] == [0, 0, 1]
     _same(flags[0], orun.field_summary(state0.q.cpu().numpy()))
+
+
+def test_restart_through_zarr_store(gpu, tmp_path):
+    """tests/test_cli_run.py:275-306: a spin-up's final state written to a
+    zarr store and read back continues the run as one uninterrupted
+    integration: the interior bitwise.  (The restart's integrate call zeroes
+    the ghost ring of its initial state, as the reference's does,
+    model.py:75; within one call the ghost values evolve with the stage
+    combination, and every RHS evaluation re-applies the boundary condition,
+    so only the ghost ring can differ.)"""
+    import torch
+
+    from somax_amd import doublegyre_baroclinic_qg, io
+
+    _log("[run] QG 64^2 restart through a zarr store")
+    model, q0 = doublegyre_baroclinic_qg(nx=64, ny=64)
+    dt = 1200.0
+    spin = model.integrate(q0, 0.0, 12 * dt, dt, solver="ssprk3")
+    final = type(q0)(q=spin.ys.q[-1])
+    io.save_dataset(io.state_to_dataset(final, time=12 * dt), tmp_path / "final_state.zarr")
+    restart = io.dataset_to_state(io.load_dataset(tmp_path / "final_state.zarr"))
+    assert type(restart) is type(q0) and restart.q.device.type == "cuda"
+    assert torch.equal(restart.q, final.q)
+    cont = model.integrate(restart, 12 * dt, 24 * dt, dt, solver="ssprk3")
+    whole = model.integrate(q0, 0.0, 24 * dt, dt, solver="ssprk3")
+    a, b = cont.ys.q[-1], whole.ys.q[-1]
+    _log(f"  ghost ring max |restart - whole| {float((a - b).abs().max()):.3e}")
+    assert torch.equal(a[..., 1:-1, 1:-1], b[..., 1:-1, 1:-1])
