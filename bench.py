@@ -15,8 +15,9 @@ events on the stream the kernels run on.
 
 Prints one JSON line.  ``roofline`` is taken from a second, per-launch
 profiled run (HIP events around every launch, somaxhip_qg_step_profile);
-``cpu_baseline`` times the FP64 NumPy oracle on the host for a bounded
-sample of the same workload.
+``cpu_baseline`` times the C++/OpenMP FP32 restatement of the step
+(oracle/cpu_step.cpp) on the host's threads for a bounded sample of the same
+workload, with the FP64 NumPy oracle on one core beside it.
 """
 
 from __future__ import annotations
@@ -117,23 +118,40 @@ def roofline_entry(name: str, st: dict) -> dict:
 
 
 def cpu_baseline(n: int, q64, dt: float, steps: int) -> dict:
+    """The same 1024^2 workload on the host: the C++/OpenMP FP32 restatement
+    of the step (oracle/cpu_step.cpp: dense DST-I, fused stencils) on every
+    host thread OpenMP is given, for ``steps`` steps, and beside it the FP64
+    NumPy oracle on one core for two steps."""
+    from threadpoolctl import threadpool_limits
+
+    from oracle import cpu as ocpu
     from oracle import qg as oqg
     from somax_amd import testcases as tc
 
     setup = oqg.baroclinic_setup(nx=n, ny=n, Lx=tc.BASIN, Ly=tc.BASIN, f0=tc.F0, beta=tc.BETA, bottom_drag=tc.DRAG,
                                  wind_amplitude=tc.wind_amplitude(), biharmonic_viscosity=tc.NU4_128 * (128 / n) ** 4)
-    from threadpoolctl import threadpool_limits
-
-    q = q64.copy()
+    nl = q64.shape[0]
+    if not ocpu.LIB.exists():
+        ocpu.build()
+    cs = ocpu.CpuStep(setup, fp64=False)
+    q = cs.step(q64, dt, 1)  # warm-up
+    t0 = time.perf_counter()
+    q = cs.step(q, dt, steps)
+    el = time.perf_counter() - t0
+    k = 2
+    qo = q64.copy()
     with threadpool_limits(1):
-        t0 = time.perf_counter()
-        for _ in range(steps):
-            q = oqg.ssprk3_step(setup, q, dt)
-        el = time.perf_counter() - t0
-    nl = q.shape[0]
-    return {"value": round(nl * n * n * steps / el, 1), "unit": "layer-cell-steps/s", "cores": 1, "kind": "port",
-            "sample": f"{steps} SSP-RK3 steps of the same {nl}x{n}x{n} workload, FP64 NumPy/SciPy oracle "
-                      f"(FFT DST-I), single thread, {el:.1f} s"}
+        t1 = time.perf_counter()
+        for _ in range(k):
+            qo = oqg.ssprk3_step(setup, qo, dt)
+        elo = time.perf_counter() - t1
+    return {"value": round(nl * n * n * steps / el, 1), "unit": "layer-cell-steps/s", "cores": cs.threads,
+            "kind": "port",
+            "sample": f"{steps} SSP-RK3 steps of the same {nl}x{n}x{n} workload by the C++/OpenMP FP32 "
+                      f"restatement of the step (oracle/cpu_step.cpp: dense DST-I GEMMs, fused stencils) on "
+                      f"{cs.threads} host threads, {el:.1f} s",
+            "oracle_fp64_numpy_1core": {"value": round(nl * n * n * k / elo, 1), "steps": k,
+                                        "sample": f"FP64 NumPy/SciPy oracle (FFT DST-I), one thread, {elo:.1f} s"}}
 
 
 def bench_ensemble(dt: float, steps: int, warmup: int) -> dict:
@@ -307,7 +325,7 @@ def main() -> int:
     ap.add_argument("--dt", type=float, default=1200.0)
     ap.add_argument("--advection", default="arakawa", choices=["arakawa", "weno"])
     ap.add_argument("--profile-steps", type=int, default=2)
-    ap.add_argument("--cpu-steps", type=int, default=15, help="oracle steps timed for cpu_baseline (0 = skip)")
+    ap.add_argument("--cpu-steps", type=int, default=5, help="C++ CPU steps timed for cpu_baseline (0 = skip)")
     ap.add_argument("--ensemble-steps", type=int, default=10, help="steps timed for the 256-member ensemble (0 = skip)")
     ap.add_argument("--swm-steps", type=int, default=50, help="steps timed for the shallow-water jet (0 = skip)")
     ap.add_argument("--cmm-solves", type=int, default=20, help="masked-domain solves timed (0 = skip)")
