@@ -121,7 +121,7 @@ def cpu_baseline(n: int, q64, dt: float, steps: int) -> dict:
     """The same 1024^2 workload on the host: the C++/OpenMP FP32 restatement
     of the step (oracle/cpu_step.cpp: dense DST-I, fused stencils) on every
     host thread OpenMP is given, for ``steps`` steps, and beside it the FP64
-    NumPy oracle on one core for two steps."""
+    NumPy oracle on one core for six steps."""
     from threadpoolctl import threadpool_limits
 
     from oracle import cpu as ocpu
@@ -138,7 +138,7 @@ def cpu_baseline(n: int, q64, dt: float, steps: int) -> dict:
     t0 = time.perf_counter()
     q = cs.step(q, dt, steps)
     el = time.perf_counter() - t0
-    k = 2
+    k = 6
     qo = q64.copy()
     with threadpool_limits(1):
         t1 = time.perf_counter()
@@ -325,7 +325,7 @@ def main() -> int:
     ap.add_argument("--dt", type=float, default=1200.0)
     ap.add_argument("--advection", default="arakawa", choices=["arakawa", "weno"])
     ap.add_argument("--profile-steps", type=int, default=2)
-    ap.add_argument("--cpu-steps", type=int, default=5, help="C++ CPU steps timed for cpu_baseline (0 = skip)")
+    ap.add_argument("--cpu-steps", type=int, default=60, help="C++ CPU steps timed for cpu_baseline (0 = skip)")
     ap.add_argument("--ensemble-steps", type=int, default=10, help="steps timed for the 256-member ensemble (0 = skip)")
     ap.add_argument("--swm-steps", type=int, default=50, help="steps timed for the shallow-water jet (0 = skip)")
     ap.add_argument("--cmm-solves", type=int, default=20, help="masked-domain solves timed (0 = skip)")
