@@ -14,7 +14,7 @@
 #include <string>
 #include <vector>
 
-#include "../../include/somaxhip.h"
+#include "somaxhip.h"
 #include "common.h"
 #include "kernels.h"
 

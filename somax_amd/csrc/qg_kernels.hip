@@ -40,6 +40,20 @@ __global__ __launch_bounds__(256) void qg_stage_arakawa_kernel(StageArgs a) {
   const float* dbeta_row = a.dbeta_rows + (long)m * Ny;
   const float* wind_row = a.wind_rows + (long)m * Ny;
 
+  // This thread's outputs (column tx, rows tyb + 4 r): the previous stage
+  // value q_n and the wind row are fetched first, so their latency overlaps
+  // the tile loads instead of following the last barrier.
+  const int tx = tid & 63;
+  const int tyb = tid >> 6;
+  float qn_r[TY / 4], wind_r[TY / 4];
+#pragma unroll
+  for (int r = 0; r < TY / 4; ++r) {
+    const int gy = y0 + tyb + 4 * r, gx = x0 + tx;
+    const bool ok = gy < Ny && gx < Nx;
+    qn_r[r] = (ok && a.qn) ? a.qn[off + (long)gy * Nx + gx] : 0.0f;
+    wind_r[r] = ok ? wind_row[gy] : 0.0f;
+  }
+
   // q with the boundary condition (ghost ring and outside the array read 0)
   for (int i = tid; i < (TY + 4) * QW; i += 256) {
     const int ly = i / QW, lx = i - ly * QW;
@@ -78,8 +92,6 @@ __global__ __launch_bounds__(256) void qg_stage_arakawa_kernel(StageArgs a) {
     __syncthreads();
   }
 
-  const int tx = tid & 63;
-  const int tyb = tid >> 6;
   const float wc = pr.tau0 * a.t.wind_scale[z];
   const bool bottom = (a.t.layer[z] == a.t.mnl[m] - 1) && (pr.kappa != 0.0f);
 #pragma unroll
@@ -113,7 +125,7 @@ __global__ __launch_bounds__(256) void qg_stage_arakawa_kernel(StageArgs a) {
       }
       if (bottom) rhs -= pr.kappa * ((pE - 2.0f * p[0] + pW) * a.idx2 + (pN - 2.0f * p[0] + pS) * a.idy2);
     }
-    rhs += wc * wind_row[gy];
+    rhs += wc * wind_r[r];
     const long idx = off + (long)gy * Nx + gx;
     // raw q_s: the tile's copy, except on the ghost ring when the boundary
     // condition zeroed it there
@@ -124,7 +136,7 @@ __global__ __launch_bounds__(256) void qg_stage_arakawa_kernel(StageArgs a) {
     // 1 + 3e-8 per step.  Without qn: c1 qs + c2 rhs.
     float o;
     if (a.qn) {
-      const float qn = a.qn[idx];
+      const float qn = qn_r[r];
       o = qn + a.c1 * (qraw() - qn) + a.c2 * rhs;
     } else {
       o = a.c2 * rhs;

@@ -24,7 +24,7 @@
 #include <utility>
 #include <vector>
 
-#include "../../include/somaxhip.h"
+#include "somaxhip.h"
 #include "common.h"
 #include "kernels.h"
 
@@ -174,6 +174,21 @@ __global__ __launch_bounds__(256) void swm_stage_kernel(SwmArgs a) {
     const float* us = a.us + off;
     const float* vs = a.vs + off;
     if (k > 0) __syncthreads();  // previous layer's tiles are no longer read
+    // This thread's raw stage and step-start values, fetched before the tile
+    // so that their latency overlaps the tile loads.
+    float hs_r[SRW], us_r[SRW], vs_r[SRW], hn_r[SRW], un_r[SRW], vn_r[SRW];
+#pragma unroll
+    for (int r = 0; r < SRW; ++r) {
+      const int gy = y0 + ty + 4 * r, gx = x0 + tx;
+      const bool ok = gy < Ny && gx < Nx;
+      const long idx = off + (long)gy * Nx + gx;
+      hs_r[r] = ok ? a.hs[idx] : 0.0f;
+      us_r[r] = ok ? a.us[idx] : 0.0f;
+      vs_r[r] = ok ? a.vs[idx] : 0.0f;
+      hn_r[r] = (ok && a.hn) ? a.hn[idx] : 0.0f;
+      un_r[r] = (ok && a.hn) ? a.un[idx] : 0.0f;
+      vn_r[r] = (ok && a.hn) ? a.vn[idx] : 0.0f;
+    }
     for (int r = ty; r < SPH; r += 4) {
       const BcAxis rm = bc_rows(y0 - 1 + r, Ny, a.bc, a.apply_bc);
 #pragma unroll
@@ -245,14 +260,14 @@ __global__ __launch_bounds__(256) void swm_stage_kernel(SwmArgs a) {
 #undef V
       const long idx = off + (long)gy * Nx + gx;
       if (a.hn) {
-        const float hn = a.hn[idx], un = a.un[idx], vn = a.vn[idx];
-        a.ho[idx] = hn + a.c1 * (a.hs[idx] - hn) + a.c2 * dh;
-        a.uo[idx] = un + a.c1 * (a.us[idx] - un) + a.c2 * du;
-        a.vo[idx] = vn + a.c1 * (a.vs[idx] - vn) + a.c2 * dv;
+        const float hn = hn_r[r], un = un_r[r], vn = vn_r[r];
+        a.ho[idx] = hn + a.c1 * (hs_r[r] - hn) + a.c2 * dh;
+        a.uo[idx] = un + a.c1 * (us_r[r] - un) + a.c2 * du;
+        a.vo[idx] = vn + a.c1 * (vs_r[r] - vn) + a.c2 * dv;
       } else {
-        a.ho[idx] = a.c1 * a.hs[idx] + a.c2 * dh;
-        a.uo[idx] = a.c1 * a.us[idx] + a.c2 * du;
-        a.vo[idx] = a.c1 * a.vs[idx] + a.c2 * dv;
+        a.ho[idx] = a.c1 * hs_r[r] + a.c2 * dh;
+        a.uo[idx] = a.c1 * us_r[r] + a.c2 * du;
+        a.vo[idx] = a.c1 * vs_r[r] + a.c2 * dv;
       }
     }
   }
