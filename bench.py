@@ -316,6 +316,83 @@ def bench_cmm(solves: int, cpu_solves: int) -> dict:
             "rel_l2_vs_oracle": err, "setup_s": round(setup_s, 2), "cpu_baseline": cpu}
 
 
+def _event_ms(fn, reps: int) -> float:
+    """Mean milliseconds of ``fn()`` over ``reps`` calls, HIP events on the
+    current stream (the stream the library launches on)."""
+    import torch
+
+    fn()
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        fn()
+    e1.record()
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) / reps
+
+
+def bench_rows(mixed, q0, dt: float) -> dict:
+    """The remaining paths on their own: the Tsit5 step of the 1024^2 model
+    (integrate's default solver, 6 RHS per step), the ReparameterizedQG step
+    (3 layers at 512^2, Tsit5: a projection per stage), the single-layer
+    nonlinear shallow-water step (1024^2), and the between-chunk field
+    statistics of the 1024^2 state (one read of 12.6 MB: an HBM-bound
+    reduction, reported against the HBM peak)."""
+    import ctypes
+
+    import numpy as np
+    import torch
+
+    from somax_amd import MultilayerSW2DState, NonlinearShallowWater2D, NonlinearSW2DState, _lib
+    from somax_amd import testcases as tc
+    from somax_amd.run import _workspace
+
+    out = {}
+    nl, Ny, Nx = q0.shape
+    cells = nl * (Ny - 2) * (Nx - 2)
+    q = q0.clone()
+    ms = _event_ms(lambda: mixed._step_tsit5_inplace(q, dt, 1), 10)
+    out["qg_tsit5"] = {"workload": f"qg{nl}x{Ny - 2}x{Nx - 2}_tsit5_mixed", "ms_per_step": round(ms, 4),
+                       "value": round(cells / (ms * 1e-3), 1), "unit": "layer-cell-steps/s"}
+
+    rq, st = tc.doublegyre_reparameterized_qg(nx=512, ny=512, precision="mixed")
+    h, u, v = (f.clone() for f in rq.swm._fields(rq.apply_boundary_conditions(st)))
+    ms = _event_ms(lambda: rq._step_inplace(h, u, v, 20.0, 1, "tsit5"), 10)  # dx 7.8 km, c 200 m/s
+    c = 3 * 512 * 512
+    out["rqg_tsit5"] = {"workload": "reparameterized_qg3x512x512_tsit5_mixed", "ms_per_step": round(ms, 4),
+                        "value": round(c / (ms * 1e-3), 1), "unit": "layer-cell-steps/s"}
+
+    sw = NonlinearShallowWater2D.create(nx=1024, ny=1024, bc="wall", lateral_viscosity=10.0)
+    shape = (sw.grid.Ny, sw.grid.Nx)
+    rng = np.random.default_rng(5)
+    h0 = torch.from_numpy((100.0 + 0.1 * rng.standard_normal(shape)).astype(np.float32)).cuda()
+    z = torch.zeros(shape, device="cuda")
+    s0 = NonlinearSW2DState(h=h0, u=z, v=z.clone())
+    ml = sw._sync()
+    fields = [f.clone() for f in ml._fields(MultilayerSW2DState(h=h0[None], u=z[None], v=z[None]))]
+    ms = _event_ms(lambda: ml._step_inplace(*fields, 5.0, 1), 20)
+    out["nonlinear_swm_ssprk3"] = {"workload": "nonlinear_swm1024x1024_wall_ssprk3", "ms_per_step": round(ms, 4),
+                                   "value": round(1024 * 1024 / (ms * 1e-3), 1), "unit": "cell-steps/s"}
+    del s0
+
+    ws = _workspace(q0.device)
+    res = torch.empty(5, dtype=torch.float64, device="cuda")
+    ptrs = (ctypes.c_void_p * 1)(q0.data_ptr())
+    counts = (ctypes.c_size_t * 1)(q0.numel())
+    stream = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    lib = _lib.load()
+    ms = _event_ms(lambda: _lib.check(lib.somaxhip_field_stats(ptrs, counts, 1, res.data_ptr(), ws.data_ptr(),
+                                                                stream), "somaxhip_field_stats"), 50)
+    nbytes = 4.0 * q0.numel()
+    gbs = nbytes / (ms * 1e-3) / 1e9
+    out["field_stats"] = {"workload": f"field statistics of the {nl}x{Ny}x{Nx} state", "us_per_call": round(1e3 * ms, 2),
+                          "roofline": {"bound": "hbm", "achieved": round(gbs, 1), "peak": PEAK["hbm_GBs"],
+                                       "unit": "GB/s", "frac": round(gbs / PEAK["hbm_GBs"], 4),
+                                       "alg_bytes_per_call": nbytes}}
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -329,6 +406,8 @@ def main() -> int:
     ap.add_argument("--ensemble-steps", type=int, default=10, help="steps timed for the 256-member ensemble (0 = skip)")
     ap.add_argument("--swm-steps", type=int, default=50, help="steps timed for the shallow-water jet (0 = skip)")
     ap.add_argument("--cmm-solves", type=int, default=20, help="masked-domain solves timed (0 = skip)")
+    ap.add_argument("--rows", type=int, default=1, help="time the Tsit5 / ReparameterizedQG / nonlinear SWM / "
+                    "field-statistics paths (0 = skip)")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -379,6 +458,12 @@ def main() -> int:
         swm = bench_swm(args.swm_steps, 2)
         log(f"[bench] swm {swm['ms_per_step']:.3f} ms/step")
 
+    rows = None
+    if args.rows:
+        log("[bench] Tsit5, ReparameterizedQG, nonlinear SWM, field statistics")
+        rows = bench_rows(mixed, q0, dt)
+        log("[bench] " + ", ".join(f"{k} {v.get('ms_per_step', v.get('us_per_call'))}" for k, v in rows.items()))
+
     cmm = None
     if args.cmm_solves > 0:
         log("[bench] masked-domain Helmholtz (octagon 3x256x256)")
@@ -416,6 +501,7 @@ def main() -> int:
         "ensemble": ens,
         "swm": swm,
         "cmm": cmm,
+        "rows": rows,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

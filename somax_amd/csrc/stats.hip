@@ -89,7 +89,7 @@ __device__ inline Acc acc_block(Acc a) {
 }
 
 // grid (ST_MAX_BLOCKS or fewer, nfields): block b of field f covers the
-// elements i = (b * 256 + t) * 4 + {0..3} strided by the field's block count.
+// float4 vectors b * 256 + t strided by the field's block count x 256.
 __global__ void __launch_bounds__(ST_THREADS) stats_partial_kernel(StatFields fs, double* __restrict__ partial) {
   const int f = blockIdx.y;
   const unsigned long long n = fs.n[f];
@@ -102,7 +102,21 @@ __global__ void __launch_bounds__(ST_THREADS) stats_partial_kernel(StatFields fs
   if ((reinterpret_cast<uintptr_t>(x) & 15u) == 0u) {
     const unsigned long long nv = n / 4;
     const float4* __restrict__ x4 = reinterpret_cast<const float4*>(x);
-    for (unsigned long long i = t0; i < nv; i += stride) {
+    unsigned long long i = t0;
+    // four independent 16-byte loads in flight per lane, then their values
+    for (; i + 3 * stride < nv; i += 4 * stride) {
+      float4 v[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) v[u] = x4[i + u * stride];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        acc_add(a, v[u].x);
+        acc_add(a, v[u].y);
+        acc_add(a, v[u].z);
+        acc_add(a, v[u].w);
+      }
+    }
+    for (; i < nv; i += stride) {
       const float4 v = x4[i];
       acc_add(a, v.x);
       acc_add(a, v.y);
