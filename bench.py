@@ -438,6 +438,12 @@ def main() -> int:
     ms_plain, _ = time_steps(plain, q0, dt, args.steps, args.warmup)
     log(f"[bench] plain {ms_plain:.3f} ms/step")
 
+    log("[bench] timing bf16x6 (three-piece operands, six products, no refinement)")
+    x6 = build_model(n, "bf16x6", args.advection)
+    ms_x6, _ = time_steps(x6, q0, dt, args.steps, args.warmup)
+    log(f"[bench] bf16x6 {ms_x6:.3f} ms/step")
+    del x6
+
     log("[bench] per-launch profile")
     prof_mixed = kernel_profile(mixed, q0, dt, args.profile_steps)
     prof_plain = kernel_profile(plain, q0, dt, args.profile_steps)
@@ -494,6 +500,9 @@ def main() -> int:
                    "advection": args.advection, "inversion": "mixed"},
         "plain_fp32": {"value": round(plain_value, 1), "ms_per_step": round(ms_plain, 4),
                        "speedup_mixed_vs_plain": round(ms_plain / ms_mixed, 3), "roofline": roof_plain},
+        "bf16x6": {"value": round(cells / (ms_x6 * 1e-3), 1), "ms_per_step": round(ms_x6, 4),
+                   "speedup_vs_plain": round(ms_plain / ms_x6, 3),
+                   "form": "bf16 hi/mid/lo operands, 6 products, FP64 accumulation flushes, no refinement pass"},
         "roofline": roof,
         "roofline_stencil": roof_stencil,
         "kernel_profile_mixed": {k: {"launches": v["launches"], "avg_us": round(1e3 * v["total_ms"] / v["launches"], 2)}

@@ -48,8 +48,11 @@ enum {
 /* Arithmetic of the DST-I Helmholtz inversion. */
 enum {
   SOMAXHIP_PREC_FP32 = 0,   /* plain: FP32 MFMA sine-transform GEMMs */
-  SOMAXHIP_PREC_MIXED = 1   /* BF16 hi/lo 3-product MFMA, FP32 accumulate, one
+  SOMAXHIP_PREC_MIXED = 1,  /* BF16 hi/lo 3-product MFMA, FP32 accumulate, one
                                iterative-refinement pass on the Helmholtz residual */
+  SOMAXHIP_PREC_BF16X6 = 2  /* extension: BF16 hi/mid/lo three-piece operands, six
+                               products per element with FP64 accumulation flushes,
+                               one direct solve (no refinement pass) */
 };
 
 /* ------------------------------------------------------------------------ */

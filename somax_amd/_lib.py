@@ -18,6 +18,7 @@ ADV_ARAKAWA = 0
 ADV_WENO = 1
 PREC_FP32 = 0
 PREC_MIXED = 1
+PREC_BF16X6 = 2
 ENS_CONCURRENT = 1
 ENS_GRAPH = 2
 SOLVER_SSPRK3 = 0

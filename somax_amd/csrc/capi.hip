@@ -200,7 +200,9 @@ static std::string check_desc(const somaxhip_qg_desc* d) {
   if (!d->cl2m || !d->cm2l || !d->lambdas || !d->beta_y_row || !d->wind_row || !d->wind_scale)
     return "a required table pointer is null";
   if (d->advection != SOMAXHIP_ADV_ARAKAWA && d->advection != SOMAXHIP_ADV_WENO) return "unknown advection";
-  if (d->precision != SOMAXHIP_PREC_FP32 && d->precision != SOMAXHIP_PREC_MIXED) return "unknown precision";
+  if (d->precision != SOMAXHIP_PREC_FP32 && d->precision != SOMAXHIP_PREC_MIXED &&
+      d->precision != SOMAXHIP_PREC_BF16X6)
+    return "unknown precision";
   if (d->mass_correction && (!d->homsol || !d->homsol_mean))
     return "mass_correction needs homsol and homsol_mean";
   return "";
@@ -727,6 +729,7 @@ void dst_solve(const Batch& p, bool interior_input, int np_fwd, int np_inv_lo, i
 
 void invert(const Batch& p, const float* q, float* psi, void* ws, hipStream_t st, bool apply_bc = true) {
   const bool mixed = p.precision == SOMAXHIP_PREC_MIXED;
+  const bool x6 = p.precision == SOMAXHIP_PREC_BF16X6;
   float* QM = wsf(ws, p.off_QM);
   float* P0 = wsf(ws, p.off_P0);
   ModalArgs ma{};
@@ -739,10 +742,10 @@ void invert(const Batch& p, const float* q, float* psi, void* ws, hipStream_t st
   ma.apply_bc = apply_bc ? 1 : 0;
   ma.t = p.t;
   ma.op = static_cast<char*>(ws) + p.off_OP1;
-  ma.op_pieces = mixed ? 2 : 0;
+  ma.op_pieces = mixed ? 2 : (x6 ? 3 : 0);
   ma.op_ld = p.wpitch;
   ma.op_piece = p.wpiece;
-  ma.op_batch = mixed ? 3 * p.wpiece : 3 * p.wpiece / 2;
+  ma.op_batch = (mixed || x6) ? 3 * p.wpiece : 3 * p.wpiece / 2;
   ma.op_kpad = (int)round_up(apply_bc ? p.Nx - 2 : p.Nx, 32);
   if (p.fold) {
     ma.fold = 1;
@@ -763,7 +766,11 @@ void invert(const Batch& p, const float* q, float* psi, void* ws, hipStream_t st
   la.t = p.t;
   la.fold = p.fold;
   la.P = p.P;
-  if (!mixed) {
+  if (x6) {
+    // three-piece operands, six products and FP64 flushes in every transform:
+    // about 2^-24 per product, accurate without a refinement pass
+    dst_solve(p, apply_bc, 6, 6, 6, P0, ws, st);
+  } else if (!mixed) {
     dst_solve(p, apply_bc, NP_F32, NP_F32, NP_F32, P0, ws, st);
   } else {
     // bf16x3 forward, x6 (Poisson-like modes) / x3 inverse; FP32 per-mode

@@ -9,7 +9,10 @@ field names, ``create()`` keyword arguments and defaults, and the
 
 Extensions beyond the reference (keyword-only, defaults reproduce the
 reference): ``biharmonic_viscosity``, ``advection="weno"``,
-``precision="mixed"``, ``mass_correction`` and ``helmholtz_eigenvalues``.
+``precision`` (``"fp32"``: FP32 MFMA transforms; ``"mixed"``: bf16 hi/lo
+three-product transforms plus one refinement pass on the Helmholtz residual;
+``"bf16x6"``: bf16 hi/mid/lo six-product transforms with FP64 accumulation,
+one direct solve), ``mass_correction`` and ``helmholtz_eigenvalues``.
 """
 
 from __future__ import annotations
@@ -35,7 +38,7 @@ from .core import (
 )
 
 _ADV = {"arakawa": _lib.ADV_ARAKAWA, "weno": _lib.ADV_WENO}
-_PREC = {"fp32": _lib.PREC_FP32, "mixed": _lib.PREC_MIXED}
+_PREC = {"fp32": _lib.PREC_FP32, "mixed": _lib.PREC_MIXED, "bf16x6": _lib.PREC_BF16X6}
 
 
 def _torch():

@@ -64,7 +64,8 @@ def _log(msg):
 
 
 @pytest.mark.parametrize("n,precision,tol", [(64, "fp32", 3e-6), (64, "mixed", 3e-7), (128, "fp32", 3e-6),
-                                             (128, "mixed", 3e-7), (1024, "fp32", 3e-6), (1024, "mixed", 3e-7)])
+                                             (128, "mixed", 3e-7), (1024, "fp32", 3e-6), (1024, "mixed", 3e-7),
+                                             (64, "bf16x6", 6e-7), (1024, "bf16x6", 6e-7)])
 def test_inversion_matches_oracle(gpu, n, precision, tol):
     _log(f"[inversion] n={n} precision={precision}")
     model, setup = _pair(n, precision=precision)
@@ -116,7 +117,7 @@ def test_build_terms_applies_bc(gpu):
 
 
 @pytest.mark.parametrize("n,precision", [(64, "fp32"), (128, "fp32"), (128, "mixed"), (1024, "fp32"),
-                                         (1024, "mixed")])
+                                         (1024, "mixed"), (128, "bf16x6"), (1024, "bf16x6")])
 def test_one_step_matches_oracle(gpu, n, precision):
     _log(f"[step] 1 SSP-RK3 step n={n} precision={precision}")
     from somax_amd import BaroclinicQGState
@@ -135,7 +136,7 @@ def test_one_step_matches_oracle(gpu, n, precision):
     assert rq < 1e-6 and rp < (2e-6 if (precision == "fp32" and n >= 1024) else 1e-6)
 
 
-@pytest.mark.parametrize("precision", ["fp32", "mixed"])
+@pytest.mark.parametrize("precision", ["fp32", "mixed", "bf16x6"])
 def test_hundred_steps_match_oracle(gpu, precision):
     _log(f"[step] 100 SSP-RK3 steps n=128 precision={precision}")
     from somax_amd import BaroclinicQGState
@@ -357,7 +358,7 @@ def test_barotropic_diagnose(gpu):
     assert rel(d.enstrophy.cpu().numpy(), ref["enstrophy"][0]) < 1e-6
 
 
-@pytest.mark.parametrize("precision,tol", [("fp32", 3e-6), ("mixed", 3e-7)])
+@pytest.mark.parametrize("precision,tol", [("fp32", 3e-6), ("mixed", 3e-7), ("bf16x6", 6e-7)])
 def test_mass_corrected_inversion(gpu, precision, tol):
     """Homogeneous-solution mass correction (mqg_demo_retangular.ipynb cells
     18-19): psi matches the oracle to the inversion bar, and every modal
