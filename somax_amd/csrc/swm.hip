@@ -58,9 +58,11 @@ struct SwmArgs {
 
 namespace {
 
-constexpr int STX = 64;      // tile width
-constexpr int SRW = 1;       // rows per thread
-constexpr int STY = 4 * SRW; // tile height (256 threads = 64 x 4)
+constexpr int STX = 64;        // tile width
+constexpr int SRW = 1;         // rows per thread
+constexpr int SWV = 4;         // waves per workgroup (one tile row each; 2 measured slower at 512^2)
+constexpr int SNT = 64 * SWV;  // threads per workgroup
+constexpr int STY = SWV * SRW; // tile height
 constexpr int SPW = STX + 2; // LDS pitch (halo 1)
 constexpr int SPH = STY + 2;
 
@@ -140,7 +142,7 @@ __device__ inline void swm_tile(int tiles_x, int& ty, int& tx) {
 
 }  // namespace
 
-__global__ __launch_bounds__(256) void swm_stage_kernel(SwmArgs a) {
+__global__ __launch_bounds__(SNT) void swm_stage_kernel(SwmArgs a) {
   __shared__ float sh[SPH * SPW];
   __shared__ float su[SPH * SPW];
   __shared__ float sv[SPH * SPW];
@@ -179,7 +181,7 @@ __global__ __launch_bounds__(256) void swm_stage_kernel(SwmArgs a) {
     float hs_r[SRW], us_r[SRW], vs_r[SRW], hn_r[SRW], un_r[SRW], vn_r[SRW];
 #pragma unroll
     for (int r = 0; r < SRW; ++r) {
-      const int gy = y0 + ty + 4 * r, gx = x0 + tx;
+      const int gy = y0 + ty + SWV * r, gx = x0 + tx;
       const bool ok = gy < Ny && gx < Nx;
       const long idx = off + (long)gy * Nx + gx;
       hs_r[r] = ok ? a.hs[idx] : 0.0f;
@@ -189,7 +191,7 @@ __global__ __launch_bounds__(256) void swm_stage_kernel(SwmArgs a) {
       un_r[r] = (ok && a.hn) ? a.un[idx] : 0.0f;
       vn_r[r] = (ok && a.hn) ? a.vn[idx] : 0.0f;
     }
-    for (int r = ty; r < SPH; r += 4) {
+    for (int r = ty; r < SPH; r += SWV) {
       const BcAxis rm = bc_rows(y0 - 1 + r, Ny, a.bc, a.apply_bc);
 #pragma unroll
       for (int half = 0; half < 2; ++half) {
@@ -207,7 +209,7 @@ __global__ __launch_bounds__(256) void swm_stage_kernel(SwmArgs a) {
     const bool top = k == 0, bottom = k == a.nl - 1;
 #pragma unroll
     for (int r = 0; r < SRW; ++r) {
-      const int ly = ty + 4 * r;
+      const int ly = ty + SWV * r;
       const int gy = y0 + ly, gx = x0 + tx;
       if (gy >= Ny || gx >= Nx) continue;
       const int c = (ly + 1) * SPW + tx + 1;  // LDS index of (gy, gx)
@@ -423,7 +425,7 @@ void swm_stage(somaxhip_swm_plan p, const float* hs, const float* us, const floa
     (void)hipEventCreate(&e1);
     (void)hipEventRecord(e0, st);
   }
-  hipLaunchKernelGGL(swm_stage_kernel, grid, dim3(256), 0, st, a);
+  hipLaunchKernelGGL(swm_stage_kernel, grid, dim3(SNT), 0, st, a);
   if (g_swm_prof) {
     (void)hipEventRecord(e1, st);
     SwmProf& pr = *g_swm_prof;
