@@ -347,6 +347,16 @@ long somaxhip_ens_member_offset(somaxhip_ens ens, int member); /* -1 on error */
 size_t somaxhip_ens_workspace_bytes(somaxhip_ens ens);
 int somaxhip_ens_num_classes(somaxhip_ens ens);
 
+/* Class cls (0 .. num_classes-1, largest grid first): interior size, planes,
+ * members, whether its transforms are folded, its stage tile height. */
+int somaxhip_ens_class_info(somaxhip_ens ens, int cls, int* nx, int* ny, int* planes, int* members, int* fold,
+                            int* stage_rows);
+
+/* Tile height (8 or 16 rows of 64 columns) of class cls's Arakawa stage
+ * kernel: a knob for the ensemble autotuner; results do not depend on it.
+ * Only before the ensemble's first step: afterwards SOMAXHIP_ERR_ARG. */
+int somaxhip_ens_set_stage_rows(somaxhip_ens ens, int cls, int rows);
+
 /* Replaces every member's parameters (n entries, in member order). */
 int somaxhip_ens_set_params(somaxhip_ens ens, const somaxhip_qg_params* params, somaxhip_stream_t stream);
 

@@ -179,6 +179,8 @@ _SIGNATURES = {
     "somaxhip_ens_member_offset": (ctypes.c_long, [_vp, ctypes.c_int]),
     "somaxhip_ens_workspace_bytes": (ctypes.c_size_t, [_vp]),
     "somaxhip_ens_num_classes": (ctypes.c_int, [_vp]),
+    "somaxhip_ens_class_info": (ctypes.c_int, [_vp, ctypes.c_int] + [ctypes.POINTER(ctypes.c_int)] * 6),
+    "somaxhip_ens_set_stage_rows": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int]),
     "somaxhip_ens_set_params": (ctypes.c_int, [_vp, ctypes.POINTER(QGParams), _vp]),
     "somaxhip_ens_step": (ctypes.c_int, [_vp, _vp, ctypes.c_int, ctypes.c_double, _vp, _vp]),
     "somaxhip_ens_step_profile": (

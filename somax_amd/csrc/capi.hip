@@ -186,6 +186,7 @@ struct Batch {
   size_t off_DIAG = 0;  // FP64 partial sums of the diagnostics and the mass constraint
   size_t off_MSUM = 0, off_ALPHA = 0, off_PWALL = 0;  // per-plane sums, alpha, wall psi
   size_t ws_bytes = 0;
+  int stage_rows = 16;  // Arakawa stage tile height (8 or 16), chosen per class
   // mass correction: homogeneous solutions (P planes) and their means (0: none)
   int any_mass = 0;
   float* homsol = nullptr;
@@ -379,6 +380,7 @@ struct somaxhip_ens_s {
   const void* g_ws = nullptr;
   double g_dt = 0.0;
   bool stale = false;
+  bool stepped = false;  // launch shapes are fixed from the first step on
   ~somaxhip_ens_s() {
     if (main) (void)hipStreamSynchronize(main);
     if (exec) (void)hipGraphExecDestroy(exec);
@@ -856,6 +858,7 @@ void stage(const Batch& p, const float* qs, const float* psi, const float* qn, f
   a.c2 = c2;
   a.advection = p.advection;
   a.apply_bc = apply_bc ? 1 : 0;
+  a.tile_rows = p.stage_rows;
   // algorithmic bytes: read qs and psi, read qn when combined, write out
   const double bytes = 4.0 * p.P * (double)p.nx * p.ny * (3.0 + (qn ? 1.0 : 0.0));
   ProfScope ps("qg_stage", 0.0, bytes, st);
@@ -1264,6 +1267,35 @@ int somaxhip_ens_set_params(somaxhip_ens e, const somaxhip_qg_params* params, so
   return SOMAXHIP_OK;
 }
 
+int somaxhip_ens_class_info(somaxhip_ens e, int cls, int* nx, int* ny, int* planes, int* members, int* fold,
+                            int* stage_rows) {
+  if (!e || cls < 0 || cls >= (int)e->cls.size()) {
+    set_error("somaxhip_ens_class_info: null ensemble or class out of range");
+    return SOMAXHIP_ERR_ARG;
+  }
+  const Batch& b = *e->cls[cls];
+  if (nx) *nx = b.nx;
+  if (ny) *ny = b.ny;
+  if (planes) *planes = b.P;
+  if (members) *members = b.M;
+  if (fold) *fold = b.fold;
+  if (stage_rows) *stage_rows = b.stage_rows;
+  return SOMAXHIP_OK;
+}
+
+int somaxhip_ens_set_stage_rows(somaxhip_ens e, int cls, int rows) {
+  if (!e || cls < 0 || cls >= (int)e->cls.size() || (rows != 8 && rows != 16)) {
+    set_error("somaxhip_ens_set_stage_rows: null ensemble, class out of range or rows not 8 or 16");
+    return SOMAXHIP_ERR_ARG;
+  }
+  if (e->stepped) {
+    set_error("somaxhip_ens_set_stage_rows: the tile height is fixed once the ensemble has stepped");
+    return SOMAXHIP_ERR_ARG;
+  }
+  e->cls[cls]->stage_rows = rows;
+  return SOMAXHIP_OK;
+}
+
 }  // extern "C"
 
 namespace {
@@ -1304,6 +1336,7 @@ int somaxhip_ens_step(somaxhip_ens e, float* q, int nsteps, double dt, void* ws,
   }
   hipStream_t st = reinterpret_cast<hipStream_t>(stream);
   if (nsteps == 0) return SOMAXHIP_OK;
+  e->stepped = true;
   if (!(e->flags & SOMAXHIP_ENS_GRAPH)) return ens_queue(e, q, nsteps, dt, ws, st);
 
   // capture one step on the private stream and replay it there, ordered
@@ -1347,6 +1380,7 @@ int somaxhip_ens_step_profile(somaxhip_ens e, float* q, int nsteps, double dt, v
   }
   // serial and eager, so every launch is timed on the caller's stream
   hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+  e->stepped = true;
   const int saved = e->flags;
   e->flags = 0;
   Profile prof;

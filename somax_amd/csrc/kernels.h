@@ -227,6 +227,7 @@ struct StageArgs {
   float c1, c2;
   int advection;
   int apply_bc;  // 1: ghost ring of qs read as 0 in the RHS (SomaxModel.build_terms)
+  int tile_rows;  // Arakawa stage tile height: 8 or 16 (0: 16)
 };
 void launch_qg_stage(const StageArgs& a, hipStream_t st);
 

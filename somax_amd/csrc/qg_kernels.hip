@@ -14,12 +14,12 @@ namespace somax {
 
 namespace {
 constexpr int TX = 64;
-constexpr int TY = 16;
+constexpr int TY_DEFAULT = 16;  // tile rows unless a batch chooses 8
 constexpr int QW = TX + 4;  // q tile pitch (halo 2)
 constexpr int PW = TX + 2;  // psi / lap tile pitch (halo 1)
 }  // namespace
 
-template <bool BIHARM>
+template <bool BIHARM, int TY>
 __global__ __launch_bounds__(256) void qg_stage_arakawa_kernel(StageArgs a) {
   __shared__ float sq[(TY + 4) * QW];
   __shared__ float sp[(TY + 2) * PW];
@@ -376,11 +376,15 @@ void launch_qg_stage(const StageArgs& a, hipStream_t st) {
     else hipLaunchKernelGGL(qg_stage_weno_kernel<false>, g, dim3(256), 0, st, a);
     return;
   }
-  dim3 grid((a.Nx + TX - 1) / TX, (a.Ny + TY - 1) / TY, a.P);
-  if (a.biharm)
-    hipLaunchKernelGGL(qg_stage_arakawa_kernel<true>, grid, dim3(256), 0, st, a);
-  else
-    hipLaunchKernelGGL(qg_stage_arakawa_kernel<false>, grid, dim3(256), 0, st, a);
+  const int ty = a.tile_rows ? a.tile_rows : TY_DEFAULT;
+  const dim3 grid((a.Nx + TX - 1) / TX, (a.Ny + ty - 1) / ty, a.P);
+  if (ty == 8) {
+    if (a.biharm) hipLaunchKernelGGL((qg_stage_arakawa_kernel<true, 8>), grid, dim3(256), 0, st, a);
+    else hipLaunchKernelGGL((qg_stage_arakawa_kernel<false, 8>), grid, dim3(256), 0, st, a);
+  } else {
+    if (a.biharm) hipLaunchKernelGGL((qg_stage_arakawa_kernel<true, 16>), grid, dim3(256), 0, st, a);
+    else hipLaunchKernelGGL((qg_stage_arakawa_kernel<false, 16>), grid, dim3(256), 0, st, a);
+  }
 }
 
 __global__ __launch_bounds__(256) void zero_ghost_kernel(float* f, int Ny, int Nx) {

@@ -8,12 +8,23 @@ advection and precision form a class whose planes every kernel of the step
 covers in one launch; classes run concurrently on their own streams, and the
 step is captured as a hipGraph and replayed.  The arithmetic per member is
 that of ``model.step``.
+
+``QGEnsemble.autotune`` picks the launch configuration by measurement: the
+transform layout (folded half-size transforms for every even grid, for the
+large grids only, or for none), concurrent class streams or one stream, and
+then, class by class, the stage kernels' tile height (8 or 16 rows).  Every
+candidate is a new ensemble, timed from its first step: launch shapes are
+fixed when an ensemble first steps.  Concurrency and tile height leave every
+result bit unchanged; the transform layout changes the order of the
+transforms' sums (both layouts meet the same parity bars against the FP64
+oracle).
 """
 
 from __future__ import annotations
 
 import ctypes
 import math
+import os
 
 from . import _lib
 
@@ -32,7 +43,7 @@ class QGEnsemble:
     ``step`` advances it in place.
     """
 
-    def __init__(self, models, *, concurrent: bool = True, graph: bool = True, device="cuda"):
+    def __init__(self, models, *, concurrent: bool = True, graph: bool = True, stage_rows=None, device="cuda"):
         torch = _torch()
         self.models = list(models)
         if not self.models:
@@ -58,6 +69,12 @@ class QGEnsemble:
         self.shapes = [(m._nl, m.grid.Ny, m.grid.Nx) for m in self.models]
         nbytes = int(lib.somaxhip_ens_workspace_bytes(ens))
         self._ws = torch.empty(max(nbytes, 256), dtype=torch.uint8, device=device)
+        if stage_rows is not None:
+            rows = [stage_rows] * self.num_classes if isinstance(stage_rows, int) else list(stage_rows)
+            if len(rows) != self.num_classes:
+                raise ValueError(f"stage_rows: expected {self.num_classes} entries (one per class), got {len(rows)}")
+            for c, r in enumerate(rows):
+                self.set_stage_rows(c, r)
 
     def __del__(self):
         ens = getattr(self, "_ens", None)
@@ -139,6 +156,93 @@ class QGEnsemble:
             m.params = p
         arr = (_lib.QGParams * len(self.models))(*[m._c_params() for m in self.models])
         _lib.check(_lib.load().somaxhip_ens_set_params(self._ens, arr, self._stream()), "somaxhip_ens_set_params")
+
+    # -- configuration ------------------------------------------------------
+    def class_info(self) -> list[dict]:
+        """Per class (largest grid first): nx, ny, planes, members, fold, stage_rows."""
+        lib = _lib.load()
+        out = []
+        for c in range(self.num_classes):
+            v = [ctypes.c_int(0) for _ in range(6)]
+            _lib.check(lib.somaxhip_ens_class_info(self._ens, c, *[ctypes.byref(x) for x in v]),
+                       "somaxhip_ens_class_info")
+            out.append(dict(zip(("nx", "ny", "planes", "members", "fold", "stage_rows"), (x.value for x in v))))
+        return out
+
+    def set_stage_rows(self, cls: int, rows: int) -> None:
+        """Stage-kernel tile height of class ``cls`` (8 or 16); only before
+        the first step."""
+        _lib.check(_lib.load().somaxhip_ens_set_stage_rows(self._ens, int(cls), int(rows)),
+                   "somaxhip_ens_set_stage_rows")
+
+    def time_step(self, packed, dt: float, steps: int = 5) -> float:
+        """Mean milliseconds of one ensemble step (HIP events; one untimed
+        step first, which also captures the graph)."""
+        torch = _torch()
+        self.step(packed, dt, 1)
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        self.step(packed, dt, steps)
+        e1.record()
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / steps
+
+    @classmethod
+    def autotune(cls, models, qs, dt: float, steps: int = 5, graph: bool = True, log=None):
+        """Build the fastest configuration of an ensemble of ``models`` for
+        steps of ``dt`` from the initial fields ``qs`` (timed on copies).
+
+        Stage 1 times every (transform layout, class concurrency) pair on a
+        new ensemble each; stage 2 tries the other tile height for each class
+        of the winner in turn, on a new ensemble each, keeping a change only
+        when the step gets faster.  Returns ``(ensemble, report)``; the
+        report lists every timing and the chosen configuration."""
+        log = log or (lambda msg: None)
+        report = {"candidates": []}
+        best = None
+        saved = os.environ.get("SOMAXHIP_FOLD")
+
+        def fold_env(fold):
+            if fold == "default":
+                os.environ.pop("SOMAXHIP_FOLD", None)
+            else:
+                os.environ["SOMAXHIP_FOLD"] = fold
+
+        try:
+            for fold in ("default", "1", "0"):
+                for conc in (True, False):
+                    fold_env(fold)
+                    ens = cls(models, concurrent=conc, graph=graph)
+                    ms = ens.time_step(ens.pack(qs), dt, steps)
+                    report["candidates"].append({"fold": fold, "concurrent": conc, "ms_per_step": round(ms, 4)})
+                    log(f"autotune: fold={fold} concurrent={conc}: {ms:.3f} ms/step")
+                    if best is None or ms < best[0]:
+                        best = (ms, ens, fold, conc)
+                    del ens
+            ms, ens, fold, conc = best
+            best = None
+            rows = [info["stage_rows"] for info in ens.class_info()]
+            fold_env(fold)
+            for c, info in enumerate(ens.class_info()):
+                trial = list(rows)
+                trial[c] = 8 if rows[c] == 16 else 16
+                cand = cls(models, concurrent=conc, graph=graph, stage_rows=trial)
+                t = cand.time_step(cand.pack(qs), dt, steps)
+                report["candidates"].append({"class": f"{info['nx']}x{info['ny']}", "stage_rows": trial[c],
+                                             "ms_per_step": round(t, 4)})
+                log(f"autotune: class {info['nx']}x{info['ny']} stage rows {trial[c]}: {t:.3f} ms/step")
+                if t < ms:
+                    ms, rows, ens = t, trial, cand
+                del cand
+        finally:
+            if saved is None:
+                os.environ.pop("SOMAXHIP_FOLD", None)
+            else:
+                os.environ["SOMAXHIP_FOLD"] = saved
+        report["chosen"] = {"fold": fold, "concurrent": conc, "ms_per_step": round(ms, 4),
+                            "classes": ens.class_info()}
+        return ens, report
 
     def profile(self, packed, dt: float, nsteps: int = 1) -> dict:
         """Eager serial step with every launch timed: {kernel: (launches, ms, flops, bytes)}."""

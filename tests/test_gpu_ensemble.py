@@ -195,3 +195,53 @@ def test_ensemble_mixed_precision_classes(gpu):
         q = torch.from_numpy(qs[i]).cuda()
         m._step_inplace(q, 1200.0, 1)
         assert rel(ens.member(packed, i).cpu().numpy(), q.cpu().numpy()) < 1e-7
+
+
+@pytest.mark.parametrize("rows", [8])
+def test_stage_tile_rows_leave_results_unchanged(gpu, rows):
+    """The autotuner's stage-tile knob: 8-row tiles give the 16-row results
+    bit for bit (same arithmetic, other launch shape); the knob is refused
+    once the ensemble has stepped."""
+    import torch
+
+    from somax_amd import QGEnsemble
+
+    _log(f"[ensemble] stage tile rows {rows} vs 16, mixed")
+    models = _models("mixed")
+    qs = _states(models)
+    ref = QGEnsemble(models, concurrent=False, graph=False)
+    alt = QGEnsemble(models, concurrent=False, graph=True)
+    for c in range(alt.num_classes):
+        alt.set_stage_rows(c, rows)
+    assert all(info["stage_rows"] == rows for info in alt.class_info())
+    pr, pa = ref.pack(qs), alt.pack(qs)
+    ref.step(pr, 1200.0, 3)
+    alt.step(pa, 1200.0, 3)
+    torch.cuda.synchronize()
+    assert torch.equal(pr, pa)
+    with pytest.raises(RuntimeError, match="fixed once the ensemble has stepped"):
+        alt.set_stage_rows(0, 16)
+
+
+def test_autotune_picks_a_valid_configuration(gpu):
+    """QGEnsemble.autotune: every candidate timed, the chosen ensemble the
+    fastest of them, and its step still matches the FP64 oracle."""
+    import torch
+
+    from somax_amd import QGEnsemble
+
+    _log("[ensemble] autotune, mixed")
+    models = _models("mixed")
+    qs = _states(models)
+    ens, report = QGEnsemble.autotune(models, qs, 1200.0, steps=2, log=_log)
+    chosen = report["chosen"]
+    timed = [c["ms_per_step"] for c in report["candidates"]]
+    assert len(report["candidates"]) >= 6 and chosen["ms_per_step"] <= min(timed) + 1e-9
+    assert len(chosen["classes"]) == ens.num_classes
+    packed = ens.pack(qs)
+    ens.step(packed, 1200.0, 1)
+    torch.cuda.synchronize()
+    for i in (0, 4):
+        setup = _oracle(i)
+        ref = oqg.ssprk3_step(setup, qs[i].astype(np.float64), 1200.0)
+        assert rel(ens.member(packed, i).cpu().numpy().reshape(ref.shape), ref) < 1e-6
