@@ -198,6 +198,7 @@ static std::string check_desc(const somaxhip_qg_desc* d) {
   if (!d) return "null description";
   if (d->nx < 2 || d->ny < 2 || d->nl < 1 || d->nl > MAX_LAYERS || !(d->dx > 0) || !(d->dy > 0))
     return "need nx, ny >= 2, 1 <= nl <= 8, dx, dy > 0";
+  if ((long)(d->nx + 2) * (d->ny + 2) > 0x7fffffffL) return "a plane of (nx + 2) * (ny + 2) cells must be below 2^31";
   if (!d->cl2m || !d->cm2l || !d->lambdas || !d->beta_y_row || !d->wind_row || !d->wind_scale)
     return "a required table pointer is null";
   if (d->advection != SOMAXHIP_ADV_ARAKAWA && d->advection != SOMAXHIP_ADV_WENO) return "unknown advection";

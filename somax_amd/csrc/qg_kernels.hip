@@ -45,36 +45,55 @@ __global__ __launch_bounds__(256) void qg_stage_arakawa_kernel(StageArgs a) {
   // the tile loads instead of following the last barrier.
   const int tx = tid & 63;
   const int tyb = tid >> 6;
-  float qn_r[TY / 4], wind_r[TY / 4];
+  // On the ghost ring under the boundary condition the tile holds 0, so the
+  // raw q_s of the stage combination is fetched here as well.
+  float qn_r[TY / 4], wind_r[TY / 4], qg_r[TY / 4];
 #pragma unroll
   for (int r = 0; r < TY / 4; ++r) {
     const int gy = y0 + tyb + 4 * r, gx = x0 + tx;
     const bool ok = gy < Ny && gx < Nx;
-    qn_r[r] = (ok && a.qn) ? a.qn[off + (long)gy * Nx + gx] : 0.0f;
+    const bool gring = gy == 0 || gx == 0 || gy == Ny - 1 || gx == Nx - 1;
+    const int idx = gy * Nx + gx;
+    qn_r[r] = (ok && a.qn) ? a.qn[off + idx] : 0.0f;
     wind_r[r] = ok ? wind_row[gy] : 0.0f;
+    qg_r[r] = (ok && gring && a.apply_bc) ? qs[idx] : 0.0f;
   }
 
-  // q with the boundary condition (ghost ring and outside the array read 0)
-  for (int i = tid; i < (TY + 4) * QW; i += 256) {
+  // Tile loads: a fixed number per thread, all issued before the first LDS
+  // store, so one memory latency covers them.  q with the boundary condition
+  // (ghost ring and outside the array read 0); psi with a 1-cell halo.
+  constexpr int NQ = ((TY + 4) * QW + 255) / 256;
+  constexpr int NPS = ((TY + 2) * PW + 255) / 256;
+  float vq[NQ], vp[NPS];
+#pragma unroll
+  for (int k = 0; k < NQ; ++k) {
+    const int i = tid + 256 * k;
     const int ly = i / QW, lx = i - ly * QW;
     const int gy = y0 - 2 + ly, gx = x0 - 2 + lx;
-    float v = 0.0f;
-    const bool inside = gy >= 0 && gy < Ny && gx >= 0 && gx < Nx;
+    const bool inside = i < (TY + 4) * QW && gy >= 0 && gy < Ny && gx >= 0 && gx < Nx;
     const bool ghost = gy == 0 || gx == 0 || gy == Ny - 1 || gx == Nx - 1;
-    if (inside && !(ghost && a.apply_bc)) v = qs[(long)gy * Nx + gx];
-    sq[i] = v;
+    vq[k] = (inside && !(ghost && a.apply_bc)) ? qs[gy * Nx + gx] : 0.0f;  // plane < 2^31 cells
   }
-  for (int i = tid; i < (TY + 2) * PW; i += 256) {
+#pragma unroll
+  for (int k = 0; k < NPS; ++k) {
+    const int i = tid + 256 * k;
     const int ly = i / PW, lx = i - ly * PW;
     const int gy = y0 - 1 + ly, gx = x0 - 1 + lx;
-    float v = 0.0f;
-    if (gy >= 0 && gy < Ny && gx >= 0 && gx < Nx) v = psi[(long)gy * Nx + gx];
-    sp[i] = v;
+    const bool inside = i < (TY + 2) * PW && gy >= 0 && gy < Ny && gx >= 0 && gx < Nx;
+    vp[k] = inside ? psi[gy * Nx + gx] : 0.0f;
   }
+  float vdb = 0.0f;
   if (tid < TY + 4) {
     const int gy = y0 - 2 + tid;
-    sdb[tid] = (gy >= 0 && gy < Ny) ? dbeta_row[gy] : 0.0f;
+    vdb = (gy >= 0 && gy < Ny) ? dbeta_row[gy] : 0.0f;
   }
+#pragma unroll
+  for (int k = 0; k < NQ; ++k)
+    if (tid + 256 * k < (TY + 4) * QW) sq[tid + 256 * k] = vq[k];
+#pragma unroll
+  for (int k = 0; k < NPS; ++k)
+    if (tid + 256 * k < (TY + 2) * PW) sp[tid + 256 * k] = vp[k];
+  if (tid < TY + 4) sdb[tid] = vdb;
   __syncthreads();
 
   if (BIHARM) {
@@ -130,7 +149,8 @@ __global__ __launch_bounds__(256) void qg_stage_arakawa_kernel(StageArgs a) {
     // raw q_s: the tile's copy, except on the ghost ring when the boundary
     // condition zeroed it there
     const bool gring = gy == 0 || gx == 0 || gy == Ny - 1 || gx == Nx - 1;
-    auto qraw = [&]() { return (gring && a.apply_bc) ? a.qs[idx] : sq[(ly + 2) * QW + tx + 2]; };
+    const float qtile = sq[(ly + 2) * QW + tx + 2];
+    auto qraw = [&]() { return (gring && a.apply_bc) ? qg_r[r] : qtile; };
     // Stage combination qn + c1 (qs - qn) + c2 rhs: the FP32 weights of the
     // Shu-Osher form (1/3 + 2/3 != 1 in FP32) would otherwise scale q by
     // 1 + 3e-8 per step.  Without qn: c1 qs + c2 rhs.

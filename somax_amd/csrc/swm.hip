@@ -183,7 +183,7 @@ __global__ __launch_bounds__(SNT) void swm_stage_kernel(SwmArgs a) {
     for (int r = 0; r < SRW; ++r) {
       const int gy = y0 + ty + SWV * r, gx = x0 + tx;
       const bool ok = gy < Ny && gx < Nx;
-      const long idx = off + (long)gy * Nx + gx;
+      const long idx = off + (gy * Nx + gx);
       hs_r[r] = ok ? a.hs[idx] : 0.0f;
       us_r[r] = ok ? a.us[idx] : 0.0f;
       vs_r[r] = ok ? a.vs[idx] : 0.0f;
@@ -191,16 +191,35 @@ __global__ __launch_bounds__(SNT) void swm_stage_kernel(SwmArgs a) {
       un_r[r] = (ok && a.hn) ? a.un[idx] : 0.0f;
       vn_r[r] = (ok && a.hn) ? a.vn[idx] : 0.0f;
     }
-    for (int r = ty; r < SPH; r += SWV) {
+    // Tile loads: a fixed count per thread, all issued before the first LDS
+    // store so that one memory latency covers them (in-plane offsets fit in
+    // 32 bits: a plane is below 2^31 cells).
+    constexpr int NR = (SPH + SWV - 1) / SWV;
+    float th[NR][2], tu[NR][2], tv[NR][2];
+#pragma unroll
+    for (int j = 0; j < NR; ++j) {
+      const int r = ty + SWV * j;
       const BcAxis rm = bc_rows(y0 - 1 + r, Ny, a.bc, a.apply_bc);
 #pragma unroll
       for (int half = 0; half < 2; ++half) {
-        const int c = tx + 64 * half;
-        if (c >= SPW) break;
+        const bool in = r < SPH && tx + 64 * half < SPW;
         const BcAxis& cm = half ? cm1 : cm0;
-        sh[r * SPW + c] = (rm.h >= 0 && cm.h >= 0) ? hs[(long)rm.h * Nx + cm.h] : 0.0f;
-        su[r * SPW + c] = (rm.u >= 0 && cm.u >= 0) ? us[(long)rm.u * Nx + cm.u] : 0.0f;
-        sv[r * SPW + c] = (rm.v >= 0 && cm.v >= 0) ? vs[(long)rm.v * Nx + cm.v] : 0.0f;
+        th[j][half] = (in && rm.h >= 0 && cm.h >= 0) ? hs[rm.h * Nx + cm.h] : 0.0f;
+        tu[j][half] = (in && rm.u >= 0 && cm.u >= 0) ? us[rm.u * Nx + cm.u] : 0.0f;
+        tv[j][half] = (in && rm.v >= 0 && cm.v >= 0) ? vs[rm.v * Nx + cm.v] : 0.0f;
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < NR; ++j) {
+      const int r = ty + SWV * j;
+#pragma unroll
+      for (int half = 0; half < 2; ++half) {
+        const int c = tx + 64 * half;
+        if (r < SPH && c < SPW) {
+          sh[r * SPW + c] = th[j][half];
+          su[r * SPW + c] = tu[j][half];
+          sv[r * SPW + c] = tv[j][half];
+        }
       }
     }
     __syncthreads();
@@ -458,6 +477,10 @@ int somaxhip_swm_plan_create(const somaxhip_swm_desc* d, somaxhip_swm_plan* out)
   *out = nullptr;
   if (d->nx < 2 || d->ny < 2 || d->nl < 1 || d->nl > SWM_MAX_LAYERS || !(d->dx > 0) || !(d->dy > 0)) {
     set_error("somaxhip_swm_plan_create: need nx, ny >= 2, 1 <= nl <= 8, dx, dy > 0");
+    return SOMAXHIP_ERR_ARG;
+  }
+  if ((long)(d->nx + 2) * (d->ny + 2) > 0x7fffffffL) {
+    set_error("somaxhip_swm_plan_create: a plane of (nx + 2) * (ny + 2) cells must be below 2^31");
     return SOMAXHIP_ERR_ARG;
   }
   if (!d->g_prime || !d->H || !d->f_row || !d->wind_row) {
