@@ -750,6 +750,7 @@ void invert(const Batch& p, const float* q, float* psi, void* ws, hipStream_t st
   ma.op_piece = p.wpiece;
   ma.op_batch = (mixed || x6) ? 3 * p.wpiece : 3 * p.wpiece / 2;
   ma.op_kpad = (int)round_up(apply_bc ? p.Nx - 2 : p.Nx, 32);
+  ma.nlmax = *std::max_element(p.nl.begin(), p.nl.end());
   if (p.fold) {
     ma.fold = 1;
     ma.P = p.P;
@@ -769,6 +770,7 @@ void invert(const Batch& p, const float* q, float* psi, void* ws, hipStream_t st
   la.t = p.t;
   la.fold = p.fold;
   la.P = p.P;
+  la.nlmax = ma.nlmax;
   if (x6) {
     // three-piece operands, six products and FP64 flushes in every transform:
     // about 2^-24 per product, accurate without a refinement pass

@@ -156,6 +156,7 @@ struct ModalArgs {
   // y = fold_phys(k'y, Ny/2), mirror(y) = Ny-1-y; qm planes are hy x hx
   // (pitch hx), the operand rows k'y < hy zero padded to op_kpad columns.
   int fold, P;
+  int nlmax;  // largest member layer count (0: MAX_LAYERS)
 };
 
 // Folded index k' in [0, h) -> physical index in the first half: the pairs
@@ -203,6 +204,7 @@ struct LayerArgs {
   // table plane m at plane b * P + m, Ny/2 x Nx/2, pitch Nx/2); the physical
   // value at (mirror^sy(y), mirror^sx(x)) is sum_b (-1)^(p sy + q sx) pm_b.
   int fold, P;
+  int nlmax;  // largest member layer count (0: MAX_LAYERS)
 };
 void launch_to_layer(const LayerArgs& a, hipStream_t st);
 

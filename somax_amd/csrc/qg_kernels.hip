@@ -486,16 +486,20 @@ __global__ __launch_bounds__(256) void residual_fold_kernel(ResidualArgs a) {
   const int blk = z / a.P, m = z - blk * a.P;
   const float sx = (blk & 1) ? -1.0f : 1.0f, sy = (blk >> 1) ? -1.0f : 1.0f;
   const float* Z = a.psi + (long)z * hx * hy;
-  const float* Zr = Z + (long)ky * hx;
-  const float c = Zr[kx];
-  // physical indices in the first half; neighbours mapped back to folded indices
+  const float* Zr = Z + ky * hx;
+  // physical indices in the first half; neighbours mapped back to folded
+  // indices (all in range, so the five loads are issued unconditionally)
   const int x = fold_phys(kx, hx), y = fold_phys(ky, hy);
-  const float e = x == hx - 1 ? sx * c : Zr[x];
-  const float w = x == 0 ? 0.0f : (x == 1 ? Zr[hx - 1] : Zr[x - 2]);
-  const float n = y == hy - 1 ? sy * c : Z[(long)y * hx + kx];
-  const float so = y == 0 ? 0.0f : (y == 1 ? Z[(long)(hy - 1) * hx + kx] : Z[(long)(y - 2) * hx + kx]);
+  const int xw = x == 0 ? 0 : (x == 1 ? hx - 1 : x - 2);
+  const int ys = y == 0 ? 0 : (y == 1 ? hy - 1 : y - 2);
+  const float c = Zr[kx], ze = Zr[x], zw = Zr[xw], zn = Z[y * hx + kx], zs = Z[ys * hx + kx];
+  const float qv = a.q[(long)z * hx * hy + ky * hx + kx];
+  const float e = x == hx - 1 ? sx * c : ze;
+  const float w = x == 0 ? 0.0f : zw;
+  const float n = y == hy - 1 ? sy * c : zn;
+  const float so = y == 0 ? 0.0f : zs;
   const float lap = (e - 2.0f * c + w) * a.idx2 + (n - 2.0f * c + so) * a.idy2;
-  const float r = a.q[(long)z * hx * hy + (long)ky * hx + kx] - 4.0f * (lap - a.lam[m] * c);
+  const float r = qv - 4.0f * (lap - a.lam[m] * c);
   store_pieces(o, a.op_piece, 2, r);
 }
 
@@ -513,6 +517,7 @@ void launch_residual(const ResidualArgs& a, hipStream_t st) {
 // layer -> mode projection in FP64, writing the first GEMM operand
 // (one thread per cell of one member: its layers are read once)
 // ---------------------------------------------------------------------------
+template <int NLM>
 __global__ __launch_bounds__(256) void to_modal_kernel(ModalArgs a) {
   const int x = blockIdx.x * blockDim.x + threadIdx.x;  // state column
   const int y = blockIdx.y;
@@ -525,19 +530,23 @@ __global__ __launch_bounds__(256) void to_modal_kernel(ModalArgs a) {
   if (!inside && !wop) return;
   const int first = a.t.mfirst[mem], nl = a.t.mnl[mem];
   const double* cl2m = a.t.cl2m + mem * MAX_LAYERS * MAX_LAYERS;
-  const long s = (long)y * a.Nx + x;
+  const int s = y * a.Nx + (inside ? x : 0);  // in-plane offset (a plane is below 2^31 cells)
   const bool ghost = x == 0 || y == 0 || x == a.Nx - 1 || y == a.Ny - 1;
-  // fully unrolled over MAX_LAYERS with guards so qk stays in registers
-  double qk[MAX_LAYERS];
+  // Every load is issued before the first use (layers past nl re-read the
+  // member's first layer and are ignored), so one memory latency covers them.
+  float qf[NLM];
 #pragma unroll
-  for (int k = 0; k < MAX_LAYERS; ++k)
-    qk[k] = (k >= nl || !inside || (ghost && a.apply_bc)) ? 0.0 : (double)a.q[(first + k) * a.plane + s];
+  for (int k = 0; k < NLM; ++k) qf[k] = a.q[(long)(first + (k < nl ? k : 0)) * a.plane + s];
+  const bool zero = !inside || (ghost && a.apply_bc);
+  double qk[NLM];
 #pragma unroll
-  for (int p = 0; p < MAX_LAYERS; ++p) {
+  for (int k = 0; k < NLM; ++k) qk[k] = zero ? 0.0 : (double)qf[k];
+#pragma unroll
+  for (int p = 0; p < NLM; ++p) {
     if (p >= nl) break;
     double acc = 0.0;
 #pragma unroll
-    for (int k = 0; k < MAX_LAYERS; ++k)
+    for (int k = 0; k < NLM; ++k)
       if (k < nl) acc += cl2m[p * MAX_LAYERS + k] * qk[k];
     const float v = (float)acc;
     const long z = first + p;
@@ -554,6 +563,7 @@ __global__ __launch_bounds__(256) void to_modal_kernel(ModalArgs a) {
 // Folded variant: one thread per folded point (k'y, k'x) of one member reads
 // the four mirror points of every layer, projects them to modes in FP64 and
 // writes the four parity fold sums of each mode.
+template <int NLM>
 __global__ __launch_bounds__(256) void to_modal_fold_kernel(ModalArgs a) {
   const int hx = a.Nx / 2, hy = a.Ny / 2;
   const int width = hx > a.op_kpad ? hx : a.op_kpad;
@@ -570,22 +580,31 @@ __global__ __launch_bounds__(256) void to_modal_fold_kernel(ModalArgs a) {
   const double* cl2m = a.t.cl2m + mem * MAX_LAYERS * MAX_LAYERS;
   const int y0 = fold_phys(ky, hy), y1 = a.Ny - 1 - y0;
   const int x0 = inside ? fold_phys(kx, hx) : 0, x1 = a.Nx - 1 - x0;
-  const long pts[4] = {(long)y0 * a.Nx + x0, (long)y0 * a.Nx + x1, (long)y1 * a.Nx + x0, (long)y1 * a.Nx + x1};
+  // in-plane offsets (a plane is below 2^31 cells)
+  const int pts[4] = {y0 * a.Nx + x0, y0 * a.Nx + x1, y1 * a.Nx + x0, y1 * a.Nx + x1};
   // the ghost pair (last folded index) holds only ghost points
   const bool zero = !inside || (a.apply_bc && (ky == hy - 1 || kx == hx - 1));
-  double qk[MAX_LAYERS][4];
+  // Every load is issued before the first use (layers past nl re-read the
+  // member's first layer and are ignored), so one memory latency covers them.
+  float qf[NLM][4];
 #pragma unroll
-  for (int k = 0; k < MAX_LAYERS; ++k)
+  for (int k = 0; k < NLM; ++k) {
+    const float* src = a.q + (long)(first + (k < nl ? k : 0)) * a.plane;
 #pragma unroll
-    for (int c = 0; c < 4; ++c)
-      qk[k][c] = (k >= nl || zero) ? 0.0 : (double)a.q[(first + k) * a.plane + pts[c]];
+    for (int c = 0; c < 4; ++c) qf[k][c] = src[pts[c]];
+  }
+  double qk[NLM][4];
+#pragma unroll
+  for (int k = 0; k < NLM; ++k)
+#pragma unroll
+    for (int c = 0; c < 4; ++c) qk[k][c] = zero ? 0.0 : (double)qf[k][c];
   const long fplane = (long)hx * hy;
 #pragma unroll
-  for (int p = 0; p < MAX_LAYERS; ++p) {
+  for (int p = 0; p < NLM; ++p) {
     if (p >= nl) break;
     double v[4] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-    for (int k = 0; k < MAX_LAYERS; ++k)
+    for (int k = 0; k < NLM; ++k)
       if (k < nl)
 #pragma unroll
         for (int c = 0; c < 4; ++c) v[c] += cl2m[p * MAX_LAYERS + k] * qk[k][c];
@@ -607,23 +626,36 @@ __global__ __launch_bounds__(256) void to_modal_fold_kernel(ModalArgs a) {
   }
 }
 
+// Layer-count bound of the projection kernels' unrolled loops: the smallest
+// instance covering the batch's deepest member.
+#define SOMAX_NL_LAUNCH(NL, KERNEL, GRID, ST, ARGS)                                      \
+  do {                                                                                  \
+    const int nl_ = (NL) > 0 ? (NL) : MAX_LAYERS;                                       \
+    if (nl_ <= 1) hipLaunchKernelGGL(KERNEL<1>, GRID, dim3(256), 0, ST, ARGS);           \
+    else if (nl_ <= 2) hipLaunchKernelGGL(KERNEL<2>, GRID, dim3(256), 0, ST, ARGS);      \
+    else if (nl_ <= 3) hipLaunchKernelGGL(KERNEL<3>, GRID, dim3(256), 0, ST, ARGS);      \
+    else if (nl_ <= 4) hipLaunchKernelGGL(KERNEL<4>, GRID, dim3(256), 0, ST, ARGS);      \
+    else hipLaunchKernelGGL(KERNEL<MAX_LAYERS>, GRID, dim3(256), 0, ST, ARGS);           \
+  } while (0)
+
 void launch_to_modal(const ModalArgs& a, hipStream_t st) {
   if (a.fold) {
     const int hx = a.Nx / 2;
     const int width = hx > a.op_kpad ? hx : a.op_kpad;
     dim3 grid((unsigned)(((long)width * (a.Ny / 2) + 255) / 256), a.M);
-    hipLaunchKernelGGL(to_modal_fold_kernel, grid, dim3(256), 0, st, a);
+    SOMAX_NL_LAUNCH(a.nlmax, to_modal_fold_kernel, grid, st, a);
     return;
   }
   const int off = a.apply_bc ? 1 : 0;
   const int width = a.Nx > a.op_kpad + off ? a.Nx : a.op_kpad + off;
   dim3 grid((width + 255) / 256, a.Ny, a.M);
-  hipLaunchKernelGGL(to_modal_kernel, grid, dim3(256), 0, st, a);
+  SOMAX_NL_LAUNCH(a.nlmax, to_modal_kernel, grid, st, a);
 }
 
 // ---------------------------------------------------------------------------
 // mode -> layer projection of psi
 // ---------------------------------------------------------------------------
+template <int NLM>
 __global__ __launch_bounds__(256) void to_layer_kernel(LayerArgs a) {
   const int x = blockIdx.x * blockDim.x + threadIdx.x;
   const int y = blockIdx.y;
@@ -631,32 +663,41 @@ __global__ __launch_bounds__(256) void to_layer_kernel(LayerArgs a) {
   if (x >= a.Nx) return;
   const int first = a.t.mfirst[mem], nl = a.t.mnl[mem];
   const float* cm2l = a.t.cm2l + mem * MAX_LAYERS * MAX_LAYERS;
-  const long s = (long)y * a.Nx + x;
+  const int s = y * a.Nx + x;  // in-plane offset (a plane is below 2^31 cells)
   const bool ghost = a.t.mask[mem] && (x == 0 || y == 0 || x == a.Nx - 1 || y == a.Ny - 1);
-  float pm[MAX_LAYERS];
+  // loads first (layers past nl re-read the member's first layer, unused)
+  float zp[NLM], zd[NLM], zh[NLM];
 #pragma unroll
-  for (int p = 0; p < MAX_LAYERS; ++p) {
+  for (int p = 0; p < NLM; ++p) {
+    const long i = (long)(first + (p < nl ? p : 0)) * a.plane + s;
+    zp[p] = a.pm[i];
+    zd[p] = a.d ? a.d[i] : 0.0f;
+    zh[p] = a.homsol ? a.homsol[i] : 0.0f;
+  }
+  float pm[NLM];
+#pragma unroll
+  for (int p = 0; p < NLM; ++p) {
     pm[p] = 0.0f;
     if (p < nl) {
-      const long i = (first + p) * a.plane + s;
-      pm[p] = a.pm[i] + (a.d ? a.d[i] : 0.0f);
-      if (a.homsol) pm[p] += (float)a.alpha[first + p] * a.homsol[i];
+      pm[p] = zp[p] + (a.d ? zd[p] : 0.0f);
+      if (a.homsol) pm[p] += (float)a.alpha[first + p] * zh[p];
     }
   }
 #pragma unroll
-  for (int l = 0; l < MAX_LAYERS; ++l) {
+  for (int l = 0; l < NLM; ++l) {
     if (l >= nl) break;
     float v = 0.0f;
 #pragma unroll
-    for (int p = 0; p < MAX_LAYERS; ++p)
+    for (int p = 0; p < NLM; ++p)
       if (p < nl) v += cm2l[l * MAX_LAYERS + p] * pm[p];
-    a.psi[(first + l) * a.plane + s] = ghost ? 0.0f : v;
+    a.psi[(long)(first + l) * a.plane + s] = ghost ? 0.0f : v;
   }
 }
 
 // Folded variant: one thread per folded point of one member unfolds the
 // parity components of every mode at the four mirror points, adds the mass
 // correction and mixes modes into layers.
+template <int NLM>
 __global__ __launch_bounds__(256) void to_layer_fold_kernel(LayerArgs a) {
   const int hx = a.Nx / 2, hy = a.Ny / 2;
   const int e = blockIdx.x * blockDim.x + threadIdx.x;  // row-major over hy x hx
@@ -667,43 +708,55 @@ __global__ __launch_bounds__(256) void to_layer_fold_kernel(LayerArgs a) {
   const float* cm2l = a.t.cm2l + mem * MAX_LAYERS * MAX_LAYERS;
   const int y0 = fold_phys(ky, hy), y1 = a.Ny - 1 - y0;
   const int x0 = fold_phys(kx, hx), x1 = a.Nx - 1 - x0;
-  const long pts[4] = {(long)y0 * a.Nx + x0, (long)y0 * a.Nx + x1, (long)y1 * a.Nx + x0, (long)y1 * a.Nx + x1};
+  // in-plane offsets (a plane is below 2^31 cells)
+  const int pts[4] = {y0 * a.Nx + x0, y0 * a.Nx + x1, y1 * a.Nx + x0, y1 * a.Nx + x1};
   const bool ghost = a.t.mask[mem] && (ky == hy - 1 || kx == hx - 1);
   const long fplane = (long)hx * hy;
-  const long fo = (long)ky * hx + kx;
-  float pm[MAX_LAYERS][4];
+  const int fo = ky * hx + kx;
+  // Every load is issued before the first use (layers past nl re-read the
+  // member's first layer and are ignored), so one memory latency covers them.
+  float zb[NLM][4], zd[NLM][4], zh[NLM][4];
 #pragma unroll
-  for (int p = 0; p < MAX_LAYERS; ++p) {
+  for (int p = 0; p < NLM; ++p) {
+    const int pp = first + (p < nl ? p : 0);
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+      const long i = ((long)b * a.P + pp) * fplane + fo;
+      zb[p][b] = a.pm[i];
+      zd[p][b] = a.d ? a.d[i] : 0.0f;
+      zh[p][b] = a.homsol ? a.homsol[(long)pp * a.plane + pts[b]] : 0.0f;
+    }
+  }
+  float pm[NLM][4];
+#pragma unroll
+  for (int p = 0; p < NLM; ++p) {
 #pragma unroll
     for (int c = 0; c < 4; ++c) pm[p][c] = 0.0f;
     if (p < nl) {
-      float zb[4];
+      float z[4];
 #pragma unroll
-      for (int b = 0; b < 4; ++b) {
-        const long i = ((long)b * a.P + first + p) * fplane + fo;
-        zb[b] = a.pm[i] + (a.d ? a.d[i] : 0.0f);
-      }
-      pm[p][0] = (zb[0] + zb[1]) + (zb[2] + zb[3]);
-      pm[p][1] = (zb[0] - zb[1]) + (zb[2] - zb[3]);
-      pm[p][2] = (zb[0] + zb[1]) - (zb[2] + zb[3]);
-      pm[p][3] = (zb[0] - zb[1]) - (zb[2] - zb[3]);
+      for (int b = 0; b < 4; ++b) z[b] = zb[p][b] + (a.d ? zd[p][b] : 0.0f);
+      pm[p][0] = (z[0] + z[1]) + (z[2] + z[3]);
+      pm[p][1] = (z[0] - z[1]) + (z[2] - z[3]);
+      pm[p][2] = (z[0] + z[1]) - (z[2] + z[3]);
+      pm[p][3] = (z[0] - z[1]) - (z[2] - z[3]);
       if (a.homsol) {
         const float al = (float)a.alpha[first + p];
 #pragma unroll
-        for (int c = 0; c < 4; ++c) pm[p][c] += al * a.homsol[(first + p) * a.plane + pts[c]];
+        for (int c = 0; c < 4; ++c) pm[p][c] += al * zh[p][c];
       }
     }
   }
 #pragma unroll
-  for (int l = 0; l < MAX_LAYERS; ++l) {
+  for (int l = 0; l < NLM; ++l) {
     if (l >= nl) break;
 #pragma unroll
     for (int c = 0; c < 4; ++c) {
       float v = 0.0f;
 #pragma unroll
-      for (int p = 0; p < MAX_LAYERS; ++p)
+      for (int p = 0; p < NLM; ++p)
         if (p < nl) v += cm2l[l * MAX_LAYERS + p] * pm[p][c];
-      a.psi[(first + l) * a.plane + pts[c]] = ghost ? 0.0f : v;
+      a.psi[(long)(first + l) * a.plane + pts[c]] = ghost ? 0.0f : v;
     }
   }
 }
@@ -711,11 +764,11 @@ __global__ __launch_bounds__(256) void to_layer_fold_kernel(LayerArgs a) {
 void launch_to_layer(const LayerArgs& a, hipStream_t st) {
   if (a.fold) {
     dim3 grid((unsigned)(((long)(a.Nx / 2) * (a.Ny / 2) + 255) / 256), a.M);
-    hipLaunchKernelGGL(to_layer_fold_kernel, grid, dim3(256), 0, st, a);
+    SOMAX_NL_LAUNCH(a.nlmax, to_layer_fold_kernel, grid, st, a);
     return;
   }
   dim3 grid((a.Nx + 255) / 256, a.Ny, a.M);
-  hipLaunchKernelGGL(to_layer_kernel, grid, dim3(256), 0, st, a);
+  SOMAX_NL_LAUNCH(a.nlmax, to_layer_kernel, grid, st, a);
 }
 
 }  // namespace somax
