@@ -456,10 +456,14 @@ __device__ inline void mfma_products(const bf16x8 (&fa)[3][TM], const bf16x8 (&f
 // ---------------------------------------------------------------------------
 __device__ inline int swz(int r) { return ((r >> 1) ^ (r >> 2)) & 3; }
 
+// k-tile stages in LDS per operand piece count (1, 2 or 3 bf16 pieces).  Five
+// stages for two pieces (160 KiB) and eight for one were measured no faster.
+__host__ __device__ constexpr int bf16_nstage(int npc) { return npc == 3 ? 3 : 4; }
+
 template <int NPROD, int TM, int TN>
 struct Bf16Cfg {
   static constexpr int NPC = NPROD == 1 ? 1 : (NPROD == 3 ? 2 : 3);
-  static constexpr int NSTAGE = NPC == 3 ? 3 : 4;
+  static constexpr int NSTAGE = bf16_nstage(NPC);
   static constexpr int TILE = 128 * 32;         // bf16 elements per operand piece
   static constexpr int STAGE = 2 * NPC * TILE;  // A pieces then B pieces
   static constexpr int NWAVES = (4 / TM) * (4 / TN);
@@ -473,6 +477,19 @@ template <int N>
 __device__ inline void wait_vmcnt() {
   // s_waitcnt vmcnt(N); expcnt and lgkmcnt fields at their maxima (no wait)
   __builtin_amdgcn_s_waitcnt((N & 0xF) | (0x7 << 4) | (0xF << 8) | ((N >> 4) << 14));
+}
+
+// Wait until at most `later` k-tiles of LPT loads each are outstanding
+// (later <= MAXL, a wave-uniform value).
+template <int LPT, int MAXL>
+__device__ inline void wait_later(int later) {
+  if constexpr (MAXL <= 0) {
+    wait_vmcnt<0>();
+  } else {
+    static_assert(MAXL * LPT < 64, "vmcnt holds at most 63");
+    if (later >= MAXL) wait_vmcnt<MAXL * LPT>();
+    else wait_later<LPT, MAXL - 1>(later);
+  }
 }
 
 template <int NPROD, bool ACC64, int TM, int TN>
@@ -557,9 +574,7 @@ __device__ inline void bf16_mainloop(const GemmArgs& a, const uint16_t* S, uint1
   // statements keep the compiler from moving LDS reads across the barrier.
   auto enter_tile = [&](int kt) {
     const int later = (nk - 1 - kt) < (C::NSTAGE - 2) ? (nk - 1 - kt) : (C::NSTAGE - 2);
-    if (later >= 2) wait_vmcnt<2 * C::LPT>();
-    else if (later == 1) wait_vmcnt<C::LPT>();
-    else wait_vmcnt<0>();
+    wait_later<C::LPT, C::NSTAGE - 2>(later);
     __builtin_amdgcn_s_waitcnt((0xF) | (0x7 << 4) | (0x0 << 8) | (0x3 << 14));  // lgkmcnt(0)
     asm volatile("" ::: "memory");
     __builtin_amdgcn_s_barrier();
@@ -656,8 +671,7 @@ __device__ inline void bf16_mainloop(const GemmArgs& a, const uint16_t* S, uint1
 
 size_t dst_gemm_lds_bytes(int max_np) {
   const int npc = max_np == 1 ? 1 : (max_np == 3 ? 2 : 3);
-  const int nstage = npc == 3 ? 3 : 4;
-  return (size_t)nstage * 2 * npc * 128 * 32 * sizeof(uint16_t);
+  return (size_t)bf16_nstage(npc) * 2 * npc * 128 * 32 * sizeof(uint16_t);
 }
 
 // Workgroup shape per instance: x1 / x3 run 16 waves of 32 x 32 (four per
