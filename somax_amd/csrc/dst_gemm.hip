@@ -83,31 +83,22 @@ struct Spectral {
   float lamz;
 };
 
-// Eigenvalues of the spectral epilogue that one thread divides by in the
-// bf16-piece stores of a 1024-thread tile (one row per half, 8 columns),
-// fetched before the k-loop so that their latency hides under it.
-struct SpectralPre {
-  float la[2];  // lamA of this thread's row in each half
-  float lb[8];  // lamB of its 8 columns (clamped to RB - 1 like the stores)
-};
-
-__device__ inline void prefetch_spectral(const GemmArgs& a, const Spectral& sp, int a0, int b0, SpectralPre& pre) {
-  const int co = b0 + (threadIdx.x & 15) * 8;
-#pragma unroll
-  for (int e = 0; e < 8; ++e) pre.lb[e] = sp.lamB[co + e < a.RB ? co + e : a.RB - 1];
-#pragma unroll
-  for (int h = 0; h < 2; ++h) {
-    const int ro = a0 + h * 64 + (threadIdx.x >> 4);
-    pre.la[h] = sp.lamA[ro < a.RA ? ro : a.RA - 1];
-  }
+// The spectral epilogue's eigenvalues of one tile (lamA of its 128 rows, then
+// lamB of its 128 columns, both clamped to the last valid index like the
+// stores) are fetched by the first 256 threads before the k-loop, so their
+// latency hides under it, and staged in LDS for the epilogue.
+__device__ inline float prefetch_spectral(const GemmArgs& a, const Spectral& sp, int a0, int b0) {
+  const int t = threadIdx.x;
+  if (t < 128) return sp.lamA[a0 + t < a.RA ? a0 + t : a.RA - 1];
+  if (t < 256) return sp.lamB[b0 + t - 128 < a.RB ? b0 + t - 128 : a.RB - 1];
+  return 0.0f;
 }
 
-template <bool SUM64, int TM, int TN, bool PRE = false>
+template <bool SUM64, int TM, int TN>
 __device__ inline void store_tile(const GemmArgs& a, long z, int a0, int b0, const floatx16 (&acc)[TM][TN],
                                   const double (&sum)[TM][TN][16], int opieces, float* stage, const Spectral& sp,
-                                  const SpectralPre* pre = nullptr) {
+                                  const float* lam = nullptr) {
   constexpr int WN = 4 / TN, WM = 4 / TM, NT = 64 * WM * WN;  // waves along N and M, threads
-  static_assert(!PRE || NT == 1024, "prefetched eigenvalues cover one row per half");
   const int tid = threadIdx.x;
   const int lane = tid & 63;
   const int wave = tid >> 6;
@@ -156,11 +147,11 @@ __device__ inline void store_tile(const GemmArgs& a, long z, int a0, int b0, con
         float v[8] = {va[0], va[1], va[2], va[3], vb[0], vb[1], vb[2], vb[3]};
         // branch free per element (columns past RB are zero padding)
         if (sp.lamA) {
-          const float la = (PRE ? pre->la[h] : sp.lamA[ro]) - lamz;
+          const float la = (lam ? lam[h * 64 + lr] : sp.lamA[ro]) - lamz;
 #pragma unroll
           for (int e = 0; e < 8; ++e) {
             const int c = co + e < a.RB ? co + e : a.RB - 1;
-            const float q = v[e] / (la + (PRE ? pre->lb[e] : sp.lamB[c]));
+            const float q = v[e] / (la + (lam ? lam[128 + lc + e] : sp.lamB[c]));
             v[e] = co + e < a.RB ? q : 0.0f;
           }
         } else {
@@ -722,10 +713,9 @@ void dst_gemm_bf16_kernel(GemmArgs a) {
   const bool x_in_a = !a.trans;
   const uint16_t* S = a.Sb + ps.s_off;
 
-  // the 1024-thread instances fetch the spectral epilogue's eigenvalues now
-  constexpr bool PRE = NT == 1024;
-  SpectralPre pre{};
-  if (PRE && ps.sp.lamA && opieces != 0) prefetch_spectral(a, ps.sp, a0, b0, pre);
+  __shared__ float lam_tile[256];
+  const bool spectral = ps.sp.lamA && opieces != 0;
+  const float lam_pre = spectral ? prefetch_spectral(a, ps.sp, a0, b0) : 0.0f;
 
   floatx16 acc[TM][TN];
   double sum[TM][TN][16];
@@ -737,8 +727,10 @@ void dst_gemm_bf16_kernel(GemmArgs a) {
   else if (a.acc64) bf16_mainloop<6, true, TM, TN>(a, S, lds16, z, a0, b0, x_in_a, acc, sum);
   else bf16_mainloop<6, false, TM, TN>(a, S, lds16, z, a0, b0, x_in_a, acc, sum);
   float* stage = reinterpret_cast<float*>(lds16);
-  if (NPMAX == 6 && np == 6 && a.acc64) store_tile<true, TM, TN>(a, z, a0, b0, acc, sum, opieces, stage, ps.sp);
-  else store_tile<false, TM, TN, PRE>(a, z, a0, b0, acc, sum, opieces, stage, ps.sp, &pre);
+  if (spectral && threadIdx.x < 256) lam_tile[threadIdx.x] = lam_pre;  // read after store_tile's first barrier
+  const float* lam = spectral ? lam_tile : nullptr;
+  if (NPMAX == 6 && np == 6 && a.acc64) store_tile<true, TM, TN>(a, z, a0, b0, acc, sum, opieces, stage, ps.sp, lam);
+  else store_tile<false, TM, TN>(a, z, a0, b0, acc, sum, opieces, stage, ps.sp, lam);
 }
 template __global__ void dst_gemm_bf16_kernel<1>(GemmArgs);
 template __global__ void dst_gemm_bf16_kernel<3>(GemmArgs);
