@@ -408,7 +408,8 @@ def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
-    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=50,
+                    help="untimed steps before each timed region (the GPU's clocks settle within a few dozen)")
     ap.add_argument("--n", type=int, default=1024, help="interior cells per side")
     ap.add_argument("--dt", type=float, default=1200.0)
     ap.add_argument("--advection", default="arakawa", choices=["arakawa", "weno"])
