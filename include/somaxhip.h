@@ -108,7 +108,8 @@ typedef struct {
   double tau0;   /* wind_amplitude (baroclinic.py:181) */
 } somaxhip_qg_params;
 
-/* Builds the plan on the current device (host tables are copied). */
+/* Builds the plan on the current device (host tables are copied).  A plane of
+ * (nx + 2) * (ny + 2) cells must stay below 2^31 (SOMAXHIP_ERR_ARG otherwise). */
 int somaxhip_qg_plan_create(const somaxhip_qg_desc* desc, somaxhip_qg_plan* plan);
 int somaxhip_qg_plan_destroy(somaxhip_qg_plan plan);
 
@@ -197,7 +198,8 @@ typedef struct {
   double tau0;   /* wind_amplitude (multilayer.py:190-191) */
 } somaxhip_swm_params;
 
-/* Plan of one MultilayerShallowWater2D.create (multilayer.py:259-377). */
+/* Plan of one MultilayerShallowWater2D.create (multilayer.py:259-377).  A
+ * plane of (nx + 2) * (ny + 2) cells must stay below 2^31. */
 int somaxhip_swm_plan_create(const somaxhip_swm_desc* desc, somaxhip_swm_plan* plan);
 int somaxhip_swm_plan_destroy(somaxhip_swm_plan plan);
 size_t somaxhip_swm_workspace_bytes(somaxhip_swm_plan plan);
