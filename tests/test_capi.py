@@ -10,6 +10,7 @@ from pathlib import Path
 import pytest
 
 ROOT = Path(__file__).resolve().parent.parent
+ERR_ARG = 1  # SOMAXHIP_ERR_ARG
 
 
 def _declared():
@@ -44,3 +45,24 @@ def test_version():
     from somax_amd import _lib
 
     assert _lib.load().somaxhip_version() >= 100
+
+
+@pytest.mark.parametrize("kind", ["qg", "swm"])
+def test_plan_create_rejects_planes_of_2_31_cells(kind):
+    """Argument checks run before any device call, so this needs no GPU: a
+    plane of (nx + 2) * (ny + 2) >= 2^31 cells is refused (the kernels index a
+    plane with 32-bit offsets), one just below passes the size check and
+    fails on the next one (null tables)."""
+    from somax_amd import _lib
+
+    lib = _lib.load()
+    plan = ctypes.c_void_p()
+    desc_t, create = ((_lib.QGDesc, lib.somaxhip_qg_plan_create) if kind == "qg"
+                      else (_lib.SWMDesc, lib.somaxhip_swm_plan_create))
+    d = desc_t()
+    d.nx, d.ny, d.nl, d.dx, d.dy = 46340, 46340, 1, 1.0, 1.0  # 46342^2 > 2^31 - 1
+    assert create(ctypes.byref(d), ctypes.byref(plan)) == ERR_ARG
+    assert "2^31" in _lib.last_error()
+    d.nx = d.ny = 46338  # 46340^2 < 2^31 - 1
+    assert create(ctypes.byref(d), ctypes.byref(plan)) == ERR_ARG
+    assert "2^31" not in _lib.last_error()
