@@ -2,11 +2,14 @@
 """Throughput of the MI355X QG time step in layer-cell-steps/s.
 
 Workload (default): the 3-layer QG double gyre at 1024 x 1024 interior cells
-(3 x 1026 x 1026 float32 state), biharmonic viscosity 2e11*(1/8)^4 m^4/s,
-bottom drag 3.6e-8 /s, tau0 = 0.08 N/m^2, dt = 1200 s, seeded k^-3 random
-initial PV, advanced by the SSP-RK3 step with the mixed-precision MFMA
-inversion (BF16 hi/lo 3-product GEMMs + one refinement pass).  The plain
-all-FP32 step on the same state is timed in the same run as the comparator.
+(3 x 1026 x 1026 float32 state), H = 400/1100/2600 m, g' = 9.81/0.025/0.0125,
+WENO-Z PV flux, biharmonic viscosity 2e11*(1/8)^4 m^4/s, bottom drag
+3.6e-8 /s, tau0 = 0.08 N/m^2, dt = 1200 s, the homogeneous-solution mass
+constraint (FP64 domain means), seeded k^-3 random initial streamfunction
+with zero layer means, advanced by the SSP-RK3 step with the mixed-precision
+MFMA inversion (BF16 hi/lo 3-product GEMMs + one refinement pass).  The
+plain all-FP32 step on the same state is timed in the same run as the
+comparator, and the 256-member uneven ensemble (same physics) beside them.
 
 A layer-cell-step is one interior cell of one layer advanced by one full
 SSP-RK3 step.  Inputs are resident in HBM before the timed region; the timed
@@ -15,9 +18,9 @@ events on the stream the kernels run on.
 
 Prints one JSON line.  ``roofline`` is taken from a second, per-launch
 profiled run (HIP events around every launch, somaxhip_qg_step_profile);
-``cpu_baseline`` times the C++/OpenMP FP32 restatement of the step
-(oracle/cpu_step.cpp) on the host's threads for a bounded sample of the same
-workload, with the FP64 NumPy oracle on one core beside it.
+``cpu_baseline`` times the FP32 host restatement of the same step with
+FFT-based DST-I (oracle/cpu.py:FftCpuStep) on all host threads and on one,
+for a bounded sample of the same workload.
 """
 
 from __future__ import annotations
@@ -36,10 +39,10 @@ def log(msg: str) -> None:
     print(msg, file=sys.stderr, flush=True)
 
 
-def build_model(n: int, precision: str, advection: str):
+def build_model(n: int, precision: str, advection: str, mass: bool):
     from somax_amd import testcases as tc
 
-    return tc.config_baroclinic(n, precision=precision, advection=advection)
+    return tc.config_baroclinic(n, precision=precision, advection=advection, mass_correction=mass)
 
 
 def time_steps(model, q0, dt: float, steps: int, warmup: int):
@@ -117,48 +120,46 @@ def roofline_entry(name: str, st: dict) -> dict:
             "alg_bytes_per_launch": per_launch, "traffic": None}
 
 
-def cpu_baseline(n: int, q64, dt: float, steps: int) -> dict:
-    """The same 1024^2 workload on the host: the C++/OpenMP FP32 restatement
-    of the step (oracle/cpu_step.cpp: dense DST-I, fused stencils) on every
-    host thread OpenMP is given, for ``steps`` steps, and beside it the FP64
-    NumPy oracle on one core for six steps."""
-    from threadpoolctl import threadpool_limits
-
+def cpu_baseline(n: int, q64, dt: float, steps: int, advection: str, mass: bool) -> dict:
+    """The same workload on the host: the FP32 restatement of the step with
+    FFT-based DST-I (oracle/cpu.py:FftCpuStep: pocketfft sine transforms,
+    OpenMP stencils), timed for ``steps`` steps on every host thread this
+    process may use, and for one step on one thread."""
     from oracle import cpu as ocpu
     from oracle import qg as oqg
     from somax_amd import testcases as tc
 
     setup = oqg.baroclinic_setup(nx=n, ny=n, Lx=tc.BASIN, Ly=tc.BASIN, f0=tc.F0, beta=tc.BETA, bottom_drag=tc.DRAG,
-                                 wind_amplitude=tc.wind_amplitude(), biharmonic_viscosity=tc.NU4_128 * (128 / n) ** 4)
+                                 wind_amplitude=tc.wind_amplitude(), biharmonic_viscosity=tc.NU4_128 * (128 / n) ** 4,
+                                 advection=advection, mass_correction=mass)
     nl = q64.shape[0]
-    if not ocpu.LIB.exists():
+    if not (ocpu.LIB.exists() and ocpu.RHS_LIB.exists()):
         ocpu.build()
-    cs = ocpu.CpuStep(setup, fp64=False)
-    q = cs.step(q64, dt, 1)  # warm-up
-    t0 = time.perf_counter()
-    q = cs.step(q, dt, steps)
-    el = time.perf_counter() - t0
-    k = 6
-    qo = q64.copy()
-    with threadpool_limits(1):
-        t1 = time.perf_counter()
-        for _ in range(k):
-            qo = oqg.ssprk3_step(setup, qo, dt)
-        elo = time.perf_counter() - t1
-    return {"value": round(nl * n * n * steps / el, 1), "unit": "layer-cell-steps/s", "cores": cs.threads,
-            "kind": "port",
-            "sample": f"{steps} SSP-RK3 steps of the same {nl}x{n}x{n} workload by the C++/OpenMP FP32 "
-                      f"restatement of the step (oracle/cpu_step.cpp: dense DST-I GEMMs, fused stencils) on "
-                      f"{cs.threads} host threads, {el:.1f} s",
-            "oracle_fp64_numpy_1core": {"value": round(nl * n * n * k / elo, 1), "steps": k,
-                                        "sample": f"FP64 NumPy/SciPy oracle (FFT DST-I), one thread, {elo:.1f} s"}}
+    q32 = q64.astype("float32")
+    out = {}
+    for label, threads, k in (("all", ocpu.host_threads(), steps), ("one", 1, 1)):
+        cs = ocpu.FftCpuStep(setup, threads=threads)
+        cs.step(q32, dt, 1)  # warm-up
+        t0 = time.perf_counter()
+        cs.step(q32, dt, k)
+        el = time.perf_counter() - t0
+        out[label] = (round(nl * n * n * k / el, 1), threads, k, el)
+    v, th, k, el = out["all"]
+    v1, _, k1, el1 = out["one"]
+    return {"value": v, "unit": "layer-cell-steps/s", "cores": th, "kind": "port",
+            "sample": f"{k} SSP-RK3 steps of the same {nl}x{n}x{n} workload ({advection}"
+                      f"{', mass constraint' if mass else ''}) by the FP32 host restatement of the step with "
+                      f"FFT-based DST-I (oracle/cpu.py:FftCpuStep: pocketfft sine transforms, OpenMP stencils) on "
+                      f"{th} host threads, {el:.1f} s",
+            "one_thread": {"value": v1, "steps": k1, "sample": f"the same step on one thread, {el1:.1f} s"}}
 
 
 def bench_ensemble(dt: float, steps: int, warmup: int) -> dict:
     """The 256-member uneven ensemble (96 x 64^2, 96 x 128^2, 48 x 256^2,
-    16 x 512^2, layers cycling 1/2/3, tau0 ~ U[0.04, 0.12]) advanced as one
-    hipGraph-captured ensemble step with the mixed inversion, against the
-    same members stepped one after another with their own plain FP32 plans."""
+    16 x 512^2, layers cycling 1/2/3, tau0 ~ U[0.04, 0.12], WENO-Z PV flux and
+    the mass constraint) advanced as one hipGraph-captured ensemble step with
+    the mixed inversion, against the same members stepped one after another
+    with their own plain FP32 plans."""
     import numpy as np
     import torch
 
@@ -169,7 +170,8 @@ def bench_ensemble(dt: float, steps: int, warmup: int) -> dict:
     models, qs = tc.config_ensemble(seed=7, precision="mixed")
     cells = sum(m._nl * m.grid.nx * m.grid.ny for m in models)
     out = {"members": len(models), "layer_cells": cells,
-           "counts": {f"{n}x{n}": c for n, c in tc.ENSEMBLE_COUNTS}}
+           "counts": {f"{n}x{n}": c for n, c in tc.ENSEMBLE_COUNTS},
+           "physics": "WENO-Z PV flux, FP64 mass constraint, zero-mean seeded k^-3 initial streamfunctions"}
 
     def timed(fn, k):
         torch.cuda.synchronize()
@@ -208,7 +210,8 @@ def bench_ensemble(dt: float, steps: int, warmup: int) -> dict:
     del ens, packed
 
     log("[bench] ensemble: per-member plain FP32 steps")
-    plain = [tc.ensemble_member(m.grid.nx, m._nl, 0.0, precision="fp32") for m in models]
+    plain = [tc.ensemble_member(m.grid.nx, m._nl, 0.0, precision="fp32", advection="weno", mass_correction=True)
+             for m in models]
     for p, m in zip(plain, models):
         p.params = m.params
     qd = [torch.from_numpy(q.astype(np.float32)).cuda() for q in qs]
@@ -412,9 +415,10 @@ def main() -> int:
                     help="untimed steps before each timed region (the GPU's clocks settle within a few dozen)")
     ap.add_argument("--n", type=int, default=1024, help="interior cells per side")
     ap.add_argument("--dt", type=float, default=1200.0)
-    ap.add_argument("--advection", default="arakawa", choices=["arakawa", "weno"])
+    ap.add_argument("--advection", default="weno", choices=["arakawa", "weno"])
+    ap.add_argument("--no-mass", action="store_true", help="leave out the homogeneous-solution mass constraint")
     ap.add_argument("--profile-steps", type=int, default=2)
-    ap.add_argument("--cpu-steps", type=int, default=60, help="C++ CPU steps timed for cpu_baseline (0 = skip)")
+    ap.add_argument("--cpu-steps", type=int, default=12, help="host steps timed for cpu_baseline (0 = skip)")
     ap.add_argument("--ensemble-steps", type=int, default=10, help="steps timed for the 256-member ensemble (0 = skip)")
     ap.add_argument("--swm-steps", type=int, default=50, help="steps timed for the shallow-water jet (0 = skip)")
     ap.add_argument("--cmm-solves", type=int, default=20, help="masked-domain solves timed (0 = skip)")
@@ -435,10 +439,11 @@ def main() -> int:
     from somax_amd import testcases as tc
 
     n, dt = args.n, args.dt
-    log(f"[bench] building models n={n}")
-    mixed = build_model(n, "mixed", args.advection)
-    plain = build_model(n, "fp32", args.advection)
-    q64 = tc.random_pv(mixed, seed=2024)
+    mass = not args.no_mass
+    log(f"[bench] building models n={n} advection={args.advection} mass_constraint={mass}")
+    mixed = build_model(n, "mixed", args.advection, mass)
+    plain = build_model(n, "fp32", args.advection, mass)
+    q64 = tc.random_pv(mixed, seed=2024, zero_mean=mass)
     q0 = torch.from_numpy(q64.astype(np.float32)).cuda()
     nl = q0.shape[0]
     cells = nl * n * n
@@ -451,7 +456,7 @@ def main() -> int:
     log(f"[bench] plain {ms_plain:.3f} ms/step")
 
     log("[bench] timing bf16x6 (three-piece operands, six products, no refinement)")
-    x6 = build_model(n, "bf16x6", args.advection)
+    x6 = build_model(n, "bf16x6", args.advection, mass)
     ms_x6, _ = time_steps(x6, q0, dt, args.steps, args.warmup)
     log(f"[bench] bf16x6 {ms_x6:.3f} ms/step")
     del x6
@@ -464,8 +469,9 @@ def main() -> int:
     roof_stencil = roofline_entry("qg_stage", prof_mixed["qg_stage"])
     dom_p = max(prof_plain, key=lambda k: prof_plain[k]["total_ms"])
     roof_plain = roofline_entry(dom_p, prof_plain[dom_p])
-    if n == 1024 and args.advection == "arakawa":
-        tr = measured_traffic()
+    workload = f"qg{nl}x{n}x{n}_{args.advection}{'_mass' if mass else ''}_ssprk3"
+    if n == 1024:
+        tr = measured_traffic(f"hbm_traffic_{workload}_mixed.json")
         roof["traffic"] = tr.get(dom)
         roof_stencil["traffic"] = tr.get("qg_stage")
 
@@ -491,7 +497,7 @@ def main() -> int:
     cpu = None
     if args.cpu_steps > 0:
         log("[bench] cpu baseline (oracle)")
-        cpu = cpu_baseline(n, q64, dt, args.cpu_steps)
+        cpu = cpu_baseline(n, q64, dt, args.cpu_steps, args.advection, mass)
 
     value = cells / (ms_mixed * 1e-3)
     plain_value = cells / (ms_plain * 1e-3)
@@ -506,10 +512,14 @@ def main() -> int:
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32 stencils; bf16x3 MFMA + f32 refinement inversion",
-        "data": "synthetic (seeded k^-3 random PV)",
-        "config": {"workload": f"qg{nl}x{n}x{n}_ssprk3_mixed", "n": n, "layers": nl, "dt": dt,
-                   "advection": args.advection, "inversion": "mixed"},
+        "dtype": "f32 stencils; inversion: bf16x3 MFMA GEMMs (bf16x6 on Poisson-like planes' first inverse) "
+                 "with f32 accumulation + one f32 refinement pass; f64 layer-to-mode projection and "
+                 "mass-constraint means",
+        "data": "synthetic (seeded k^-3 random streamfunction" + (", zero layer means)" if mass else ")"),
+        "config": {"workload": f"{workload}_mixed", "n": n, "layers": nl, "dt": dt, "advection": args.advection,
+                   "mass_constraint": mass, "inversion": "mixed",
+                   "value_is": "the single 3-layer 1024^2 mixed step; the plain FP32 step and the 256-member "
+                               "ensemble are the plain_fp32 and ensemble objects"},
         "plain_fp32": {"value": round(plain_value, 1), "ms_per_step": round(ms_plain, 4),
                        "speedup_mixed_vs_plain": round(ms_plain / ms_mixed, 3), "roofline": roof_plain},
         "bf16x6": {"value": round(cells / (ms_x6 * 1e-3), 1), "ms_per_step": round(ms_x6, 4),

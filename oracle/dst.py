@@ -37,12 +37,15 @@ def dst1_eigenvalues(n: int, d: float) -> np.ndarray:
     return (2.0 * np.cos(np.pi * k / (n + 1)) - 2.0) / (d * d)
 
 
-def dst2(f: np.ndarray) -> np.ndarray:
-    """Orthonormal 2-D DST-I over the last two axes (textbook ``scipy.fft.dstn``)."""
-    return scipy.fft.dstn(f, type=1, axes=(-2, -1), norm="ortho")
+def dst2(f: np.ndarray, workers: int | None = None) -> np.ndarray:
+    """Orthonormal 2-D DST-I over the last two axes (textbook ``scipy.fft.dstn``).
+
+    ``workers`` threads split the independent 1-D transforms, which leaves
+    every result bit unchanged."""
+    return scipy.fft.dstn(f, type=1, axes=(-2, -1), norm="ortho", workers=workers)
 
 
-def helmholtz_dst(f: np.ndarray, dx: float, dy: float, lam: float = 0.0) -> np.ndarray:
+def helmholtz_dst(f: np.ndarray, dx: float, dy: float, lam: float = 0.0, workers: int | None = None) -> np.ndarray:
     """Solve ``(nabla_h^2 - lam) phi = f`` by DST-I over the full array (ghosts included).
 
     ``f`` has shape ``(..., Ny, Nx)``; ``lam`` may be a scalar or broadcast
@@ -54,7 +57,7 @@ def helmholtz_dst(f: np.ndarray, dx: float, dy: float, lam: float = 0.0) -> np.n
     lam = np.asarray(lam, dtype=np.float64)
     denom = (dst1_eigenvalues(ny, dy)[:, None] + dst1_eigenvalues(nx, dx)[None, :])
     denom = denom - lam[..., None, None] if lam.ndim else denom - lam
-    return dst2(dst2(f) / denom)
+    return dst2(dst2(f, workers) / denom, workers)
 
 
 def helmholtz_apply(phi: np.ndarray, dx: float, dy: float, lam: float = 0.0) -> np.ndarray:

@@ -168,12 +168,13 @@ def apply_bc(q: np.ndarray) -> np.ndarray:
     return stencils.zero_boundaries(q)
 
 
-def invert_pv(s: QGSetup, q: np.ndarray, return_modal=False):
+def invert_pv(s: QGSetup, q: np.ndarray, return_modal=False, workers: int | None = None):
     """``_invert_pv`` (baroclinic.py:135-159): to_modal, per-mode DST-I
-    Helmholtz solve, [mass correction], to_layer, [ghost ring of psi = 0]."""
+    Helmholtz solve, [mass correction], to_layer, [ghost ring of psi = 0].
+    ``workers``: threads of the sine transforms (results unchanged)."""
     q = np.asarray(q, dtype=np.float64).reshape(s.nl, s.Ny, s.Nx)
     qm = np.einsum("lm,m...->l...", s.Cl2m, q)
-    pm = dst.helmholtz_dst(qm, s.dx, s.dy, s.lambdas)
+    pm = dst.helmholtz_dst(qm, s.dx, s.dy, s.lambdas, workers=workers)
     alpha = np.zeros(s.nl)
     if s.mass_correction:
         alpha = -pm.mean(axis=(-2, -1)) / s.homsol_mean
@@ -192,12 +193,22 @@ def psi_wall(s: QGSetup, alpha: np.ndarray) -> np.ndarray:
     return s.Cm2l @ alpha
 
 
-def rhs(s: QGSetup, q: np.ndarray, return_psi=False):
+def rhs(s: QGSetup, q: np.ndarray, return_psi=False, fast=False):
     """``vector_field`` (baroclinic.py:161-190 / barotropic.py:123-152) on a
-    state whose boundary condition has already been applied."""
+    state whose boundary condition has already been applied.
+
+    ``fast``: the same arithmetic with threaded sine transforms and the
+    stencil terms from ``oracle/qgrhs.c`` (FP64, OpenMP), which restates the
+    NumPy expressions below operation for operation (same bits; pinned in
+    ``tests/test_oracle_pins.py``).  Used where the grid is large."""
     q = np.asarray(q, dtype=np.float64).reshape(s.nl, s.Ny, s.Nx)
-    psi, _, alpha = invert_pv(s, q, return_modal=True)
+    psi, _, alpha = invert_pv(s, q, return_modal=True, workers=-1 if fast else None)
     qt = q + s.beta_y[None]
+    if fast:
+        from . import cpu
+
+        dq = cpu.qg_stencil_rhs(s, psi, q, qt, psi_wall(s, alpha))
+        return (dq, psi) if return_psi else dq
     if s.advection == "weno":
         dq = stencils.weno_advection(psi, qt, s.dx, s.dy, psi_wall(s, alpha))
     else:
@@ -215,12 +226,12 @@ def rhs(s: QGSetup, q: np.ndarray, return_psi=False):
     return dq
 
 
-def ssprk3_step(s: QGSetup, q: np.ndarray, dt: float) -> np.ndarray:
+def ssprk3_step(s: QGSetup, q: np.ndarray, dt: float, fast=False) -> np.ndarray:
     """One SSP-RK3 step (Shu-Osher form) with the BC applied before each RHS."""
     q0 = np.asarray(q, dtype=np.float64)
-    q1 = q0 + dt * rhs(s, apply_bc(q0))
-    q2 = 0.75 * q0 + 0.25 * (q1 + dt * rhs(s, apply_bc(q1)))
-    return q0 / 3.0 + 2.0 / 3.0 * (q2 + dt * rhs(s, apply_bc(q2)))
+    q1 = q0 + dt * rhs(s, apply_bc(q0), fast=fast)
+    q2 = 0.75 * q0 + 0.25 * (q1 + dt * rhs(s, apply_bc(q1), fast=fast))
+    return q0 / 3.0 + 2.0 / 3.0 * (q2 + dt * rhs(s, apply_bc(q2), fast=fast))
 
 
 def tsit5_step(s: QGSetup, q: np.ndarray, dt: float) -> np.ndarray:
@@ -231,9 +242,9 @@ def tsit5_step(s: QGSetup, q: np.ndarray, dt: float) -> np.ndarray:
     return rk.explicit_step(lambda y: (rhs(s, apply_bc(y[0])),), (q,), dt)[0]
 
 
-def run(s: QGSetup, q: np.ndarray, dt: float, nsteps: int) -> np.ndarray:
+def run(s: QGSetup, q: np.ndarray, dt: float, nsteps: int, fast=False) -> np.ndarray:
     for _ in range(nsteps):
-        q = ssprk3_step(s, q, dt)
+        q = ssprk3_step(s, q, dt, fast=fast)
     return q
 
 

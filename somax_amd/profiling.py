@@ -6,8 +6,12 @@ kernel (its FETCH_SIZE / WRITE_SIZE give the counters' scale against a known
 byte count), then advances the named workload by ``--steps`` SSP-RK3 steps.
 Workloads:
 
-* ``qg1024-mixed`` / ``qg1024-fp32``: 3-layer QG double gyre, 1024 x 1024
-  interior cells, mixed or plain FP32 inversion, Arakawa advection.
+* ``qg1024-weno-mass-mixed`` / ``qg1024-weno-mass-fp32`` (default): the
+  3-layer QG double gyre at 1024 x 1024 interior cells with WENO-Z PV flux
+  and the mass constraint, zero-mean initial streamfunction, mixed or plain
+  FP32 inversion (the benchmark's headline workload).
+* ``qg1024-mixed`` / ``qg1024-fp32``: the same grid with Arakawa advection
+  and no mass constraint.
 * ``ensemble``: the 256-member uneven ensemble, mixed inversion, one
   eager step (every launch visible to the profiler).
 * ``swm512``: the 2-layer shallow-water baroclinic jet, 512 x 512 channel,
@@ -16,10 +20,11 @@ Workloads:
 Each step is preceded by one untimed warm-up step.  Prints one line per
 phase to stdout.
 
-``python -m somax_amd.profiling summarize-trace TRACE.csv`` and
-``summarize-pmc FETCH.csv WRITE.csv COPY_MIB [OUT.json WORKLOAD]`` turn
-rocprofv3 CSV output into the markdown tables kept with the profiles (and the
-per-kernel HBM bytes per launch that ``bench.py`` reports as ``traffic``).
+``python -m somax_amd.profiling summarize-trace TRACE.csv``,
+``summarize-pmc FETCH.csv WRITE.csv COPY_MIB [OUT.json WORKLOAD]`` and
+``summarize-counters COUNTERS.csv [COUNTERS.csv ...]`` turn rocprofv3 CSV
+output into the markdown tables kept with the profiles (and the per-kernel
+HBM bytes per launch that ``bench.py`` reports as ``traffic``).
 """
 
 from __future__ import annotations
@@ -109,11 +114,40 @@ def summarize_pmc(fetch_csv: str, write_csv: str, copy_mib: int, per_name: dict 
     return "\n".join(lines)
 
 
+def summarize_counters(paths) -> str:
+    """Markdown table of any rocprofv3 ``--pmc`` counters: per (kernel, grid)
+    the dispatches and the mean of each counter per dispatch, over one or
+    more counter CSVs (one per pass)."""
+    import collections
+    import csv
+
+    agg = collections.defaultdict(lambda: collections.defaultdict(list))
+    names = []
+    for path in paths:
+        with open(path) as f:
+            for r in csv.DictReader(f):
+                c = r["Counter_Name"]
+                if c not in names:
+                    names.append(c)
+                agg[(r["Kernel_Name"], _grid(r))][c].append(float(r["Counter_Value"]))
+    lines = ["| kernel | grid | dispatches | " + " | ".join(names) + " |",
+             "|---|---:|---:|" + "---:|" * len(names)]
+    for key in sorted(agg, key=lambda k: -max((sum(v) / len(v) for v in agg[k].values()), default=0.0)):
+        row = agg[key]
+        n = max(len(v) for v in row.values())
+        cells = [f"{sum(row[c]) / len(row[c]):.4g}" if row.get(c) else "" for c in names]
+        lines.append(f"| `{key[0][:70]}` | {key[1]} | {n} | " + " | ".join(cells) + " |")
+    return "\n".join(lines)
+
+
 def main(argv=None) -> int:
     if argv is None:
         argv = sys.argv[1:]
     if argv and argv[0] == "summarize-trace":
         print(summarize_trace(argv[1]))
+        return 0
+    if argv and argv[0] == "summarize-counters":
+        print(summarize_counters(argv[1:]))
         return 0
     if argv and argv[0] == "summarize-pmc":
         # summarize-pmc FETCH.csv WRITE.csv COPY_MIB [OUT.json WORKLOAD]
@@ -127,7 +161,9 @@ def main(argv=None) -> int:
                 json.dump(per, f, indent=1)
         return 0
     ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
-    ap.add_argument("--workload", default="qg1024-mixed", choices=["qg1024-mixed", "qg1024-fp32", "ensemble", "swm512", "cmm256"])
+    ap.add_argument("--workload", default="qg1024-weno-mass-mixed",
+                    choices=["qg1024-weno-mass-mixed", "qg1024-weno-mass-fp32", "qg1024-mixed", "qg1024-fp32",
+                             "ensemble", "swm512", "cmm256"])
     ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--copy-mib", type=int, default=1024)
     ap.add_argument("--dt", type=float, default=1200.0)
@@ -200,8 +236,10 @@ def main(argv=None) -> int:
         return 0
 
     precision = "mixed" if args.workload.endswith("mixed") else "fp32"
-    model = tc.config_baroclinic(1024, precision=precision)
-    q = torch.from_numpy(tc.random_pv(model, seed=2024).astype(np.float32)).cuda()
+    weno_mass = "weno-mass" in args.workload
+    model = tc.config_baroclinic(1024, precision=precision, advection="weno" if weno_mass else "arakawa",
+                                 mass_correction=weno_mass)
+    q = torch.from_numpy(tc.random_pv(model, seed=2024, zero_mean=weno_mass).astype(np.float32)).cuda()
     model._step_inplace(q, args.dt, 1)
     torch.cuda.synchronize()
     model._step_inplace(q, args.dt, args.steps)

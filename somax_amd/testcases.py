@@ -174,11 +174,14 @@ def ensemble_member(nx: int, nl: int, tau0: float, ny: int | None = None, **kw) 
     return BaroclinicQG.create(**ensemble_member_args(nx, nl, tau0, ny), **kw)
 
 
-def config_ensemble(seed: int = 0, counts=ENSEMBLE_COUNTS, precision: str = "mixed", **kw):
+def config_ensemble(seed: int = 0, counts=ENSEMBLE_COUNTS, precision: str = "mixed", advection: str = "weno",
+                    mass_correction: bool = True, **kw):
     """The uneven ensemble: ``counts`` = (grid size, members) pairs; layer
     counts cycle 1/2/3 over the members in order; tau0 ~ U[0.04, 0.12] N/m^2
-    and a seeded k^-3 random initial streamfunction per member.  Returns
-    ``(models, member PV fields as float32 host arrays)``."""
+    and a seeded k^-3 random initial streamfunction per member (zero layer
+    means when the mass constraint is on).  The physics defaults to the
+    multilayer double gyre's: WENO-Z PV flux and the FP64 mass constraint.
+    Returns ``(models, member PV fields as float32 host arrays)``."""
     rng = np.random.default_rng(seed)
     models, qs = [], []
     i = 0
@@ -186,9 +189,10 @@ def config_ensemble(seed: int = 0, counts=ENSEMBLE_COUNTS, precision: str = "mix
         for _ in range(count):
             nl = 1 + i % 3
             tau0 = float(rng.uniform(0.04, 0.12))
-            m = ensemble_member(n, nl, tau0, precision=precision, **kw)
+            m = ensemble_member(n, nl, tau0, precision=precision, advection=advection,
+                                mass_correction=mass_correction, **kw)
             models.append(m)
-            qs.append(random_pv(m, seed * 100003 + i).astype(np.float32))
+            qs.append(random_pv(m, seed * 100003 + i, zero_mean=mass_correction).astype(np.float32))
             i += 1
     return models, qs
 

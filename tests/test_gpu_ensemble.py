@@ -245,3 +245,52 @@ def test_autotune_picks_a_valid_configuration(gpu):
         setup = _oracle(i)
         ref = oqg.ssprk3_step(setup, qs[i].astype(np.float64), 1200.0)
         assert rel(ens.member(packed, i).cpu().numpy().reshape(ref.shape), ref) < 1e-6
+
+
+def test_uneven_256_member_ensemble(gpu):
+    """The 256-member ensemble: 96 members at 64^2, 96 at 128^2, 48 at 256^2
+    and 16 at 512^2 (the 256^2 and 512^2 classes with folded transforms),
+    layer counts cycling 1/2/3, tau0 ~ U[0.04, 0.12] N/m^2, WENO-Z PV flux
+    with the FP64 mass constraint, mixed inversion, concurrent classes in one
+    captured graph.  After one step every member equals its single model bit
+    for bit, and one member per (grid, layer count) meets the one-step bar
+    (1e-6) against the FP64 oracle."""
+    import torch
+
+    from somax_amd import QGEnsemble
+    from somax_amd import testcases as tc
+
+    _log("[ensemble] 256 uneven members, WENO-Z + mass constraint, mixed, graph + concurrent")
+    models, qs = tc.config_ensemble(seed=3, precision="mixed")
+    assert len(models) == 256
+    ens = QGEnsemble(models, concurrent=True, graph=True)
+    info = ens.class_info()
+    assert [(c["nx"], c["members"], c["fold"]) for c in info] == [(512, 16, 1), (256, 48, 1), (128, 96, 0),
+                                                                  (64, 96, 0)]
+    packed = ens.pack(qs)
+    dt = 1200.0
+    ens.step(packed, dt, 1)
+    torch.cuda.synchronize()
+    assert torch.isfinite(packed).all()
+    mismatched = []
+    for i, m in enumerate(models):
+        q = torch.from_numpy(qs[i]).cuda()
+        m._step_inplace(q, dt, 1)
+        if not torch.equal(ens.member(packed, i), q.view(ens.shapes[i])):
+            mismatched.append(i)
+    _log(f"  members differing from their single model: {mismatched}")
+    assert not mismatched
+    seen = set()
+    for i, m in enumerate(models):
+        key = (m.grid.nx, m._nl)
+        if key in seen:
+            continue
+        seen.add(key)
+        a = {k: v for k, v in tc.ensemble_member_args(m.grid.nx, m._nl, 0.0).items() if k != "n_layers"}
+        a["wind_amplitude"] = m.params.wind_amplitude  # the member's own tau0
+        setup = oqg.baroclinic_setup(advection="weno", mass_correction=True, **a)
+        ref = oqg.ssprk3_step(setup, qs[i].astype(np.float64), dt, fast=True)
+        r = rel(ens.member(packed, i).cpu().numpy(), ref)
+        _log(f"  member {i} ({m.grid.nx}^2, {m._nl} layers): rel-L2 q vs oracle = {r:.3e}")
+        assert r < 1e-6
+    assert len(seen) == 12

@@ -449,3 +449,105 @@ def test_fold_mass_corrected(gpu, fold_mode, mode):
     means = np.abs(modal.mean(axis=(-2, -1))) / np.abs(modal).mean(axis=(-2, -1))
     _log(f"  rel-L2 psi {rel(psi, ref):.3e}, modal means {means}")
     assert rel(psi, ref) < 3e-7 and means.max() < 1e-6
+
+
+# ---------------------------------------------------------------------------
+# The 3-layer 1024^2 double gyre with WENO-Z PV flux and the FP64 mass
+# constraint, plain FP32 and mixed, one and 100 SSP-RK3 steps against the
+# FP64 oracle.  The oracle runs its stencil terms in C (oracle/qgrhs.c, the
+# same bits as its NumPy form) so that 100 steps at 1024^2 take about a
+# minute on the host; its result is shared by both precisions.
+# ---------------------------------------------------------------------------
+_BIG = {}
+
+
+def _weno_mass_1024():
+    if _BIG:
+        return _BIG
+    from somax_amd import testcases as tc
+
+    n, dt = 1024, 1200.0
+    args = dict(nx=n, ny=n, Lx=tc.BASIN, Ly=tc.BASIN, f0=tc.F0, beta=tc.BETA, H=(400.0, 1100.0, 2600.0),
+                g_prime=(9.81, 0.025, 0.0125), bottom_drag=tc.DRAG, wind_amplitude=tc.wind_amplitude(),
+                biharmonic_viscosity=tc.NU4_128 * (128.0 / n) ** 4)
+    setup = oqg.baroclinic_setup(advection="weno", mass_correction=True, **args)
+    psi0 = tc.random_streamfunction(n, n, tc.BASIN, tc.BASIN, 3, 31, 0.05, zero_mean=True)
+    q = tc.pv_from_streamfunction(psi0, setup.dx, setup.dy, setup.lambdas, setup.Cl2m, setup.Cm2l)
+    q = q.astype(np.float32)
+    _log("  oracle: 1 step")
+    ref1 = oqg.ssprk3_step(setup, q.astype(np.float64), dt, fast=True)
+    ref = ref1
+    for k in range(2, 101):
+        ref = oqg.ssprk3_step(setup, ref, dt, fast=True)
+        if k % 20 == 0:
+            _log(f"  oracle: {k} steps")
+    _BIG.update(args=args, setup=setup, q=q, dt=dt, ref1=ref1, ref100=ref,
+                psi1=oqg.invert_pv(setup, ref1, workers=-1), psi100=oqg.invert_pv(setup, ref, workers=-1))
+    return _BIG
+
+
+@pytest.mark.timeout(900)
+@pytest.mark.parametrize("precision", ["fp32", "mixed"])
+def test_weno_mass_1024_one_and_hundred_steps(gpu, precision):
+    import torch
+
+    from somax_amd import BaroclinicQG
+
+    _log(f"[weno+mass 1024] 1 and 100 steps precision={precision}")
+    big = _weno_mass_1024()
+    model = BaroclinicQG.create(n_layers=3, precision=precision, advection="weno", mass_correction=True,
+                                **big["args"])
+    q = torch.from_numpy(big["q"]).cuda()
+    model._step_inplace(q, big["dt"], 1)
+    torch.cuda.synchronize()
+    rq1 = rel(q.cpu().numpy(), big["ref1"])
+    rp1 = rel(model._invert_pv(q).cpu().numpy(), big["psi1"])
+    model._step_inplace(q, big["dt"], 99)
+    torch.cuda.synchronize()
+    rq100 = rel(q.cpu().numpy(), big["ref100"])
+    rp100 = rel(model._invert_pv(q).cpu().numpy(), big["psi100"])
+    _log(f"  1 step: q {rq1:.3e} psi {rp1:.3e}; 100 steps: q {rq100:.3e} psi {rp100:.3e}")
+    # the plain FP32 comparator's psi at 1026-point transforms sits at its
+    # sequential-accumulation floor (~1.2e-6); the mixed step meets 1e-6
+    assert rq1 < 1e-6 and rp1 < (2e-6 if precision == "fp32" else 1e-6)
+    assert rq100 < 1e-4 and rp100 < 1e-4
+
+
+# ---------------------------------------------------------------------------
+# The reference's own Helmholtz parameters: jnp.linalg.eigh of the
+# non-symmetric stretching matrix (somax/_src/core/transforms.py:192-194),
+# i.e. the eigenvalues of (A + A^T)/2.  The barotropic one is negative, so the
+# barotropic operator is indefinite; at 128^2 its conditioning (1.1e4) is that
+# of a Poisson mode, at 64^2 it is nearly resonant (tests/test_oracle_pins.py).
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("precision", ["fp32", "mixed"])
+def test_reference_eigenvalues_match_oracle(gpu, precision):
+    import torch
+
+    from somax_amd import BaroclinicQG
+    from somax_amd import testcases as tc
+
+    _log(f"[reference eigenvalues] n=128 inversion, 1 and 100 steps precision={precision}")
+    n, dt = 128, 1200.0
+    args = dict(nx=n, ny=n, Lx=tc.BASIN, Ly=tc.BASIN, f0=tc.F0, beta=tc.BETA, H=(400.0, 1100.0, 2600.0),
+                g_prime=(9.81, 0.025, 0.0125), bottom_drag=tc.DRAG, wind_amplitude=tc.wind_amplitude(),
+                biharmonic_viscosity=tc.NU4_128)
+    model = BaroclinicQG.create(n_layers=3, precision=precision, helmholtz_eigenvalues="reference", **args)
+    setup = oqg.baroclinic_setup(eigenvalues="reference", **args)
+    np.testing.assert_allclose(model.helmholtz_lambdas, setup.lambdas, rtol=1e-14)
+    assert setup.lambdas[0] < 0.0
+    q, qd = _state(model, seed=13)
+    inv = rel(model._invert_pv(qd).cpu().numpy(), oqg.invert_pv(setup, q.astype(np.float64)))
+    q1 = qd.clone()
+    model._step_inplace(q1, dt, 1)
+    ref1 = oqg.ssprk3_step(setup, q.astype(np.float64), dt)
+    q100 = qd.clone()
+    model._step_inplace(q100, dt, 100)
+    torch.cuda.synchronize()
+    ref100 = oqg.run(setup, q.astype(np.float64), dt, 100, fast=True)
+    r1, r100 = rel(q1.cpu().numpy(), ref1), rel(q100.cpu().numpy(), ref100)
+    p1 = rel(model._invert_pv(q1).cpu().numpy(), oqg.invert_pv(setup, ref1))
+    p100 = rel(model._invert_pv(q100).cpu().numpy(), oqg.invert_pv(setup, ref100))
+    _log(f"  inversion {inv:.3e}; 1 step q {r1:.3e} psi {p1:.3e}; 100 steps q {r100:.3e} psi {p100:.3e}")
+    assert inv < (3e-6 if precision == "fp32" else 3e-7)
+    assert r1 < 1e-6 and p1 < 1e-6 and r100 < 1e-4 and p100 < 1e-4

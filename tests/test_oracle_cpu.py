@@ -1,6 +1,10 @@
-"""The C++/OpenMP restatement of the QG step (oracle/cpu_step.cpp, the
-multi-core CPU baseline) against the NumPy oracle: FP64 to round-off, FP32
-to its precision, with viscosity, hyperviscosity, drag, beta and wind on."""
+"""The C restatements of the oracle against the NumPy oracle.
+
+* oracle/cpu_step.cpp (the multi-core CPU baseline): FP64 to round-off, FP32
+  to its precision, with viscosity, hyperviscosity, drag, beta and wind on.
+* oracle/qgrhs.c (the FP64 stencil terms behind ``rhs(..., fast=True)``, used
+  for the large-grid parity checks): bit for bit, for both advection schemes,
+  with and without the mass correction, on square and non-square grids."""
 
 from __future__ import annotations
 
@@ -12,7 +16,7 @@ from oracle import cpu, qg as oqg
 
 @pytest.fixture(scope="module")
 def built():
-    if not cpu.LIB.exists():
+    if not (cpu.LIB.exists() and cpu.RHS_LIB.exists()):
         cpu.build()
     return True
 
@@ -32,3 +36,46 @@ def test_cpu_step_matches_oracle(built, n, ny):
 def test_cpu_step_rejects_extensions(built):
     with pytest.raises(ValueError):
         cpu.CpuStep(oqg.baroclinic_setup(nx=16, ny=16, advection="weno"))
+
+
+@pytest.mark.parametrize("advection", ["arakawa", "weno"])
+@pytest.mark.parametrize("mass", [False, True])
+@pytest.mark.parametrize("n,ny", [(32, 32), (40, 24)])
+def test_fast_rhs_is_the_numpy_rhs_bit_for_bit(built, advection, mass, n, ny):
+    s = oqg.baroclinic_setup(nx=n, ny=ny, Lx=4e6, Ly=4e6 * ny / n, lateral_viscosity=50.0, bottom_drag=1e-7,
+                             wind_amplitude=1e-11, biharmonic_viscosity=1e11, advection=advection,
+                             mass_correction=mass)
+    q = oqg.apply_bc(np.random.default_rng(n + 7).standard_normal((3, ny + 2, n + 2)) * 1e-6)
+    ref, psi_ref = oqg.rhs(s, q, return_psi=True)
+    got, psi = oqg.rhs(s, q, return_psi=True, fast=True)
+    assert np.array_equal(psi, psi_ref)
+    assert np.array_equal(got, ref)
+    # a non-trivial field: every term contributes
+    assert np.abs(got).max() > 0 and np.abs(got[:, 1:-1, 1:-1]).min() > 0
+
+
+def test_fast_step_is_the_numpy_step(built):
+    s = oqg.baroclinic_setup(nx=24, ny=24, Lx=4e6, Ly=4e6, bottom_drag=1e-7, wind_amplitude=1e-11,
+                             biharmonic_viscosity=1e11, advection="weno", mass_correction=True)
+    q = np.random.default_rng(3).standard_normal((3, 26, 26)) * 1e-6
+    assert np.array_equal(oqg.run(s, q, 1200.0, 2, fast=True), oqg.run(s, q, 1200.0, 2))
+
+
+@pytest.mark.parametrize("advection,mass", [("arakawa", False), ("weno", True)])
+def test_fft_cpu_step_matches_oracle(built, advection, mass):
+    """The FP32 FFT-DST CPU step (the CPU baseline) against the FP64 oracle:
+    two steps within FP32 precision, on 1 and on several threads (equal)."""
+    from somax_amd import testcases as tc
+
+    n = 48
+    s = oqg.baroclinic_setup(nx=n, ny=n, Lx=tc.BASIN, Ly=tc.BASIN, f0=tc.F0, beta=tc.BETA, bottom_drag=tc.DRAG,
+                             wind_amplitude=tc.wind_amplitude(), biharmonic_viscosity=tc.NU4_128 * (128 / n) ** 4,
+                             advection=advection, mass_correction=mass)
+    psi = tc.random_streamfunction(n, n, tc.BASIN, tc.BASIN, 3, 5, zero_mean=mass)
+    q = tc.pv_from_streamfunction(psi, s.dx, s.dy, s.lambdas, s.Cl2m, s.Cm2l).astype(np.float32)
+    ref = oqg.run(s, q.astype(np.float64), 1200.0, 2)
+    got1 = cpu.FftCpuStep(s, threads=1).step(q, 1200.0, 2)
+    got4 = cpu.FftCpuStep(s, threads=4).step(q, 1200.0, 2)
+    assert got1.dtype == np.float32
+    assert np.array_equal(got1, got4)
+    assert float(np.linalg.norm(got1 - ref) / np.linalg.norm(ref)) < 1e-6

@@ -138,6 +138,29 @@ def test_reference_eigenvalues_are_symmetrised():
     assert not np.allclose(ref, np.sort(np.linalg.eigvals(A).real))
 
 
+def test_reference_eigenvalues_make_the_barotropic_solve_indefinite():
+    """With the reference's negative barotropic parameter (f0^2 * -0.00644) the
+    mode's operator lap_h - lambda_0 is indefinite: Lam_x[i] + Lam_y[j] +
+    |lambda_0| changes sign over the spectrum.  How close a discrete eigenvalue
+    comes to resonance depends on the grid: on the 5120 km basin at 64^2 the
+    smallest |denominator| is 2.8e-15 (condition 4.2e5, 40x a Poisson mode's),
+    at 128^2 it is 4.6e-13 (condition 1.1e4, twice a Poisson mode's 5.3e3)."""
+    A = modes.coupling_matrix([400.0, 1100.0, 2600.0], [9.81, 0.025, 0.0125])
+    f0, L = 9.375e-5, 5120e3
+    lam_ref = f0 * f0 * modes.reference_eigenvalues(A)[0]
+    lam_poisson = f0 * f0 * modes.vertical_modes(A, f0)[0][0]
+    cond = {}
+    for n in (64, 128):
+        lam_n = dst.dst1_eigenvalues(n + 2, L / n)
+        den = lam_n[:, None] + lam_n[None, :]
+        assert (den - lam_ref).max() > 0 > (den - lam_ref).min()  # indefinite
+        assert (den - lam_poisson).max() < 0  # the consistent mode stays definite
+        c = lambda d: np.abs(d).max() / np.abs(d).min()  # noqa: E731
+        cond[n] = (c(den - lam_ref), c(den - lam_poisson))
+    assert cond[64][0] > 1e5 and cond[64][0] > 30 * cond[64][1]
+    assert cond[128][0] < 2e4 and cond[128][0] < 2.5 * cond[128][1]
+
+
 def test_homsol_notebook_known_answers():
     """notebooks/dev/QG_Multilayer/mqg_demo_retangular.ipynb cell 18 (x64):
     rectangle of 256 nodes, L = 4000 km, dx = L/N, f0 = 9.4e-5, rigid-lid A.
