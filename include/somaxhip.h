@@ -178,6 +178,15 @@ enum {
   SOMAXHIP_BC_CHANNEL = 2   /* extension: periodic in x, the wall rules in y */
 };
 
+/* PV flux of the momentum equations. */
+enum {
+  SOMAXHIP_VFLUX_AVERAGE = 0,          /* reference: q averaged to the face times the four-point average of
+                                          the other mass flux (multilayer.py:172-187) */
+  SOMAXHIP_VFLUX_ENERGY_ENSTROPHY = 1  /* extension: Arakawa & Lamb (1981) triads (the "EEN" form); does no
+                                          work, and keeps the potential enstrophy of a non-divergent mass
+                                          flux */
+};
+
 typedef struct somaxhip_swm_plan_s* somaxhip_swm_plan;
 
 typedef struct {
@@ -189,6 +198,7 @@ typedef struct {
   const double* f_row;      /* ny+2 values f0 + beta*(y_j - Ly/2), y_j = j*dy (multilayer.py:345-349) */
   const double* wind_row;   /* ny+2 values of the x wind-stress pattern (multilayer.py:352-356) */
   int bc;                   /* SOMAXHIP_BC_* */
+  int vorticity_flux;       /* SOMAXHIP_VFLUX_* */
 } somaxhip_swm_desc;
 
 /* MultilayerSW2DParams (multilayer.py:43-55), passed per call. */

@@ -59,6 +59,7 @@ class SWMSetup:
     kappa: float = 0.0
     tau0: float = 0.0
     bc: str = "periodic"
+    vorticity_flux: str = "average"  # or "energy_enstrophy" (extension)
 
     @property
     def Nx(self):
@@ -83,10 +84,12 @@ class SWMSetup:
 
 def setup(nx=64, ny=64, Lx=4e6, Ly=4e6, f0=9.375e-5, beta=1.754e-11, H=(400.0, 1100.0, 2600.0),
           g_prime=(9.81, 0.025, 0.0125), lateral_viscosity=0.0, bottom_drag=0.0, wind_amplitude=0.0,
-          wind_profile="doublegyre", bc="periodic") -> SWMSetup:
+          wind_profile="doublegyre", bc="periodic", vorticity_flux="average") -> SWMSetup:
     """``MultilayerShallowWater2D.create`` (multilayer.py:259-377) in FP64:
     ``f = f0 + beta (y - Ly/2)`` (:345-349), ``tau_x = -cos(2 pi y / Ly)``
-    (doublegyre) or ``-cos(pi y / Ly)`` (single) (:352-356)."""
+    (doublegyre) or ``-cos(pi y / Ly)`` (single) (:352-356).
+    ``vorticity_flux``: "average" (the reference's, multilayer.py:172-187) or
+    "energy_enstrophy" (extension, ``vorticity_flux_terms``)."""
     H = np.asarray(H, dtype=np.float64)
     gp = np.asarray(g_prime, dtype=np.float64)
     if H.size != gp.size:
@@ -101,8 +104,11 @@ def setup(nx=64, ny=64, Lx=4e6, Ly=4e6, f0=9.375e-5, beta=1.754e-11, H=(400.0, 1
         wind = -np.cos(2.0 * np.pi * y / Ly)
     if bc not in ("periodic", "wall", "channel"):
         raise ValueError(f"unknown bc {bc!r}")
+    if vorticity_flux not in ("average", "energy_enstrophy"):
+        raise ValueError(f"unknown vorticity_flux {vorticity_flux!r}")
     return SWMSetup(nx=nx, ny=ny, Lx=Lx, Ly=Ly, nl=H.size, g_prime=gp, H=H, f_row=f_row, wind_row=wind,
-                    nu=lateral_viscosity, kappa=bottom_drag, tau0=wind_amplitude, bc=bc)
+                    nu=lateral_viscosity, kappa=bottom_drag, tau0=wind_amplitude, bc=bc,
+                    vorticity_flux=vorticity_flux)
 
 
 # ---------------------------------------------------------------------------
@@ -215,6 +221,66 @@ def _interior(a):
     return a[..., 1:-1, 1:-1]
 
 
+def extend_pv(s: SWMSetup, q):
+    """PV on the X points of the whole array.  ``potential_vorticity`` fills
+    rows and columns 0..N-2; the last X column (row) is the periodic image of
+    column (row) 1 where x (y) is periodic, and 0 beside a wall."""
+    q = np.array(q, dtype=np.float64)
+    if s.bc in ("periodic", "channel"):
+        q[..., :, -1] = q[..., :, 1]
+    if s.bc == "periodic":
+        q[..., -1, :] = q[..., 1, :]
+    return q
+
+
+def vorticity_flux_terms(s: SWMSetup, q, uh, vh):
+    """The PV-flux part of the momentum tendency on interior U and V points:
+    ``(+ q (vh) at U, - q (uh) at V)``.
+
+    * ``"average"`` (the reference, multilayer.py:172-187): the two X values
+      beside the face times the four-point average of the other flux.
+    * ``"energy_enstrophy"`` (extension): the triad form of Arakawa & Lamb
+      (1981) as written for a C grid with PV at the corners (the "EEN" form):
+      each T cell carries four triads, the sum of its corner values except the
+      one opposite a corner, /12; a U face takes each of its four neighbouring
+      V fluxes with the triad of the cell that holds both, at the corner they
+      share, and a V face the U fluxes likewise, with the opposite sign.  Both
+      equations use the same triad for each (U, V) pair, so the term does no
+      work (sum U F_u + V F_v = 0); for a non-divergent mass flux it also
+      leaves the potential enstrophy sum h_X q^2 / 2 unchanged.  With uniform
+      q it equals the average form.
+
+    T cell [j, i] has the corners NE = X[j, i], NW = X[j, i-1], SE = X[j-1, i],
+    SW = X[j-1, i-1]; U[j, i] is its east face and V[j, i] its north face."""
+    if s.vorticity_flux == "average":
+        qU = 0.5 * (q[..., 1:-1, 1:-1] + q[..., :-2, 1:-1])
+        vhU = 0.25 * (vh[..., 1:-1, 1:-1] + vh[..., 1:-1, 2:] + vh[..., :-2, 1:-1] + vh[..., :-2, 2:])
+        qV = 0.5 * (q[..., 1:-1, 1:-1] + q[..., 1:-1, :-2])
+        uhV = 0.25 * (uh[..., 1:-1, 1:-1] + uh[..., 1:-1, :-2] + uh[..., 2:, 1:-1] + uh[..., 2:, :-2])
+        return qU * vhU, -qV * uhV
+    q = extend_pv(s, q)
+    # triads of every T cell [j, i], j, i >= 1 (array index [j-1, i-1])
+    ne, nw, se, sw = q[..., 1:, 1:], q[..., 1:, :-1], q[..., :-1, 1:], q[..., :-1, :-1]
+    t_ne = (nw + ne + se) / 12.0
+    t_nw = (sw + nw + ne) / 12.0
+    t_se = (ne + se + sw) / 12.0
+    t_sw = (se + sw + nw) / 12.0
+
+    def T(t, dj, di):  # triad of cell [j+dj, i+di] for interior [j, i]
+        ny, nx = t.shape[-2:]
+        return t[..., dj : ny - 1 + dj, di : nx - 1 + di]
+
+    def F(f, dj, di):  # flux at [j+dj, i+di] for interior [j, i]
+        ny, nx = f.shape[-2:]
+        return f[..., 1 + dj : ny - 1 + dj, 1 + di : nx - 1 + di]
+
+    fu = (T(t_ne, 0, 0) * F(vh, 0, 0) + T(t_nw, 0, 1) * F(vh, 0, 1) + T(t_se, 0, 0) * F(vh, -1, 0)
+          + T(t_sw, 0, 1) * F(vh, -1, 1))
+    fv = -(T(t_sw, 1, 0) * F(uh, 1, -1) + T(t_se, 1, 0) * F(uh, 1, 0) + T(t_nw, 0, 0) * F(uh, 0, -1)
+           + T(t_ne, 0, 0) * F(uh, 0, 0))
+    return fu, fv
+
+
 def rhs(s: SWMSetup, h, u, v):
     """``vector_field`` (multilayer.py:150-201) on the state as given."""
     h = np.asarray(h, dtype=np.float64)
@@ -237,14 +303,11 @@ def rhs(s: SWMSetup, h, u, v):
     # 4. Bernoulli potential: KE + cumulative g' h (multilayer.py:178-183)
     P = kinetic_energy(u, v) + np.cumsum(s.g_prime[:, None, None] * h, axis=0)
     # 5. momentum on interior U / V points
-    qU = 0.5 * (q[..., 1:-1, 1:-1] + q[..., :-2, 1:-1])
-    vhU = 0.25 * (vh[..., 1:-1, 1:-1] + vh[..., 1:-1, 2:] + vh[..., :-2, 1:-1] + vh[..., :-2, 2:])
-    qV = 0.5 * (q[..., 1:-1, 1:-1] + q[..., 1:-1, :-2])
-    uhV = 0.25 * (uh[..., 1:-1, 1:-1] + uh[..., 1:-1, :-2] + uh[..., 2:, 1:-1] + uh[..., 2:, :-2])
+    fu, fv = vorticity_flux_terms(s, q, uh, vh)
     du = np.zeros_like(u)
     dv = np.zeros_like(v)
-    du[..., 1:-1, 1:-1] = qU * vhU - (P[..., 1:-1, 2:] - P[..., 1:-1, 1:-1]) / dx
-    dv[..., 1:-1, 1:-1] = -qV * uhV - (P[..., 2:, 1:-1] - P[..., 1:-1, 1:-1]) / dy
+    du[..., 1:-1, 1:-1] = fu - (P[..., 1:-1, 2:] - P[..., 1:-1, 1:-1]) / dx
+    dv[..., 1:-1, 1:-1] = fv - (P[..., 2:, 1:-1] - P[..., 1:-1, 1:-1]) / dy
     # 6. wind on the top layer, full array (multilayer.py:190-191)
     du[0] += s.tau0 * s.wind_row[:, None] / s.H[0]
     # 7. diffusion (all layers)

@@ -80,6 +80,7 @@ class SWMDesc(ctypes.Structure):
         ("f_row", c_double_p),
         ("wind_row", c_double_p),
         ("bc", ctypes.c_int),
+        ("vorticity_flux", ctypes.c_int),
     ]
 
 

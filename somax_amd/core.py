@@ -213,6 +213,22 @@ class Solution:
     stats: dict = field(default_factory=dict)
 
 
+def save_times(t0: float, t1: float, dt: float, saveat) -> list[float]:
+    """The save times of ``integrate`` (``t1`` alone when ``saveat`` is None),
+    checked as diffrax checks them: a positive step, and save times that do
+    not decrease and lie at or after ``t0``."""
+    ts = [float(t1)] if saveat is None else [float(t) for t in saveat]
+    if not (float(dt) > 0.0):
+        raise ValueError(f"dt must be positive, got {dt!r}")
+    if not ts:
+        raise ValueError("saveat holds no save times")
+    if any(t < float(t0) for t in ts):
+        raise ValueError(f"save times must lie at or after t0 = {t0!r}, got {ts}")
+    if any(b < a for a, b in zip(ts, ts[1:])):
+        raise ValueError(f"save times must not decrease, got {ts}")
+    return ts
+
+
 def solver_name(solver) -> str:
     """``"tsit5"`` or ``"ssprk3"`` from a solver name or a solver object
     (``diffrax.Tsit5()`` -> ``"tsit5"``): the schemes the HIP path runs."""

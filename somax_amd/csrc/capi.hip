@@ -1004,6 +1004,10 @@ int somaxhip_qg_step(somaxhip_qg_plan p, float* q, float* psi_out, int nsteps, d
   return SOMAXHIP_OK;
 }
 
+}  // extern "C"
+
+extern "C" {
+
 int somaxhip_qg_project(somaxhip_qg_plan p, double f0, const double* H, const double* M, const float* h,
                         const float* u, const float* v, int h_deviation, float* ho, float* uo, float* vo,
                         float* psi_out, void* ws, somaxhip_stream_t stream) {
@@ -1308,6 +1312,13 @@ namespace {
 // turns into parallel graph branches), else all on `st`.
 int ens_queue(somaxhip_ens e, float* q, int nsteps, double dt, void* ws, hipStream_t st) {
   const int C = (int)e->cls.size();
+  // every per-class table is indexed by the class below
+  const size_t Cs = (size_t)C;
+  if (e->cparams.size() != Cs || e->cbiharm.size() != Cs || e->cstate.size() != Cs || e->cws.size() != Cs ||
+      e->side.size() != Cs || e->ev_join.size() != Cs) {
+    set_error("somaxhip_ens_step: inconsistent ensemble tables");
+    return SOMAXHIP_ERR_ARG;
+  }
   const bool conc = (e->flags & SOMAXHIP_ENS_CONCURRENT) && C > 1;
   if (conc) {
     SOMAX_HIP_CHECK(hipEventRecord(e->ev_fork, st));

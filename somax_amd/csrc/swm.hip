@@ -142,11 +142,38 @@ __device__ inline void swm_tile(int tiles_x, int& ty, int& tx) {
 
 }  // namespace
 
+// PV (zeta + f_X) / h_X at X point (gy, gx) from the global arrays of one
+// layer, each value read through the boundary condition when `apply` (the
+// same mapping as the tile loads).  Used only for the X points past the last
+// row / column that the energy-enstrophy flux needs (their periodic images).
+__device__ float pv_global(const SwmArgs& a, const float* h, const float* u, const float* v, float hk, int gy,
+                           int gx) {
+  const int Ny = a.Ny, Nx = a.Nx, bc = a.bc, ap = a.apply_bc;
+  const float zeta = (bc_value(v, F_V, gy, gx + 1, Ny, Nx, bc, ap) - bc_value(v, F_V, gy, gx, Ny, Nx, bc, ap)) * a.idx -
+                     (bc_value(u, F_U, gy + 1, gx, Ny, Nx, bc, ap) - bc_value(u, F_U, gy, gx, Ny, Nx, bc, ap)) * a.idy;
+  const float hx = hk + 0.25f * (bc_value(h, F_H, gy, gx, Ny, Nx, bc, ap) + bc_value(h, F_H, gy, gx + 1, Ny, Nx, bc, ap) +
+                                 bc_value(h, F_H, gy + 1, gx, Ny, Nx, bc, ap) +
+                                 bc_value(h, F_H, gy + 1, gx + 1, Ny, Nx, bc, ap));
+  return (zeta + 0.5f * (a.f_row[gy] + a.f_row[gy + 1])) * __builtin_amdgcn_rcpf(hx);
+}
+
+// EEN: the energy-enstrophy-conserving (triad) vorticity flux instead of the
+// reference's average form.  Its stencil reaches the X points one row and
+// one column further north / east, so the tile carries a 2-cell halo on those
+// sides and the tile's PV is formed once into LDS.  X points past the last
+// row / column take the value of their periodic image (row / column 1) in a
+// periodic direction and 0 beside a wall.
+template <bool EEN>
 __global__ __launch_bounds__(SNT) void swm_stage_kernel(SwmArgs a) {
-  __shared__ float sh[SPH * SPW];
-  __shared__ float su[SPH * SPW];
-  __shared__ float sv[SPH * SPW];
-  __shared__ float sfx[STY + 1];  // Coriolis at the corner rows y0-1 .. y0+STY-1
+  constexpr int PW = EEN ? STX + 3 : SPW;  // LDS pitch
+  constexpr int PH = EEN ? STY + 3 : SPH;  // LDS rows
+  constexpr int QW = STX + 2;              // X points of the tile and its 1-point halo
+  constexpr int QH = STY + 2;
+  __shared__ float sh[PH * PW];
+  __shared__ float su[PH * PW];
+  __shared__ float sv[PH * PW];
+  __shared__ float sfx[STY + 2];  // Coriolis at the corner rows y0-1 .. y0+STY
+  __shared__ float sq[EEN ? QH * QW : 1];
 
   const int tid = threadIdx.x;
   const int tx = tid & 63;
@@ -157,7 +184,7 @@ __global__ __launch_bounds__(SNT) void swm_stage_kernel(SwmArgs a) {
   const int x0 = txi * STX;
   const int y0 = tyi * STY;
 
-  if (tid <= STY) {
+  if (tid <= STY + 1) {
     const int gy = y0 - 1 + tid;
     sfx[tid] = (gy >= 0 && gy + 1 < Ny) ? 0.5f * (a.f_row[gy] + a.f_row[gy + 1]) : 0.0f;
   }
@@ -194,7 +221,7 @@ __global__ __launch_bounds__(SNT) void swm_stage_kernel(SwmArgs a) {
     // Tile loads: a fixed count per thread, all issued before the first LDS
     // store so that one memory latency covers them (in-plane offsets fit in
     // 32 bits: a plane is below 2^31 cells).
-    constexpr int NR = (SPH + SWV - 1) / SWV;
+    constexpr int NR = (PH + SWV - 1) / SWV;
     float th[NR][2], tu[NR][2], tv[NR][2];
 #pragma unroll
     for (int j = 0; j < NR; ++j) {
@@ -202,7 +229,7 @@ __global__ __launch_bounds__(SNT) void swm_stage_kernel(SwmArgs a) {
       const BcAxis rm = bc_rows(y0 - 1 + r, Ny, a.bc, a.apply_bc);
 #pragma unroll
       for (int half = 0; half < 2; ++half) {
-        const bool in = r < SPH && tx + 64 * half < SPW;
+        const bool in = r < PH && tx + 64 * half < PW;
         const BcAxis& cm = half ? cm1 : cm0;
         th[j][half] = (in && rm.h >= 0 && cm.h >= 0) ? hs[rm.h * Nx + cm.h] : 0.0f;
         tu[j][half] = (in && rm.u >= 0 && cm.u >= 0) ? us[rm.u * Nx + cm.u] : 0.0f;
@@ -215,10 +242,10 @@ __global__ __launch_bounds__(SNT) void swm_stage_kernel(SwmArgs a) {
 #pragma unroll
       for (int half = 0; half < 2; ++half) {
         const int c = tx + 64 * half;
-        if (r < SPH && c < SPW) {
-          sh[r * SPW + c] = th[j][half];
-          su[r * SPW + c] = tu[j][half];
-          sv[r * SPW + c] = tv[j][half];
+        if (r < PH && c < PW) {
+          sh[r * PW + c] = th[j][half];
+          su[r * PW + c] = tu[j][half];
+          sv[r * PW + c] = tv[j][half];
         }
       }
     }
@@ -226,15 +253,46 @@ __global__ __launch_bounds__(SNT) void swm_stage_kernel(SwmArgs a) {
     const float gp = a.gprime[k];
     const float hk = a.hoff[k];
     const bool top = k == 0, bottom = k == a.nl - 1;
+    if (EEN) {
+      // PV of the tile's X points (rows y0-1 .. y0+STY, columns x0-1 .. x0+STX);
+      // X point (y0-1+xr, x0-1+xc) reads tile rows xr, xr+1 and columns xc, xc+1
+      for (int i = tid; i < QH * QW; i += SNT) {
+        const int xr = i / QW, xc = i - xr * QW;
+        const int gy = y0 - 1 + xr, gx = x0 - 1 + xc;
+        float qv = 0.0f;
+        if (gy >= 0 && gx >= 0 && gy <= Ny - 2 && gx <= Nx - 2) {
+          const int c = xr * PW + xc;
+          const float zeta = (sv[c + 1] - sv[c]) * a.idx - (su[c + PW] - su[c]) * a.idy;
+          const float hx = hk + 0.25f * (sh[c] + sh[c + 1] + sh[c + PW] + sh[c + PW + 1]);
+          qv = (zeta + sfx[xr]) * __builtin_amdgcn_rcpf(hx);
+        } else if (gy >= 0 && gx >= 0 && gy <= Ny - 1 && gx <= Nx - 1) {
+          // past the last row / column: periodic image or 0 beside a wall
+          const bool xper = a.bc != SOMAXHIP_BC_WALL, yper = a.bc == SOMAXHIP_BC_PERIODIC;
+          int sy = gy, sx = gx;
+          bool zero = false;
+          if (gx == Nx - 1) {
+            zero |= !xper;
+            sx = 1;
+          }
+          if (gy == Ny - 1) {
+            zero |= !yper;
+            sy = 1;
+          }
+          if (!zero) qv = pv_global(a, hs, us, vs, hk, sy, sx);
+        }
+        sq[i] = qv;
+      }
+      __syncthreads();
+    }
 #pragma unroll
     for (int r = 0; r < SRW; ++r) {
       const int ly = ty + SWV * r;
       const int gy = y0 + ly, gx = x0 + tx;
       if (gy >= Ny || gx >= Nx) continue;
-      const int c = (ly + 1) * SPW + tx + 1;  // LDS index of (gy, gx)
-#define H(dj, di) sh[c + (dj) * SPW + (di)]
-#define U(dj, di) su[c + (dj) * SPW + (di)]
-#define V(dj, di) sv[c + (dj) * SPW + (di)]
+      const int c = (ly + 1) * PW + tx + 1;  // LDS index of (gy, gx)
+#define H(dj, di) sh[c + (dj) * PW + (di)]
+#define U(dj, di) su[c + (dj) * PW + (di)]
+#define V(dj, di) sv[c + (dj) * PW + (di)]
       const bool interior = gy >= 1 && gy <= Ny - 2 && gx >= 1 && gx <= Nx - 2;
       float dh = 0.0f, du = 0.0f, dv = 0.0f;
       if (interior) {
@@ -246,26 +304,46 @@ __global__ __launch_bounds__(SNT) void swm_stage_kernel(SwmArgs a) {
         const float gn = V(0, 0) * (hk + (V(0, 0) > 0.0f ? H(0, 0) : H(1, 0)));
         const float gs = V(-1, 0) * (hk + (V(-1, 0) > 0.0f ? H(-1, 0) : H(0, 0)));
         dh = -((fe - fw) * a.idx + (gn - gs) * a.idy);
-        // PV at X points (dj, di) relative to (gy, gx)
-        auto q = [&](int dj, int di) {
-          const float zeta = (V(dj, di + 1) - V(dj, di)) * a.idx - (U(dj + 1, di) - U(dj, di)) * a.idy;
-          const float hx = hk + 0.25f * (H(dj, di) + H(dj, di + 1) + H(dj + 1, di) + H(dj + 1, di + 1));
-          return (zeta + sfx[ly + 1 + dj]) * __builtin_amdgcn_rcpf(hx);
-        };
         auto vh = [&](int dj, int di) { return (hk + 0.5f * (H(dj, di) + H(dj + 1, di))) * V(dj, di); };
         auto uh = [&](int dj, int di) { return (hk + 0.5f * (H(dj, di) + H(dj, di + 1))) * U(dj, di); };
         auto ke = [&](int dj, int di) {
           return 0.25f * (U(dj, di) * U(dj, di) + U(dj, di - 1) * U(dj, di - 1) + V(dj, di) * V(dj, di) +
                           V(dj - 1, di) * V(dj - 1, di));
         };
-        const float q00 = q(0, 0);
-        const float qU = 0.5f * (q00 + q(-1, 0));
-        const float vhU = 0.25f * (vh(0, 0) + vh(0, 1) + vh(-1, 0) + vh(-1, 1));
-        const float qV = 0.5f * (q00 + q(0, -1));
-        const float uhV = 0.25f * (uh(0, 0) + uh(0, -1) + uh(1, 0) + uh(1, -1));
+        float fu, fv;
+        if (EEN) {
+          // triads (three corners of a cell, /12) at the corner each U / V pair shares
+          auto Q = [&](int dj, int di) { return sq[(ly + 1 + dj) * QW + tx + 1 + di]; };
+          const float q00 = Q(0, 0), q0m = Q(0, -1), qm0 = Q(-1, 0), qmm = Q(-1, -1);
+          const float q01 = Q(0, 1), qm1 = Q(-1, 1), q10 = Q(1, 0), q1m = Q(1, -1);
+          const float t_ne = q0m + q00 + qm0;  // cell (gy, gx): NE triad
+          const float t_se = q00 + qm0 + qmm;  //                SE triad
+          const float t_nw = qmm + q0m + q00;  //                NW triad
+          const float t_nw_e = qm0 + q00 + q01;  // cell (gy, gx+1): NW triad
+          const float t_sw_e = qm1 + qm0 + q00;  //                  SW triad
+          const float t_sw_n = q00 + q0m + q1m;  // cell (gy+1, gx): SW triad
+          const float t_se_n = q10 + q00 + q0m;  //                  SE triad
+          constexpr float r12 = 1.0f / 12.0f;
+          fu = r12 * (t_ne * vh(0, 0) + t_nw_e * vh(0, 1) + t_se * vh(-1, 0) + t_sw_e * vh(-1, 1));
+          fv = -r12 * (t_sw_n * uh(1, -1) + t_se_n * uh(1, 0) + t_nw * uh(0, -1) + t_ne * uh(0, 0));
+        } else {
+          // PV at X points (dj, di) relative to (gy, gx)
+          auto q = [&](int dj, int di) {
+            const float zeta = (V(dj, di + 1) - V(dj, di)) * a.idx - (U(dj + 1, di) - U(dj, di)) * a.idy;
+            const float hx = hk + 0.25f * (H(dj, di) + H(dj, di + 1) + H(dj + 1, di) + H(dj + 1, di + 1));
+            return (zeta + sfx[ly + 1 + dj]) * __builtin_amdgcn_rcpf(hx);
+          };
+          const float q00 = q(0, 0);
+          const float qU = 0.5f * (q00 + q(-1, 0));
+          const float vhU = 0.25f * (vh(0, 0) + vh(0, 1) + vh(-1, 0) + vh(-1, 1));
+          const float qV = 0.5f * (q00 + q(0, -1));
+          const float uhV = 0.25f * (uh(0, 0) + uh(0, -1) + uh(1, 0) + uh(1, -1));
+          fu = qU * vhU;
+          fv = -qV * uhV;
+        }
         const float ke0 = ke(0, 0);
-        du = qU * vhU - ((ke(0, 1) - ke0) + dpx[r]) * a.idx;
-        dv = -qV * uhV - ((ke(1, 0) - ke0) + dpy[r]) * a.idy;
+        du = fu - ((ke(0, 1) - ke0) + dpx[r]) * a.idx;
+        dv = fv - ((ke(1, 0) - ke0) + dpy[r]) * a.idy;
         if (a.nu != 0.0f) {
           du += a.nu * ((U(0, 1) - 2.0f * U(0, 0) + U(0, -1)) * a.idx2 + (U(1, 0) - 2.0f * U(0, 0) + U(-1, 0)) * a.idy2);
           dv += a.nu * ((V(0, 1) - 2.0f * V(0, 0) + V(0, -1)) * a.idx2 + (V(1, 0) - 2.0f * V(0, 0) + V(-1, 0)) * a.idy2);
@@ -375,6 +453,7 @@ using namespace somax;
 
 struct somaxhip_swm_plan_s {
   int nx = 0, ny = 0, nl = 0, Nx = 0, Ny = 0, bc = 0;
+  int vflux = SOMAXHIP_VFLUX_AVERAGE;
   double dx = 0.0, dy = 0.0;
   std::vector<float> gprime;
   std::vector<float> Hk;
@@ -444,7 +523,10 @@ void swm_stage(somaxhip_swm_plan p, const float* hs, const float* us, const floa
     (void)hipEventCreate(&e1);
     (void)hipEventRecord(e0, st);
   }
-  hipLaunchKernelGGL(swm_stage_kernel, grid, dim3(SNT), 0, st, a);
+  if (p->vflux == SOMAXHIP_VFLUX_ENERGY_ENSTROPHY)
+    hipLaunchKernelGGL(swm_stage_kernel<true>, grid, dim3(SNT), 0, st, a);
+  else
+    hipLaunchKernelGGL(swm_stage_kernel<false>, grid, dim3(SNT), 0, st, a);
   if (g_swm_prof) {
     (void)hipEventRecord(e1, st);
     SwmProf& pr = *g_swm_prof;
@@ -495,6 +577,10 @@ int somaxhip_swm_plan_create(const somaxhip_swm_desc* d, somaxhip_swm_plan* out)
     set_error("somaxhip_swm_plan_create: H[0] must be positive");
     return SOMAXHIP_ERR_ARG;
   }
+  if (d->vorticity_flux != SOMAXHIP_VFLUX_AVERAGE && d->vorticity_flux != SOMAXHIP_VFLUX_ENERGY_ENSTROPHY) {
+    set_error("somaxhip_swm_plan_create: unknown vorticity flux");
+    return SOMAXHIP_ERR_ARG;
+  }
   somaxhip_swm_plan p = new (std::nothrow) somaxhip_swm_plan_s();
   if (!p) {
     set_error("somaxhip_swm_plan_create: out of host memory");
@@ -508,6 +594,7 @@ int somaxhip_swm_plan_create(const somaxhip_swm_desc* d, somaxhip_swm_plan* out)
   p->dx = d->dx;
   p->dy = d->dy;
   p->bc = d->bc;
+  p->vflux = d->vorticity_flux;
   p->H0 = d->H[0];
   for (int k = 0; k < d->nl; ++k) {
     p->gprime.push_back((float)d->g_prime[k]);

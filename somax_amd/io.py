@@ -29,6 +29,7 @@ import gzip
 import importlib
 import itertools
 import json
+import os
 import shutil
 from pathlib import Path
 
@@ -228,18 +229,36 @@ def save_dataset(ds: Dataset, path, *, mode: str = "w") -> None:
     """Writes ``ds`` as a zarr v3 directory store (xarray.py:235-249);
     ``mode`` ``"w"`` overwrites, ``"w-"`` fails if the store exists."""
     path = Path(path)
+    if mode not in ("w", "w-"):
+        raise ValueError(f"mode must be 'w' or 'w-', got {mode!r}")
+    if path.exists() and mode == "w-":
+        raise FileExistsError(f"{path} exists")
+    # The store is written whole into a sibling directory first and swapped in
+    # afterwards, so an error or interrupt while writing leaves any existing
+    # store at `path` as it was.
+    path.parent.mkdir(parents=True, exist_ok=True)
+    tmp = path.with_name(f".{path.name}.tmp-{os.getpid()}")
+    if tmp.exists():
+        shutil.rmtree(tmp)
+    tmp.mkdir()
+    try:
+        (tmp / "zarr.json").write_text(json.dumps(
+            {"zarr_format": 3, "node_type": "group",
+             "attributes": {k: _json_attr(v) for k, v in ds.attrs.items()}}, indent=2))
+        for name, var in ds.variables.items():
+            _write_array(tmp, name, var)
+    except BaseException:
+        shutil.rmtree(tmp, ignore_errors=True)
+        raise
     if path.exists():
-        if mode == "w-":
-            raise FileExistsError(f"{path} exists")
-        if mode != "w":
-            raise ValueError(f"mode must be 'w' or 'w-', got {mode!r}")
-        shutil.rmtree(path)
-    path.mkdir(parents=True)
-    (path / "zarr.json").write_text(json.dumps(
-        {"zarr_format": 3, "node_type": "group", "attributes": {k: _json_attr(v) for k, v in ds.attrs.items()}},
-        indent=2))
-    for name, var in ds.variables.items():
-        _write_array(path, name, var)
+        old = path.with_name(f".{path.name}.old-{os.getpid()}")
+        if old.exists():
+            shutil.rmtree(old)
+        os.replace(path, old)
+        os.replace(tmp, path)
+        shutil.rmtree(old)
+    else:
+        os.replace(tmp, path)
 
 
 def _decode_chunk(raw: bytes, codecs, dtype: np.dtype, chunk_shape) -> np.ndarray:
@@ -314,7 +333,9 @@ def load_dataset(path) -> Dataset:
 
 def append_to_dataset(ds: Dataset, path, *, append_dim: str = "time") -> None:
     """Extends a store along ``append_dim`` (xarray.py:269-286): every
-    variable with that dimension is concatenated; the others must match."""
+    variable with that dimension is concatenated; the others must match.
+    The merged store replaces the old one only once it is written whole
+    (``save_dataset``), so a failed append leaves the old store intact."""
     old = load_dataset(path)
     merged_data, merged_coords = {}, {}
     for target, src_old, src_new in ((merged_data, old.data_vars, ds.data_vars),

@@ -15,7 +15,7 @@ Workloads:
 * ``ensemble``: the 256-member uneven ensemble, mixed inversion, one
   eager step (every launch visible to the profiler).
 * ``swm512``: the 2-layer shallow-water baroclinic jet, 512 x 512 channel,
-  SSP-RK3 at CFL 0.5.
+  energy-enstrophy-conserving PV flux, SSP-RK3 at CFL 0.5.
 
 Each step is preceded by one untimed warm-up step.  Prints one line per
 phase to stdout.
@@ -200,7 +200,8 @@ def main(argv=None) -> int:
     if args.workload == "swm512":
         from . import MultilayerSW2DState
 
-        model, (h, u, v) = tc.baroclinic_jet_swm(512, seed=3, lateral_viscosity=50.0)
+        model, (h, u, v) = tc.baroclinic_jet_swm(512, seed=3, lateral_viscosity=50.0,
+                                                  vorticity_flux="energy_enstrophy")
         h, u, v = (torch.from_numpy(np.asarray(a, dtype=np.float32)).cuda() for a in (h, u, v))
         st = MultilayerSW2DState(h=h, u=u, v=v)
         dt = tc.jet_dt(model)

@@ -34,6 +34,7 @@ from .core import (
     SomaxModel,
     State,
     StratificationProfile,
+    save_times,
     solver_name,
 )
 
@@ -336,7 +337,7 @@ class _QGBase(SomaxModel):
         saveat = kw.pop("saveat", None)
         if kw:
             raise TypeError(f"unexpected keyword arguments {sorted(kw)}")
-        ts = [float(t1)] if saveat is None else [float(t) for t in saveat]
+        ts = save_times(t0, t1, dt, saveat)
         q = self._zero_ghost(state0.q)
         t = float(t0)
         out = []
@@ -429,7 +430,18 @@ class BaroclinicQG(_QGBase):
                wind_profile: str = "doublegyre", poisson_bc: str = "dst", *, biharmonic_viscosity: float = 0.0,
                advection: str = "arakawa", precision: str = "fp32", mass_correction: bool = False,
                helmholtz_eigenvalues: str = "consistent", device="cuda") -> "BaroclinicQG":
-        """``BaroclinicQG.create`` (baroclinic.py:231-332), same arguments and errors."""
+        """``BaroclinicQG.create`` (baroclinic.py:231-332), same arguments and errors.
+
+        One default differs from the reference: the Helmholtz parameters are
+        ``f0^2`` times the eigenvalues of the stretching matrix ``A`` that
+        belong to ``Cm2l`` (``helmholtz_eigenvalues="consistent"``).  The
+        reference takes ``jnp.linalg.eigh(A)`` of the non-symmetric ``A``
+        (transforms.py:192-194), i.e. the eigenvalues of ``(A + A^T)/2``, whose
+        barotropic one is negative and makes that mode's operator indefinite;
+        ``helmholtz_eigenvalues="reference"`` reproduces them.  Keyword-only
+        extensions: ``biharmonic_viscosity``, ``advection`` ("arakawa" or
+        "weno"), ``precision`` ("fp32", "mixed" or "bf16x6"),
+        ``mass_correction``."""
         grid = ArakawaCGrid2D.from_interior(nx, ny, Lx, Ly)
         if stratification is not None:
             strat = stratification

@@ -18,7 +18,7 @@ from dataclasses import dataclass
 import numpy as np
 
 from . import _lib
-from .core import Diagnostics, SomaxModel, Solution, solver_name
+from .core import Diagnostics, SomaxModel, Solution, save_times, solver_name
 from .qg import BaroclinicQG
 from .swm import MultilayerShallowWater2D, MultilayerSW2DState
 
@@ -145,7 +145,7 @@ class ReparameterizedQG(SomaxModel):
         saveat = kw.pop("saveat", None)
         if kw:
             raise TypeError(f"unexpected keyword arguments {sorted(kw)}")
-        ts = [float(t1)] if saveat is None else [float(t) for t in saveat]
+        ts = save_times(t0, t1, dt, saveat)
         st = self.apply_boundary_conditions(state0)
         h, u, v = (f.clone() for f in self.swm._fields(st))
         t = float(t0)

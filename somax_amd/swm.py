@@ -3,8 +3,11 @@
 Drop-in mirror of ``somax/_src/models/swm/multilayer.py``: class names,
 state/param/const fields, ``create()`` arguments and defaults, and the
 ``SomaxModel`` contract.  Every evaluation runs in ``libsomaxhip.so``
-(gfx950); there is no fallback.  Extension (keyword-compatible):
-``bc="channel"`` -- periodic in x, free-slip walls in y.
+(gfx950); there is no fallback.  Extensions (keyword-compatible):
+``bc="channel"`` -- periodic in x, free-slip walls in y;
+``vorticity_flux="energy_enstrophy"`` -- the Arakawa & Lamb (1981) triad form
+of the PV flux (no work done, potential enstrophy of non-divergent flow
+kept) instead of the reference's average form (the default, "average").
 """
 
 from __future__ import annotations
@@ -26,10 +29,12 @@ from .core import (
     SomaxModel,
     State,
     StratificationProfile,
+    save_times,
     solver_name,
 )
 
 _BC = {"periodic": _lib.BC_PERIODIC, "wall": _lib.BC_WALL, "channel": _lib.BC_CHANNEL}
+_VFLUX = {"average": 0, "energy_enstrophy": 1}
 
 
 def _torch():
@@ -83,9 +88,12 @@ class MultilayerShallowWater2D(SomaxModel):
     """Multilayer vector-invariant shallow-water model (multilayer.py:95-377)."""
 
     def __init__(self, params, consts, grid, strat, modal, f_row, wind_row, bc_type="periodic", method="upwind1",
-                 device="cuda"):
+                 device="cuda", vorticity_flux="average"):
         if bc_type not in _BC:
             raise ValueError(f"bc must be one of {sorted(_BC)}, got {bc_type!r}")
+        if vorticity_flux not in _VFLUX:
+            raise ValueError(f"vorticity_flux must be one of {sorted(_VFLUX)}, got {vorticity_flux!r}")
+        self.vorticity_flux = vorticity_flux
         if method != "upwind1":
             raise ValueError("only method='upwind1' is implemented on the HIP path")
         self.params = params
@@ -132,6 +140,7 @@ class MultilayerShallowWater2D(SomaxModel):
         d.dx, d.dy = g.dx, g.dy
         d.g_prime, d.H, d.f_row, d.wind_row = (a.ctypes.data_as(_lib.c_double_p) for a in keep)
         d.bc = _BC[self.bc_type]
+        d.vorticity_flux = _VFLUX[self.vorticity_flux]
         plan = ctypes.c_void_p()
         _lib.check(lib.somaxhip_swm_plan_create(ctypes.byref(d), ctypes.byref(plan)), "somaxhip_swm_plan_create")
         self._plan = plan
@@ -247,7 +256,7 @@ class MultilayerShallowWater2D(SomaxModel):
         saveat = kw.pop("saveat", None)
         if kw:
             raise TypeError(f"unexpected keyword arguments {sorted(kw)}")
-        ts = [float(t1)] if saveat is None else [float(t) for t in saveat]
+        ts = save_times(t0, t1, dt, saveat)
         h, u, v = self._fields(self.apply_boundary_conditions(state0))  # model.py:71
         t = float(t0)
         out = []
@@ -290,8 +299,9 @@ class MultilayerShallowWater2D(SomaxModel):
                g_prime=(9.81, 0.025, 0.0125), stratification: StratificationProfile | None = None,
                lateral_viscosity: float = 0.0, bottom_drag: float = 0.0, wind_amplitude: float = 0.0,
                wind_profile: str = "doublegyre", bc: str = "periodic", method: str = "upwind1",
-               *, device="cuda") -> "MultilayerShallowWater2D":
-        """``MultilayerShallowWater2D.create`` (multilayer.py:259-377), same arguments and errors."""
+               *, vorticity_flux: str = "average", device="cuda") -> "MultilayerShallowWater2D":
+        """``MultilayerShallowWater2D.create`` (multilayer.py:259-377), same arguments and errors;
+        ``vorticity_flux`` (extension): "average" (the reference's) or "energy_enstrophy"."""
         grid = ArakawaCGrid2D.from_interior(nx, ny, Lx, Ly)
         if stratification is not None:
             strat = stratification
@@ -313,7 +323,7 @@ class MultilayerShallowWater2D(SomaxModel):
         else:
             wind = -np.cos(2.0 * np.pi * y / Ly)
         return MultilayerShallowWater2D(params, consts, grid, strat, modal, f_row, wind, bc_type=bc, method=method,
-                                        device=device)
+                                        device=device, vorticity_flux=vorticity_flux)
 
 
 # ---------------------------------------------------------------------------

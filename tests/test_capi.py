@@ -66,3 +66,4 @@ def test_plan_create_rejects_planes_of_2_31_cells(kind):
     d.nx = d.ny = 46338  # 46340^2 < 2^31 - 1
     assert create(ctypes.byref(d), ctypes.byref(plan)) == ERR_ARG
     assert "2^31" not in _lib.last_error()
+

@@ -294,3 +294,39 @@ def test_uneven_256_member_ensemble(gpu):
         _log(f"  member {i} ({m.grid.nx}^2, {m._nl} layers): rel-L2 q vs oracle = {r:.3e}")
         assert r < 1e-6
     assert len(seen) == 12
+
+
+def test_ensemble_graph_recaptures(gpu):
+    """A concurrent, graph-captured ensemble recaptures its step whenever the
+    parameters switch the stage kernel (biharmonic on / off), dt changes or
+    the state buffer moves; five captures in a row on one ensemble, each
+    followed by steps checked bit for bit against the single models."""
+    import dataclasses
+
+    import torch
+
+    from somax_amd import QGEnsemble
+
+    _log("[ensemble] five graph captures on one concurrent ensemble")
+    models = _models("mixed")
+    qs = _states(models)
+    ens = QGEnsemble(models, concurrent=True, graph=True)
+    base = [m.params for m in models]
+    no_bih = [dataclasses.replace(p, biharmonic_viscosity=0.0) for p in base]
+    packed = ens.pack(qs)
+    singles = [torch.from_numpy(q).cuda() for q in qs]
+    plan = [("first capture", None, 1200.0, False), ("biharmonic off", no_bih, 1200.0, False),
+            ("dt changed", None, 900.0, False), ("state moved", None, 900.0, True),
+            ("biharmonic on", base, 900.0, False)]
+    for k, (what, params, dt, move) in enumerate(plan):
+        if params is not None:
+            ens.set_params(params)
+        if move:
+            packed = packed.clone()
+        ens.step(packed, dt, 2)
+        for m, q in zip(models, singles):
+            m._step_inplace(q, dt, 2)
+        torch.cuda.synchronize()
+        same = all(torch.equal(ens.member(packed, i), q.view(ens.shapes[i])) for i, q in enumerate(singles))
+        _log(f"  capture {k + 1} ({what}): members equal their single models: {same}")
+        assert same

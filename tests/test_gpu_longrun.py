@@ -83,3 +83,46 @@ def test_long_run_statistics_match_oracle(gpu):
     _log(f"  time-mean KE rel diff {r_ke:.3e}; spectrum max rel diff over {int(keep.sum())} bins {r_sp:.3e}; "
          f"final-state rel-L2 {final:.3e}")
     assert r_ke < 0.01 and r_sp < 0.01
+
+
+def test_long_run_statistics_qg128_weno_match_oracle(gpu):
+    """The multilayer double gyre at 128^2 with WENO-Z PV flux, 10,000 mixed
+    SSP-RK3 steps of 1200 s (139 days) sampled every 50 steps, against the
+    FP64 oracle's statistics of the same run held in
+    tests/golden/longrun_qg128_weno.json (written by
+    tests/golden/make_longrun_qg128_weno.py): time-mean total kinetic energy
+    and time-mean isotropic KE spectrum within 1%."""
+    import json
+    from pathlib import Path
+
+    import torch
+
+    from somax_amd import BaroclinicQG, BaroclinicQGState
+    from somax_amd import testcases as tc
+
+    fx = json.loads((Path(__file__).resolve().parent / "golden" / "longrun_qg128_weno.json").read_text())
+    n, nsteps, every, dt = fx["n"], fx["nsteps"], fx["every"], fx["dt"]
+    args = dict(fx["args"])
+    H = tuple(args["H"])
+    model = BaroclinicQG.create(n_layers=3, precision="mixed", advection=fx["advection"], **args)
+    psi0 = tc.random_streamfunction(n, n, args["Lx"], args["Ly"], 3, fx["seed"], 0.05)
+    q = tc.pv_from_streamfunction(psi0, model.grid.dx, model.grid.dy, model._lambdas, model._Cl2m, model._Cm2l)
+    qd = torch.from_numpy(q.astype(np.float32)).cuda()
+    dx, dy = model.grid.dx, model.grid.dy
+    ke, sp = [], []
+    _log(f"[longrun] QG {n}^2 WENO-Z mixed, {nsteps} steps of {dt:.0f} s, sample every {every}")
+    for k in range(nsteps // every):
+        model._step_inplace(qd, dt, every)
+        d = model.diagnose(BaroclinicQGState(q=qd))
+        ke.append(float(d.total_kinetic_energy))
+        sp.append(_spectrum(d.psi.cpu().numpy(), H, dx, dy))
+        if (k + 1) % 40 == 0:
+            _log(f"  step {(k + 1) * every}: KE gpu {ke[-1]:.6e} oracle {fx['total_kinetic_energy'][k]:.6e}")
+    assert np.all(np.isfinite(ke))
+    mean_g, mean_o = np.mean(ke), np.mean(fx["total_kinetic_energy"])
+    r_ke = abs(mean_g - mean_o) / mean_o
+    sg, so = np.mean(sp, axis=0), np.asarray(fx["spectrum_mean"])
+    keep = so > 1e-4 * so.sum()
+    r_sp = float(np.max(np.abs(sg[keep] - so[keep]) / so[keep]))
+    _log(f"  time-mean KE rel diff {r_ke:.3e}; spectrum max rel diff over {int(keep.sum())} bins {r_sp:.3e}")
+    assert r_ke < 0.01 and r_sp < 0.01

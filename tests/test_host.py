@@ -126,3 +126,19 @@ def test_solver_names():
     assert solver_name("SSP-RK3") == "ssprk3" and solver_name("ssprk3") == "ssprk3"
     with pytest.raises(NotImplementedError):
         solver_name("dopri5")
+
+
+def test_save_times_are_checked_like_diffrax():
+    """integrate's time span: positive dt, save times at or after t0 and not
+    decreasing (a negative dt or a backward span raises instead of taking
+    one step over the whole span or returning the state unchanged)."""
+    import pytest
+
+    from somax_amd.core import save_times
+
+    assert save_times(0.0, 3600.0, 600.0, None) == [3600.0]
+    assert save_times(0.0, 1.0, 0.5, [0.0, 0.5, 0.5, 2.0]) == [0.0, 0.5, 0.5, 2.0]
+    for t0, t1, dt, saveat in ((0.0, 3600.0, -600.0, None), (0.0, 3600.0, 0.0, None), (3600.0, 0.0, 600.0, None),
+                               (0.0, 1.0, 0.5, [1.0, 0.5]), (0.0, 1.0, 0.5, [])):
+        with pytest.raises(ValueError):
+            save_times(t0, t1, dt, saveat)

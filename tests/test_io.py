@@ -167,3 +167,31 @@ def test_reads_multichunk_gzip_and_missing_chunks(tmp_path):
     (root / "a" / "zarr.json").write_text(json.dumps(meta))
     with pytest.raises(ValueError, match="zstd"):
         io.load_dataset(root)
+
+
+def test_failed_write_leaves_the_existing_store(tmp_path, monkeypatch):
+    """save_dataset / append_to_dataset write into a sibling directory and swap
+    it in only when complete: an error while writing keeps the old store."""
+    from somax_amd import io
+
+    path = tmp_path / "run.zarr"
+    ds = io.Dataset({"q": io.DataArray(("time", "y"), np.arange(6.0).reshape(2, 3))},
+                    {"time": io.DataArray(("time",), np.array([0.0, 1.0]))}, {"note": "first"})
+    io.save_dataset(ds, path)
+    before = io.load_dataset(path)
+
+    def boom(*a, **k):
+        raise OSError("disk full")
+
+    monkeypatch.setattr(io, "_write_array", boom)
+    new = io.Dataset({"q": io.DataArray(("time", "y"), np.ones((1, 3)))},
+                     {"time": io.DataArray(("time",), np.array([2.0]))}, {})
+    with pytest.raises(OSError, match="disk full"):
+        io.append_to_dataset(new, path)
+    monkeypatch.undo()
+    after = io.load_dataset(path)
+    np.testing.assert_array_equal(after.data_vars["q"].values, before.data_vars["q"].values)
+    assert after.attrs["note"] == "first"
+    assert sorted(p.name for p in tmp_path.iterdir()) == ["run.zarr"]  # no temporary left behind
+    io.append_to_dataset(new, path)
+    assert io.load_dataset(path).data_vars["q"].values.shape == (3, 3)

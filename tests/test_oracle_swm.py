@@ -12,6 +12,12 @@
   any of the linear operators moves these eigenvalues.
 * Flux-form mass conservation for every boundary condition, and the sign of
   the Coriolis term on uniform flow.
+* The energy-enstrophy-conserving vorticity flux (extension): on a doubly
+  periodic grid its work sum U F_u + V F_v vanishes for any PV and fluxes,
+  and its potential-enstrophy tendency sum q curl(F) vanishes for any PV and
+  a non-divergent mass flux, both to round-off; the reference's average form
+  fails the first, which makes the check discriminating.  Uniform PV gives the
+  average form, and the linear dispersion relation is unchanged.
 """
 
 from __future__ import annotations
@@ -88,10 +94,11 @@ def _jacobian_eigs(s):
     return np.linalg.eigvals(J)
 
 
+@pytest.mark.parametrize("flux", ["average", "energy_enstrophy"])
 @pytest.mark.parametrize("nl", [1, 2])
-def test_linear_dispersion_matches_c_grid_theory(nl):
+def test_linear_dispersion_matches_c_grid_theory(nl, flux):
     nx = ny = 6
-    s = _setup("periodic", nl=nl, n=nx, Lx=6e5, Ly=6e5)
+    s = _setup("periodic", nl=nl, n=nx, Lx=6e5, Ly=6e5, vorticity_flux=flux)
     lam = _jacobian_eigs(s)
     assert np.abs(lam.real).max() < 1e-9 * np.abs(lam).max()
     # pressure matrix of the cumulative coupling: P_k = sum_{j<=k} g'_j h_j
@@ -154,3 +161,87 @@ def test_nonlinear_single_layer_wind_not_divided():
     want = 3e-6 * -np.cos(2 * np.pi * y / s.Ly)
     np.testing.assert_allclose(du[0, 1:-1, 1:-1], np.broadcast_to(want[1:-1, None], (s.ny, s.nx)), rtol=1e-12)
     assert np.abs(dh).max() < 1e-12 and np.abs(dv).max() < 1e-18
+
+
+# ---------------------------------------------------------------------------
+# energy-enstrophy-conserving vorticity flux (extension)
+# ---------------------------------------------------------------------------
+def _torus_fields(s, seed):
+    """Random PV on X points and mass fluxes on U / V points of a doubly
+    periodic grid (ghost ring = periodic images)."""
+    rng = np.random.default_rng(seed)
+    nl, ny, nx = s.nl, s.ny, s.nx
+
+    def x_field():
+        a = np.zeros(s.shape)
+        a[:, 1:-1, 1:-1] = rng.standard_normal((nl, ny, nx))
+        a[:, 0, :] = a[:, -2, :]
+        a[:, :, 0] = a[:, :, -2]
+        return a
+
+    def t_field():
+        a = np.zeros(s.shape)
+        a[:, 1:-1, 1:-1] = rng.standard_normal((nl, ny, nx))
+        return swm.enforce_periodic(a)
+
+    return x_field, t_field
+
+
+@pytest.mark.parametrize("flux,conserves", [("energy_enstrophy", True), ("average", False)])
+def test_vorticity_flux_does_no_work(flux, conserves):
+    s = _setup("periodic", nl=2, n=12, ny=10, Ly=1e6 * 10 / 12, vorticity_flux=flux)
+    x_field, t_field = _torus_fields(s, 4)
+    q, uh, vh = x_field(), t_field(), t_field()
+    fu, fv = swm.vorticity_flux_terms(s, q, uh, vh)
+    work = (uh[:, 1:-1, 1:-1] * fu).sum() + (vh[:, 1:-1, 1:-1] * fv).sum()
+    scale = np.abs(uh[:, 1:-1, 1:-1] * fu).sum()
+    assert (abs(work) < 1e-14 * scale) == conserves
+
+
+@pytest.mark.parametrize("flux", ["energy_enstrophy", "average"])
+def test_vorticity_flux_keeps_enstrophy_of_nondivergent_flow(flux):
+    """sum_X q curl(F) = 0 when div(uh, vh) = 0 (then h_X is steady and the
+    potential enstrophy sum h_X q^2 / 2 changes only through curl(F))."""
+    s = _setup("periodic", nl=2, n=12, ny=10, Ly=1e6 * 10 / 12, vorticity_flux=flux)
+    x_field, _ = _torus_fields(s, 5)
+    q, psi = x_field(), x_field()
+    uh = np.zeros(s.shape)
+    vh = np.zeros(s.shape)
+    uh[:, 1:, :] = -(psi[:, 1:, :] - psi[:, :-1, :]) / s.dy
+    vh[:, :, 1:] = (psi[:, :, 1:] - psi[:, :, :-1]) / s.dx
+    uh, vh = swm.enforce_periodic(uh), swm.enforce_periodic(vh)
+    div = (uh[:, 1:-1, 1:-1] - uh[:, 1:-1, :-2]) / s.dx + (vh[:, 1:-1, 1:-1] - vh[:, :-2, 1:-1]) / s.dy
+    assert np.abs(div).max() < 1e-12 * np.abs(uh).max() / s.dx
+    fu, fv = swm.vorticity_flux_terms(s, q, uh, vh)
+    FU, FV = np.zeros(s.shape), np.zeros(s.shape)
+    FU[:, 1:-1, 1:-1], FV[:, 1:-1, 1:-1] = fu, fv
+    FU, FV = swm.enforce_periodic(FU), swm.enforce_periodic(FV)
+    curl = (FV[:, 1:-1, 2:] - FV[:, 1:-1, 1:-1]) / s.dx - (FU[:, 2:, 1:-1] - FU[:, 1:-1, 1:-1]) / s.dy
+    dz = (q[:, 1:-1, 1:-1] * curl).sum()
+    assert abs(dz) < 1e-14 * np.abs(q[:, 1:-1, 1:-1] * curl).sum()
+
+
+def test_energy_enstrophy_flux_with_uniform_pv_is_the_average_form():
+    s_een = _setup("periodic", nl=1, n=8, vorticity_flux="energy_enstrophy")
+    s_avg = _setup("periodic", nl=1, n=8)
+    _, t_field = _torus_fields(s_een, 6)
+    q = np.full(s_een.shape, 2.5e-7)
+    uh, vh = t_field(), t_field()
+    for a, b in zip(swm.vorticity_flux_terms(s_een, q, uh, vh), swm.vorticity_flux_terms(s_avg, q, uh, vh)):
+        np.testing.assert_allclose(a, b, rtol=1e-13, atol=1e-22)
+
+
+@pytest.mark.parametrize("bc", ["periodic", "wall", "channel"])
+def test_energy_enstrophy_flux_rest_and_mass(bc):
+    s = _setup(bc, nl=2, n=12, vorticity_flux="energy_enstrophy")
+    dh, du, dv = swm.rhs(s, *swm.apply_bc(s, *_rest(s)))
+    assert max(np.abs(dh).max(), np.abs(du).max(), np.abs(dv).max()) < 1e-10
+    rng = np.random.default_rng(8)
+    h, u, v = _rest(s)
+    h = h + rng.standard_normal(h.shape)
+    u, v = 0.1 * rng.standard_normal(u.shape), 0.1 * rng.standard_normal(v.shape)
+    dh, du, _ = swm.rhs(s, *swm.apply_bc(s, h, u, v))
+    assert np.abs(dh[:, 1:-1, 1:-1].sum(axis=(-2, -1))).max() < 1e-12 * np.abs(dh).max() * dh[0].size
+    s_avg = _setup(bc, nl=2, n=12)
+    _, du_avg, _ = swm.rhs(s_avg, *swm.apply_bc(s_avg, h, u, v))
+    assert np.abs(du - du_avg).max() > 1e-6 * np.abs(du).max()  # the two forms differ on a real flow
