@@ -119,6 +119,16 @@ def _load_rhs():
             fn.restype = ctypes.c_int
             fn.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_int,
                            p, p, p, p, _dp, ctypes.c_double, ctypes.c_double, ctypes.c_double, _dp, p]
+        lib.fftcpu_prepare.restype = None
+        lib.fftcpu_prepare.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int] + [_fp] * 6
+        lib.fftcpu_divide.restype = None
+        lib.fftcpu_divide.argtypes = [ctypes.c_long, _fp, _fp]
+        lib.fftcpu_finish.restype = None
+        lib.fftcpu_finish.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, _fp, _fp, _dp, _fp, _dp, ctypes.c_int,
+                                      _fp, _dp]
+        lib.fftcpu_combine.restype = None
+        lib.fftcpu_combine.argtypes = [ctypes.c_long, _fp, ctypes.c_float, _fp, ctypes.c_float, _fp, ctypes.c_float,
+                                       _fp]
         lib.qgrhs_set_threads.argtypes = [ctypes.c_int]
         lib.qgrhs_set_threads.restype = None
         lib.qgrhs_max_threads.restype = ctypes.c_int
@@ -170,51 +180,50 @@ class FftCpuStep:
         s = setup
         self.s = s
         self.threads = int(threads or host_threads())
-        _load_rhs().qgrhs_set_threads(self.threads)
+        self._lib = _load_rhs()
+        self._lib.qgrhs_set_threads(self.threads)
         f32 = np.float32
-        self.cl2m = np.asarray(s.Cl2m, dtype=f32)
-        self.cm2l = np.asarray(s.Cm2l, dtype=f32)
+        self.cl2m = np.ascontiguousarray(s.Cl2m, dtype=f32)
+        self.cm2l = np.ascontiguousarray(s.Cm2l, dtype=f32)
+        self.cm2l64 = np.ascontiguousarray(s.Cm2l, dtype=np.float64)
         den = dst.dst1_eigenvalues(s.Ny, s.dy)[:, None] + dst.dst1_eigenvalues(s.Nx, s.dx)[None, :]
-        self.den = (den[None] - np.asarray(s.lambdas)[:, None, None]).astype(f32)
-        self.beta_y = np.broadcast_to(s.beta_y, (s.Ny, s.Nx)).astype(f32)
+        self.den = np.ascontiguousarray((den[None] - np.asarray(s.lambdas)[:, None, None]).astype(f32))
+        self.beta_row = np.ascontiguousarray(np.broadcast_to(s.beta_y, (s.Ny, s.Nx))[:, 0], dtype=f32)
         self.mass = bool(s.mass_correction)
         if self.mass:
-            self.homsol = np.asarray(s.homsol, dtype=f32)
-            self.hmean = np.asarray(s.homsol_mean, dtype=np.float64)
+            self.homsol = np.ascontiguousarray(s.homsol, dtype=f32)
+            self.hmean = np.ascontiguousarray(s.homsol_mean, dtype=np.float64)
         self.shape = (s.nl, s.Ny, s.Nx)
 
-    def _mix(self, M, f):
-        return (M @ f.reshape(self.s.nl, -1)).reshape(self.shape)
-
     def rhs(self, q: np.ndarray) -> np.ndarray:
+        """The RHS of the boundary-conditioned state BC(q): elementwise passes
+        in C++ (OpenMP), sine transforms by pocketfft."""
         import scipy.fft
 
-        s, w = self.s, self.threads
-        qm = self._mix(self.cl2m, q)
-        f = scipy.fft.dstn(qm, type=1, axes=(-2, -1), norm="ortho", workers=w)
-        pm = scipy.fft.dstn(f / self.den, type=1, axes=(-2, -1), norm="ortho", workers=w)
+        s, w, lib = self.s, self.threads, self._lib
+        p = lambda a: a.ctypes.data_as(_fp)  # noqa: E731
+        qbc, qt, qm, psi = (np.empty(self.shape, dtype=np.float32) for _ in range(4))
+        lib.fftcpu_prepare(s.nl, s.Ny, s.Nx, p(q), p(self.beta_row), p(self.cl2m), p(qbc), p(qt), p(qm))
+        f = scipy.fft.dstn(qm, type=1, axes=(-2, -1), norm="ortho", workers=w, overwrite_x=True)
+        lib.fftcpu_divide(f.size, p(f), p(self.den))
+        pm = np.ascontiguousarray(scipy.fft.dstn(f, type=1, axes=(-2, -1), norm="ortho", workers=w,
+                                                 overwrite_x=True), dtype=np.float32)
         wall = np.zeros(s.nl)
-        if self.mass:
-            alpha = -pm.mean(axis=(-2, -1), dtype=np.float64) / self.hmean
-            pm = pm + alpha.astype(np.float32)[:, None, None] * self.homsol
-            wall = np.asarray(s.Cm2l) @ alpha
-        psi = self._mix(self.cm2l, pm)
-        if s.zero_psi_ghost:
-            psi[:, 0, :] = psi[:, -1, :] = 0.0
-            psi[:, :, 0] = psi[:, :, -1] = 0.0
-        return qg_stencil_rhs(s, psi, q, q + self.beta_y, wall, dtype=np.float32)
-
-    def _bc(self, q):
-        g = q.copy()
-        g[:, 0, :] = g[:, -1, :] = 0.0
-        g[:, :, 0] = g[:, :, -1] = 0.0
-        return g
+        lib.fftcpu_finish(s.nl, s.Ny, s.Nx, p(pm), p(self.homsol) if self.mass else None,
+                          self.hmean.ctypes.data_as(_dp) if self.mass else None, p(self.cm2l),
+                          self.cm2l64.ctypes.data_as(_dp), int(s.zero_psi_ghost), p(psi), wall.ctypes.data_as(_dp))
+        return qg_stencil_rhs(s, psi, qbc, qt, wall, dtype=np.float32)
 
     def step(self, q: np.ndarray, dt: float, nsteps: int = 1) -> np.ndarray:
-        h = np.float32(dt)
-        q = np.ascontiguousarray(np.asarray(q, dtype=np.float32).reshape(self.shape))
+        """``nsteps`` SSP-RK3 steps (Shu-Osher form) from ``q``."""
+        h = float(dt)
+        q = np.array(np.asarray(q, dtype=np.float32).reshape(self.shape), order="C")
+        q1, q2 = np.empty_like(q), np.empty_like(q)
+        n = q.size
+        p = lambda a: a.ctypes.data_as(_fp)  # noqa: E731
+        comb = self._lib.fftcpu_combine
         for _ in range(nsteps):
-            q1 = q + h * self.rhs(self._bc(q))
-            q2 = np.float32(0.75) * q + np.float32(0.25) * (q1 + h * self.rhs(self._bc(q1)))
-            q = q / np.float32(3.0) + np.float32(2.0 / 3.0) * (q2 + h * self.rhs(self._bc(q2)))
+            comb(n, p(q1), 0.0, p(q), 1.0, p(q), h, p(self.rhs(q)))
+            comb(n, p(q2), 0.75, p(q), 0.25, p(q1), h, p(self.rhs(q1)))
+            comb(n, p(q), 1.0 / 3.0, p(q), 2.0 / 3.0, p(q2), h, p(self.rhs(q2)))
         return q

@@ -224,6 +224,86 @@ int qgrhs_stencil_f32(int nl, int Ny, int Nx, double dx, double dy, int advectio
   return stencil_rhs<float>(nl, Ny, Nx, dx, dy, advection, psi, q, qt, wind, wind_coef, kappa, nu, nu4, psi_wall, dq);
 }
 
+// ---------------------------------------------------------------------------
+// FP32 elementwise passes of the CPU baseline's step (oracle/cpu.py:FftCpuStep);
+// the sine transforms between them are pocketfft's.
+// ---------------------------------------------------------------------------
+
+// qbc = q with its ghost ring set to 0; qt = qbc + beta (beta: Ny values per
+// row); qm = cl2m . qbc (nl x nl, row-major).
+void fftcpu_prepare(int nl, int Ny, int Nx, const float* q, const float* beta_row, const float* cl2m, float* qbc,
+                    float* qt, float* qm) {
+  const long plane = (long)Ny * Nx;
+#pragma omp parallel for schedule(static)
+  for (long j = 0; j < Ny; ++j) {
+    for (long i = 0; i < Nx; ++i) {
+      const bool ghost = j == 0 || j == Ny - 1 || i == 0 || i == Nx - 1;
+      float v[8];
+      for (int l = 0; l < nl; ++l) {
+        const long k = l * plane + j * Nx + i;
+        v[l] = ghost ? 0.0f : q[k];
+        qbc[k] = v[l];
+        qt[k] = v[l] + beta_row[j];
+      }
+      for (int m = 0; m < nl; ++m) {
+        float a = 0.0f;
+        for (int l = 0; l < nl; ++l) a += cl2m[m * nl + l] * v[l];
+        qm[m * plane + j * Nx + i] = a;
+      }
+    }
+  }
+}
+
+// f /= den elementwise over n values.
+void fftcpu_divide(long n, float* f, const float* den) {
+#pragma omp parallel for schedule(static)
+  for (long k = 0; k < n; ++k) f[k] = f[k] / den[k];
+}
+
+// Mass constraint and layer mix: alpha[m] = -mean(pm[m]) / hmean[m] (FP64
+// means; skipped when homsol is null), psi = cm2l . (pm + alpha homsol),
+// ghost ring of psi 0 when zero_ghost; wall[l] = (cm2l . alpha)[l].
+void fftcpu_finish(int nl, int Ny, int Nx, const float* pm, const float* homsol, const double* hmean,
+                   const float* cm2l, const double* cm2l64, int zero_ghost, float* psi, double* wall) {
+  const long plane = (long)Ny * Nx;
+  double alpha[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  if (homsol) {
+    for (int m = 0; m < nl; ++m) {
+      double sum = 0.0;
+#pragma omp parallel for schedule(static) reduction(+ : sum)
+      for (long k = 0; k < plane; ++k) sum += (double)pm[m * plane + k];
+      alpha[m] = -(sum / (double)plane) / hmean[m];
+    }
+  }
+  for (int l = 0; l < nl; ++l) {
+    double w = 0.0;
+    for (int m = 0; m < nl; ++m) w += cm2l64[l * nl + m] * alpha[m];
+    wall[l] = w;
+  }
+  float af[8];
+  for (int m = 0; m < nl; ++m) af[m] = (float)alpha[m];
+#pragma omp parallel for schedule(static)
+  for (long j = 0; j < Ny; ++j) {
+    for (long i = 0; i < Nx; ++i) {
+      const long k = j * Nx + i;
+      float v[8];
+      for (int m = 0; m < nl; ++m) v[m] = pm[m * plane + k] + (homsol ? af[m] * homsol[m * plane + k] : 0.0f);
+      const bool ghost = j == 0 || j == Ny - 1 || i == 0 || i == Nx - 1;
+      for (int l = 0; l < nl; ++l) {
+        float a = 0.0f;
+        for (int m = 0; m < nl; ++m) a += cm2l[l * nl + m] * v[m];
+        psi[l * plane + k] = (zero_ghost && ghost) ? 0.0f : a;
+      }
+    }
+  }
+}
+
+// out = a * x + b * (y + h * k) elementwise (the SSP-RK3 stage forms).
+void fftcpu_combine(long n, float* out, float a, const float* x, float b, const float* y, float h, const float* k) {
+#pragma omp parallel for schedule(static)
+  for (long i = 0; i < n; ++i) out[i] = a * x[i] + b * (y[i] + h * k[i]);
+}
+
 void qgrhs_set_threads(int n) { omp_set_num_threads(n > 0 ? n : 1); }
 
 int qgrhs_max_threads(void) { return omp_get_max_threads(); }

@@ -321,14 +321,14 @@ void launch_qg_diag(const QGDiagArgs& a, hipStream_t st);
 void launch_pair_reduce(const double* partial, int n, int P, double scale, double* out, hipStream_t st);
 
 // Mass correction of the modal streamfunction (homogeneous solution method):
-// per plane the FP64 sum of (a + b) over the full array (b nullable) into
-// out[p][0] (fixed order, via launch_pair_reduce), then
-//   alpha[p] = -sum[p] / (Nx*Ny) / hmean[p]   (0 where hmean[p] == 0)
+// per plane FP64 block partials of the sum of (a + b) over the full array (b
+// nullable), then in one block their fixed-order sum and
+//   alpha[p] = -sum[p] * inv_count / hmean[p]   (0 where hmean[p] == 0)
 // and the wall value of the layer streamfunction psi_wall[first + l] =
 // sum_p cm2l[l][p] * alpha[first + p] for the WENO corner streamfunction.
-void launch_plane_sums(const float* a, const float* b, int Nx, int Ny, int P, long plane, double* partial,
-                       double* out, hipStream_t st);
-void launch_mass_alpha(const double* sums, const double* hmean, Tables t, int P, double inv_count, double* alpha,
-                       float* psi_wall, hipStream_t st);
+int launch_plane_sums(const float* a, const float* b, int Nx, int Ny, int P, long plane, double* partial,
+                      hipStream_t st);  // returns the partials per plane
+void launch_mass_alpha(const double* partial, int nblk, const double* hmean, Tables t, int P, double inv_count,
+                       double* alpha, float* psi_wall, hipStream_t st);
 
 }  // namespace somax

@@ -193,22 +193,24 @@ __device__ inline float weno5_corr(float d1, float d2, float d3, float d4) {
   const float b1 = (13.0f / 12.0f) * (d3 - d2) * (d3 - d2) + 0.25f * (d2 + d3) * (d2 + d3);
   const float b2 = (13.0f / 12.0f) * (d4 - d3) * (d4 - d3) + 0.25f * (d4 - 3.0f * d3) * (d4 - 3.0f * d3);
   const float tau = fabsf(b0 - b2);
-  const float a0 = 0.1f * (1.0f + tau / (b0 + WENO_EPS));
-  const float a1 = 0.6f * (1.0f + tau / (b1 + WENO_EPS));
-  const float a2 = 0.3f * (1.0f + tau / (b2 + WENO_EPS));
+  // the nonlinear weights need no correctly rounded division: the hardware
+  // reciprocal (1 ulp) moves a face value by about one FP32 ulp
+  const float a0 = 0.1f * (1.0f + tau * __builtin_amdgcn_rcpf(b0 + WENO_EPS));
+  const float a1 = 0.6f * (1.0f + tau * __builtin_amdgcn_rcpf(b1 + WENO_EPS));
+  const float a2 = 0.3f * (1.0f + tau * __builtin_amdgcn_rcpf(b2 + WENO_EPS));
   const float c0 = (5.0f * d2 - 2.0f * d1) * (1.0f / 6.0f);
   const float c1 = (d2 + 2.0f * d3) * (1.0f / 6.0f);
   const float c2 = (4.0f * d3 - d4) * (1.0f / 6.0f);
-  return (a0 * c0 + a1 * c1 + a2 * c2) / (a0 + a1 + a2);
+  return (a0 * c0 + a1 * c1 + a2 * c2) * __builtin_amdgcn_rcpf(a0 + a1 + a2);
 }
 
 // 3rd-order WENO-Z correction from d2 = q_i - q_{i-1}, d3 = q_{i+1} - q_i.
 __device__ inline float weno3_corr(float d2, float d3) {
   const float b0 = d2 * d2, b1 = d3 * d3;
   const float tau = fabsf(b0 - b1);
-  const float a0 = (1.0f / 3.0f) * (1.0f + tau / (b0 + WENO_EPS));
-  const float a1 = (2.0f / 3.0f) * (1.0f + tau / (b1 + WENO_EPS));
-  return (a0 * (0.5f * d2) + a1 * (0.5f * d3)) / (a0 + a1);
+  const float a0 = (1.0f / 3.0f) * (1.0f + tau * __builtin_amdgcn_rcpf(b0 + WENO_EPS));
+  const float a1 = (2.0f / 3.0f) * (1.0f + tau * __builtin_amdgcn_rcpf(b1 + WENO_EPS));
+  return (a0 * (0.5f * d2) + a1 * (0.5f * d3)) * __builtin_amdgcn_rcpf(a0 + a1);
 }
 
 // Upwind face value minus the upwind cell value for the face between cells
@@ -254,27 +256,59 @@ __global__ __launch_bounds__(256) void qg_stage_weno_kernel(StageArgs a) {
   const float* wind_row = a.wind_rows + (long)m * Ny;
   const float pw = a.t.psi_wall ? a.t.psi_wall[z] : 0.0f;
   constexpr int PW2 = WX + 2;
+  const int tx = tid & 63;
+  const int tyb = tid >> 6;
 
-  for (int i = tid; i < (WY + 6) * WQ; i += 256) {
+  // This thread's outputs (column tx, rows tyb + 4 r): the step-start value,
+  // the wind row and (on the ghost ring under the boundary condition, where
+  // the tile holds 0) the raw stage value are fetched first, so their latency
+  // overlaps the tile loads.
+  float qn_r[WY / 4], wind_r[WY / 4], qg_r[WY / 4];
+#pragma unroll
+  for (int r = 0; r < WY / 4; ++r) {
+    const int gy = y0 + tyb + 4 * r, gx = x0 + tx;
+    const bool ok = gy < Ny && gx < Nx;
+    const bool gring = gy == 0 || gx == 0 || gy == Ny - 1 || gx == Nx - 1;
+    const int idx = gy * Nx + gx;  // plane < 2^31 cells
+    qn_r[r] = (ok && a.qn) ? a.qn[off + idx] : 0.0f;
+    wind_r[r] = ok ? wind_row[gy] : 0.0f;
+    qg_r[r] = (ok && gring && a.apply_bc) ? qs[idx] : 0.0f;
+  }
+  // Tile loads: a fixed number per thread, all issued before the first LDS
+  // store, so one memory latency covers them.  q with the boundary condition
+  // (ghost ring and outside the array read 0), halo 3; psi with halo 1.
+  constexpr int NQ = ((WY + 6) * WQ + 255) / 256;
+  constexpr int NPS = ((WY + 2) * PW2 + 255) / 256;
+  float vq[NQ], vp[NPS];
+#pragma unroll
+  for (int k = 0; k < NQ; ++k) {
+    const int i = tid + 256 * k;
     const int ly = i / WQ, lx = i - ly * WQ;
     const int gy = y0 - 3 + ly, gx = x0 - 3 + lx;
-    float v = 0.0f;
-    const bool inside = gy >= 0 && gy < Ny && gx >= 0 && gx < Nx;
+    const bool inside = i < (WY + 6) * WQ && gy >= 0 && gy < Ny && gx >= 0 && gx < Nx;
     const bool ghost = gy == 0 || gx == 0 || gy == Ny - 1 || gx == Nx - 1;
-    if (inside && !(ghost && a.apply_bc)) v = qs[(long)gy * Nx + gx];
-    sq[i] = v;
+    vq[k] = (inside && !(ghost && a.apply_bc)) ? qs[gy * Nx + gx] : 0.0f;
   }
-  for (int i = tid; i < (WY + 2) * PW2; i += 256) {
+#pragma unroll
+  for (int k = 0; k < NPS; ++k) {
+    const int i = tid + 256 * k;
     const int ly = i / PW2, lx = i - ly * PW2;
     const int gy = y0 - 1 + ly, gx = x0 - 1 + lx;
-    float v = 0.0f;
-    if (gy >= 0 && gy < Ny && gx >= 0 && gx < Nx) v = psi[(long)gy * Nx + gx];
-    sp[i] = v;
+    const bool inside = i < (WY + 2) * PW2 && gy >= 0 && gy < Ny && gx >= 0 && gx < Nx;
+    vp[k] = inside ? psi[gy * Nx + gx] : 0.0f;
   }
+  float vdb = 0.0f;
   if (tid < WY + 6) {
     const int gy = y0 - 3 + tid;
-    sdb[tid] = (gy >= 0 && gy < Ny) ? dbeta_row[gy] : 0.0f;
+    vdb = (gy >= 0 && gy < Ny) ? dbeta_row[gy] : 0.0f;
   }
+#pragma unroll
+  for (int k = 0; k < NQ; ++k)
+    if (tid + 256 * k < (WY + 6) * WQ) sq[tid + 256 * k] = vq[k];
+#pragma unroll
+  for (int k = 0; k < NPS; ++k)
+    if (tid + 256 * k < (WY + 2) * PW2) sp[tid + 256 * k] = vp[k];
+  if (tid < WY + 6) sdb[tid] = vdb;
   __syncthreads();
 
   // corners: corner (cy, cx) lies between T rows cy, cy+1 and columns cx, cx+1;
@@ -345,8 +379,6 @@ __global__ __launch_bounds__(256) void qg_stage_weno_kernel(StageArgs a) {
   }
   __syncthreads();
 
-  const int tx = tid & 63;
-  const int tyb = tid >> 6;
   const float wc = pr.tau0 * a.t.wind_scale[z];
   const bool bottom = (a.t.layer[z] == a.t.mnl[m] - 1) && (pr.kappa != 0.0f);
 #pragma unroll
@@ -373,17 +405,18 @@ __global__ __launch_bounds__(256) void qg_stage_weno_kernel(StageArgs a) {
       }
       if (bottom) rhs -= pr.kappa * ((p[1] - 2.0f * p[0] + p[-1]) * a.idx2 + (p[PW2] - 2.0f * p[0] + p[-PW2]) * a.idy2);
     }
-    rhs += wc * wind_row[gy];
+    rhs += wc * wind_r[r];
     const long idx = off + (long)gy * Nx + gx;
     const bool gring = gy == 0 || gx == 0 || gy == Ny - 1 || gx == Nx - 1;
-    auto qraw = [&]() { return (gring && a.apply_bc) ? a.qs[idx] : sq[(ly + 3) * WQ + tx + 3]; };
+    const float qtile = sq[(ly + 3) * WQ + tx + 3];
+    const float qraw = (gring && a.apply_bc) ? qg_r[r] : qtile;
     float o;
     if (a.qn) {
-      const float qn = a.qn[idx];
-      o = qn + a.c1 * (qraw() - qn) + a.c2 * rhs;
+      const float qn = qn_r[r];
+      o = qn + a.c1 * (qraw - qn) + a.c2 * rhs;
     } else {
       o = a.c2 * rhs;
-      if (a.c1 != 0.0f) o += a.c1 * qraw();
+      if (a.c1 != 0.0f) o += a.c1 * qraw;
     }
     a.out[idx] = o;
   }
@@ -873,25 +906,41 @@ namespace somax {
 // ---------------------------------------------------------------------------
 // mass correction (homogeneous solution method)
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void plane_sums_kernel(const float* a, const float* b, int Nx, long plane,
-                                                          double* partial) {
-  const int x = blockIdx.x * 256 + threadIdx.x;
-  const int y = blockIdx.y;
-  const int z = blockIdx.z;
-  double v = 0.0;
-  if (x < Nx) {
-    const long i = z * plane + (long)y * Nx + x;
-    v = (double)a[i] + (b ? (double)b[i] : 0.0);
+// Per-block FP64 partial sums of a[i] + b[i] (b nullable) over planes of n
+// contiguous floats at stride `plane`: block k of plane z takes the chunk
+// [k * chunk, (k + 1) * chunk) with four loads per thread in flight and a
+// fixed summation order, so the result is deterministic.
+__global__ __launch_bounds__(256) void plane_sums_kernel(const float* a, const float* b, long n, long plane,
+                                                          long chunk, double* partial) {
+  const int z = blockIdx.y;
+  const long lo = (long)blockIdx.x * chunk;
+  const long hi = lo + chunk < n ? lo + chunk : n;
+  const float* pa = a + z * plane;
+  const float* pb = b ? b + z * plane : nullptr;
+  double s[4] = {0.0, 0.0, 0.0, 0.0};
+  for (long i = lo + threadIdx.x; i < hi; i += 4 * 256) {
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const long j = i + u * 256;
+      if (j < hi) s[u] += (double)pa[j] + (pb ? (double)pb[j] : 0.0);
+    }
   }
-  const long blk = (long)z * gridDim.y * gridDim.x + (long)y * gridDim.x + blockIdx.x;
-  block_pair_sum(v, 0.0, partial + 2 * blk);
+  block_pair_sum((s[0] + s[1]) + (s[2] + s[3]), 0.0, partial + 2 * ((long)z * gridDim.x + blockIdx.x));
 }
 
-__global__ __launch_bounds__(256) void mass_alpha_kernel(const double* sums, const double* hmean, Tables t, int P,
-                                                          double inv_count, double* alpha, float* psi_wall) {
+// One block: the fixed-order sum of every plane's nblk partials, then
+// alpha[z] = -sum * inv_count / hmean[z] (0 where hmean[z] == 0) and the wall
+// value of the layer streamfunction psi_wall[first + l] = sum_p cm2l[l][p] *
+// alpha[first + p].
+__global__ __launch_bounds__(256) void mass_alpha_kernel(const double* partial, int nblk, const double* hmean,
+                                                          Tables t, int P, double inv_count, double* alpha,
+                                                          float* psi_wall) {
   for (int z = threadIdx.x; z < P; z += 256) {
+    const double* q = partial + 2L * nblk * z;
+    double sum = 0.0;
+    for (int k = 0; k < nblk; ++k) sum += q[2 * k];
     const double hm = hmean[z];
-    alpha[z] = hm != 0.0 ? -sums[2 * z] * inv_count / hm : 0.0;
+    alpha[z] = hm != 0.0 ? -sum * inv_count / hm : 0.0;
   }
   __syncthreads();
   for (int z = threadIdx.x; z < P; z += 256) {
@@ -903,16 +952,21 @@ __global__ __launch_bounds__(256) void mass_alpha_kernel(const double* sums, con
   }
 }
 
-void launch_plane_sums(const float* a, const float* b, int Nx, int Ny, int P, long plane, double* partial,
-                       double* out, hipStream_t st) {
-  const int bx = (Nx + 255) / 256;
-  hipLaunchKernelGGL(plane_sums_kernel, dim3(bx, Ny, P), dim3(256), 0, st, a, b, Nx, plane, partial);
-  launch_pair_reduce(partial, bx * Ny, P, 1.0, out, st);
+int launch_plane_sums(const float* a, const float* b, int Nx, int Ny, int P, long plane, double* partial,
+                      hipStream_t st) {
+  // at most 64 blocks per plane and never more than Ny (the partial buffer
+  // holds 2 * P * Ny * ceil(Nx / 256) doubles)
+  const long n = (long)Nx * Ny;
+  const int nblk = Ny < 64 ? Ny : 64;
+  const long chunk = (n + nblk - 1) / nblk;
+  hipLaunchKernelGGL(plane_sums_kernel, dim3(nblk, P), dim3(256), 0, st, a, b, n, plane, chunk, partial);
+  return nblk;
 }
 
-void launch_mass_alpha(const double* sums, const double* hmean, Tables t, int P, double inv_count, double* alpha,
-                       float* psi_wall, hipStream_t st) {
-  hipLaunchKernelGGL(mass_alpha_kernel, dim3(1), dim3(256), 0, st, sums, hmean, t, P, inv_count, alpha, psi_wall);
+void launch_mass_alpha(const double* partial, int nblk, const double* hmean, Tables t, int P, double inv_count,
+                       double* alpha, float* psi_wall, hipStream_t st) {
+  hipLaunchKernelGGL(mass_alpha_kernel, dim3(1), dim3(256), 0, st, partial, nblk, hmean, t, P, inv_count, alpha,
+                     psi_wall);
 }
 
 }  // namespace somax
