@@ -167,6 +167,25 @@ int somaxhip_qg_step(somaxhip_qg_plan plan, float* q, float* psi_out, int nsteps
                      const somaxhip_qg_params* params, void* workspace,
                      somaxhip_stream_t stream);
 
+/* Size of the library-owned device array that somaxhip_qg_step_noalloc uses
+ * as its workspace (64 MiB: plans up to 3 layers at 512 x 512). */
+#define SOMAXHIP_NOALLOC_SCRATCH_BYTES (64u << 20)
+
+/* somaxhip_qg_step without a caller workspace.  No counterpart in the
+ * reference: for callers that cannot allocate device memory (code under
+ * hipGraph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * The step's workspace is a static device array of
+ * SOMAXHIP_NOALLOC_SCRATCH_BYTES that each GPU the library runs on holds from
+ * module load until the process ends; the call itself allocates nothing.
+ * Results equal somaxhip_qg_step's bit for bit.  A plan whose
+ * somaxhip_qg_workspace_bytes exceeds the array is refused with
+ * SOMAXHIP_ERR_ARG.  Every call on one GPU shares that GPU's array, which
+ * keeps the last call's contents (the step needs no zeroed workspace): calls
+ * that overlap in time on one GPU -- other streams or other host threads --
+ * race and must be serialised by the caller. */
+int somaxhip_qg_step_noalloc(somaxhip_qg_plan plan, float* q, float* psi_out, int nsteps, double dt,
+                             const somaxhip_qg_params* params, somaxhip_stream_t stream);
+
 /* ------------------------------------------------------------------------ */
 /* Multilayer shallow water                                                 */
 /* ------------------------------------------------------------------------ */

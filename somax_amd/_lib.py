@@ -26,6 +26,7 @@ SOLVER_TSIT5 = 1
 BC_PERIODIC = 0
 BC_WALL = 1
 BC_CHANNEL = 2
+NOALLOC_SCRATCH_BYTES = 64 << 20  # SOMAXHIP_NOALLOC_SCRATCH_BYTES
 STATS_MAX_FIELDS = 64
 
 
@@ -116,6 +117,10 @@ _SIGNATURES = {
     "somaxhip_qg_step": (
         ctypes.c_int,
         [_vp, _vp, _vp, ctypes.c_int, ctypes.c_double, ctypes.POINTER(QGParams), _vp, _vp],
+    ),
+    "somaxhip_qg_step_noalloc": (
+        ctypes.c_int,
+        [_vp, _vp, _vp, ctypes.c_int, ctypes.c_double, ctypes.POINTER(QGParams), _vp],
     ),
     "somaxhip_qg_tsit5_workspace_bytes": (ctypes.c_size_t, [_vp]),
     "somaxhip_qg_step_tsit5": (

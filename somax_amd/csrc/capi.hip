@@ -10,6 +10,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <new>
 #include <string>
 #include <vector>
@@ -393,6 +394,45 @@ struct somaxhip_ens_s {
   }
 };
 
+// ---------------------------------------------------------------------------
+// Step without a caller workspace: the workspace is `scratch`, a device array
+// of SOMAXHIP_NOALLOC_SCRATCH_BYTES that every GPU the library runs on holds
+// from module load until the process ends.  In host code `scratch` names a
+// host-side stand-in; its device address on the current device comes from
+// hipGetSymbolAddress once per device and is the only form passed to kernels.
+// ---------------------------------------------------------------------------
+static __device__ __align__(256) unsigned char scratch[SOMAXHIP_NOALLOC_SCRATCH_BYTES];
+
+namespace {
+
+constexpr int MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+void* g_scratch[MAX_DEVICES] = {};
+
+// Device address of `scratch` on the current device (looked up once per device).
+int scratch_address(void** out, const char* who) {
+  int dev = 0;
+  const hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess || dev < 0 || dev >= MAX_DEVICES) {
+    set_error(std::string(who) + ": no current device (" + hipGetErrorString(e) + ")");
+    return SOMAXHIP_ERR_HIP;
+  }
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch[dev]) {
+    void* ptr = nullptr;
+    const hipError_t es = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (es != hipSuccess || !ptr) {
+      set_error(std::string(who) + ": hipGetSymbolAddress(scratch) failed: " + hipGetErrorString(es));
+      return SOMAXHIP_ERR_HIP;
+    }
+    g_scratch[dev] = ptr;
+  }
+  *out = g_scratch[dev];
+  return SOMAXHIP_OK;
+}
+
+}  // namespace
+
 extern "C" {
 
 size_t somaxhip_last_error(char* buf, size_t len) {
@@ -416,7 +456,8 @@ int somaxhip_init(void) {
     set_error(std::string("somaxhip is built for gfx950; current device is ") + prop.gcnArchName);
     return SOMAXHIP_ERR_DEVICE;
   }
-  return SOMAXHIP_OK;
+  void* ptr = nullptr;  // this device's address of the no-allocation step's scratch array
+  return scratch_address(&ptr, "somaxhip_init");
 }
 
 int somaxhip_qg_plan_create(const somaxhip_qg_desc* d, somaxhip_qg_plan* out) {
@@ -1005,6 +1046,27 @@ int somaxhip_qg_step(somaxhip_qg_plan p, float* q, float* psi_out, int nsteps, d
 }  // extern "C"
 
 extern "C" {
+
+int somaxhip_qg_step_noalloc(somaxhip_qg_plan p, float* q, float* psi_out, int nsteps, double dt,
+                             const somaxhip_qg_params* prm, somaxhip_stream_t stream) {
+  if (!p || !q || !prm || nsteps < 0) {
+    set_error("somaxhip_qg_step_noalloc: null plan, state or params, or negative nsteps");
+    return SOMAXHIP_ERR_ARG;
+  }
+  if (p->b.ws_bytes > (size_t)SOMAXHIP_NOALLOC_SCRATCH_BYTES) {
+    set_error("somaxhip_qg_step_noalloc: the plan needs " + std::to_string(p->b.ws_bytes) +
+              " bytes of workspace, more than the " + std::to_string((size_t)SOMAXHIP_NOALLOC_SCRATCH_BYTES) +
+              " bytes of the library's scratch array; use somaxhip_qg_step with a caller workspace");
+    return SOMAXHIP_ERR_ARG;
+  }
+  void* ws = nullptr;
+  const int rc = scratch_address(&ws, "somaxhip_qg_step_noalloc");
+  if (rc) return rc;
+  hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+  rk3(p->b, q, psi_out, nsteps, dt, uniform_params(prm), ws, st);
+  SOMAX_HIP_CHECK(hipGetLastError());
+  return SOMAXHIP_OK;
+}
 
 int somaxhip_qg_project(somaxhip_qg_plan p, double f0, const double* H, const double* M, const float* h,
                         const float* u, const float* v, int h_deviation, float* ho, float* uo, float* vo,

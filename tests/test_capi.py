@@ -67,3 +67,14 @@ def test_plan_create_rejects_planes_of_2_31_cells(kind):
     assert create(ctypes.byref(d), ctypes.byref(plan)) == ERR_ARG
     assert "2^31" not in _lib.last_error()
 
+
+def test_noalloc_step_checks_arguments_first():
+    """The no-allocation step refuses a null plan before touching a device."""
+    from somax_amd import _lib
+
+    lib = _lib.load()
+    prm = _lib.QGParams(0.0, 0.0, 0.0, 0.0)
+    assert lib.somaxhip_qg_step_noalloc(None, None, None, 1, 1200.0, ctypes.byref(prm), None) == ERR_ARG
+    assert "somaxhip_qg_step_noalloc" in _lib.last_error()
+    header = (ROOT / "include" / "somaxhip.h").read_text()
+    assert f"#define SOMAXHIP_NOALLOC_SCRATCH_BYTES ({_lib.NOALLOC_SCRATCH_BYTES >> 20}u << 20)" in header
