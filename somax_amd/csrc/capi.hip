@@ -185,7 +185,7 @@ struct Batch {
   long splane = 0;  // Ny*Nx
   size_t off_OP1 = 0, off_OP2 = 0, off_QM = 0, off_P0 = 0, off_D = 0, off_Q1 = 0, off_Q2 = 0, off_PSI = 0;
   size_t off_DIAG = 0;  // FP64 partial sums of the diagnostics and the mass constraint
-  size_t off_ALPHA = 0, off_PWALL = 0;  // per-plane alpha, wall psi
+  size_t off_PWALL = 0;  // per-plane wall psi
   size_t ws_bytes = 0;
   int stage_rows = 16;  // Arakawa stage tile height (8 or 16), chosen per class
   // mass correction: homogeneous solutions (P planes) and their means (0: none)
@@ -331,7 +331,6 @@ static hipError_t batch_build(Batch& b, const std::vector<const somaxhip_qg_desc
   b.off_Q2 = take((size_t)P * b.splane * sizeof(float));
   b.off_PSI = take((size_t)P * b.splane * sizeof(float));
   b.off_DIAG = take(qg_diag_partial_doubles(b.Nx, b.Ny, P) * sizeof(double));
-  b.off_ALPHA = take((size_t)P * sizeof(double));
   b.off_PWALL = take((size_t)P * sizeof(float));
   b.ws_bytes = off;
   return prepare_dst_gemm();
@@ -848,17 +847,18 @@ void invert(const Batch& p, const float* q, float* psi, void* ws, hipStream_t st
   if (p.any_mass) {
     // FP64 domain mean of every modal streamfunction, alpha = -mean / mean(homsol)
     double* partial = reinterpret_cast<double*>(static_cast<char*>(ws) + p.off_DIAG);
-    double* alpha = reinterpret_cast<double*>(static_cast<char*>(ws) + p.off_ALPHA);
     {
       ProfScope ps("mass_constraint", 0.0, (mixed ? 8.0 : 4.0) * p.P * (double)p.splane, st);
       // folded: the array sum is 4x the sum of the even-even block (planes 0..P-1)
-      const int nblk = launch_plane_sums(P0, la.d, p.fold ? p.sx.h : p.Nx, p.fold ? p.sy.h : p.Ny, p.P,
-                                         p.fold ? p.fplane : p.splane, partial, st);
-      launch_mass_alpha(partial, nblk, p.hmean, p.t, p.P, (p.fold ? 4.0 : 1.0) / (double)p.splane, alpha,
-                        wsf(ws, p.off_PWALL), st);
+      la.nblk = launch_plane_sums(P0, la.d, p.fold ? p.sx.h : p.Nx, p.fold ? p.sy.h : p.Ny, p.P,
+                                  p.fold ? p.fplane : p.splane, partial, st);
     }
+    // to_layer forms alpha = -mean / mean(homsol) from the partials and the wall psi
     la.homsol = p.homsol;
-    la.alpha = alpha;
+    la.partial = partial;
+    la.hmean = p.hmean;
+    la.inv_count = (p.fold ? 4.0 : 1.0) / (double)p.splane;
+    la.psi_wall = wsf(ws, p.off_PWALL);
   }
   {
     ProfScope ps("to_layer", 0.0, (mixed ? 12.0 : 8.0) * p.P * (double)p.splane, st);

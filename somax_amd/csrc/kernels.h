@@ -197,9 +197,18 @@ struct LayerArgs {
   int Nx, Ny, M;
   long plane;
   Tables t;
-  // mass correction (nullable): pm += alpha[p] * homsol[p] before the projection
+  // mass correction (homsol nullable): pm += alpha[p] * homsol[p] before the
+  // projection, with alpha[p] = -sum[p] * inv_count / hmean[p] (0 where
+  // hmean[p] == 0) and sum[p] the fixed-order sum of plane p's nblk FP64
+  // partials (pairs, first element, from launch_plane_sums); each block forms
+  // the alphas of its member, and the member's first block writes the wall
+  // value psi_wall[first + l] = sum_p cm2l[l][p] * alpha[first + p]
   const float* homsol;
-  const double* alpha;
+  const double* partial;
+  int nblk;
+  const double* hmean;
+  double inv_count;
+  float* psi_wall;
   // folded (fold != 0): pm and d hold parity components (block b = 2p + q of
   // table plane m at plane b * P + m, Ny/2 x Nx/2, pitch Nx/2); the physical
   // value at (mirror^sy(y), mirror^sx(x)) is sum_b (-1)^(p sy + q sx) pm_b.
@@ -322,13 +331,8 @@ void launch_pair_reduce(const double* partial, int n, int P, double scale, doubl
 
 // Mass correction of the modal streamfunction (homogeneous solution method):
 // per plane FP64 block partials of the sum of (a + b) over the full array (b
-// nullable), then in one block their fixed-order sum and
-//   alpha[p] = -sum[p] * inv_count / hmean[p]   (0 where hmean[p] == 0)
-// and the wall value of the layer streamfunction psi_wall[first + l] =
-// sum_p cm2l[l][p] * alpha[first + p] for the WENO corner streamfunction.
+// nullable); launch_to_layer turns them into the correction (LayerArgs).
 int launch_plane_sums(const float* a, const float* b, int Nx, int Ny, int P, long plane, double* partial,
-                      hipStream_t st);  // returns the partials per plane
-void launch_mass_alpha(const double* partial, int nblk, const double* hmean, Tables t, int P, double inv_count,
-                       double* alpha, float* psi_wall, hipStream_t st);
+                      hipStream_t st);  // returns the partials per plane (at most 64)
 
 }  // namespace somax

@@ -688,14 +688,49 @@ void launch_to_modal(const ModalArgs& a, hipStream_t st) {
 // ---------------------------------------------------------------------------
 // mode -> layer projection of psi
 // ---------------------------------------------------------------------------
+// Mass correction of a block's member: the alphas of its nl modes from the
+// FP64 partial sums (each mode's partials summed in order by one thread, so
+// every block and every run gets the same bits), into s_alpha; the member's
+// first block also writes the wall value of every layer.  Every thread of the
+// block calls this before any returns.
+__device__ inline void member_alpha(const LayerArgs& a, int first, int nl, const float* cm2l, bool write_wall,
+                                    float* s_alpha) {
+  __shared__ double sp[MAX_LAYERS][64];
+  __shared__ double sal[MAX_LAYERS];
+  const int t = threadIdx.x;
+  for (int i = t; i < nl * a.nblk; i += blockDim.x) {
+    const int p = i / a.nblk, k = i - p * a.nblk;
+    sp[p][k] = a.partial[2L * ((long)(first + p) * a.nblk + k)];
+  }
+  __syncthreads();
+  if (t < nl) {
+    double sum = 0.0;
+    for (int k = 0; k < a.nblk; ++k) sum += sp[t][k];
+    const double hm = a.hmean[first + t];
+    sal[t] = hm != 0.0 ? -sum * a.inv_count / hm : 0.0;
+  }
+  __syncthreads();
+  if (t < nl) {
+    s_alpha[t] = (float)sal[t];
+    if (write_wall) {
+      double w = 0.0;
+      for (int p = 0; p < nl; ++p) w += (double)cm2l[t * MAX_LAYERS + p] * sal[p];
+      a.psi_wall[first + t] = (float)w;
+    }
+  }
+  __syncthreads();
+}
+
 template <int NLM>
 __global__ __launch_bounds__(256) void to_layer_kernel(LayerArgs a) {
   const int x = blockIdx.x * blockDim.x + threadIdx.x;
   const int y = blockIdx.y;
   const int mem = blockIdx.z;
-  if (x >= a.Nx) return;
   const int first = a.t.mfirst[mem], nl = a.t.mnl[mem];
   const float* cm2l = a.t.cm2l + mem * MAX_LAYERS * MAX_LAYERS;
+  __shared__ float s_alpha[MAX_LAYERS];
+  if (a.homsol) member_alpha(a, first, nl, cm2l, blockIdx.x == 0 && y == 0, s_alpha);
+  if (x >= a.Nx) return;
   const int s = y * a.Nx + x;  // in-plane offset (a plane is below 2^31 cells)
   const bool ghost = a.t.mask[mem] && (x == 0 || y == 0 || x == a.Nx - 1 || y == a.Ny - 1);
   // loads first (layers past nl re-read the member's first layer, unused)
@@ -713,7 +748,7 @@ __global__ __launch_bounds__(256) void to_layer_kernel(LayerArgs a) {
     pm[p] = 0.0f;
     if (p < nl) {
       pm[p] = zp[p] + (a.d ? zd[p] : 0.0f);
-      if (a.homsol) pm[p] += (float)a.alpha[first + p] * zh[p];
+      if (a.homsol) pm[p] += s_alpha[p] * zh[p];
     }
   }
 #pragma unroll
@@ -735,10 +770,12 @@ __global__ __launch_bounds__(256) void to_layer_fold_kernel(LayerArgs a) {
   const int hx = a.Nx / 2, hy = a.Ny / 2;
   const int e = blockIdx.x * blockDim.x + threadIdx.x;  // row-major over hy x hx
   const int mem = blockIdx.y;
-  if (e >= hy * hx) return;
-  const int ky = e / hx, kx = e - ky * hx;
   const int first = a.t.mfirst[mem], nl = a.t.mnl[mem];
   const float* cm2l = a.t.cm2l + mem * MAX_LAYERS * MAX_LAYERS;
+  __shared__ float s_alpha[MAX_LAYERS];
+  if (a.homsol) member_alpha(a, first, nl, cm2l, blockIdx.x == 0, s_alpha);
+  if (e >= hy * hx) return;
+  const int ky = e / hx, kx = e - ky * hx;
   const int y0 = fold_phys(ky, hy), y1 = a.Ny - 1 - y0;
   const int x0 = fold_phys(kx, hx), x1 = a.Nx - 1 - x0;
   // in-plane offsets (a plane is below 2^31 cells)
@@ -774,7 +811,7 @@ __global__ __launch_bounds__(256) void to_layer_fold_kernel(LayerArgs a) {
       pm[p][2] = (z[0] + z[1]) - (z[2] + z[3]);
       pm[p][3] = (z[0] - z[1]) - (z[2] - z[3]);
       if (a.homsol) {
-        const float al = (float)a.alpha[first + p];
+        const float al = s_alpha[p];
 #pragma unroll
         for (int c = 0; c < 4; ++c) pm[p][c] += al * zh[p][c];
       }
@@ -928,30 +965,6 @@ __global__ __launch_bounds__(256) void plane_sums_kernel(const float* a, const f
   block_pair_sum((s[0] + s[1]) + (s[2] + s[3]), 0.0, partial + 2 * ((long)z * gridDim.x + blockIdx.x));
 }
 
-// One block: the fixed-order sum of every plane's nblk partials, then
-// alpha[z] = -sum * inv_count / hmean[z] (0 where hmean[z] == 0) and the wall
-// value of the layer streamfunction psi_wall[first + l] = sum_p cm2l[l][p] *
-// alpha[first + p].
-__global__ __launch_bounds__(256) void mass_alpha_kernel(const double* partial, int nblk, const double* hmean,
-                                                          Tables t, int P, double inv_count, double* alpha,
-                                                          float* psi_wall) {
-  for (int z = threadIdx.x; z < P; z += 256) {
-    const double* q = partial + 2L * nblk * z;
-    double sum = 0.0;
-    for (int k = 0; k < nblk; ++k) sum += q[2 * k];
-    const double hm = hmean[z];
-    alpha[z] = hm != 0.0 ? -sum * inv_count / hm : 0.0;
-  }
-  __syncthreads();
-  for (int z = threadIdx.x; z < P; z += 256) {
-    const int m = t.member[z], l = t.layer[z];
-    const int first = t.mfirst[m], nl = t.mnl[m];
-    double w = 0.0;
-    for (int p = 0; p < nl; ++p) w += (double)t.cm2l[m * MAX_LAYERS * MAX_LAYERS + l * MAX_LAYERS + p] * alpha[first + p];
-    psi_wall[z] = (float)w;
-  }
-}
-
 int launch_plane_sums(const float* a, const float* b, int Nx, int Ny, int P, long plane, double* partial,
                       hipStream_t st) {
   // at most 64 blocks per plane and never more than Ny (the partial buffer
@@ -963,11 +976,6 @@ int launch_plane_sums(const float* a, const float* b, int Nx, int Ny, int P, lon
   return nblk;
 }
 
-void launch_mass_alpha(const double* partial, int nblk, const double* hmean, Tables t, int P, double inv_count,
-                       double* alpha, float* psi_wall, hipStream_t st) {
-  hipLaunchKernelGGL(mass_alpha_kernel, dim3(1), dim3(256), 0, st, partial, nblk, hmean, t, P, inv_count, alpha,
-                     psi_wall);
-}
 
 }  // namespace somax
 
