@@ -218,14 +218,17 @@ __device__ inline float weno3_corr(float d2, float d3) {
 // increment q_{f+k} - q_{f+k-1} for k = -1..3 (dm index k+1), i.e. the
 // increments from cell f-2 up to cell f+3.  vel > 0: upwind cell f.
 __device__ inline float face_corr(int f, int n, float vel, const float* dm) {
-  if (vel > 0.0f) {
-    if (f - 2 >= 1 && f + 2 <= n) return weno5_corr(dm[0], dm[1], dm[2], dm[3]);
-    if (f - 1 >= 1 && f + 1 <= n) return weno3_corr(dm[1], dm[2]);
-    return 0.0f;
-  }
-  const int g = f + 1;  // mirror image around cell f+1
-  if (g - 2 >= 1 && g + 2 <= n) return weno5_corr(-dm[4], -dm[3], -dm[2], -dm[1]);
-  if (g - 1 >= 1 && g + 1 <= n) return weno3_corr(-dm[3], -dm[2]);
+  // the upwind side by selects rather than branches, so the lanes of a wave
+  // with velocities of both signs share one reconstruction (mirror image
+  // around cell f+1 when vel <= 0)
+  const bool pos = vel > 0.0f;
+  const int c = pos ? f : f + 1;  // upwind cell
+  const float e1 = pos ? dm[0] : -dm[4];
+  const float e2 = pos ? dm[1] : -dm[3];
+  const float e3 = pos ? dm[2] : -dm[2];
+  const float e4 = pos ? dm[3] : -dm[1];
+  if (c - 2 >= 1 && c + 2 <= n) return weno5_corr(e1, e2, e3, e4);
+  if (c - 1 >= 1 && c + 1 <= n) return weno3_corr(e2, e3);
   return 0.0f;
 }
 }  // namespace
@@ -234,7 +237,6 @@ template <bool BIHARM>
 __global__ __launch_bounds__(256) void qg_stage_weno_kernel(StageArgs a) {
   __shared__ float sq[(WY + 6) * WQ];           // q (BC), halo 3
   __shared__ float sp[(WY + 2) * (WX + 2)];     // psi, halo 1
-  __shared__ float sc[(WY + 1) * (WX + 1)];     // corner psi for corners (y0-1.., x0-1..)
   __shared__ float sfx[WY * (WX + 1)];          // x-face fluxes u*qface
   __shared__ float sfy[(WY + 1) * WX];          // y-face fluxes v*(qface - b_row)
   __shared__ float sl[BIHARM ? (WY + 2) * (WX + 2) : 1];
@@ -311,18 +313,17 @@ __global__ __launch_bounds__(256) void qg_stage_weno_kernel(StageArgs a) {
   if (tid < WY + 6) sdb[tid] = vdb;
   __syncthreads();
 
-  // corners: corner (cy, cx) lies between T rows cy, cy+1 and columns cx, cx+1;
-  // on the boundary lines of the interior block it takes the wall value
-  for (int i = tid; i < (WY + 1) * (WX + 1); i += 256) {
-    const int ly = i / (WX + 1), lx = i - ly * (WX + 1);
-    const int cy = y0 - 1 + ly, cx = x0 - 1 + lx;
+  // psi at corner (cy, cx), between T rows cy, cy+1 and columns cx, cx+1
+  // (cy in y0-1 .. y0+WY-1, cx in x0-1 .. x0+WX-1): the 4-point mean inside
+  // the interior block, the wall value on its boundary lines
+  auto corner = [&](int cy, int cx) {
     float v = pw;
     if (cy >= 1 && cy <= Ny - 3 && cx >= 1 && cx <= Nx - 3) {
-      const float* p = &sp[ly * PW2 + lx];  // T (cy, cx) at local (ly, lx) of the psi tile
+      const float* p = &sp[(cy - y0 + 1) * PW2 + (cx - x0 + 1)];
       v = 0.25f * (p[0] + p[1] + p[PW2] + p[PW2 + 1]);
     }
-    sc[i] = v;
-  }
+    return v;
+  };
   if (BIHARM) {
     for (int i = tid; i < (WY + 2) * PW2; i += 256) {
       const int ly = i / PW2, lx = i - ly * PW2;
@@ -335,7 +336,6 @@ __global__ __launch_bounds__(256) void qg_stage_weno_kernel(StageArgs a) {
       sl[i] = v;
     }
   }
-  __syncthreads();
 
   // x-face fluxes: face between T columns f and f+1 on row gy, f = x0-1 .. x0+WX-1
   for (int i = tid; i < WY * (WX + 1); i += 256) {
@@ -343,8 +343,7 @@ __global__ __launch_bounds__(256) void qg_stage_weno_kernel(StageArgs a) {
     const int gy = y0 + ly, f = x0 - 1 + lf;
     float flux = 0.0f;
     if (gy >= 1 && gy <= nyi && f >= 1 && f <= nxi - 1) {
-      // u = -(C(gy, f) - C(gy-1, f)) / dy ; corner (gy, f) at local (ly+1, lf)
-      const float u = -(sc[(ly + 1) * (WX + 1) + lf] - sc[ly * (WX + 1) + lf]) * a.idy;
+      const float u = -(corner(gy, f) - corner(gy - 1, f)) * a.idy;
       const float* qr = &sq[(ly + 3) * WQ + (f - x0 + 3)];  // cell f
       float dm[5];
 #pragma unroll
@@ -361,8 +360,7 @@ __global__ __launch_bounds__(256) void qg_stage_weno_kernel(StageArgs a) {
     const int g = y0 - 1 + lg, gx = x0 + lx;
     float flux = 0.0f;
     if (gx >= 1 && gx <= nxi && g >= 1 && g <= nyi - 1) {
-      // v = (C(g, gx) - C(g, gx-1)) / dx ; corner (g, gx) at local (lg, lx+1)
-      const float v = (sc[lg * (WX + 1) + lx + 1] - sc[lg * (WX + 1) + lx]) * a.idx;
+      const float v = (corner(g, gx) - corner(g, gx - 1)) * a.idx;
       const int lq = g - y0 + 3;  // local q row of cell g
       float dm[5];
 #pragma unroll
@@ -393,7 +391,7 @@ __global__ __launch_bounds__(256) void qg_stage_weno_kernel(StageArgs a) {
       // the upper face is relative to b_gy already, the lower one to b_{gy-1},
       // so its b offset db_{gy-1} times v_s completes the b_gy reference.
       const float gn = sfy[(ly + 1) * WX + tx], gs = sfy[ly * WX + tx];
-      const float vs = (sc[ly * (WX + 1) + tx + 1] - sc[ly * (WX + 1) + tx]) * a.idx;
+      const float vs = (corner(gy - 1, gx) - corner(gy - 1, gx - 1)) * a.idx;
       const float gs_rel = gs - vs * sdb[ly + 2];
       rhs = -((fe - fw) * a.idx + (gn - gs_rel) * a.idy);
       const float* q = &sq[(ly + 3) * WQ + tx + 3];
@@ -945,24 +943,41 @@ namespace somax {
 // ---------------------------------------------------------------------------
 // Per-block FP64 partial sums of a[i] + b[i] (b nullable) over planes of n
 // contiguous floats at stride `plane`: block k of plane z takes the chunk
-// [k * chunk, (k + 1) * chunk) with four loads per thread in flight and a
-// fixed summation order, so the result is deterministic.
-__global__ __launch_bounds__(256) void plane_sums_kernel(const float* a, const float* b, long n, long plane,
-                                                          long chunk, double* partial) {
+// [k * chunk, (k + 1) * chunk), 1024 threads with four loads each in flight,
+// and a fixed summation order, so the result is deterministic.
+constexpr int SUM_THREADS = 1024;
+__global__ __launch_bounds__(SUM_THREADS) void plane_sums_kernel(const float* a, const float* b, long n, long plane,
+                                                                  long chunk, double* partial) {
+  __shared__ double red[SUM_THREADS];
   const int z = blockIdx.y;
+  const int t = threadIdx.x;
   const long lo = (long)blockIdx.x * chunk;
   const long hi = lo + chunk < n ? lo + chunk : n;
   const float* pa = a + z * plane;
   const float* pb = b ? b + z * plane : nullptr;
   double s[4] = {0.0, 0.0, 0.0, 0.0};
-  for (long i = lo + threadIdx.x; i < hi; i += 4 * 256) {
+  for (long i = lo + t; i < hi; i += 4 * SUM_THREADS) {
+    float va[4], vb[4];
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
-      const long j = i + u * 256;
-      if (j < hi) s[u] += (double)pa[j] + (pb ? (double)pb[j] : 0.0);
+      const long j = i + u * SUM_THREADS;
+      va[u] = j < hi ? pa[j] : 0.0f;
+      vb[u] = (pb && j < hi) ? pb[j] : 0.0f;
     }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) s[u] += (double)va[u] + (double)vb[u];
   }
-  block_pair_sum((s[0] + s[1]) + (s[2] + s[3]), 0.0, partial + 2 * ((long)z * gridDim.x + blockIdx.x));
+  red[t] = (s[0] + s[1]) + (s[2] + s[3]);
+  __syncthreads();
+  for (int w = SUM_THREADS / 2; w > 0; w >>= 1) {
+    if (t < w) red[t] += red[t + w];
+    __syncthreads();
+  }
+  if (t == 0) {
+    double* dst = partial + 2 * ((long)z * gridDim.x + blockIdx.x);
+    dst[0] = red[0];
+    dst[1] = 0.0;
+  }
 }
 
 int launch_plane_sums(const float* a, const float* b, int Nx, int Ny, int P, long plane, double* partial,
@@ -972,7 +987,7 @@ int launch_plane_sums(const float* a, const float* b, int Nx, int Ny, int P, lon
   const long n = (long)Nx * Ny;
   const int nblk = Ny < 64 ? Ny : 64;
   const long chunk = (n + nblk - 1) / nblk;
-  hipLaunchKernelGGL(plane_sums_kernel, dim3(nblk, P), dim3(256), 0, st, a, b, n, plane, chunk, partial);
+  hipLaunchKernelGGL(plane_sums_kernel, dim3(nblk, P), dim3(SUM_THREADS), 0, st, a, b, n, plane, chunk, partial);
   return nblk;
 }
 
