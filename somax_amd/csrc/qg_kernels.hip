@@ -235,12 +235,16 @@ __device__ inline float face_corr(int f, int n, float vel, const float* dm) {
 
 template <bool BIHARM>
 __global__ __launch_bounds__(256) void qg_stage_weno_kernel(StageArgs a) {
-  __shared__ float sq[(WY + 6) * WQ];           // q (BC), halo 3
-  __shared__ float sp[(WY + 2) * (WX + 2)];     // psi, halo 1
+  constexpr int QH = WY + 6;    // q tile rows (halo 3)
+  constexpr int PH = WY + 2;    // psi / Laplacian tile rows (halo 1)
+  constexpr int PW2 = WX + 2;   // psi / Laplacian tile pitch
+  __shared__ float sq[QH * WQ];                 // q (BC), halo 3
+  __shared__ float sp[PH * PW2];                // psi, halo 1
   __shared__ float sfx[WY * (WX + 1)];          // x-face fluxes u*qface
-  __shared__ float sfy[(WY + 1) * WX];          // y-face fluxes v*(qface - b_row)
-  __shared__ float sl[BIHARM ? (WY + 2) * (WX + 2) : 1];
-  __shared__ float sdb[WY + 6];                 // b[j+1] - b[j] for rows y0-3 ..
+  __shared__ float sfl[(WY + 1) * WX];          // y-face fluxes v*(qface - b of the lower cell)
+  __shared__ float sfh[(WY + 1) * WX];          // the same relative to b of the upper cell
+  __shared__ float sl[BIHARM ? PH * PW2 : 1];
+  __shared__ float sdb[QH];                     // b[j+1] - b[j] for rows y0-3 ..
 
   const int tid = threadIdx.x;
   int bx, by, z;
@@ -257,90 +261,106 @@ __global__ __launch_bounds__(256) void qg_stage_weno_kernel(StageArgs a) {
   const float* dbeta_row = a.dbeta_rows + (long)m * Ny;
   const float* wind_row = a.wind_rows + (long)m * Ny;
   const float pw = a.t.psi_wall ? a.t.psi_wall[z] : 0.0f;
-  constexpr int PW2 = WX + 2;
+  // Lane tx owns column gx = x0 + tx of every tile; wave wv owns rows
+  // wv, wv + 4, ...  Row indices and row tests are therefore wave-uniform
+  // (scalar), and the only per-lane index is the column.
   const int tx = tid & 63;
-  const int tyb = tid >> 6;
+  const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int gx = x0 + tx;
+  const bool gx_in = gx < Nx;
+  const bool gx_ring = gx == 0 || gx == Nx - 1;
+  const bool bc = a.apply_bc != 0;
 
-  // This thread's outputs (column tx, rows tyb + 4 r): the step-start value,
-  // the wind row and (on the ghost ring under the boundary condition, where
-  // the tile holds 0) the raw stage value are fetched first, so their latency
-  // overlaps the tile loads.
+  // This thread's outputs (rows wv + 4 r): the step-start value, the wind row
+  // and (on the ghost ring under the boundary condition, where the tile holds
+  // 0) the raw stage value are fetched first, so their latency overlaps the
+  // tile loads.
   float qn_r[WY / 4], wind_r[WY / 4], qg_r[WY / 4];
 #pragma unroll
   for (int r = 0; r < WY / 4; ++r) {
-    const int gy = y0 + tyb + 4 * r, gx = x0 + tx;
-    const bool ok = gy < Ny && gx < Nx;
-    const bool gring = gy == 0 || gx == 0 || gy == Ny - 1 || gx == Nx - 1;
+    const int gy = y0 + wv + 4 * r;
+    const bool ok = gy < Ny && gx_in;
+    const bool gring = gy == 0 || gy == Ny - 1 || gx_ring;
     const int idx = gy * Nx + gx;  // plane < 2^31 cells
     qn_r[r] = (ok && a.qn) ? a.qn[off + idx] : 0.0f;
     wind_r[r] = ok ? wind_row[gy] : 0.0f;
-    qg_r[r] = (ok && gring && a.apply_bc) ? qs[idx] : 0.0f;
+    qg_r[r] = (ok && gring && bc) ? qs[idx] : 0.0f;
   }
-  // Tile loads: a fixed number per thread, all issued before the first LDS
-  // store, so one memory latency covers them.  q with the boundary condition
-  // (ghost ring and outside the array read 0), halo 3; psi with halo 1.
-  constexpr int NQ = ((WY + 6) * WQ + 255) / 256;
-  constexpr int NPS = ((WY + 2) * PW2 + 255) / 256;
-  float vq[NQ], vp[NPS];
+  // Tile loads, all issued before the first LDS store so that one memory
+  // latency covers them.  q with the boundary condition (ghost ring and
+  // outside the array read 0), halo 3: the lane's own column for the wave's
+  // rows, then the six halo columns of every row by threads 0..6*QH-1.  psi
+  // with halo 1 the same way (two halo columns).
+  constexpr int NQM = (QH + 3) / 4;
+  constexpr int NPM = (PH + 3) / 4;
+  float vq[NQM], vp[NPM];
 #pragma unroll
-  for (int k = 0; k < NQ; ++k) {
-    const int i = tid + 256 * k;
-    const int ly = i / WQ, lx = i - ly * WQ;
-    const int gy = y0 - 3 + ly, gx = x0 - 3 + lx;
-    const bool inside = i < (WY + 6) * WQ && gy >= 0 && gy < Ny && gx >= 0 && gx < Nx;
-    const bool ghost = gy == 0 || gx == 0 || gy == Ny - 1 || gx == Nx - 1;
-    vq[k] = (inside && !(ghost && a.apply_bc)) ? qs[gy * Nx + gx] : 0.0f;
+  for (int k = 0; k < NQM; ++k) {
+    const int gy = y0 - 3 + wv + 4 * k;
+    const bool row_ok = wv + 4 * k < QH && gy >= 0 && gy < Ny;
+    const bool ghost = gy == 0 || gy == Ny - 1 || gx_ring;
+    vq[k] = (row_ok && gx_in && !(ghost && bc)) ? qs[gy * Nx + gx] : 0.0f;
   }
 #pragma unroll
-  for (int k = 0; k < NPS; ++k) {
-    const int i = tid + 256 * k;
-    const int ly = i / PW2, lx = i - ly * PW2;
-    const int gy = y0 - 1 + ly, gx = x0 - 1 + lx;
-    const bool inside = i < (WY + 2) * PW2 && gy >= 0 && gy < Ny && gx >= 0 && gx < Nx;
-    vp[k] = inside ? psi[gy * Nx + gx] : 0.0f;
+  for (int k = 0; k < NPM; ++k) {
+    const int gy = y0 - 1 + wv + 4 * k;
+    const bool row_ok = wv + 4 * k < PH && gy >= 0 && gy < Ny;
+    vp[k] = (row_ok && gx_in) ? psi[gy * Nx + gx] : 0.0f;
   }
-  float vdb = 0.0f;
-  if (tid < WY + 6) {
+  float vqe = 0.0f, vpe = 0.0f, vdb = 0.0f;
+  int qe_at = -1, pe_at = -1;
+  if (tid < 6 * QH) {
+    const int ly = tid / 6, e = tid - 6 * ly;
+    const int lx = e < 3 ? e : WX + e;  // tile columns 0..2 and WX+3..WX+5
+    const int gy = y0 - 3 + ly, hx = x0 - 3 + lx;
+    const bool inside = gy >= 0 && gy < Ny && hx >= 0 && hx < Nx;
+    const bool ghost = gy == 0 || hx == 0 || gy == Ny - 1 || hx == Nx - 1;
+    vqe = (inside && !(ghost && bc)) ? qs[gy * Nx + hx] : 0.0f;
+    qe_at = ly * WQ + lx;
+  }
+  if (tid < 2 * PH) {
+    const int ly = tid >> 1;
+    const int lx = (tid & 1) ? WX + 1 : 0;
+    const int gy = y0 - 1 + ly, hx = x0 - 1 + lx;
+    const bool inside = gy >= 0 && gy < Ny && hx >= 0 && hx < Nx;
+    vpe = inside ? psi[gy * Nx + hx] : 0.0f;
+    pe_at = ly * PW2 + lx;
+  }
+  if (tid < QH) {
     const int gy = y0 - 3 + tid;
     vdb = (gy >= 0 && gy < Ny) ? dbeta_row[gy] : 0.0f;
   }
 #pragma unroll
-  for (int k = 0; k < NQ; ++k)
-    if (tid + 256 * k < (WY + 6) * WQ) sq[tid + 256 * k] = vq[k];
+  for (int k = 0; k < NQM; ++k)
+    if (wv + 4 * k < QH) sq[(wv + 4 * k) * WQ + tx + 3] = vq[k];
 #pragma unroll
-  for (int k = 0; k < NPS; ++k)
-    if (tid + 256 * k < (WY + 2) * PW2) sp[tid + 256 * k] = vp[k];
-  if (tid < WY + 6) sdb[tid] = vdb;
+  for (int k = 0; k < NPM; ++k)
+    if (wv + 4 * k < PH) sp[(wv + 4 * k) * PW2 + tx + 1] = vp[k];
+  if (qe_at >= 0) sq[qe_at] = vqe;
+  if (pe_at >= 0) sp[pe_at] = vpe;
+  if (tid < QH) sdb[tid] = vdb;
   __syncthreads();
 
   // psi at corner (cy, cx), between T rows cy, cy+1 and columns cx, cx+1
-  // (cy in y0-1 .. y0+WY-1, cx in x0-1 .. x0+WX-1): the 4-point mean inside
-  // the interior block, the wall value on its boundary lines
+  // (cy in y0-1 .. y0+WY-1, cx in x0-1 .. x0+WX-1, always inside the tile):
+  // the 4-point mean inside the interior block, the wall value on its
+  // boundary lines
   auto corner = [&](int cy, int cx) {
-    float v = pw;
-    if (cy >= 1 && cy <= Ny - 3 && cx >= 1 && cx <= Nx - 3) {
-      const float* p = &sp[(cy - y0 + 1) * PW2 + (cx - x0 + 1)];
-      v = 0.25f * (p[0] + p[1] + p[PW2] + p[PW2 + 1]);
-    }
-    return v;
+    const float* p = &sp[(cy - y0 + 1) * PW2 + (cx - x0 + 1)];
+    const float v = 0.25f * (p[0] + p[1] + p[PW2] + p[PW2 + 1]);
+    return (cy >= 1 && cy <= Ny - 3 && cx >= 1 && cx <= Nx - 3) ? v : pw;
   };
-  if (BIHARM) {
-    for (int i = tid; i < (WY + 2) * PW2; i += 256) {
-      const int ly = i / PW2, lx = i - ly * PW2;
-      const int gy = y0 - 1 + ly, gx = x0 - 1 + lx;
-      float v = 0.0f;
-      if (gy >= 1 && gy <= Ny - 2 && gx >= 1 && gx <= Nx - 2) {
-        const float* c = &sq[(ly + 2) * WQ + lx + 2];
-        v = (c[1] - 2.0f * c[0] + c[-1]) * a.idx2 + (c[WQ] - 2.0f * c[0] + c[-WQ]) * a.idy2;
-      }
-      sl[i] = v;
-    }
-  }
-
-  // x-face fluxes: face between T columns f and f+1 on row gy, f = x0-1 .. x0+WX-1
-  for (int i = tid; i < WY * (WX + 1); i += 256) {
-    const int ly = i / (WX + 1), lf = i - ly * (WX + 1);
-    const int gy = y0 + ly, f = x0 - 1 + lf;
+  // inner Laplacian of q at tile position (ly, lx) of the 1-halo region
+  // (gy = y0-1+ly, hx = x0-1+lx); its ghost ring is 0
+  auto inner_lap = [&](int ly, int lx) {
+    const int gy = y0 - 1 + ly, hx = x0 - 1 + lx;
+    const float* c = &sq[(ly + 2) * WQ + lx + 2];
+    const float v = (c[1] - 2.0f * c[0] + c[-1]) * a.idx2 + (c[WQ] - 2.0f * c[0] + c[-WQ]) * a.idy2;
+    return (gy >= 1 && gy <= Ny - 2 && hx >= 1 && hx <= Nx - 2) ? v : 0.0f;
+  };
+  // flux through the x-face between T columns f and f+1 on row gy = y0+ly
+  auto xface = [&](int ly, int f) {
+    const int gy = y0 + ly;
     float flux = 0.0f;
     if (gy >= 1 && gy <= nyi && f >= 1 && f <= nxi - 1) {
       const float u = -(corner(gy, f) - corner(gy - 1, f)) * a.idy;
@@ -351,49 +371,68 @@ __global__ __launch_bounds__(256) void qg_stage_weno_kernel(StageArgs a) {
       const float qup = u > 0.0f ? qr[0] : qr[1];
       flux = u * (qup + face_corr(f, nxi, u, dm));
     }
-    sfx[i] = flux;
-  }
-  // y-face fluxes: face between T rows g and g+1 on column gx, g = y0-1 .. y0+WY-1;
-  // stored as v * (face value of q + b minus b of the lower cell g)
-  for (int i = tid; i < (WY + 1) * WX; i += 256) {
-    const int lg = i / WX, lx = i - lg * WX;
-    const int g = y0 - 1 + lg, gx = x0 + lx;
+    return flux;
+  };
+  // y-face between T rows g and g+1 on column gx (g = y0-1 .. y0+WY-1):
+  // v * (face value of q + b minus b of the lower cell g) into sfl, and the
+  // same minus b of the upper cell into sfh
+  auto yface = [&](int g) {
+    const float v = (corner(g, gx) - corner(g, gx - 1)) * a.idx;
+    const int lq = g - y0 + 3;  // local q row of cell g
     float flux = 0.0f;
     if (gx >= 1 && gx <= nxi && g >= 1 && g <= nyi - 1) {
-      const float v = (corner(g, gx) - corner(g, gx - 1)) * a.idx;
-      const int lq = g - y0 + 3;  // local q row of cell g
       float dm[5];
 #pragma unroll
       for (int k = 0; k < 5; ++k) {
         // increment of q + b from cell g+k-2 to g+k-1
         const int r = lq + k - 1;
-        dm[k] = (sq[r * WQ + lx + 3] - sq[(r - 1) * WQ + lx + 3]) + sdb[r - 1];
+        dm[k] = (sq[r * WQ + tx + 3] - sq[(r - 1) * WQ + tx + 3]) + sdb[r - 1];
       }
       // face value of q+b minus b_g: upwind g -> q_g + corr ; upwind g+1 -> q_{g+1} + db_g + corr
-      const float base = v > 0.0f ? sq[lq * WQ + lx + 3] : sq[(lq + 1) * WQ + lx + 3] + sdb[lq];
+      const float base = v > 0.0f ? sq[lq * WQ + tx + 3] : sq[(lq + 1) * WQ + tx + 3] + sdb[lq];
       flux = v * (base + face_corr(g, nyi, v, dm));
     }
-    sfy[i] = flux;
+    const int lg = g - y0 + 1;
+    sfl[lg * WX + tx] = flux;
+    sfh[lg * WX + tx] = flux - v * sdb[lq];
+  };
+
+  if (BIHARM) {
+#pragma unroll
+    for (int k = 0; k < NPM; ++k)
+      if (wv + 4 * k < PH) sl[(wv + 4 * k) * PW2 + tx + 1] = inner_lap(wv + 4 * k, tx + 1);
+    if (tid < 2 * PH) {
+      const int ly = tid >> 1;
+      const int lx = (tid & 1) ? WX + 1 : 0;
+      sl[ly * PW2 + lx] = inner_lap(ly, lx);
+    }
   }
+  // faces of the lane's cells: east x-faces and north y-faces of rows
+  // wv + 4 r; wave 0 adds the south y-faces of the tile's first row, wave 1
+  // (16 lanes) the west x-faces of its first column
+#pragma unroll
+  for (int r = 0; r < WY / 4; ++r) {
+    const int ly = wv + 4 * r;
+    sfx[ly * (WX + 1) + tx + 1] = xface(ly, gx);
+    yface(y0 + ly);
+  }
+  if (wv == 0) yface(y0 - 1);
+  if (wv == 1 && tx < WY) sfx[tx * (WX + 1)] = xface(tx, x0 - 1);
   __syncthreads();
 
   const float wc = pr.tau0 * a.t.wind_scale[z];
   const bool bottom = (a.t.layer[z] == a.t.mnl[m] - 1) && (pr.kappa != 0.0f);
 #pragma unroll
   for (int r = 0; r < WY / 4; ++r) {
-    const int ly = tyb + 4 * r;
-    const int gy = y0 + ly, gx = x0 + tx;
-    if (gy >= Ny || gx >= Nx) continue;
+    const int ly = wv + 4 * r;
+    const int gy = y0 + ly;
+    if (gy >= Ny || !gx_in) continue;
     float rhs = 0.0f;
     if (gy >= 1 && gy <= Ny - 2 && gx >= 1 && gx <= Nx - 2) {
       const float fe = sfx[ly * (WX + 1) + tx + 1], fw = sfx[ly * (WX + 1) + tx];
-      // y-faces above/below cell gy: stored relative to b of their lower cell;
-      // the upper face is relative to b_gy already, the lower one to b_{gy-1},
-      // so its b offset db_{gy-1} times v_s completes the b_gy reference.
-      const float gn = sfy[(ly + 1) * WX + tx], gs = sfy[ly * WX + tx];
-      const float vs = (corner(gy - 1, gx) - corner(gy - 1, gx - 1)) * a.idx;
-      const float gs_rel = gs - vs * sdb[ly + 2];
-      rhs = -((fe - fw) * a.idx + (gn - gs_rel) * a.idy);
+      // y-faces above/below cell gy, both relative to b of cell gy
+      const float gn = sfl[(ly + 1) * WX + tx], gs = sfh[ly * WX + tx];
+      rhs = -((fe - fw) * a.idx + (gn - gs) * a.idy);
       const float* q = &sq[(ly + 3) * WQ + tx + 3];
       const float* p = &sp[(ly + 1) * PW2 + tx + 1];
       if (pr.nu != 0.0f) rhs += pr.nu * ((q[1] - 2.0f * q[0] + q[-1]) * a.idx2 + (q[WQ] - 2.0f * q[0] + q[-WQ]) * a.idy2);
@@ -405,9 +444,9 @@ __global__ __launch_bounds__(256) void qg_stage_weno_kernel(StageArgs a) {
     }
     rhs += wc * wind_r[r];
     const long idx = off + (long)gy * Nx + gx;
-    const bool gring = gy == 0 || gx == 0 || gy == Ny - 1 || gx == Nx - 1;
+    const bool gring = gy == 0 || gy == Ny - 1 || gx_ring;
     const float qtile = sq[(ly + 3) * WQ + tx + 3];
-    const float qraw = (gring && a.apply_bc) ? qg_r[r] : qtile;
+    const float qraw = (gring && bc) ? qg_r[r] : qtile;
     float o;
     if (a.qn) {
       const float qn = qn_r[r];
@@ -982,10 +1021,15 @@ __global__ __launch_bounds__(SUM_THREADS) void plane_sums_kernel(const float* a,
 
 int launch_plane_sums(const float* a, const float* b, int Nx, int Ny, int P, long plane, double* partial,
                       hipStream_t st) {
-  // at most 64 blocks per plane and never more than Ny (the partial buffer
-  // holds 2 * P * Ny * ceil(Nx / 256) doubles)
+  // about 4096 values per block (one round of loads for 1024 threads), at
+  // most 64 blocks per plane and never more than Ny (the partial buffer holds
+  // 2 * P * Ny * ceil(Nx / 256) doubles); small planes of a large batch thus
+  // take one or two blocks each rather than 64 nearly idle ones
   const long n = (long)Nx * Ny;
-  const int nblk = Ny < 64 ? Ny : 64;
+  long want = (n + 4095) / 4096;
+  if (want > 64) want = 64;
+  if (want > Ny) want = Ny;
+  const int nblk = want < 1 ? 1 : (int)want;
   const long chunk = (n + nblk - 1) / nblk;
   hipLaunchKernelGGL(plane_sums_kernel, dim3(nblk, P), dim3(SUM_THREADS), 0, st, a, b, n, plane, chunk, partial);
   return nblk;
