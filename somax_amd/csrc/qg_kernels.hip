@@ -725,19 +725,43 @@ void launch_to_modal(const ModalArgs& a, hipStream_t st) {
 // ---------------------------------------------------------------------------
 // mode -> layer projection of psi
 // ---------------------------------------------------------------------------
+// The mass-constraint partials that thread t stores for member_alpha
+// (indices t and t + blockDim; nl * nblk <= 8 * 64 = 2 * blockDim), loaded by
+// member_partials before the caller's other loads so that the wait for them
+// does not wait for those.
+struct AlphaPartials {
+  double v[2];
+};
+__device__ inline AlphaPartials member_partials(const LayerArgs& a, int first, int nl) {
+  AlphaPartials r{{0.0, 0.0}};
+#pragma unroll
+  for (int j = 0; j < 2; ++j) {
+    const int i = threadIdx.x + j * (int)blockDim.x;
+    if (i < nl * a.nblk) {
+      const int p = i / a.nblk, k = i - p * a.nblk;
+      r.v[j] = a.partial[2L * ((long)(first + p) * a.nblk + k)];
+    }
+  }
+  return r;
+}
+
 // Mass correction of a block's member: the alphas of its nl modes from the
 // FP64 partial sums (each mode's partials summed in order by one thread, so
 // every block and every run gets the same bits), into s_alpha; the member's
 // first block also writes the wall value of every layer.  Every thread of the
 // block calls this before any returns.
 __device__ inline void member_alpha(const LayerArgs& a, int first, int nl, const float* cm2l, bool write_wall,
-                                    float* s_alpha) {
+                                    const AlphaPartials& part, float* s_alpha) {
   __shared__ double sp[MAX_LAYERS][64];
   __shared__ double sal[MAX_LAYERS];
   const int t = threadIdx.x;
-  for (int i = t; i < nl * a.nblk; i += blockDim.x) {
-    const int p = i / a.nblk, k = i - p * a.nblk;
-    sp[p][k] = a.partial[2L * ((long)(first + p) * a.nblk + k)];
+#pragma unroll
+  for (int j = 0; j < 2; ++j) {
+    const int i = t + j * (int)blockDim.x;
+    if (i < nl * a.nblk) {
+      const int p = i / a.nblk, k = i - p * a.nblk;
+      sp[p][k] = part.v[j];
+    }
   }
   __syncthreads();
   if (t < nl) {
@@ -765,12 +789,13 @@ __global__ __launch_bounds__(256) void to_layer_kernel(LayerArgs a) {
   const int mem = blockIdx.z;
   const int first = a.t.mfirst[mem], nl = a.t.mnl[mem];
   const float* cm2l = a.t.cm2l + mem * MAX_LAYERS * MAX_LAYERS;
-  __shared__ float s_alpha[MAX_LAYERS];
-  if (a.homsol) member_alpha(a, first, nl, cm2l, blockIdx.x == 0 && y == 0, s_alpha);
-  if (x >= a.Nx) return;
-  const int s = y * a.Nx + x;  // in-plane offset (a plane is below 2^31 cells)
+  const bool in = x < a.Nx;
+  const int s = in ? y * a.Nx + x : 0;  // in-plane offset (a plane is below 2^31 cells)
   const bool ghost = a.t.mask[mem] && (x == 0 || y == 0 || x == a.Nx - 1 || y == a.Ny - 1);
-  // loads first (layers past nl re-read the member's first layer, unused)
+  // loads first, before the mass constraint's barriers, so their latency
+  // overlaps the alphas (layers past nl re-read the member's first layer,
+  // unused)
+  const AlphaPartials part = a.homsol ? member_partials(a, first, nl) : AlphaPartials{{0.0, 0.0}};
   float zp[NLM], zd[NLM], zh[NLM];
 #pragma unroll
   for (int p = 0; p < NLM; ++p) {
@@ -779,6 +804,9 @@ __global__ __launch_bounds__(256) void to_layer_kernel(LayerArgs a) {
     zd[p] = a.d ? a.d[i] : 0.0f;
     zh[p] = a.homsol ? a.homsol[i] : 0.0f;
   }
+  __shared__ float s_alpha[MAX_LAYERS];
+  if (a.homsol) member_alpha(a, first, nl, cm2l, blockIdx.x == 0 && y == 0, part, s_alpha);
+  if (!in) return;
   float pm[NLM];
 #pragma unroll
   for (int p = 0; p < NLM; ++p) {
@@ -809,10 +837,9 @@ __global__ __launch_bounds__(256) void to_layer_fold_kernel(LayerArgs a) {
   const int mem = blockIdx.y;
   const int first = a.t.mfirst[mem], nl = a.t.mnl[mem];
   const float* cm2l = a.t.cm2l + mem * MAX_LAYERS * MAX_LAYERS;
-  __shared__ float s_alpha[MAX_LAYERS];
-  if (a.homsol) member_alpha(a, first, nl, cm2l, blockIdx.x == 0, s_alpha);
-  if (e >= hy * hx) return;
-  const int ky = e / hx, kx = e - ky * hx;
+  const bool in = e < hy * hx;
+  const int ee = in ? e : 0;  // threads past the array load point 0 and return after the alphas
+  const int ky = ee / hx, kx = ee - ky * hx;
   const int y0 = fold_phys(ky, hy), y1 = a.Ny - 1 - y0;
   const int x0 = fold_phys(kx, hx), x1 = a.Nx - 1 - x0;
   // in-plane offsets (a plane is below 2^31 cells)
@@ -820,8 +847,10 @@ __global__ __launch_bounds__(256) void to_layer_fold_kernel(LayerArgs a) {
   const bool ghost = a.t.mask[mem] && (ky == hy - 1 || kx == hx - 1);
   const long fplane = (long)hx * hy;
   const int fo = ky * hx + kx;
-  // Every load is issued before the first use (layers past nl re-read the
-  // member's first layer and are ignored), so one memory latency covers them.
+  // Every load is issued before the first use and before the mass
+  // constraint's barriers (layers past nl re-read the member's first layer and
+  // are ignored), so one memory latency covers them and the alphas.
+  const AlphaPartials part = a.homsol ? member_partials(a, first, nl) : AlphaPartials{{0.0, 0.0}};
   float zb[NLM][4], zd[NLM][4], zh[NLM][4];
 #pragma unroll
   for (int p = 0; p < NLM; ++p) {
@@ -834,6 +863,9 @@ __global__ __launch_bounds__(256) void to_layer_fold_kernel(LayerArgs a) {
       zh[p][b] = a.homsol ? a.homsol[(long)pp * a.plane + pts[b]] : 0.0f;
     }
   }
+  __shared__ float s_alpha[MAX_LAYERS];
+  if (a.homsol) member_alpha(a, first, nl, cm2l, blockIdx.x == 0, part, s_alpha);
+  if (!in) return;
   float pm[NLM][4];
 #pragma unroll
   for (int p = 0; p < NLM; ++p) {
