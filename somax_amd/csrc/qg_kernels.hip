@@ -249,8 +249,10 @@ __global__ __launch_bounds__(256) void qg_stage_weno_kernel(StageArgs a) {
   const int tid = threadIdx.x;
   int bx, by, z;
   xcd_tile3(bx, by, z);
-  const int x0 = bx * WX;
-  const int y0 = by * WY;
+  // Tiles cover the interior block (rows and columns 1 .. N-2); the edge
+  // tiles also write the ghost ring (after the interior outputs).
+  const int x0 = 1 + bx * WX;
+  const int y0 = 1 + by * WY;
   const int Nx = a.Nx, Ny = a.Ny;
   const int nxi = Nx - 2, nyi = Ny - 2;
   const long off = (long)z * a.plane;
@@ -268,23 +270,20 @@ __global__ __launch_bounds__(256) void qg_stage_weno_kernel(StageArgs a) {
   const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int gx = x0 + tx;
   const bool gx_in = gx < Nx;
-  const bool gx_ring = gx == 0 || gx == Nx - 1;
+  const bool gx_out = gx <= Nx - 2;  // an interior output column
+  const bool gx_ring = gx == Nx - 1;  // (gx >= 1)
   const bool bc = a.apply_bc != 0;
 
-  // This thread's outputs (rows wv + 4 r): the step-start value, the wind row
-  // and (on the ghost ring under the boundary condition, where the tile holds
-  // 0) the raw stage value are fetched first, so their latency overlaps the
-  // tile loads.
-  float qn_r[WY / 4], wind_r[WY / 4], qg_r[WY / 4];
+  // This thread's outputs (rows wv + 4 r): the step-start value and the wind
+  // row are fetched first, so their latency overlaps the tile loads.
+  float qn_r[WY / 4], wind_r[WY / 4];
 #pragma unroll
   for (int r = 0; r < WY / 4; ++r) {
     const int gy = y0 + wv + 4 * r;
-    const bool ok = gy < Ny && gx_in;
-    const bool gring = gy == 0 || gy == Ny - 1 || gx_ring;
+    const bool ok = gy <= Ny - 2 && gx_out;
     const int idx = gy * Nx + gx;  // plane < 2^31 cells
     qn_r[r] = (ok && a.qn) ? a.qn[off + idx] : 0.0f;
     wind_r[r] = ok ? wind_row[gy] : 0.0f;
-    qg_r[r] = (ok && gring && bc) ? qs[idx] : 0.0f;
   }
   // Tile loads, all issued before the first LDS store so that one memory
   // latency covers them.  q with the boundary condition (ghost ring and
@@ -422,13 +421,25 @@ __global__ __launch_bounds__(256) void qg_stage_weno_kernel(StageArgs a) {
 
   const float wc = pr.tau0 * a.t.wind_scale[z];
   const bool bottom = (a.t.layer[z] == a.t.mnl[m] - 1) && (pr.kappa != 0.0f);
+  // Stage combination qn + c1 (qs - qn) + c2 rhs (the FP32 weights of the
+  // Shu-Osher form would otherwise not sum to 1); without qn: c1 qs + c2 rhs.
+  auto combine = [&](float qn, float qraw, float rhs) {
+    float o;
+    if (a.qn) {
+      o = qn + a.c1 * (qraw - qn) + a.c2 * rhs;
+    } else {
+      o = a.c2 * rhs;
+      if (a.c1 != 0.0f) o += a.c1 * qraw;
+    }
+    return o;
+  };
 #pragma unroll
   for (int r = 0; r < WY / 4; ++r) {
     const int ly = wv + 4 * r;
     const int gy = y0 + ly;
-    if (gy >= Ny || !gx_in) continue;
+    if (gy > Ny - 2 || !gx_out) continue;
     float rhs = 0.0f;
-    if (gy >= 1 && gy <= Ny - 2 && gx >= 1 && gx <= Nx - 2) {
+    {
       const float fe = sfx[ly * (WX + 1) + tx + 1], fw = sfx[ly * (WX + 1) + tx];
       // y-faces above/below cell gy, both relative to b of cell gy
       const float gn = sfl[(ly + 1) * WX + tx], gs = sfh[ly * WX + tx];
@@ -444,24 +455,46 @@ __global__ __launch_bounds__(256) void qg_stage_weno_kernel(StageArgs a) {
     }
     rhs += wc * wind_r[r];
     const long idx = off + (long)gy * Nx + gx;
-    const bool gring = gy == 0 || gy == Ny - 1 || gx_ring;
-    const float qtile = sq[(ly + 3) * WQ + tx + 3];
-    const float qraw = (gring && bc) ? qg_r[r] : qtile;
-    float o;
-    if (a.qn) {
-      const float qn = qn_r[r];
-      o = qn + a.c1 * (qraw - qn) + a.c2 * rhs;
-    } else {
-      o = a.c2 * rhs;
-      if (a.c1 != 0.0f) o += a.c1 * qraw;
+    a.out[idx] = combine(qn_r[r], sq[(ly + 3) * WQ + tx + 3], rhs);
+  }
+
+  // Ghost ring: no flux or viscous terms, only the wind row, and the raw
+  // stage value (the tile holds 0 there under the boundary condition).
+  // Wave 0: row 0 under the first tile row; wave 1: row Ny-1 under the last;
+  // wave 2: column 0 (lanes 0..WY-1, first tile column) and column Nx-1
+  // (lanes 32..32+WY-1, last tile column); wave 3: the four corners.
+  const bool first_x = bx == 0, last_x = bx == (int)gridDim.x - 1;
+  const bool first_y = by == 0, last_y = by == (int)gridDim.y - 1;
+  int ry = -1, rx = -1;
+  if (wv == 0 && first_y && gx_out) {
+    ry = 0;
+    rx = gx;
+  } else if (wv == 1 && last_y && gx_out) {
+    ry = Ny - 1;
+    rx = gx;
+  } else if (wv == 2 && tx < WY && first_x && y0 + tx <= Ny - 2) {
+    ry = y0 + tx;
+    rx = 0;
+  } else if (wv == 2 && tx >= 32 && tx < 32 + WY && last_x && y0 + tx - 32 <= Ny - 2) {
+    ry = y0 + tx - 32;
+    rx = Nx - 1;
+  } else if (wv == 3 && tx < 4) {
+    const bool top = tx < 2, left = (tx & 1) == 0;
+    if ((top ? first_y : last_y) && (left ? first_x : last_x)) {
+      ry = top ? 0 : Ny - 1;
+      rx = left ? 0 : Nx - 1;
     }
-    a.out[idx] = o;
+  }
+  if (ry >= 0) {
+    const long idx = off + (long)ry * Nx + rx;
+    const float qn = a.qn ? a.qn[idx] : 0.0f;
+    a.out[idx] = combine(qn, a.qs[idx], wc * wind_row[ry]);
   }
 }
 
 void launch_qg_stage(const StageArgs& a, hipStream_t st) {
   if (a.advection == 1) {
-    dim3 g((a.Nx + WX - 1) / WX, (a.Ny + WY - 1) / WY, a.P);
+    dim3 g((a.Nx - 2 + WX - 1) / WX, (a.Ny - 2 + WY - 1) / WY, a.P);  // interior tiles
     if (a.biharm) hipLaunchKernelGGL(qg_stage_weno_kernel<true>, g, dim3(256), 0, st, a);
     else hipLaunchKernelGGL(qg_stage_weno_kernel<false>, g, dim3(256), 0, st, a);
     return;
