@@ -29,8 +29,10 @@ __global__ __launch_bounds__(256) void qg_stage_arakawa_kernel(StageArgs a) {
   const int tid = threadIdx.x;
   int bx, by, z;
   xcd_tile3(bx, by, z);
-  const int x0 = bx * TX;
-  const int y0 = by * TY;
+  // Tiles cover the interior block (rows and columns 1 .. N-2); the edge
+  // tiles also write the ghost ring (after the interior outputs).
+  const int x0 = 1 + bx * TX;
+  const int y0 = 1 + by * TY;
   const int Nx = a.Nx, Ny = a.Ny;
   const long off = (long)z * a.plane;
   const float* qs = a.qs + off;
@@ -45,18 +47,14 @@ __global__ __launch_bounds__(256) void qg_stage_arakawa_kernel(StageArgs a) {
   // the tile loads instead of following the last barrier.
   const int tx = tid & 63;
   const int tyb = tid >> 6;
-  // On the ghost ring under the boundary condition the tile holds 0, so the
-  // raw q_s of the stage combination is fetched here as well.
-  float qn_r[TY / 4], wind_r[TY / 4], qg_r[TY / 4];
+  float qn_r[TY / 4], wind_r[TY / 4];
 #pragma unroll
   for (int r = 0; r < TY / 4; ++r) {
     const int gy = y0 + tyb + 4 * r, gx = x0 + tx;
-    const bool ok = gy < Ny && gx < Nx;
-    const bool gring = gy == 0 || gx == 0 || gy == Ny - 1 || gx == Nx - 1;
+    const bool ok = gy <= Ny - 2 && gx <= Nx - 2;
     const int idx = gy * Nx + gx;
     qn_r[r] = (ok && a.qn) ? a.qn[off + idx] : 0.0f;
     wind_r[r] = ok ? wind_row[gy] : 0.0f;
-    qg_r[r] = (ok && gring && a.apply_bc) ? qs[idx] : 0.0f;
   }
 
   // Tile loads: a fixed number per thread, all issued before the first LDS
@@ -113,13 +111,26 @@ __global__ __launch_bounds__(256) void qg_stage_arakawa_kernel(StageArgs a) {
 
   const float wc = pr.tau0 * a.t.wind_scale[z];
   const bool bottom = (a.t.layer[z] == a.t.mnl[m] - 1) && (pr.kappa != 0.0f);
+  // Stage combination qn + c1 (qs - qn) + c2 rhs: the FP32 weights of the
+  // Shu-Osher form (1/3 + 2/3 != 1 in FP32) would otherwise scale q by
+  // 1 + 3e-8 per step.  Without qn: c1 qs + c2 rhs.
+  auto combine = [&](float qn, float qraw, float rhs) {
+    float o;
+    if (a.qn) {
+      o = qn + a.c1 * (qraw - qn) + a.c2 * rhs;
+    } else {
+      o = a.c2 * rhs;
+      if (a.c1 != 0.0f) o += a.c1 * qraw;
+    }
+    return o;
+  };
 #pragma unroll
   for (int r = 0; r < TY / 4; ++r) {
     const int ly = tyb + 4 * r;
     const int gy = y0 + ly, gx = x0 + tx;
-    if (gy >= Ny || gx >= Nx) continue;
+    if (gy > Ny - 2 || gx > Nx - 2) continue;
     float rhs = 0.0f;
-    if (gy >= 1 && gy <= Ny - 2 && gx >= 1 && gx <= Nx - 2) {
+    {
       const float* q = &sq[(ly + 2) * QW + tx + 2];
       const float* p = &sp[(ly + 1) * PW + tx + 1];
       // J(psi, q + b) = J(psi, q) + J(psi, b), b = beta*(y - y0) a function of
@@ -146,23 +157,42 @@ __global__ __launch_bounds__(256) void qg_stage_arakawa_kernel(StageArgs a) {
     }
     rhs += wc * wind_r[r];
     const long idx = off + (long)gy * Nx + gx;
-    // raw q_s: the tile's copy, except on the ghost ring when the boundary
-    // condition zeroed it there
-    const bool gring = gy == 0 || gx == 0 || gy == Ny - 1 || gx == Nx - 1;
-    const float qtile = sq[(ly + 2) * QW + tx + 2];
-    auto qraw = [&]() { return (gring && a.apply_bc) ? qg_r[r] : qtile; };
-    // Stage combination qn + c1 (qs - qn) + c2 rhs: the FP32 weights of the
-    // Shu-Osher form (1/3 + 2/3 != 1 in FP32) would otherwise scale q by
-    // 1 + 3e-8 per step.  Without qn: c1 qs + c2 rhs.
-    float o;
-    if (a.qn) {
-      const float qn = qn_r[r];
-      o = qn + a.c1 * (qraw() - qn) + a.c2 * rhs;
-    } else {
-      o = a.c2 * rhs;
-      if (a.c1 != 0.0f) o += a.c1 * qraw();
+    a.out[idx] = combine(qn_r[r], sq[(ly + 2) * QW + tx + 2], rhs);
+  }
+
+  // Ghost ring: no Jacobian or viscous terms, only the wind row, and the raw
+  // stage value (the tile holds 0 there under the boundary condition).
+  // Wave 0: row 0 under the first tile row; wave 1: row Ny-1 under the last;
+  // wave 2: column 0 (lanes 0..TY-1, first tile column) and column Nx-1
+  // (lanes 32..32+TY-1, last tile column); wave 3: the four corners.
+  const int wv = __builtin_amdgcn_readfirstlane(tyb);
+  const int gx = x0 + tx;
+  const bool first_x = bx == 0, last_x = bx == (int)gridDim.x - 1;
+  const bool first_y = by == 0, last_y = by == (int)gridDim.y - 1;
+  int ry = -1, rx = -1;
+  if (wv == 0 && first_y && gx <= Nx - 2) {
+    ry = 0;
+    rx = gx;
+  } else if (wv == 1 && last_y && gx <= Nx - 2) {
+    ry = Ny - 1;
+    rx = gx;
+  } else if (wv == 2 && tx < TY && first_x && y0 + tx <= Ny - 2) {
+    ry = y0 + tx;
+    rx = 0;
+  } else if (wv == 2 && tx >= 32 && tx < 32 + TY && last_x && y0 + tx - 32 <= Ny - 2) {
+    ry = y0 + tx - 32;
+    rx = Nx - 1;
+  } else if (wv == 3 && tx < 4) {
+    const bool top = tx < 2, left = (tx & 1) == 0;
+    if ((top ? first_y : last_y) && (left ? first_x : last_x)) {
+      ry = top ? 0 : Ny - 1;
+      rx = left ? 0 : Nx - 1;
     }
-    a.out[idx] = o;
+  }
+  if (ry >= 0) {
+    const long idx = off + (long)ry * Nx + rx;
+    const float qn = a.qn ? a.qn[idx] : 0.0f;
+    a.out[idx] = combine(qn, a.qs[idx], wc * wind_row[ry]);
   }
 }
 
@@ -500,7 +530,7 @@ void launch_qg_stage(const StageArgs& a, hipStream_t st) {
     return;
   }
   const int ty = a.tile_rows ? a.tile_rows : TY_DEFAULT;
-  const dim3 grid((a.Nx + TX - 1) / TX, (a.Ny + ty - 1) / ty, a.P);
+  const dim3 grid((a.Nx - 2 + TX - 1) / TX, (a.Ny - 2 + ty - 1) / ty, a.P);  // interior tiles
   if (ty == 8) {
     if (a.biharm) hipLaunchKernelGGL((qg_stage_arakawa_kernel<true, 8>), grid, dim3(256), 0, st, a);
     else hipLaunchKernelGGL((qg_stage_arakawa_kernel<false, 8>), grid, dim3(256), 0, st, a);
