@@ -91,7 +91,7 @@ def kernel_profile(model, q0, dt: float, steps: int):
     return out
 
 
-def measured_traffic(name: str = "hbm_traffic_qg1024_mixed.json") -> dict:
+def measured_traffic(name: str) -> dict:
     """HBM bytes per launch of a workload's kernels, measured with rocprofv3
     FETCH_SIZE / WRITE_SIZE passes and scaled by a copy of known size
     (profiles/hbm_traffic_*.json); empty if absent."""

@@ -140,6 +140,51 @@ def summarize_counters(paths) -> str:
     return "\n".join(lines)
 
 
+def summarize_issue(paths, cus: int = 256, xcds: int = 8) -> str:
+    """Markdown table of the instruction-issue counters per (kernel, grid):
+    VALU busy = SQ_ACTIVE_INST_VALU / CUs / (GRBM_GUI_ACTIVE / XCDs) (the
+    counter is in quad-cycles, one per wave64 VALU instruction on a 16-lane
+    SIMD, and GRBM_GUI_ACTIVE sums the XCDs' active cycles), VALU / SALU / LDS
+    instructions per wave, the share of wave cycles spent waiting for an
+    instruction's operands (SQ_WAIT_INST_ANY / SQ_WAVE_CYCLES), and, when the
+    second pass is given, active lanes per VALU instruction and the LDS
+    bank-conflict share of LDS cycles."""
+    import collections
+    import csv
+
+    agg = collections.defaultdict(lambda: collections.defaultdict(list))
+    for path in paths:
+        with open(path) as f:
+            for r in csv.DictReader(f):
+                agg[(r["Kernel_Name"], _grid(r))][r["Counter_Name"]].append(float(r["Counter_Value"]))
+
+    def mean(row, c):
+        v = row.get(c)
+        return sum(v) / len(v) if v else None
+
+    lines = ["| kernel | grid | dispatches | VALU busy | VALU / wave | SALU / wave | LDS / wave | waiting | "
+             "lanes active | LDS conflicts |", "|---|---:|---:|---:|---:|---:|---:|---:|---:|---:|"]
+    for key in sorted(agg, key=lambda k: -(mean(agg[k], "SQ_ACTIVE_INST_VALU") or 0.0)):
+        row = agg[key]
+        gui, act, waves = mean(row, "GRBM_GUI_ACTIVE"), mean(row, "SQ_ACTIVE_INST_VALU"), mean(row, "SQ_WAVES")
+        if not gui or act is None or not waves:
+            continue
+        n = max(len(v) for v in row.values())
+        busy = act / cus / (gui / xcds)
+        cells = [f"{100 * busy:.0f}%"]
+        for c in ("SQ_INSTS_VALU", "SQ_INSTS_SALU", "SQ_INSTS_LDS"):
+            v = mean(row, c)
+            cells.append(f"{v / waves:.0f}" if v is not None else "")
+        wt, wc = mean(row, "SQ_WAIT_INST_ANY"), mean(row, "SQ_WAVE_CYCLES")
+        cells.append(f"{100 * wt / wc:.0f}%" if wt is not None and wc else "")
+        tc = mean(row, "SQ_THREAD_CYCLES_VALU")
+        cells.append(f"{100 * tc / (act * 64):.0f}%" if tc is not None and act else "")
+        bc, li = mean(row, "SQ_LDS_BANK_CONFLICT"), mean(row, "SQ_LDS_IDX_ACTIVE")
+        cells.append(f"{100 * bc / li:.0f}%" if bc is not None and li else "")
+        lines.append(f"| `{key[0][:70]}` | {key[1]} | {n} | " + " | ".join(cells) + " |")
+    return "\n".join(lines)
+
+
 def main(argv=None) -> int:
     if argv is None:
         argv = sys.argv[1:]
@@ -148,6 +193,9 @@ def main(argv=None) -> int:
         return 0
     if argv and argv[0] == "summarize-counters":
         print(summarize_counters(argv[1:]))
+        return 0
+    if argv and argv[0] == "summarize-issue":
+        print(summarize_issue(argv[1:]))
         return 0
     if argv and argv[0] == "summarize-pmc":
         # summarize-pmc FETCH.csv WRITE.csv COPY_MIB [OUT.json WORKLOAD]

@@ -1,10 +1,13 @@
 """The C restatements of the oracle against the NumPy oracle.
 
-* oracle/cpu_step.cpp (the multi-core CPU baseline): FP64 to round-off, FP32
-  to its precision, with viscosity, hyperviscosity, drag, beta and wind on.
-* oracle/qgrhs.c (the FP64 stencil terms behind ``rhs(..., fast=True)``, used
+* oracle/cpu_step.cpp (the dense-GEMM C++ restatement of the Arakawa step):
+  FP64 to round-off, FP32 to its precision, with viscosity, hyperviscosity,
+  drag, beta and wind on.
+* oracle/qgrhs.cpp (the FP64 stencil terms behind ``rhs(..., fast=True)``, used
   for the large-grid parity checks): bit for bit, for both advection schemes,
-  with and without the mass correction, on square and non-square grids."""
+  with and without the mass correction, on square and non-square grids.
+* oracle/cpu.py:FftCpuStep (the CPU baseline: FFT-based DST-I and the C++
+  stencils in FP32) against the oracle, on one and on several threads."""
 
 from __future__ import annotations
 
