@@ -361,8 +361,12 @@ int somaxhip_field_stats(const float* const* fields, const size_t* counts, int n
 typedef struct somaxhip_ens_s* somaxhip_ens;
 
 enum {
-  SOMAXHIP_ENS_CONCURRENT = 1, /* each class on its own stream, forked from and joined to the caller's */
-  SOMAXHIP_ENS_GRAPH = 2       /* capture one step as a hipGraph and replay it */
+  SOMAXHIP_ENS_CONCURRENT = 1, /* each class (or launch group) on its own stream, forked from and joined to the caller's */
+  SOMAXHIP_ENS_GRAPH = 2,      /* capture one step as a hipGraph and replay it */
+  SOMAXHIP_ENS_MERGED = 4      /* classes that share precision, advection, biharmonic flag and stage tile
+                                  height form a launch group (up to 8 classes): every kernel phase of a
+                                  Runge-Kutta stage is one launch over the whole group (a work table of
+                                  the classes' blocks); the arithmetic per member is unchanged */
 };
 
 /* members[i] / params[i], i < n: one QG model each (as for plan_create);

@@ -21,6 +21,7 @@ PREC_MIXED = 1
 PREC_BF16X6 = 2
 ENS_CONCURRENT = 1
 ENS_GRAPH = 2
+ENS_MERGED = 4
 SOLVER_SSPRK3 = 0
 SOLVER_TSIT5 = 1
 BC_PERIODIC = 0

@@ -543,6 +543,22 @@ void gemm(const GemmArgs& a, int batch, hipStream_t st) {
 
 // Products of a GEMM class: NP_F32 selects the FP32 kernels; 1, 3, 6 bf16.
 constexpr int NP_F32 = 0;
+
+// The four transforms of one solve (F1, F2, I1, I2) with their batch sizes,
+// recorded instead of launched when a merged launch will carry them.
+struct SolveGemms {
+  GemmArgs g[4];
+  int batch[4];
+};
+// Launch transform k of a solve, or record it into `rec`.
+inline void emit_gemm(const GemmArgs& a, int batch, int k, SolveGemms* rec, hipStream_t st) {
+  if (rec) {
+    rec->g[k] = a;
+    rec->batch[k] = batch;
+    return;
+  }
+  gemm(a, batch, st);
+}
 inline int pieces_for(int np) { return np == 1 ? 1 : (np == 3 ? 2 : 3); }
 
 // Folded solve: the four parity blocks of every plane as 4P half-size planes
@@ -550,7 +566,7 @@ inline int pieces_for(int np) { return np == 1 ? 1 : (np == 3 ? 2 : 3); }
 // pair, last in the folded order, is zero and skipped).  Out: `out` receives
 // the parity components of the solution, 4P planes of hy x hx.
 void dst_solve_folded(const Batch& p, bool interior_input, int np_fwd, int np_inv_lo, int np_inv_hi, float* out,
-                      void* ws, hipStream_t st, bool acc64_fwd, bool acc64_inv) {
+                      void* ws, hipStream_t st, bool acc64_fwd, bool acc64_inv, SolveGemms* rec) {
   const int hx = p.sx.h, hy = p.sy.h, P = p.P;
   const bool f32 = np_fwd == NP_F32;
   char* OP1 = static_cast<char*>(ws) + p.off_OP1;
@@ -613,7 +629,7 @@ void dst_solve_folded(const Batch& p, bool interior_input, int np_fwd, int np_in
   g.RA = hx;
   g.RB = R0;
   g.RBpad = (int)round_up(R0, 32);
-  gemm(g, 4 * P, st);
+  emit_gemm(g, 4 * P, 0, rec, st);
 
   // F2: W[ix][iy] = sum_y' T[ix][y'] Sy_p[iy][y'] / (lamx_q[ix] + lamy_p[iy] - lambda)
   GemmArgs h{};
@@ -629,7 +645,7 @@ void dst_solve_folded(const Batch& p, bool interior_input, int np_fwd, int np_in
   h.lam_var = hx;
   h.lamB_var = hy;
   h.lam = p.t.lam;
-  gemm(h, 4 * P, st);
+  emit_gemm(h, 4 * P, 1, rec, st);
 
   // I1: V[y'][ix] = sum_iy Sy_p^T[y'][iy] W[ix][iy]
   GemmArgs u{};
@@ -640,7 +656,7 @@ void dst_solve_folded(const Batch& p, bool interior_input, int np_fwd, int np_in
   u.RA = hy;
   u.RB = hx;
   u.RBpad = (int)round_up(hx, 32);
-  gemm(u, 4 * P, st);
+  emit_gemm(u, 4 * P, 2, rec, st);
 
   // I2: out[y'][x'] = sum_ix V[y'][ix] Sx_q^T[x'][ix]
   GemmArgs w{};
@@ -655,7 +671,7 @@ void dst_solve_folded(const Batch& p, bool interior_input, int np_fwd, int np_in
   w.RA = hy;
   w.RB = hx;
   w.RBpad = hx;
-  gemm(w, 4 * P, st);
+  emit_gemm(w, 4 * P, 3, rec, st);
 }
 
 // One DST-I solve per plane of (lap_h - lambda_z) y_z = x_z, from the GEMM
@@ -665,9 +681,9 @@ void dst_solve_folded(const Batch& p, bool interior_input, int np_fwd, int np_in
 // components).  np_fwd: products of the forward transforms; np_inv_lo /
 // np_inv_hi: products of the inverse transforms of Poisson-like / other planes.
 void dst_solve(const Batch& p, bool interior_input, int np_fwd, int np_inv_lo, int np_inv_hi, float* out,
-               void* ws, hipStream_t st, bool acc64_fwd = true, bool acc64_inv = true) {
+               void* ws, hipStream_t st, bool acc64_fwd = true, bool acc64_inv = true, SolveGemms* rec = nullptr) {
   if (p.fold) {
-    dst_solve_folded(p, interior_input, np_fwd, np_inv_lo, np_inv_hi, out, ws, st, acc64_fwd, acc64_inv);
+    dst_solve_folded(p, interior_input, np_fwd, np_inv_lo, np_inv_hi, out, ws, st, acc64_fwd, acc64_inv, rec);
     return;
   }
   const int Nx = p.Nx, Ny = p.Ny, P = p.P;
@@ -726,7 +742,7 @@ void dst_solve(const Batch& p, bool interior_input, int np_fwd, int np_inv_lo, i
   g.RA = Nx;
   g.RB = R0;
   g.RBpad = (int)round_up(R0, 32);
-  gemm(g, P, st);
+  emit_gemm(g, P, 0, rec, st);
 
   // F2: W[xm][ym] = sum_y T[xm][y] Sy[ym][y] / (lamx + lamy - lambda)
   GemmArgs h{};
@@ -740,7 +756,7 @@ void dst_solve(const Batch& p, bool interior_input, int np_fwd, int np_inv_lo, i
   h.lamA = p.sx.lam;
   h.lamB = p.sy.lam;
   h.lam = p.t.lam;
-  gemm(h, P, st);
+  emit_gemm(h, P, 1, rec, st);
 
   // I1: V[y][xm] = sum_ym Sy[y][ym] W[xm][ym]        (S in the A slot)
   GemmArgs u{};
@@ -751,7 +767,7 @@ void dst_solve(const Batch& p, bool interior_input, int np_fwd, int np_inv_lo, i
   u.RA = Ny;
   u.RB = Nx;
   u.RBpad = (int)round_up(Nx, 32);
-  gemm(u, P, st);
+  emit_gemm(u, P, 2, rec, st);
 
   // I2: out[y][x] = sum_xm V[y][xm] Sx[x][xm]
   GemmArgs w{};
@@ -766,17 +782,16 @@ void dst_solve(const Batch& p, bool interior_input, int np_fwd, int np_inv_lo, i
   w.RA = Ny;
   w.RB = Nx;
   w.RBpad = Nx;
-  gemm(w, P, st);
+  emit_gemm(w, P, 3, rec, st);
 }
 
-void invert(const Batch& p, const float* q, float* psi, void* ws, hipStream_t st, bool apply_bc = true) {
+// Arguments of the inversion's kernels for one batch.
+ModalArgs modal_args(const Batch& p, const float* q, void* ws, bool apply_bc) {
   const bool mixed = p.precision == SOMAXHIP_PREC_MIXED;
   const bool x6 = p.precision == SOMAXHIP_PREC_BF16X6;
-  float* QM = wsf(ws, p.off_QM);
-  float* P0 = wsf(ws, p.off_P0);
   ModalArgs ma{};
   ma.q = q;
-  ma.qm = QM;
+  ma.qm = wsf(ws, p.off_QM);
   ma.Nx = p.Nx;
   ma.Ny = p.Ny;
   ma.M = p.M;
@@ -795,12 +810,34 @@ void invert(const Batch& p, const float* q, float* psi, void* ws, hipStream_t st
     ma.P = p.P;
     ma.op_kpad = (int)round_up(apply_bc ? p.sx.h - 1 : p.sx.h, 32);
   }
-  {
-    ProfScope ps("to_modal", 0.0, 8.0 * p.P * (double)p.splane, st);
-    launch_to_modal(ma, st);
-  }
+  return ma;
+}
+
+ResidualArgs residual_args(const Batch& p, void* ws) {
+  ResidualArgs r{};
+  r.q = wsf(ws, p.off_QM);
+  r.psi = wsf(ws, p.off_P0);
+  r.r = reinterpret_cast<uint16_t*>(static_cast<char*>(ws) + p.off_OP1);
+  r.op_ld = p.wpitch;
+  r.op_piece = p.wpiece;
+  r.op_batch = 3 * p.wpiece;
+  r.op_kpad = (int)round_up(p.fold ? p.sx.h : p.Nx, 32);
+  r.fold = p.fold;
+  r.plane = p.splane;
+  r.Nx = p.Nx;
+  r.Ny = p.Ny;
+  r.P = p.P;
+  r.idx2 = (float)(1.0 / (p.dx * p.dx));
+  r.idy2 = (float)(1.0 / (p.dy * p.dy));
+  r.lam = p.t.lam;
+  return r;
+}
+
+// to_layer arguments; the correction plane when `with_correction`
+LayerArgs layer_args(const Batch& p, float* psi, void* ws, bool with_correction) {
   LayerArgs la{};
-  la.pm = P0;
+  la.pm = wsf(ws, p.off_P0);
+  la.d = with_correction ? wsf(ws, p.off_D) : nullptr;
   la.psi = psi;
   la.Nx = p.Nx;
   la.Ny = p.Ny;
@@ -809,7 +846,37 @@ void invert(const Batch& p, const float* q, float* psi, void* ws, hipStream_t st
   la.t = p.t;
   la.fold = p.fold;
   la.P = p.P;
-  la.nlmax = ma.nlmax;
+  la.nlmax = *std::max_element(p.nl.begin(), p.nl.end());
+  return la;
+}
+
+// The mass constraint's plane sums of the modal psi (first guess plus the
+// correction), FP64 partials in the diagnostics area; folded: the array sum
+// is 4x the sum of the even-even block (planes 0..P-1).
+PlaneSumArgs mass_sums_args(const Batch& p, const LayerArgs& la, void* ws) {
+  double* partial = reinterpret_cast<double*>(static_cast<char*>(ws) + p.off_DIAG);
+  return plane_sums_args(la.pm, la.d, p.fold ? p.sx.h : p.Nx, p.fold ? p.sy.h : p.Ny, p.P,
+                         p.fold ? p.fplane : p.splane, partial);
+}
+// to_layer forms alpha = -mean / mean(homsol) from the partials and the wall psi
+void mass_layer_args(const Batch& p, LayerArgs& la, const PlaneSumArgs& ps, void* ws) {
+  la.nblk = ps.nblk;
+  la.homsol = p.homsol;
+  la.partial = ps.partial;
+  la.hmean = p.hmean;
+  la.inv_count = (p.fold ? 4.0 : 1.0) / (double)p.splane;
+  la.psi_wall = wsf(ws, p.off_PWALL);
+}
+
+void invert(const Batch& p, const float* q, float* psi, void* ws, hipStream_t st, bool apply_bc = true) {
+  const bool mixed = p.precision == SOMAXHIP_PREC_MIXED;
+  const bool x6 = p.precision == SOMAXHIP_PREC_BF16X6;
+  float* P0 = wsf(ws, p.off_P0);
+  {
+    const ModalArgs ma = modal_args(p, q, ws, apply_bc);
+    ProfScope ps("to_modal", 0.0, 8.0 * p.P * (double)p.splane, st);
+    launch_to_modal(ma, st);
+  }
   if (x6) {
     // three-piece operands, six products and FP64 flushes in every transform:
     // about 2^-24 per product, accurate without a refinement pass
@@ -820,49 +887,107 @@ void invert(const Batch& p, const float* q, float* psi, void* ws, hipStream_t st
     // bf16x3 forward, x6 (Poisson-like modes) / x3 inverse; FP32 per-mode
     // Helmholtz residual; correction with bf16x3 forward and bf16 inverse.
     dst_solve(p, apply_bc, 3, 6, 3, P0, ws, st);
-    float* D = wsf(ws, p.off_D);
-    ResidualArgs r{};
-    r.q = QM;
-    r.psi = P0;
-    r.r = reinterpret_cast<uint16_t*>(static_cast<char*>(ws) + p.off_OP1);
-    r.op_ld = p.wpitch;
-    r.op_piece = p.wpiece;
-    r.op_batch = 3 * p.wpiece;
-    r.op_kpad = (int)round_up(p.fold ? p.sx.h : p.Nx, 32);
-    r.fold = p.fold;
-    r.plane = p.splane;
-    r.Nx = p.Nx;
-    r.Ny = p.Ny;
-    r.P = p.P;
-    r.idx2 = (float)(1.0 / (p.dx * p.dx));
-    r.idy2 = (float)(1.0 / (p.dy * p.dy));
-    r.lam = p.t.lam;
     {
+      const ResidualArgs r = residual_args(p, ws);
       ProfScope ps("residual", 0.0, 12.0 * p.P * (double)p.splane, st);
       launch_residual(r, st);
     }
-    dst_solve(p, false, 3, 1, 1, D, ws, st, true, false);
-    la.d = D;
+    dst_solve(p, false, 3, 1, 1, wsf(ws, p.off_D), ws, st, true, false);
   }
+  LayerArgs la = layer_args(p, psi, ws, mixed);
   if (p.any_mass) {
     // FP64 domain mean of every modal streamfunction, alpha = -mean / mean(homsol)
-    double* partial = reinterpret_cast<double*>(static_cast<char*>(ws) + p.off_DIAG);
+    const PlaneSumArgs sums = mass_sums_args(p, la, ws);
     {
       ProfScope ps("mass_constraint", 0.0, (mixed ? 8.0 : 4.0) * p.P * (double)p.splane, st);
-      // folded: the array sum is 4x the sum of the even-even block (planes 0..P-1)
-      la.nblk = launch_plane_sums(P0, la.d, p.fold ? p.sx.h : p.Nx, p.fold ? p.sy.h : p.Ny, p.P,
-                                  p.fold ? p.fplane : p.splane, partial, st);
+      launch_plane_sums(sums.a, sums.b, p.fold ? p.sx.h : p.Nx, p.fold ? p.sy.h : p.Ny, p.P, sums.plane,
+                        sums.partial, st);
     }
-    // to_layer forms alpha = -mean / mean(homsol) from the partials and the wall psi
-    la.homsol = p.homsol;
-    la.partial = partial;
-    la.hmean = p.hmean;
-    la.inv_count = (p.fold ? 4.0 : 1.0) / (double)p.splane;
-    la.psi_wall = wsf(ws, p.off_PWALL);
+    mass_layer_args(p, la, sums, ws);
   }
   {
     ProfScope ps("to_layer", 0.0, (mixed ? 12.0 : 8.0) * p.P * (double)p.splane, st);
     launch_to_layer(la, st);
+  }
+}
+
+// One batch of a merged launch group: its state, streamfunction and workspace.
+struct GroupItem {
+  const Batch* b;
+  const float* q;
+  float* psi;
+  void* ws;
+};
+
+// invert() for the batches of a group (same precision): every kernel phase is
+// one launch over all of them; each batch's arithmetic is that of invert().
+void invert_group(const std::vector<GroupItem>& it, hipStream_t st) {
+  const int n = (int)it.size();
+  const int prec = it[0].b->precision;
+  const bool mixed = prec == SOMAXHIP_PREC_MIXED;
+  const bool x6 = prec == SOMAXHIP_PREC_BF16X6;
+  double bytes = 0.0;
+  for (const GroupItem& g : it) bytes += (double)g.b->P * g.b->splane;
+  {
+    ModalArgs ma[MAX_GROUP];
+    for (int i = 0; i < n; ++i) ma[i] = modal_args(*it[i].b, it[i].q, it[i].ws, true);
+    ProfScope ps("to_modal", 0.0, 8.0 * bytes, st);
+    launch_to_modal_group(ma, n, st);
+  }
+  auto solves = [&](bool interior, int npf, int npl, int nph, bool correction, bool acc64f, bool acc64i) {
+    SolveGemms rec[MAX_GROUP];
+    for (int i = 0; i < n; ++i) {
+      const Batch& p = *it[i].b;
+      dst_solve(p, interior, npf, npl, nph, wsf(it[i].ws, correction ? p.off_D : p.off_P0), it[i].ws, st, acc64f,
+                acc64i, &rec[i]);
+    }
+    for (int k = 0; k < 4; ++k) {
+      GemmArgs g[MAX_GROUP];
+      int batch[MAX_GROUP];
+      double flops = 0.0, gbytes = 0.0;
+      for (int i = 0; i < n; ++i) {
+        g[i] = rec[i].g[k];
+        batch[i] = rec[i].batch[k];
+        const double R = g[i].trans ? g[i].RB : g[i].RA, C = g[i].trans ? g[i].RA : g[i].RB;
+        flops += 2.0 * R * C * g[i].K * batch[i];
+        gbytes += 4.0 * batch[i] * (R * g[i].K + R * C);
+      }
+      ProfScope ps(gemm_name(g[0]), flops, gbytes, st);
+      launch_dst_gemm_group(g, batch, n, st);
+    }
+  };
+  if (x6) {
+    solves(true, 6, 6, 6, false, true, true);
+  } else if (!mixed) {
+    solves(true, NP_F32, NP_F32, NP_F32, false, true, true);
+  } else {
+    solves(true, 3, 6, 3, false, true, true);
+    {
+      ResidualArgs r[MAX_GROUP];
+      for (int i = 0; i < n; ++i) r[i] = residual_args(*it[i].b, it[i].ws);
+      ProfScope ps("residual", 0.0, 12.0 * bytes, st);
+      launch_residual_group(r, n, st);
+    }
+    solves(false, 3, 1, 1, true, true, false);
+  }
+  LayerArgs la[MAX_GROUP];
+  PlaneSumArgs sums[MAX_GROUP];
+  int nm = 0;
+  for (int i = 0; i < n; ++i) {
+    la[i] = layer_args(*it[i].b, it[i].psi, it[i].ws, mixed);
+    if (it[i].b->any_mass) {
+      sums[nm] = mass_sums_args(*it[i].b, la[i], it[i].ws);
+      mass_layer_args(*it[i].b, la[i], sums[nm], it[i].ws);
+      ++nm;
+    }
+  }
+  if (nm) {
+    ProfScope ps("mass_constraint", 0.0, (mixed ? 8.0 : 4.0) * bytes, st);
+    launch_plane_sums_group(sums, nm, st);
+  }
+  {
+    ProfScope ps("to_layer", 0.0, (mixed ? 12.0 : 8.0) * bytes, st);
+    launch_to_layer_group(la, n, st);
   }
 }
 
@@ -873,8 +998,8 @@ struct CallParams {
   int biharm;
 };
 
-void stage(const Batch& p, const float* qs, const float* psi, const float* qn, float* out, const CallParams& cp,
-           float c1, float c2, bool apply_bc, void* ws, hipStream_t st) {
+StageArgs stage_args(const Batch& p, const float* qs, const float* psi, const float* qn, float* out,
+                     const CallParams& cp, float c1, float c2, bool apply_bc, void* ws) {
   StageArgs a{};
   a.qs = qs;
   a.psi = psi;
@@ -901,9 +1026,16 @@ void stage(const Batch& p, const float* qs, const float* psi, const float* qn, f
   a.advection = p.advection;
   a.apply_bc = apply_bc ? 1 : 0;
   a.tile_rows = p.stage_rows;
-  // algorithmic bytes: read qs and psi, read qn when combined, write out
-  const double bytes = 4.0 * p.P * (double)p.nx * p.ny * (3.0 + (qn ? 1.0 : 0.0));
-  ProfScope ps("qg_stage", 0.0, bytes, st);
+  return a;
+}
+
+// algorithmic bytes of a stage: read qs and psi, read qn when combined, write out
+double stage_bytes(const Batch& p, bool qn) { return 4.0 * p.P * (double)p.nx * p.ny * (3.0 + (qn ? 1.0 : 0.0)); }
+
+void stage(const Batch& p, const float* qs, const float* psi, const float* qn, float* out, const CallParams& cp,
+           float c1, float c2, bool apply_bc, void* ws, hipStream_t st) {
+  const StageArgs a = stage_args(p, qs, psi, qn, out, cp, c1, c2, apply_bc, ws);
+  ProfScope ps("qg_stage", 0.0, stage_bytes(p, qn != nullptr), st);
   launch_qg_stage(a, st);
 }
 
@@ -922,6 +1054,46 @@ void rk3(const Batch& p, float* q, float* psi_last, int nsteps, double dt, const
     stage(p, q1, psi, q, q2, cp, 0.25f, 0.25f * h, true, ws, st);
     invert(p, q2, psi, ws, st);
     stage(p, q2, psi, q, q, cp, 2.0f / 3.0f, 2.0f / 3.0f * h, true, ws, st);
+  }
+}
+
+// A batch of a merged SSP-RK3 group: its state and parameters.
+struct RkItem {
+  const Batch* b;
+  float* q;
+  CallParams cp;
+  void* ws;
+};
+
+// rk3() for the batches of a group (same precision, advection, biharmonic
+// flag and stage tile height): every kernel phase of every stage is one
+// launch over the group.
+void rk3_group(const std::vector<RkItem>& it, int nsteps, double dt, hipStream_t st) {
+  const int n = (int)it.size();
+  const float h = (float)dt;
+  std::vector<GroupItem> gi(n);
+  for (int i = 0; i < n; ++i) gi[i] = GroupItem{it[i].b, nullptr, wsf(it[i].ws, it[i].b->off_PSI), it[i].ws};
+  auto q1 = [&](int i) { return wsf(it[i].ws, it[i].b->off_Q1); };
+  auto q2 = [&](int i) { return wsf(it[i].ws, it[i].b->off_Q2); };
+  // one stage: inversion of `in`, then out = qn + c1 (in - qn) + c2 rhs
+  auto rk_stage = [&](auto in, auto qn, auto out, float c1, float c2) {
+    for (int i = 0; i < n; ++i) gi[i].q = in(i);
+    invert_group(gi, st);
+    StageArgs a[MAX_GROUP];
+    double bytes = 0.0;
+    for (int i = 0; i < n; ++i) {
+      a[i] = stage_args(*it[i].b, in(i), gi[i].psi, qn(i), out(i), it[i].cp, c1, c2, true, it[i].ws);
+      bytes += stage_bytes(*it[i].b, qn(i) != nullptr);
+    }
+    ProfScope ps("qg_stage", 0.0, bytes, st);
+    launch_qg_stage_group(a, n, st);
+  };
+  auto q0 = [&](int i) { return it[i].q; };
+  auto none = [](int) -> float* { return nullptr; };
+  for (int s = 0; s < nsteps; ++s) {
+    rk_stage(q0, none, q1, 1.0f, h);
+    rk_stage(q1, q0, q2, 0.25f, 0.25f * h);
+    rk_stage(q2, q0, q0, 2.0f / 3.0f, 2.0f / 3.0f * h);
   }
 }
 
@@ -1198,7 +1370,7 @@ int somaxhip_ens_create(const somaxhip_qg_desc* members, const somaxhip_qg_param
     return SOMAXHIP_ERR_ARG;
   }
   *out = nullptr;
-  if (flags & ~(SOMAXHIP_ENS_CONCURRENT | SOMAXHIP_ENS_GRAPH)) {
+  if (flags & ~(SOMAXHIP_ENS_CONCURRENT | SOMAXHIP_ENS_GRAPH | SOMAXHIP_ENS_MERGED)) {
     set_error("somaxhip_ens_create: unknown flags");
     return SOMAXHIP_ERR_ARG;
   }
@@ -1367,9 +1539,31 @@ int somaxhip_ens_set_stage_rows(somaxhip_ens e, int cls, int rows) {
 
 namespace {
 
-// One ensemble step queued behind `st`: each class advances on its own
-// stream when concurrent (fork/join through events, which stream capture
-// turns into parallel graph branches), else all on `st`.
+// Launch groups of a merged ensemble: classes in order, each joining the
+// first group with the same precision, advection, biharmonic flag and (for
+// the Arakawa stage) tile height that still has room.
+std::vector<std::vector<int>> ens_groups(somaxhip_ens e) {
+  std::vector<std::vector<int>> groups;
+  auto key_eq = [&](int a, int b) {
+    const Batch& x = *e->cls[a];
+    const Batch& y = *e->cls[b];
+    return x.precision == y.precision && x.advection == y.advection && e->cbiharm[a] == e->cbiharm[b] &&
+           (x.advection == 1 || x.stage_rows == y.stage_rows);
+  };
+  for (int c = 0; c < (int)e->cls.size(); ++c) {
+    size_t g = 0;
+    while (g < groups.size() && !(groups[g].size() < (size_t)MAX_GROUP && key_eq(groups[g][0], c))) ++g;
+    if (g == groups.size()) groups.emplace_back();
+    groups[g].push_back(c);
+  }
+  return groups;
+}
+
+// One ensemble step queued behind `st`.  Per class (default): each class
+// advances with its own launches, on its own stream when concurrent (fork /
+// join through events, which stream capture turns into parallel graph
+// branches), else all on `st`.  Merged: each launch group advances with one
+// launch per kernel phase, the groups on their own streams when concurrent.
 int ens_queue(somaxhip_ens e, float* q, int nsteps, double dt, void* ws, hipStream_t st) {
   const int C = (int)e->cls.size();
   // every per-class table is indexed by the class below
@@ -1379,18 +1573,31 @@ int ens_queue(somaxhip_ens e, float* q, int nsteps, double dt, void* ws, hipStre
     set_error("somaxhip_ens_step: inconsistent ensemble tables");
     return SOMAXHIP_ERR_ARG;
   }
-  const bool conc = (e->flags & SOMAXHIP_ENS_CONCURRENT) && C > 1;
+  const bool merged = (e->flags & SOMAXHIP_ENS_MERGED) != 0;
+  const std::vector<std::vector<int>> groups = merged ? ens_groups(e) : std::vector<std::vector<int>>{};
+  const int lanes = merged ? (int)groups.size() : C;  // independent chains (<= C)
+  const bool conc = (e->flags & SOMAXHIP_ENS_CONCURRENT) && lanes > 1;
   if (conc) {
     SOMAX_HIP_CHECK(hipEventRecord(e->ev_fork, st));
-    for (int c = 0; c < C; ++c) SOMAX_HIP_CHECK(hipStreamWaitEvent(e->side[c], e->ev_fork, 0));
+    for (int c = 0; c < lanes; ++c) SOMAX_HIP_CHECK(hipStreamWaitEvent(e->side[c], e->ev_fork, 0));
   }
-  for (int c = 0; c < C; ++c) {
-    const Batch& b = *e->cls[c];
-    const CallParams cp{e->cparams[c], MemberParams{0, 0, 0, 0}, e->cbiharm[c]};
-    rk3(b, q + e->cstate[c], nullptr, nsteps, dt, cp, static_cast<char*>(ws) + e->cws[c], conc ? e->side[c] : st);
+  for (int c = 0; c < lanes; ++c) {
+    hipStream_t s = conc ? e->side[c] : st;
+    if (merged) {
+      std::vector<RkItem> it;
+      for (int k : groups[c]) {
+        const CallParams cp{e->cparams[k], MemberParams{0, 0, 0, 0}, e->cbiharm[k]};
+        it.push_back(RkItem{e->cls[k].get(), q + e->cstate[k], cp, static_cast<char*>(ws) + e->cws[k]});
+      }
+      rk3_group(it, nsteps, dt, s);
+    } else {
+      const Batch& b = *e->cls[c];
+      const CallParams cp{e->cparams[c], MemberParams{0, 0, 0, 0}, e->cbiharm[c]};
+      rk3(b, q + e->cstate[c], nullptr, nsteps, dt, cp, static_cast<char*>(ws) + e->cws[c], s);
+    }
   }
   if (conc) {
-    for (int c = 0; c < C; ++c) {
+    for (int c = 0; c < lanes; ++c) {
       SOMAX_HIP_CHECK(hipEventRecord(e->ev_join[c], e->side[c]));
       SOMAX_HIP_CHECK(hipStreamWaitEvent(st, e->ev_join[c], 0));
     }
@@ -1452,11 +1659,12 @@ int somaxhip_ens_step_profile(somaxhip_ens e, float* q, int nsteps, double dt, v
     set_error("somaxhip_ens_step_profile: null argument or negative nsteps");
     return SOMAXHIP_ERR_ARG;
   }
-  // serial and eager, so every launch is timed on the caller's stream
+  // serial and eager, so every launch is timed on the caller's stream (merged
+  // launch groups stay merged)
   hipStream_t st = reinterpret_cast<hipStream_t>(stream);
   e->stepped = true;
   const int saved = e->flags;
-  e->flags = 0;
+  e->flags = saved & SOMAXHIP_ENS_MERGED;
   Profile prof;
   g_prof = &prof;
   const int rc = ens_queue(e, q, nsteps, dt, ws, st);

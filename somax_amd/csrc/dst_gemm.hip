@@ -55,14 +55,16 @@ __device__ inline void zero_acc(floatx16 (&acc)[TM][TN], double (&sum)[TM][TN][1
       }
 }
 
-// Linear block -> (plane, A-tile, B-tile).  Blocks are dealt to the 8 XCDs
-// round-robin; this remap hands each XCD a contiguous run of tiles (same
-// plane, neighbouring A rows, all B tiles) so they share its L2.
-__device__ inline void tile_of_block(int GA, int GB, int T, int& z, int& ta, int& tb) {
-  const int b = blockIdx.x;
+// Blocks are dealt to the 8 XCDs round-robin; this remap of block b of the
+// T tile blocks hands each XCD a contiguous run of tiles (same plane,
+// neighbouring A rows, all B tiles) so they share its L2.
+__device__ inline int remap_tiles(int b, int T) {
   const int xcd = b & 7, slot = b >> 3;
   const int per = T >> 3, extra = T & 7;
-  const int t = xcd * per + (xcd < extra ? xcd : extra) + slot;
+  return xcd * per + (xcd < extra ? xcd : extra) + slot;
+}
+// Tile t -> (plane, A-tile, B-tile).
+__device__ inline void tile_of(int t, int GA, int GB, int& z, int& ta, int& tb) {
   z = t / (GA * GB);
   const int rem = t - z * GA * GB;
   ta = rem / GB;
@@ -680,22 +682,56 @@ struct KernelShape {
 
 // One instantiation per largest product count of a launch (x1, x3, x6), so
 // each is a kernel symbol of its own; planes may use fewer products.
-template <int NPMAX>
+// The tile count of a transform (its fringe workgroups follow).
+__device__ inline int gemm_tiles(const GemmArgs& a) {
+  const GemmCut cut = gemm_cut(a.RA, a.RB, a.K);
+  return ((cut.am + BM - 1) / BM) * ((cut.bm + BN - 1) / BN) * a.nplanes;
+}
+
+// Transforms of one launch: G == 1 for a single transform, else up to
+// MAX_GROUP (the batches of an ensemble), each transform's blocks mapped to
+// tiles and XCDs as in its own launch.  Block lb of a transform is its tile
+// lb (in the XCD-contiguous tile order) when lb < T, else its fringe
+// workgroup lb - T; -1 marks the padding blocks of a merged launch.
+template <int G>
+struct GemmTable {
+  BlockTable t;
+  GemmArgs a[G];
+};
+template <int G>
+__device__ inline int gemm_block(const GemmTable<G>& g, int& c) {
+  if (G == 1) {
+    c = 0;
+    const int T = gemm_tiles(g.a[0]);
+    const int b = blockIdx.x;
+    return b < T ? remap_tiles(b, T) : b;
+  }
+  int l;
+  if (!table_class(g.t, blockIdx.x, c, l)) return -1;  // padding between transforms
+  const int T = gemm_tiles(g.a[c]);
+  return l < T ? remap_tiles(l, T) : l;
+}
+
+template <int NPMAX, int G>
 __global__ __launch_bounds__(KernelShape<NPMAX>::NT)
 __attribute__((amdgpu_waves_per_eu(KernelShape<NPMAX>::WPE, KernelShape<NPMAX>::WPE)))
-void dst_gemm_bf16_kernel(GemmArgs a) {
+void dst_gemm_bf16_kernel(GemmTable<G> g) {
   constexpr int TM = KernelShape<NPMAX>::TM, TN = KernelShape<NPMAX>::TN, NT = KernelShape<NPMAX>::NT;
   extern __shared__ __attribute__((aligned(16))) uint16_t lds16[];
+  int c;
+  const int lb = gemm_block<G>(g, c);
+  if (lb < 0) return;
+  const GemmArgs& a = g.a[c];
   const GemmCut cut = gemm_cut(a.RA, a.RB, a.K);
   const int GA = (cut.am + BM - 1) / BM, GB = (cut.bm + BN - 1) / BN;
   const int T = GA * GB * a.nplanes;
-  const bool fringe = (int)blockIdx.x >= T;
+  const bool fringe = lb >= T;
   int zi, ta = 0, tb = 0, part = 0;
   if (fringe) {
-    zi = ((int)blockIdx.x - T) / cut.fpw;
-    part = ((int)blockIdx.x - T) - zi * cut.fpw;
+    zi = (lb - T) / cut.fpw;
+    part = (lb - T) - zi * cut.fpw;
   } else {
-    tile_of_block(GA, GB, T, zi, ta, tb);
+    tile_of(lb, GA, GB, zi, ta, tb);
   }
   const long z = zi;
   const PlaneSel ps = select_plane(a, z);
@@ -732,26 +768,29 @@ void dst_gemm_bf16_kernel(GemmArgs a) {
   if (NPMAX == 6 && np == 6 && a.acc64) store_tile<true, TM, TN>(a, z, a0, b0, acc, sum, opieces, stage, ps.sp, lam);
   else store_tile<false, TM, TN>(a, z, a0, b0, acc, sum, opieces, stage, ps.sp, lam);
 }
-template __global__ void dst_gemm_bf16_kernel<1>(GemmArgs);
-template __global__ void dst_gemm_bf16_kernel<3>(GemmArgs);
-template __global__ void dst_gemm_bf16_kernel<6>(GemmArgs);
+
 
 // ---------------------------------------------------------------------------
 // FP32 kernel (plain step): register-staged double buffer, FP64 flush every
 // 4 k-tiles of 16.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 2))) void dst_gemm_f32_kernel(GemmArgs a) {
+template <int G>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 2))) void dst_gemm_f32_kernel(GemmTable<G> g) {
   __shared__ __attribute__((aligned(16))) float lds[2 * 2 * TILE32];
+  int c;
+  const int lb = gemm_block<G>(g, c);
+  if (lb < 0) return;
+  const GemmArgs& a = g.a[c];
   const GemmCut cut = gemm_cut(a.RA, a.RB, a.K);
   const int GA = (cut.am + BM - 1) / BM, GB = (cut.bm + BN - 1) / BN;
   const int T = GA * GB * a.nplanes;
-  if ((int)blockIdx.x >= T) {
-    const int zf = ((int)blockIdx.x - T) / cut.fpw;
-    gemm_fringe<true, 1, 256>(a, zf, ((int)blockIdx.x - T) - zf * cut.fpw, cut, select_plane(a, zf), 0, lds);
+  if (lb >= T) {
+    const int zf = (lb - T) / cut.fpw;
+    gemm_fringe<true, 1, 256>(a, zf, (lb - T) - zf * cut.fpw, cut, select_plane(a, zf), 0, lds);
     return;
   }
   int zi, ta, tb;
-  tile_of_block(GA, GB, T, zi, ta, tb);
+  tile_of(lb, GA, GB, zi, ta, tb);
   const long z = zi;
   const PlaneSel ps = select_plane(a, z);
   const int a0 = ta * BM;
@@ -854,41 +893,90 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 2))) voi
   else store_tile<false, 2, 2>(a, z, a0, b0, acc, sum, 0, lds, ps.sp);
 }
 
+
+template __global__ void dst_gemm_bf16_kernel<1, 1>(GemmTable<1>);
+template __global__ void dst_gemm_bf16_kernel<3, 1>(GemmTable<1>);
+template __global__ void dst_gemm_bf16_kernel<6, 1>(GemmTable<1>);
+template __global__ void dst_gemm_bf16_kernel<1, MAX_GROUP>(GemmTable<MAX_GROUP>);
+template __global__ void dst_gemm_bf16_kernel<3, MAX_GROUP>(GemmTable<MAX_GROUP>);
+template __global__ void dst_gemm_bf16_kernel<6, MAX_GROUP>(GemmTable<MAX_GROUP>);
+
 hipError_t prepare_dst_gemm() {
   // up to 144 KiB of the 160 KiB LDS of a gfx950 CU (3 pieces, 3 stages);
   // set outside any stream capture, before the first launch
   static hipError_t status = [] {
-    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(dst_gemm_bf16_kernel<1>),
+    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(dst_gemm_bf16_kernel<1, 1>),
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)dst_gemm_lds_bytes(1));
     if (e == hipSuccess)
-      e = hipFuncSetAttribute(reinterpret_cast<const void*>(dst_gemm_bf16_kernel<3>),
+      e = hipFuncSetAttribute(reinterpret_cast<const void*>(dst_gemm_bf16_kernel<3, 1>),
                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)dst_gemm_lds_bytes(3));
     if (e == hipSuccess)
-      e = hipFuncSetAttribute(reinterpret_cast<const void*>(dst_gemm_bf16_kernel<6>),
+      e = hipFuncSetAttribute(reinterpret_cast<const void*>(dst_gemm_bf16_kernel<6, 1>),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)dst_gemm_lds_bytes(6));
+    if (e == hipSuccess)
+      e = hipFuncSetAttribute(reinterpret_cast<const void*>(dst_gemm_bf16_kernel<1, MAX_GROUP>),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)dst_gemm_lds_bytes(1));
+    if (e == hipSuccess)
+      e = hipFuncSetAttribute(reinterpret_cast<const void*>(dst_gemm_bf16_kernel<3, MAX_GROUP>),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)dst_gemm_lds_bytes(3));
+    if (e == hipSuccess)
+      e = hipFuncSetAttribute(reinterpret_cast<const void*>(dst_gemm_bf16_kernel<6, MAX_GROUP>),
                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)dst_gemm_lds_bytes(6));
     return e;
   }();
   return status;
 }
 
-void launch_dst_gemm(const GemmArgs& args, int batch, hipStream_t st) {
-  GemmArgs a = args;
-  a.nplanes = batch;
+namespace {
+int gemm_blocks(const GemmArgs& a) {
   const GemmCut cut = gemm_cut(a.RA, a.RB, a.K);
-  const int tiles = ((cut.bm + BN - 1) / BN) * ((cut.am + BM - 1) / BM) * batch;
-  const int blocks = tiles + cut.fpw * batch;
+  const int tiles = ((cut.bm + BN - 1) / BN) * ((cut.am + BM - 1) / BM) * a.nplanes;
+  return tiles + cut.fpw * a.nplanes;
+}
+}  // namespace
+
+void launch_dst_gemm_group(const GemmArgs* args, const int* batch, int n, hipStream_t st) {
+  GemmTable<MAX_GROUP> g{};
+  int max_np = 0;
+  for (int i = 0; i < n; ++i) {
+    g.a[i] = args[i];
+    g.a[i].nplanes = batch[i];
+    table_add(g.t, gemm_blocks(g.a[i]), 1, 1);
+    const int np = args[i].np_lo > args[i].np_hi ? args[i].np_lo : args[i].np_hi;
+    max_np = np > max_np ? np : max_np;
+  }
+  const dim3 grid(g.t.start[g.t.n]);
+  if (args[0].Xf) {
+    hipLaunchKernelGGL(dst_gemm_f32_kernel<MAX_GROUP>, grid, dim3(256), 0, st, g);
+    return;
+  }
+  const size_t lds = dst_gemm_lds_bytes(max_np);
+  if (max_np == 1)
+    hipLaunchKernelGGL((dst_gemm_bf16_kernel<1, MAX_GROUP>), grid, dim3(KernelShape<1>::NT), lds, st, g);
+  else if (max_np == 3)
+    hipLaunchKernelGGL((dst_gemm_bf16_kernel<3, MAX_GROUP>), grid, dim3(KernelShape<3>::NT), lds, st, g);
+  else
+    hipLaunchKernelGGL((dst_gemm_bf16_kernel<6, MAX_GROUP>), grid, dim3(KernelShape<6>::NT), lds, st, g);
+}
+
+void launch_dst_gemm(const GemmArgs& args, int batch, hipStream_t st) {
+  GemmTable<1> g{};
+  g.a[0] = args;
+  g.a[0].nplanes = batch;
+  const GemmArgs& a = g.a[0];
+  const int blocks = gemm_blocks(a);
   if (a.Xf) {
-    hipLaunchKernelGGL(dst_gemm_f32_kernel, dim3(blocks), dim3(256), 0, st, a);
+    hipLaunchKernelGGL(dst_gemm_f32_kernel<1>, dim3(blocks), dim3(256), 0, st, g);
     return;
   }
   const int max_np = a.np_lo > a.np_hi ? a.np_lo : a.np_hi;
   const size_t lds = dst_gemm_lds_bytes(max_np);
   if (max_np == 1)
-    hipLaunchKernelGGL(dst_gemm_bf16_kernel<1>, dim3(blocks), dim3(KernelShape<1>::NT), lds, st, a);
+    hipLaunchKernelGGL((dst_gemm_bf16_kernel<1, 1>), dim3(blocks), dim3(KernelShape<1>::NT), lds, st, g);
   else if (max_np == 3)
-    hipLaunchKernelGGL(dst_gemm_bf16_kernel<3>, dim3(blocks), dim3(KernelShape<3>::NT), lds, st, a);
+    hipLaunchKernelGGL((dst_gemm_bf16_kernel<3, 1>), dim3(blocks), dim3(KernelShape<3>::NT), lds, st, g);
   else
-    hipLaunchKernelGGL(dst_gemm_bf16_kernel<6>, dim3(blocks), dim3(KernelShape<6>::NT), lds, st, a);
+    hipLaunchKernelGGL((dst_gemm_bf16_kernel<6, 1>), dim3(blocks), dim3(KernelShape<6>::NT), lds, st, g);
 }
 
 }  // namespace somax

@@ -29,6 +29,64 @@ __device__ inline void xcd_tile3(int& bx, int& by, int& bz) {
   bz = (int)(t / (nx * ny));
 }
 
+// Merged launches.  One launch may cover the blocks of up to MAX_GROUP
+// batches (the grid classes of an ensemble), so that a kernel phase of a
+// Runge-Kutta stage is one launch for all of them: batch c owns the linear
+// blocks [start[c], start[c] + count[c]) and sees them as its own
+// gx x gy x gz grid, x fastest.  Every start is a multiple of 8, so a batch's
+// block l runs on the XCD that its own launch would give it (blocks are dealt
+// to the 8 XCDs round-robin); the blocks between count[c] and the next start
+// exit at once.  Each batch's arithmetic is exactly that of its own launch.
+constexpr int MAX_GROUP = 8;
+struct BlockTable {
+  int n;
+  int start[MAX_GROUP + 1];
+  int count[MAX_GROUP];
+  int gx[MAX_GROUP], gy[MAX_GROUP], gxy[MAX_GROUP];
+  float rgx[MAX_GROUP], rgxy[MAX_GROUP];  // 1 / gx, 1 / (gx * gy)
+};
+inline void table_add(BlockTable& t, int gx, int gy, int gz) {
+  const int count = gx * gy * gz;  // below 2^24 (exact in FP32), as every launch here
+  t.gx[t.n] = gx;
+  t.gy[t.n] = gy;
+  t.gxy[t.n] = gx * gy;
+  t.rgx[t.n] = 1.0f / (float)gx;
+  t.rgxy[t.n] = 1.0f / (float)(gx * gy);
+  t.count[t.n] = count;
+  t.start[t.n + 1] = t.start[t.n] + (count + 7) / 8 * 8;
+  ++t.n;
+}
+// Batch c of linear block b and the block's index l in that batch; false for
+// the padding blocks between batches.  The starts are read with constant
+// indices, so they arrive in one batch of scalar loads.
+__device__ inline bool table_class(const BlockTable& t, int b, int& c, int& l) {
+  c = 0;
+#pragma unroll
+  for (int i = 1; i < MAX_GROUP; ++i) c += (i < t.n && b >= t.start[i]) ? 1 : 0;
+  l = b - t.start[c];
+  return l < t.count[c];
+}
+// v / d for 0 <= v < 2^24 from the FP32 reciprocal r = 1/d (one correction step)
+__device__ inline int small_div(int v, int d, float r) {
+  int q = (int)((float)v * r);
+  if (q * d > v) --q;
+  else if ((q + 1) * d <= v) ++q;
+  return q;
+}
+// Block (x, y, z) of batch c's grid for its block index l.
+__device__ inline void class_block(const BlockTable& t, int c, int l, int& x, int& y, int& z) {
+  z = small_div(l, t.gxy[c], t.rgxy[c]);
+  const int r = l - z * t.gxy[c];
+  y = small_div(r, t.gx[c], t.rgx[c]);
+  x = r - y * t.gx[c];
+}
+// Block b of T dealt to the XCDs round-robin -> the index that hands each XCD
+// one contiguous run (the order xcd_tile3 gives a 3-D grid).
+__device__ inline int xcd_order(int b, int T) {
+  const unsigned xcd = (unsigned)b & 7u, slot = (unsigned)b >> 3, per = (unsigned)T >> 3, extra = (unsigned)T & 7u;
+  return (int)(xcd * per + (xcd < extra ? xcd : extra) + slot);
+}
+
 // Planes, rows and columns (ghosts included) of a QG plan's fields.
 int qg_plan_shape(somaxhip_qg_plan_s* plan, int* planes, int* Ny, int* Nx);
 
@@ -130,6 +188,9 @@ __host__ __device__ inline GemmCut gemm_cut(int RA, int RB, int K) {
 }
 
 void launch_dst_gemm(const GemmArgs& a, int batch, hipStream_t st);
+// One launch for n (<= MAX_GROUP) transforms of the same product count and
+// operand type; transform i covers batch[i] planes.
+void launch_dst_gemm_group(const GemmArgs* a, const int* batch, int n, hipStream_t st);
 // Raises the bf16 kernel's dynamic LDS limit; call before the first launch.
 hipError_t prepare_dst_gemm();
 
@@ -163,6 +224,7 @@ struct ModalArgs {
 // (k'+1, N-2-k') first, the ghost pair (0, N-1) last.
 __host__ __device__ inline int fold_phys(int k, int h) { return k == h - 1 ? 0 : k + 1; }
 void launch_to_modal(const ModalArgs& a, hipStream_t st);
+void launch_to_modal_group(const ModalArgs* a, int n, hipStream_t st);  // n <= MAX_GROUP batches
 
 // Per-mode Helmholtz residual on the full ghosted array,
 //   r_z = qm_z - lap_h(psi_z) + lambda_z psi_z,
@@ -186,6 +248,7 @@ struct ResidualArgs {
   int fold;
 };
 void launch_residual(const ResidualArgs& a, hipStream_t st);
+void launch_residual_group(const ResidualArgs* a, int n, hipStream_t st);
 
 // Mode -> layer projection of the streamfunction:
 //   psi[first + l] = sum_p cm2l[l][p] * (pm[first + p] + d[first + p])   (d nullable),
@@ -216,6 +279,7 @@ struct LayerArgs {
   int nlmax;  // largest member layer count (0: MAX_LAYERS)
 };
 void launch_to_layer(const LayerArgs& a, hipStream_t st);
+void launch_to_layer_group(const LayerArgs* a, int n, hipStream_t st);
 
 // One QG RHS evaluation fused with the RK stage combination:
 //   out = qn + c1*(qs - qn) + c2*rhs(BC(qs), psi)   (qn given)
@@ -241,6 +305,8 @@ struct StageArgs {
   int tile_rows;  // Arakawa stage tile height: 8 or 16 (0: 16)
 };
 void launch_qg_stage(const StageArgs& a, hipStream_t st);
+// n batches sharing advection, biharm and (Arakawa) tile_rows
+void launch_qg_stage_group(const StageArgs* a, int n, hipStream_t st);
 
 // Ghost ring of nl planes of Ny x Nx set to 0 (finitevolx.zero_boundaries).
 void launch_zero_ghost(float* f, int nl, int Ny, int Nx, hipStream_t st);
@@ -332,7 +398,16 @@ void launch_pair_reduce(const double* partial, int n, int P, double scale, doubl
 // Mass correction of the modal streamfunction (homogeneous solution method):
 // per plane FP64 block partials of the sum of (a + b) over the full array (b
 // nullable); launch_to_layer turns them into the correction (LayerArgs).
+struct PlaneSumArgs {
+  const float* a;
+  const float* b;
+  long n, plane, chunk;
+  double* partial;
+  int P, nblk;  // planes; blocks (partials) per plane, at most 64
+};
+PlaneSumArgs plane_sums_args(const float* a, const float* b, int Nx, int Ny, int P, long plane, double* partial);
 int launch_plane_sums(const float* a, const float* b, int Nx, int Ny, int P, long plane, double* partial,
                       hipStream_t st);  // returns the partials per plane (at most 64)
+void launch_plane_sums_group(const PlaneSumArgs* a, int n, hipStream_t st);
 
 }  // namespace somax

@@ -19,16 +19,15 @@ constexpr int QW = TX + 4;  // q tile pitch (halo 2)
 constexpr int PW = TX + 2;  // psi / lap tile pitch (halo 1)
 }  // namespace
 
+// Tile (bx, by) of plane z, of a grid of nbx x nby tiles per plane.
 template <bool BIHARM, int TY>
-__global__ __launch_bounds__(256) void qg_stage_arakawa_kernel(StageArgs a) {
+__device__ __forceinline__ void qg_stage_arakawa(const StageArgs& a, int bx, int by, int z, int nbx, int nby) {
   __shared__ float sq[(TY + 4) * QW];
   __shared__ float sp[(TY + 2) * PW];
   __shared__ float sl[BIHARM ? (TY + 2) * PW : 1];
   __shared__ float sdb[TY + 4];
 
   const int tid = threadIdx.x;
-  int bx, by, z;
-  xcd_tile3(bx, by, z);
   // Tiles cover the interior block (rows and columns 1 .. N-2); the edge
   // tiles also write the ghost ring (after the interior outputs).
   const int x0 = 1 + bx * TX;
@@ -167,8 +166,8 @@ __global__ __launch_bounds__(256) void qg_stage_arakawa_kernel(StageArgs a) {
   // (lanes 32..32+TY-1, last tile column); wave 3: the four corners.
   const int wv = __builtin_amdgcn_readfirstlane(tyb);
   const int gx = x0 + tx;
-  const bool first_x = bx == 0, last_x = bx == (int)gridDim.x - 1;
-  const bool first_y = by == 0, last_y = by == (int)gridDim.y - 1;
+  const bool first_x = bx == 0, last_x = bx == nbx - 1;
+  const bool first_y = by == 0, last_y = by == nby - 1;
   int ry = -1, rx = -1;
   if (wv == 0 && first_y && gx <= Nx - 2) {
     ry = 0;
@@ -263,8 +262,9 @@ __device__ inline float face_corr(int f, int n, float vel, const float* dm) {
 }
 }  // namespace
 
+// Tile (bx, by) of plane z, of a grid of nbx x nby tiles per plane.
 template <bool BIHARM>
-__global__ __launch_bounds__(256) void qg_stage_weno_kernel(StageArgs a) {
+__device__ __forceinline__ void qg_stage_weno(const StageArgs& a, int bx, int by, int z, int nbx, int nby) {
   constexpr int QH = WY + 6;    // q tile rows (halo 3)
   constexpr int PH = WY + 2;    // psi / Laplacian tile rows (halo 1)
   constexpr int PW2 = WX + 2;   // psi / Laplacian tile pitch
@@ -277,8 +277,6 @@ __global__ __launch_bounds__(256) void qg_stage_weno_kernel(StageArgs a) {
   __shared__ float sdb[QH];                     // b[j+1] - b[j] for rows y0-3 ..
 
   const int tid = threadIdx.x;
-  int bx, by, z;
-  xcd_tile3(bx, by, z);
   // Tiles cover the interior block (rows and columns 1 .. N-2); the edge
   // tiles also write the ghost ring (after the interior outputs).
   const int x0 = 1 + bx * WX;
@@ -493,8 +491,8 @@ __global__ __launch_bounds__(256) void qg_stage_weno_kernel(StageArgs a) {
   // Wave 0: row 0 under the first tile row; wave 1: row Ny-1 under the last;
   // wave 2: column 0 (lanes 0..WY-1, first tile column) and column Nx-1
   // (lanes 32..32+WY-1, last tile column); wave 3: the four corners.
-  const bool first_x = bx == 0, last_x = bx == (int)gridDim.x - 1;
-  const bool first_y = by == 0, last_y = by == (int)gridDim.y - 1;
+  const bool first_x = bx == 0, last_x = bx == nbx - 1;
+  const bool first_y = by == 0, last_y = by == nby - 1;
   int ry = -1, rx = -1;
   if (wv == 0 && first_y && gx_out) {
     ry = 0;
@@ -522,21 +520,85 @@ __global__ __launch_bounds__(256) void qg_stage_weno_kernel(StageArgs a) {
   }
 }
 
+template <bool BIHARM, int TY>
+__global__ __launch_bounds__(256) void qg_stage_arakawa_kernel(StageArgs a) {
+  int bx, by, z;
+  xcd_tile3(bx, by, z);
+  qg_stage_arakawa<BIHARM, TY>(a, bx, by, z, gridDim.x, gridDim.y);
+}
+template <bool BIHARM>
+__global__ __launch_bounds__(256) void qg_stage_weno_kernel(StageArgs a) {
+  int bx, by, z;
+  xcd_tile3(bx, by, z);
+  qg_stage_weno<BIHARM>(a, bx, by, z, gridDim.x, gridDim.y);
+}
+
+// Merged launch over the batches of a group: the tiles of all batches in one
+// XCD-contiguous order (the order xcd_tile3 gives one batch).
+struct StageGroup {
+  BlockTable t;
+  StageArgs a[MAX_GROUP];
+};
+template <bool BIHARM, int TY>
+__global__ __launch_bounds__(256) void qg_stage_arakawa_group_kernel(StageGroup g) {
+  int c, l, bx, by, z;
+  if (!table_class(g.t, blockIdx.x, c, l)) return;
+  class_block(g.t, c, xcd_order(l, g.t.count[c]), bx, by, z);
+  qg_stage_arakawa<BIHARM, TY>(g.a[c], bx, by, z, g.t.gx[c], g.t.gy[c]);
+}
+template <bool BIHARM>
+__global__ __launch_bounds__(256) void qg_stage_weno_group_kernel(StageGroup g) {
+  int c, l, bx, by, z;
+  if (!table_class(g.t, blockIdx.x, c, l)) return;
+  class_block(g.t, c, xcd_order(l, g.t.count[c]), bx, by, z);
+  qg_stage_weno<BIHARM>(g.a[c], bx, by, z, g.t.gx[c], g.t.gy[c]);
+}
+
+namespace {
+dim3 stage_grid(const StageArgs& a) {
+  const int tw = a.advection == 1 ? WX : TX;
+  const int th = a.advection == 1 ? WY : (a.tile_rows ? a.tile_rows : TY_DEFAULT);
+  return dim3((a.Nx - 2 + tw - 1) / tw, (a.Ny - 2 + th - 1) / th, a.P);  // interior tiles
+}
+}  // namespace
+
 void launch_qg_stage(const StageArgs& a, hipStream_t st) {
+  const dim3 g = stage_grid(a);
   if (a.advection == 1) {
-    dim3 g((a.Nx - 2 + WX - 1) / WX, (a.Ny - 2 + WY - 1) / WY, a.P);  // interior tiles
     if (a.biharm) hipLaunchKernelGGL(qg_stage_weno_kernel<true>, g, dim3(256), 0, st, a);
     else hipLaunchKernelGGL(qg_stage_weno_kernel<false>, g, dim3(256), 0, st, a);
     return;
   }
   const int ty = a.tile_rows ? a.tile_rows : TY_DEFAULT;
-  const dim3 grid((a.Nx - 2 + TX - 1) / TX, (a.Ny - 2 + ty - 1) / ty, a.P);  // interior tiles
   if (ty == 8) {
-    if (a.biharm) hipLaunchKernelGGL((qg_stage_arakawa_kernel<true, 8>), grid, dim3(256), 0, st, a);
-    else hipLaunchKernelGGL((qg_stage_arakawa_kernel<false, 8>), grid, dim3(256), 0, st, a);
+    if (a.biharm) hipLaunchKernelGGL((qg_stage_arakawa_kernel<true, 8>), g, dim3(256), 0, st, a);
+    else hipLaunchKernelGGL((qg_stage_arakawa_kernel<false, 8>), g, dim3(256), 0, st, a);
   } else {
-    if (a.biharm) hipLaunchKernelGGL((qg_stage_arakawa_kernel<true, 16>), grid, dim3(256), 0, st, a);
-    else hipLaunchKernelGGL((qg_stage_arakawa_kernel<false, 16>), grid, dim3(256), 0, st, a);
+    if (a.biharm) hipLaunchKernelGGL((qg_stage_arakawa_kernel<true, 16>), g, dim3(256), 0, st, a);
+    else hipLaunchKernelGGL((qg_stage_arakawa_kernel<false, 16>), g, dim3(256), 0, st, a);
+  }
+}
+
+void launch_qg_stage_group(const StageArgs* a, int n, hipStream_t st) {
+  StageGroup g{};
+  for (int i = 0; i < n; ++i) {
+    g.a[i] = a[i];
+    const dim3 d = stage_grid(a[i]);
+    table_add(g.t, (int)d.x, (int)d.y, (int)d.z);
+  }
+  const dim3 grid(g.t.start[g.t.n]);
+  if (a[0].advection == 1) {
+    if (a[0].biharm) hipLaunchKernelGGL(qg_stage_weno_group_kernel<true>, grid, dim3(256), 0, st, g);
+    else hipLaunchKernelGGL(qg_stage_weno_group_kernel<false>, grid, dim3(256), 0, st, g);
+    return;
+  }
+  const int ty = a[0].tile_rows ? a[0].tile_rows : TY_DEFAULT;
+  if (ty == 8) {
+    if (a[0].biharm) hipLaunchKernelGGL((qg_stage_arakawa_group_kernel<true, 8>), grid, dim3(256), 0, st, g);
+    else hipLaunchKernelGGL((qg_stage_arakawa_group_kernel<false, 8>), grid, dim3(256), 0, st, g);
+  } else {
+    if (a[0].biharm) hipLaunchKernelGGL((qg_stage_arakawa_group_kernel<true, 16>), grid, dim3(256), 0, st, g);
+    else hipLaunchKernelGGL((qg_stage_arakawa_group_kernel<false, 16>), grid, dim3(256), 0, st, g);
   }
 }
 
@@ -578,11 +640,12 @@ __device__ inline void store_pieces(uint16_t* o, long piece, int n, float v) {
 }
 }  // namespace
 
-__global__ __launch_bounds__(256) void residual_kernel(ResidualArgs a) {
-  const int x = blockIdx.x * blockDim.x + threadIdx.x;  // up to op_kpad
-  const int y = blockIdx.y;
-  const int z = blockIdx.z;
-  if (x >= a.op_kpad) return;
+// block (bx, z) of the (ceil(Ny * op_kpad / 256), P) grid: row-major over
+// the operand's Ny x op_kpad
+__device__ __forceinline__ void residual_plain(const ResidualArgs& a, int bx, int z) {
+  const int i = bx * blockDim.x + threadIdx.x;
+  const int y = i / a.op_kpad, x = i - y * a.op_kpad;
+  if (y >= a.Ny) return;
   uint16_t* o = a.r + z * a.op_batch + (long)y * a.op_ld + x;
   if (x >= a.Nx) {
     store_pieces(o, a.op_piece, 2, 0.0f);
@@ -605,10 +668,10 @@ __global__ __launch_bounds__(256) void residual_kernel(ResidualArgs a) {
 // b * P + m): the 5-point Laplacian of a component on the half array, with
 // zero exterior and the mirror rule at the fold line (the neighbour past the
 // last physical index h-1 is +-the value itself: even / odd parity).
-__global__ __launch_bounds__(256) void residual_fold_kernel(ResidualArgs a) {
+// block (bx, z) of the (ceil(hy * op_kpad / 256), 4P) grid
+__device__ __forceinline__ void residual_fold(const ResidualArgs& a, int bx, int z) {
   const int hx = a.Nx / 2, hy = a.Ny / 2;
-  const int idx = blockIdx.x * blockDim.x + threadIdx.x;  // row-major over hy x op_kpad
-  const int z = blockIdx.y;
+  const int idx = bx * blockDim.x + threadIdx.x;  // row-major over hy x op_kpad
   if (idx >= hy * a.op_kpad) return;
   const int ky = idx / a.op_kpad, kx = idx - ky * a.op_kpad;
   uint16_t* o = a.r + z * a.op_batch + (long)ky * a.op_ld + kx;
@@ -636,26 +699,57 @@ __global__ __launch_bounds__(256) void residual_fold_kernel(ResidualArgs a) {
   store_pieces(o, a.op_piece, 2, r);
 }
 
+__global__ __launch_bounds__(256) void residual_kernel(ResidualArgs a) { residual_plain(a, blockIdx.x, blockIdx.y); }
+__global__ __launch_bounds__(256) void residual_fold_kernel(ResidualArgs a) { residual_fold(a, blockIdx.x, blockIdx.y); }
+
+struct ResidualGroup {
+  BlockTable t;
+  ResidualArgs a[MAX_GROUP];
+};
+__global__ __launch_bounds__(256) void residual_group_kernel(ResidualGroup g) {
+  int c, l, x, y, z;
+  if (!table_class(g.t, blockIdx.x, c, l)) return;
+  class_block(g.t, c, l, x, y, z);
+  if (g.a[c].fold) residual_fold(g.a[c], x, y);
+  else residual_plain(g.a[c], x, y);
+}
+
+namespace {
+dim3 residual_grid(const ResidualArgs& a) {
+  if (a.fold) return dim3((unsigned)(((long)a.op_kpad * (a.Ny / 2) + 255) / 256), 4 * a.P);
+  return dim3((unsigned)(((long)a.op_kpad * a.Ny + 255) / 256), a.P);
+}
+}  // namespace
+
 void launch_residual(const ResidualArgs& a, hipStream_t st) {
-  if (a.fold) {
-    dim3 grid((unsigned)(((long)a.op_kpad * (a.Ny / 2) + 255) / 256), 4 * a.P);
-    hipLaunchKernelGGL(residual_fold_kernel, grid, dim3(256), 0, st, a);
-    return;
+  const dim3 g = residual_grid(a);
+  if (a.fold) hipLaunchKernelGGL(residual_fold_kernel, g, dim3(256), 0, st, a);
+  else hipLaunchKernelGGL(residual_kernel, g, dim3(256), 0, st, a);
+}
+
+void launch_residual_group(const ResidualArgs* a, int n, hipStream_t st) {
+  ResidualGroup g{};
+  for (int i = 0; i < n; ++i) {
+    g.a[i] = a[i];
+    const dim3 d = residual_grid(a[i]);
+    table_add(g.t, (int)d.x, (int)d.y, (int)d.z);
   }
-  dim3 grid((a.op_kpad + 255) / 256, a.Ny, a.P);
-  hipLaunchKernelGGL(residual_kernel, grid, dim3(256), 0, st, a);
+  hipLaunchKernelGGL(residual_group_kernel, dim3(g.t.start[g.t.n]), dim3(256), 0, st, g);
 }
 
 // ---------------------------------------------------------------------------
 // layer -> mode projection in FP64, writing the first GEMM operand
 // (one thread per cell of one member: its layers are read once)
 // ---------------------------------------------------------------------------
+// block (bx, mem) of the (ceil(Ny * width / 256), M) grid: row-major over
+// Ny x width, width = max(Nx, op_kpad + off)
 template <int NLM>
-__global__ __launch_bounds__(256) void to_modal_kernel(ModalArgs a) {
-  const int x = blockIdx.x * blockDim.x + threadIdx.x;  // state column
-  const int y = blockIdx.y;
-  const int mem = blockIdx.z;
+__device__ __forceinline__ void to_modal_plain(const ModalArgs& a, int bx, int mem) {
   const int off = a.apply_bc ? 1 : 0;
+  const int width = a.Nx > a.op_kpad + off ? a.Nx : a.op_kpad + off;
+  const int e = bx * blockDim.x + threadIdx.x;
+  const int y = e / width, x = e - y * width;  // x: state column
+  if (y >= a.Ny) return;
   const int rows = a.Ny - 2 * off, cols = a.Nx - 2 * off;
   const int ox = x - off, oy = y - off;  // operand coordinates
   const bool inside = x < a.Nx;
@@ -696,12 +790,12 @@ __global__ __launch_bounds__(256) void to_modal_kernel(ModalArgs a) {
 // Folded variant: one thread per folded point (k'y, k'x) of one member reads
 // the four mirror points of every layer, projects them to modes in FP64 and
 // writes the four parity fold sums of each mode.
+// block (bx, mem) of the (ceil(hy * width / 256), M) grid
 template <int NLM>
-__global__ __launch_bounds__(256) void to_modal_fold_kernel(ModalArgs a) {
+__device__ __forceinline__ void to_modal_fold(const ModalArgs& a, int bx, int mem) {
   const int hx = a.Nx / 2, hy = a.Ny / 2;
   const int width = hx > a.op_kpad ? hx : a.op_kpad;
-  const int e = blockIdx.x * blockDim.x + threadIdx.x;  // row-major over hy x width
-  const int mem = blockIdx.y;
+  const int e = bx * blockDim.x + threadIdx.x;  // row-major over hy x width
   if (e >= hy * width) return;
   const int ky = e / width, kx = e - ky * width;
   const int K0 = a.apply_bc ? hx - 1 : hx;  // operand columns carrying data
@@ -771,18 +865,57 @@ __global__ __launch_bounds__(256) void to_modal_fold_kernel(ModalArgs a) {
     else hipLaunchKernelGGL(KERNEL<MAX_LAYERS>, GRID, dim3(256), 0, ST, ARGS);           \
   } while (0)
 
-void launch_to_modal(const ModalArgs& a, hipStream_t st) {
+template <int NLM>
+__global__ __launch_bounds__(256) void to_modal_kernel(ModalArgs a) {
+  to_modal_plain<NLM>(a, blockIdx.x, blockIdx.y);
+}
+template <int NLM>
+__global__ __launch_bounds__(256) void to_modal_fold_kernel(ModalArgs a) {
+  to_modal_fold<NLM>(a, blockIdx.x, blockIdx.y);
+}
+struct ModalGroup {
+  BlockTable t;
+  ModalArgs a[MAX_GROUP];
+};
+template <int NLM>
+__global__ __launch_bounds__(256) void to_modal_group_kernel(ModalGroup g) {
+  int c, l, x, y, z;
+  if (!table_class(g.t, blockIdx.x, c, l)) return;
+  class_block(g.t, c, l, x, y, z);
+  if (g.a[c].fold) to_modal_fold<NLM>(g.a[c], x, y);
+  else to_modal_plain<NLM>(g.a[c], x, y);
+}
+
+namespace {
+dim3 to_modal_grid(const ModalArgs& a) {
   if (a.fold) {
     const int hx = a.Nx / 2;
     const int width = hx > a.op_kpad ? hx : a.op_kpad;
-    dim3 grid((unsigned)(((long)width * (a.Ny / 2) + 255) / 256), a.M);
-    SOMAX_NL_LAUNCH(a.nlmax, to_modal_fold_kernel, grid, st, a);
-    return;
+    return dim3((unsigned)(((long)width * (a.Ny / 2) + 255) / 256), a.M);
   }
   const int off = a.apply_bc ? 1 : 0;
   const int width = a.Nx > a.op_kpad + off ? a.Nx : a.op_kpad + off;
-  dim3 grid((width + 255) / 256, a.Ny, a.M);
-  SOMAX_NL_LAUNCH(a.nlmax, to_modal_kernel, grid, st, a);
+  return dim3((unsigned)(((long)width * a.Ny + 255) / 256), a.M);
+}
+int nl_of(int nlmax) { return nlmax > 0 ? nlmax : MAX_LAYERS; }
+}  // namespace
+
+void launch_to_modal(const ModalArgs& a, hipStream_t st) {
+  const dim3 grid = to_modal_grid(a);
+  if (a.fold) SOMAX_NL_LAUNCH(a.nlmax, to_modal_fold_kernel, grid, st, a);
+  else SOMAX_NL_LAUNCH(a.nlmax, to_modal_kernel, grid, st, a);
+}
+
+void launch_to_modal_group(const ModalArgs* a, int n, hipStream_t st) {
+  ModalGroup g{};
+  int nl = 0;
+  for (int i = 0; i < n; ++i) {
+    g.a[i] = a[i];
+    const dim3 d = to_modal_grid(a[i]);
+    table_add(g.t, (int)d.x, (int)d.y, (int)d.z);
+    nl = nl_of(a[i].nlmax) > nl ? nl_of(a[i].nlmax) : nl;
+  }
+  SOMAX_NL_LAUNCH(nl, to_modal_group_kernel, dim3(g.t.start[g.t.n]), st, g);
 }
 
 // ---------------------------------------------------------------------------
@@ -845,15 +978,15 @@ __device__ inline void member_alpha(const LayerArgs& a, int first, int nl, const
   __syncthreads();
 }
 
+// block (bx, mem) of the (ceil(Ny * Nx / 256), M) grid: row-major over the plane
 template <int NLM>
-__global__ __launch_bounds__(256) void to_layer_kernel(LayerArgs a) {
-  const int x = blockIdx.x * blockDim.x + threadIdx.x;
-  const int y = blockIdx.y;
-  const int mem = blockIdx.z;
+__device__ __forceinline__ void to_layer_plain(const LayerArgs& a, int bx, int mem) {
+  const int e = bx * blockDim.x + threadIdx.x;
+  const int y = e / a.Nx, x = e - y * a.Nx;
   const int first = a.t.mfirst[mem], nl = a.t.mnl[mem];
   const float* cm2l = a.t.cm2l + mem * MAX_LAYERS * MAX_LAYERS;
-  const bool in = x < a.Nx;
-  const int s = in ? y * a.Nx + x : 0;  // in-plane offset (a plane is below 2^31 cells)
+  const bool in = y < a.Ny;
+  const int s = in ? e : 0;  // in-plane offset (a plane is below 2^31 cells)
   const bool ghost = a.t.mask[mem] && (x == 0 || y == 0 || x == a.Nx - 1 || y == a.Ny - 1);
   // loads first, before the mass constraint's barriers, so their latency
   // overlaps the alphas (layers past nl re-read the member's first layer,
@@ -868,7 +1001,7 @@ __global__ __launch_bounds__(256) void to_layer_kernel(LayerArgs a) {
     zh[p] = a.homsol ? a.homsol[i] : 0.0f;
   }
   __shared__ float s_alpha[MAX_LAYERS];
-  if (a.homsol) member_alpha(a, first, nl, cm2l, blockIdx.x == 0 && y == 0, part, s_alpha);
+  if (a.homsol) member_alpha(a, first, nl, cm2l, bx == 0, part, s_alpha);
   if (!in) return;
   float pm[NLM];
 #pragma unroll
@@ -893,11 +1026,11 @@ __global__ __launch_bounds__(256) void to_layer_kernel(LayerArgs a) {
 // Folded variant: one thread per folded point of one member unfolds the
 // parity components of every mode at the four mirror points, adds the mass
 // correction and mixes modes into layers.
+// block (bx, mem) of the (ceil(hy * hx / 256), M) grid
 template <int NLM>
-__global__ __launch_bounds__(256) void to_layer_fold_kernel(LayerArgs a) {
+__device__ __forceinline__ void to_layer_fold(const LayerArgs& a, int bx, int mem) {
   const int hx = a.Nx / 2, hy = a.Ny / 2;
-  const int e = blockIdx.x * blockDim.x + threadIdx.x;  // row-major over hy x hx
-  const int mem = blockIdx.y;
+  const int e = bx * blockDim.x + threadIdx.x;  // row-major over hy x hx
   const int first = a.t.mfirst[mem], nl = a.t.mnl[mem];
   const float* cm2l = a.t.cm2l + mem * MAX_LAYERS * MAX_LAYERS;
   const bool in = e < hy * hx;
@@ -927,7 +1060,7 @@ __global__ __launch_bounds__(256) void to_layer_fold_kernel(LayerArgs a) {
     }
   }
   __shared__ float s_alpha[MAX_LAYERS];
-  if (a.homsol) member_alpha(a, first, nl, cm2l, blockIdx.x == 0, part, s_alpha);
+  if (a.homsol) member_alpha(a, first, nl, cm2l, bx == 0, part, s_alpha);
   if (!in) return;
   float pm[NLM][4];
 #pragma unroll
@@ -963,14 +1096,50 @@ __global__ __launch_bounds__(256) void to_layer_fold_kernel(LayerArgs a) {
   }
 }
 
+template <int NLM>
+__global__ __launch_bounds__(256) void to_layer_kernel(LayerArgs a) {
+  to_layer_plain<NLM>(a, blockIdx.x, blockIdx.y);
+}
+template <int NLM>
+__global__ __launch_bounds__(256) void to_layer_fold_kernel(LayerArgs a) {
+  to_layer_fold<NLM>(a, blockIdx.x, blockIdx.y);
+}
+struct LayerGroup {
+  BlockTable t;
+  LayerArgs a[MAX_GROUP];
+};
+template <int NLM>
+__global__ __launch_bounds__(256) void to_layer_group_kernel(LayerGroup g) {
+  int c, l, x, y, z;
+  if (!table_class(g.t, blockIdx.x, c, l)) return;
+  class_block(g.t, c, l, x, y, z);
+  if (g.a[c].fold) to_layer_fold<NLM>(g.a[c], x, y);
+  else to_layer_plain<NLM>(g.a[c], x, y);
+}
+
+namespace {
+dim3 to_layer_grid(const LayerArgs& a) {
+  if (a.fold) return dim3((unsigned)(((long)(a.Nx / 2) * (a.Ny / 2) + 255) / 256), a.M);
+  return dim3((unsigned)(((long)a.Nx * a.Ny + 255) / 256), a.M);
+}
+}  // namespace
+
 void launch_to_layer(const LayerArgs& a, hipStream_t st) {
-  if (a.fold) {
-    dim3 grid((unsigned)(((long)(a.Nx / 2) * (a.Ny / 2) + 255) / 256), a.M);
-    SOMAX_NL_LAUNCH(a.nlmax, to_layer_fold_kernel, grid, st, a);
-    return;
+  const dim3 grid = to_layer_grid(a);
+  if (a.fold) SOMAX_NL_LAUNCH(a.nlmax, to_layer_fold_kernel, grid, st, a);
+  else SOMAX_NL_LAUNCH(a.nlmax, to_layer_kernel, grid, st, a);
+}
+
+void launch_to_layer_group(const LayerArgs* a, int n, hipStream_t st) {
+  LayerGroup g{};
+  int nl = 0;
+  for (int i = 0; i < n; ++i) {
+    g.a[i] = a[i];
+    const dim3 d = to_layer_grid(a[i]);
+    table_add(g.t, (int)d.x, (int)d.y, (int)d.z);
+    nl = nl_of(a[i].nlmax) > nl ? nl_of(a[i].nlmax) : nl;
   }
-  dim3 grid((a.Nx + 255) / 256, a.Ny, a.M);
-  SOMAX_NL_LAUNCH(a.nlmax, to_layer_kernel, grid, st, a);
+  SOMAX_NL_LAUNCH(nl, to_layer_group_kernel, dim3(g.t.start[g.t.n]), st, g);
 }
 
 }  // namespace somax
@@ -1080,15 +1249,14 @@ namespace somax {
 // [k * chunk, (k + 1) * chunk), 1024 threads with four loads each in flight,
 // and a fixed summation order, so the result is deterministic.
 constexpr int SUM_THREADS = 1024;
-__global__ __launch_bounds__(SUM_THREADS) void plane_sums_kernel(const float* a, const float* b, long n, long plane,
-                                                                  long chunk, double* partial) {
+// block k of plane z
+__device__ __forceinline__ void plane_sums_block(const PlaneSumArgs& p, int k, int z) {
   __shared__ double red[SUM_THREADS];
-  const int z = blockIdx.y;
   const int t = threadIdx.x;
-  const long lo = (long)blockIdx.x * chunk;
-  const long hi = lo + chunk < n ? lo + chunk : n;
-  const float* pa = a + z * plane;
-  const float* pb = b ? b + z * plane : nullptr;
+  const long lo = (long)k * p.chunk;
+  const long hi = lo + p.chunk < p.n ? lo + p.chunk : p.n;
+  const float* pa = p.a + z * p.plane;
+  const float* pb = p.b ? p.b + z * p.plane : nullptr;
   double s[4] = {0.0, 0.0, 0.0, 0.0};
   for (long i = lo + t; i < hi; i += 4 * SUM_THREADS) {
     float va[4], vb[4];
@@ -1108,14 +1276,27 @@ __global__ __launch_bounds__(SUM_THREADS) void plane_sums_kernel(const float* a,
     __syncthreads();
   }
   if (t == 0) {
-    double* dst = partial + 2 * ((long)z * gridDim.x + blockIdx.x);
+    double* dst = p.partial + 2 * ((long)z * p.nblk + k);
     dst[0] = red[0];
     dst[1] = 0.0;
   }
 }
 
-int launch_plane_sums(const float* a, const float* b, int Nx, int Ny, int P, long plane, double* partial,
-                      hipStream_t st) {
+__global__ __launch_bounds__(SUM_THREADS) void plane_sums_kernel(PlaneSumArgs p) {
+  plane_sums_block(p, blockIdx.x, blockIdx.y);
+}
+struct PlaneSumGroup {
+  BlockTable t;
+  PlaneSumArgs a[MAX_GROUP];
+};
+__global__ __launch_bounds__(SUM_THREADS) void plane_sums_group_kernel(PlaneSumGroup g) {
+  int c, l, k, z, unused;
+  if (!table_class(g.t, blockIdx.x, c, l)) return;
+  class_block(g.t, c, l, k, z, unused);
+  plane_sums_block(g.a[c], k, z);
+}
+
+PlaneSumArgs plane_sums_args(const float* a, const float* b, int Nx, int Ny, int P, long plane, double* partial) {
   // about 4096 values per block (one round of loads for 1024 threads), at
   // most 64 blocks per plane and never more than Ny (the partial buffer holds
   // 2 * P * Ny * ceil(Nx / 256) doubles); small planes of a large batch thus
@@ -1125,11 +1306,24 @@ int launch_plane_sums(const float* a, const float* b, int Nx, int Ny, int P, lon
   if (want > 64) want = 64;
   if (want > Ny) want = Ny;
   const int nblk = want < 1 ? 1 : (int)want;
-  const long chunk = (n + nblk - 1) / nblk;
-  hipLaunchKernelGGL(plane_sums_kernel, dim3(nblk, P), dim3(SUM_THREADS), 0, st, a, b, n, plane, chunk, partial);
-  return nblk;
+  return PlaneSumArgs{a, b, n, plane, (n + nblk - 1) / nblk, partial, P, nblk};
 }
 
+int launch_plane_sums(const float* a, const float* b, int Nx, int Ny, int P, long plane, double* partial,
+                      hipStream_t st) {
+  const PlaneSumArgs p = plane_sums_args(a, b, Nx, Ny, P, plane, partial);
+  hipLaunchKernelGGL(plane_sums_kernel, dim3(p.nblk, P), dim3(SUM_THREADS), 0, st, p);
+  return p.nblk;
+}
+
+void launch_plane_sums_group(const PlaneSumArgs* a, int n, hipStream_t st) {
+  PlaneSumGroup g{};
+  for (int i = 0; i < n; ++i) {
+    g.a[i] = a[i];
+    table_add(g.t, a[i].nblk, a[i].P, 1);
+  }
+  hipLaunchKernelGGL(plane_sums_group_kernel, dim3(g.t.start[g.t.n]), dim3(SUM_THREADS), 0, st, g);
+}
 
 }  // namespace somax
 

@@ -9,15 +9,22 @@ covers in one launch; classes run concurrently on their own streams, and the
 step is captured as a hipGraph and replayed.  The arithmetic per member is
 that of ``model.step``.
 
+With ``merged=True`` the classes are packed into launch groups (those
+sharing precision, advection, biharmonic flag and stage tile height, up to
+eight classes each): every kernel phase of a Runge-Kutta stage is then one
+launch over the whole group, whose blocks a work table assigns to the
+classes, instead of one launch per class.
+
 ``QGEnsemble.autotune`` picks the launch configuration by measurement: the
 transform layout (folded half-size transforms for every even grid, for the
-large grids only, or for none), concurrent class streams or one stream, and
+large grids only, or for none), the member packing (one launch per class on
+concurrent class streams or on one stream, or merged launch groups), and
 then, class by class, the stage kernels' tile height (8 or 16 rows).  Every
 candidate is a new ensemble, timed from its first step: launch shapes are
-fixed when an ensemble first steps.  Concurrency and tile height leave every
-result bit unchanged; the transform layout changes the order of the
-transforms' sums (both layouts meet the same parity bars against the FP64
-oracle).
+fixed when an ensemble first steps.  Packing, concurrency and tile height
+leave every result bit unchanged; the transform layout changes the order of
+the transforms' sums (both layouts meet the same parity bars against the
+FP64 oracle).
 """
 
 from __future__ import annotations
@@ -43,7 +50,8 @@ class QGEnsemble:
     ``step`` advances it in place.
     """
 
-    def __init__(self, models, *, concurrent: bool = True, graph: bool = True, stage_rows=None, device="cuda"):
+    def __init__(self, models, *, concurrent: bool = True, graph: bool = True, merged: bool = False,
+                 stage_rows=None, device="cuda"):
         torch = _torch()
         self.models = list(models)
         if not self.models:
@@ -58,7 +66,10 @@ class QGEnsemble:
             descs[i] = d
             keep.extend(k)
         params = (_lib.QGParams * n)(*[m._c_params() for m in self.models])
-        flags = (_lib.ENS_CONCURRENT if concurrent else 0) | (_lib.ENS_GRAPH if graph else 0)
+        flags = ((_lib.ENS_CONCURRENT if concurrent else 0) | (_lib.ENS_GRAPH if graph else 0)
+                 | (_lib.ENS_MERGED if merged else 0))
+        self.merged = bool(merged)
+        self.concurrent = bool(concurrent)
         ens = ctypes.c_void_p()
         _lib.check(lib.somaxhip_ens_create(descs, params, n, flags, ctypes.byref(ens)), "somaxhip_ens_create")
         self._ens = ens
@@ -193,11 +204,13 @@ class QGEnsemble:
         """Build the fastest configuration of an ensemble of ``models`` for
         steps of ``dt`` from the initial fields ``qs`` (timed on copies).
 
-        Stage 1 times every (transform layout, class concurrency) pair on a
-        new ensemble each; stage 2 tries the other tile height for each class
-        of the winner in turn, on a new ensemble each, keeping a change only
-        when the step gets faster.  Returns ``(ensemble, report)``; the
-        report lists every timing and the chosen configuration."""
+        Stage 1 times every (transform layout, member packing) pair on a
+        new ensemble each, the packings being one launch per class on
+        concurrent streams or on one stream, and merged launch groups; stage
+        2 tries the other tile height for each class of the winner in turn,
+        on a new ensemble each, keeping a change only when the step gets
+        faster.  Returns ``(ensemble, report)``; the report lists every
+        timing and the chosen configuration."""
         log = log or (lambda msg: None)
         report = {"candidates": []}
         best = None
@@ -211,23 +224,26 @@ class QGEnsemble:
 
         try:
             for fold in ("default", "1", "0"):
-                for conc in (True, False):
+                for conc, merged in ((True, False), (False, False), (True, True)):
                     fold_env(fold)
-                    ens = cls(models, concurrent=conc, graph=graph)
+                    ens = cls(models, concurrent=conc, graph=graph, merged=merged)
                     ms = ens.time_step(ens.pack(qs), dt, steps)
-                    report["candidates"].append({"fold": fold, "concurrent": conc, "ms_per_step": round(ms, 4)})
-                    log(f"autotune: fold={fold} concurrent={conc}: {ms:.3f} ms/step")
+                    report["candidates"].append({"fold": fold, "concurrent": conc, "merged": merged,
+                                                 "ms_per_step": round(ms, 4)})
+                    log(f"autotune: fold={fold} concurrent={conc} merged={merged}: {ms:.3f} ms/step")
                     if best is None or ms < best[0]:
-                        best = (ms, ens, fold, conc)
+                        best = (ms, ens, fold, conc, merged)
                     del ens
-            ms, ens, fold, conc = best
+            ms, ens, fold, conc, merged = best
             best = None
             rows = [info["stage_rows"] for info in ens.class_info()]
             fold_env(fold)
-            for c, info in enumerate(ens.class_info()):
+            # the tile height only shapes the Arakawa stage kernel
+            tunable = any(getattr(m, "advection", "arakawa") == "arakawa" for m in models)
+            for c, info in enumerate(ens.class_info() if tunable else []):
                 trial = list(rows)
                 trial[c] = 8 if rows[c] == 16 else 16
-                cand = cls(models, concurrent=conc, graph=graph, stage_rows=trial)
+                cand = cls(models, concurrent=conc, graph=graph, merged=merged, stage_rows=trial)
                 t = cand.time_step(cand.pack(qs), dt, steps)
                 report["candidates"].append({"class": f"{info['nx']}x{info['ny']}", "stage_rows": trial[c],
                                              "ms_per_step": round(t, 4)})
@@ -240,7 +256,7 @@ class QGEnsemble:
                 os.environ.pop("SOMAXHIP_FOLD", None)
             else:
                 os.environ["SOMAXHIP_FOLD"] = saved
-        report["chosen"] = {"fold": fold, "concurrent": conc, "ms_per_step": round(ms, 4),
+        report["chosen"] = {"fold": fold, "concurrent": conc, "merged": merged, "ms_per_step": round(ms, 4),
                             "classes": ens.class_info()}
         return ens, report
 

@@ -12,8 +12,9 @@ Workloads:
   FP32 inversion (the benchmark's headline workload).
 * ``qg1024-mixed`` / ``qg1024-fp32``: the same grid with Arakawa advection
   and no mass constraint.
-* ``ensemble``: the 256-member uneven ensemble, mixed inversion, one
-  eager step (every launch visible to the profiler).
+* ``ensemble`` / ``ensemble-merged``: the 256-member uneven ensemble,
+  mixed inversion, one eager step (every launch visible to the profiler),
+  one launch per class or merged launch groups.
 * ``swm512``: the 2-layer shallow-water baroclinic jet, 512 x 512 channel,
   energy-enstrophy-conserving PV flux, SSP-RK3 at CFL 0.5.
 
@@ -211,7 +212,7 @@ def main(argv=None) -> int:
     ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
     ap.add_argument("--workload", default="qg1024-weno-mass-mixed",
                     choices=["qg1024-weno-mass-mixed", "qg1024-weno-mass-fp32", "qg1024-mixed", "qg1024-fp32",
-                             "ensemble", "swm512", "cmm256"])
+                             "ensemble", "ensemble-merged", "swm512", "cmm256"])
     ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--copy-mib", type=int, default=1024)
     ap.add_argument("--dt", type=float, default=1200.0)
@@ -232,11 +233,11 @@ def main(argv=None) -> int:
         print(f"copy: {args.copy_mib} MiB read + {args.copy_mib} MiB written", flush=True)
         del src, dst
 
-    if args.workload == "ensemble":
+    if args.workload in ("ensemble", "ensemble-merged"):
         from .ensemble import QGEnsemble
 
         models, qs = tc.config_ensemble(seed=7, precision="mixed")
-        ens = QGEnsemble(models, concurrent=False, graph=False)
+        ens = QGEnsemble(models, concurrent=False, graph=False, merged=args.workload == "ensemble-merged")
         packed = ens.pack(qs)
         ens.step(packed, args.dt, 1)
         torch.cuda.synchronize()

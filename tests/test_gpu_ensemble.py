@@ -74,16 +74,18 @@ def _states(models, seed=21):
 
 
 @pytest.mark.parametrize("precision", ["fp32", "mixed"])
-@pytest.mark.parametrize("concurrent,graph", [(False, False), (True, False), (True, True)])
-def test_ensemble_matches_single_models(gpu, precision, concurrent, graph):
+@pytest.mark.parametrize("concurrent,graph,merged", [(False, False, False), (True, False, False),
+                                                     (True, True, False), (False, False, True),
+                                                     (True, True, True)])
+def test_ensemble_matches_single_models(gpu, precision, concurrent, graph, merged):
     import torch
 
     from somax_amd import QGEnsemble
 
-    _log(f"[ensemble] precision={precision} concurrent={concurrent} graph={graph}")
+    _log(f"[ensemble] precision={precision} concurrent={concurrent} graph={graph} merged={merged}")
     models = _models(precision)
     qs = _states(models)
-    ens = QGEnsemble(models, concurrent=concurrent, graph=graph)
+    ens = QGEnsemble(models, concurrent=concurrent, graph=graph, merged=merged)
     assert len(ens) == len(MEMBERS)
     packed = ens.pack(qs)
     dt, nsteps = 1200.0, 3
@@ -247,23 +249,25 @@ def test_autotune_picks_a_valid_configuration(gpu):
         assert rel(ens.member(packed, i).cpu().numpy().reshape(ref.shape), ref) < 1e-6
 
 
-def test_uneven_256_member_ensemble(gpu):
+@pytest.mark.parametrize("merged", [False, True])
+def test_uneven_256_member_ensemble(gpu, merged):
     """The 256-member ensemble: 96 members at 64^2, 96 at 128^2, 48 at 256^2
     and 16 at 512^2 (the 256^2 and 512^2 classes with folded transforms),
     layer counts cycling 1/2/3, tau0 ~ U[0.04, 0.12] N/m^2, WENO-Z PV flux
-    with the FP64 mass constraint, mixed inversion, concurrent classes in one
-    captured graph.  After one step every member equals its single model bit
-    for bit, and one member per (grid, layer count) meets the one-step bar
-    (1e-6) against the FP64 oracle."""
+    with the FP64 mass constraint, mixed inversion, in one captured graph:
+    concurrent class streams, or one launch group whose every kernel phase
+    is a single launch over the four classes.  After one step every member
+    equals its single model bit for bit, and one member per (grid, layer
+    count) meets the one-step bar (1e-6) against the FP64 oracle."""
     import torch
 
     from somax_amd import QGEnsemble
     from somax_amd import testcases as tc
 
-    _log("[ensemble] 256 uneven members, WENO-Z + mass constraint, mixed, graph + concurrent")
+    _log(f"[ensemble] 256 uneven members, WENO-Z + mass constraint, mixed, graph, merged={merged}")
     models, qs = tc.config_ensemble(seed=3, precision="mixed")
     assert len(models) == 256
-    ens = QGEnsemble(models, concurrent=True, graph=True)
+    ens = QGEnsemble(models, concurrent=True, graph=True, merged=merged)
     info = ens.class_info()
     assert [(c["nx"], c["members"], c["fold"]) for c in info] == [(512, 16, 1), (256, 48, 1), (128, 96, 0),
                                                                   (64, 96, 0)]
@@ -294,23 +298,34 @@ def test_uneven_256_member_ensemble(gpu):
         _log(f"  member {i} ({m.grid.nx}^2, {m._nl} layers): rel-L2 q vs oracle = {r:.3e}")
         assert r < 1e-6
     assert len(seen) == 12
+    if merged:
+        # one launch per kernel phase per RK stage for all four classes
+        prof = ens.profile(packed, dt, 1)
+        launches = {k: v[0] for k, v in prof.items()}
+        _log(f"  launches per step: {launches}")
+        for k in ("to_modal", "residual", "mass_constraint", "to_layer", "qg_stage"):
+            assert launches[k] == 3, k
+        assert launches["dst_gemm_bf16x3"] == 12 and launches["dst_gemm_bf16x6"] == 6
+        assert launches["dst_gemm_bf16x1"] == 6
 
 
-def test_ensemble_graph_recaptures(gpu):
+@pytest.mark.parametrize("merged", [False, True])
+def test_ensemble_graph_recaptures(gpu, merged):
     """A concurrent, graph-captured ensemble recaptures its step whenever the
-    parameters switch the stage kernel (biharmonic on / off), dt changes or
-    the state buffer moves; five captures in a row on one ensemble, each
-    followed by steps checked bit for bit against the single models."""
+    parameters switch the stage kernel (biharmonic on / off, which also
+    regroups merged launch groups), dt changes or the state buffer moves; five
+    captures in a row on one ensemble, each followed by steps checked bit for
+    bit against the single models."""
     import dataclasses
 
     import torch
 
     from somax_amd import QGEnsemble
 
-    _log("[ensemble] five graph captures on one concurrent ensemble")
+    _log(f"[ensemble] five graph captures on one concurrent ensemble, merged={merged}")
     models = _models("mixed")
     qs = _states(models)
-    ens = QGEnsemble(models, concurrent=True, graph=True)
+    ens = QGEnsemble(models, concurrent=True, graph=True, merged=merged)
     base = [m.params for m in models]
     no_bih = [dataclasses.replace(p, biharmonic_viscosity=0.0) for p in base]
     packed = ens.pack(qs)
