@@ -199,8 +199,9 @@ def bench_ensemble(dt: float, steps: int, warmup: int) -> dict:
         torch.cuda.synchronize()
         return e0.elapsed_time(e1) / k
 
-    for label, conc, graph in (("graph", True, True), ("eager", False, False)):
-        ens = QGEnsemble(models, concurrent=conc, graph=graph)
+    for label, conc, graph, merged in (("graph", True, True, False), ("merged_graph", True, True, True),
+                                        ("eager", False, False, False)):
+        ens = QGEnsemble(models, concurrent=conc, graph=graph, merged=merged)
         packed = ens.pack(qs)
         ens.step(packed, dt, max(warmup, 1))
         ms = timed(lambda k: ens.step(packed, dt, k), steps)
@@ -208,13 +209,13 @@ def bench_ensemble(dt: float, steps: int, warmup: int) -> dict:
             raise RuntimeError("non-finite ensemble state")
         out[f"mixed_{label}"] = {"ms_per_step": round(ms, 4), "value": round(cells / (ms * 1e-3), 1)}
         log(f"[bench] ensemble mixed {label}: {ms:.3f} ms/step")
-        if label == "graph":
+        if label in ("graph", "merged_graph"):
             prof = ens.profile(packed, dt, 1)
-            out["kernel_profile_eager"] = {k: {"launches": v[0], "total_us": round(1e3 * v[1], 1)}
-                                           for k, v in prof.items()}
+            key = "kernel_profile_eager" if label == "graph" else "kernel_profile_merged_eager"
+            out[key] = {k: {"launches": v[0], "total_us": round(1e3 * v[1], 1)} for k, v in prof.items()}
         del ens, packed
 
-    log("[bench] ensemble: autotune (transform layout, class streams, stage tile rows)")
+    log("[bench] ensemble: autotune (transform layout, member packing, stage tile rows)")
     ens, report = QGEnsemble.autotune(models, qs, dt, steps=3)
     packed = ens.pack(qs)
     ms = ens.time_step(packed, dt, steps)
