@@ -179,6 +179,30 @@ def test_ensemble_set_params(gpu):
         assert rel(ens.member(packed, i).cpu().numpy(), q.cpu().numpy()) < 1e-7
 
 
+def test_merged_groups_beyond_eight_classes(gpu):
+    """Ten grid classes merge into two launch groups (at most eight classes
+    per merged launch); every member still equals its single model bit for
+    bit after two steps."""
+    import torch
+
+    from somax_amd import QGEnsemble
+    from somax_amd import testcases as tc
+
+    _log("[ensemble] 10 classes, merged launch groups, mixed")
+    models = [tc.ensemble_member(n, 1 + i % 3, 0.04 + 0.008 * i, precision="mixed")
+              for i, n in enumerate(range(32, 112, 8))]
+    ens = QGEnsemble(models, concurrent=True, graph=True, merged=True)
+    assert ens.num_classes == 10
+    qs = _states(models)
+    packed = ens.pack(qs)
+    ens.step(packed, 1200.0, 2)
+    torch.cuda.synchronize()
+    for i, m in enumerate(models):
+        q = torch.from_numpy(qs[i]).cuda()
+        m._step_inplace(q, 1200.0, 2)
+        assert torch.equal(ens.member(packed, i), q.view(ens.shapes[i])), i
+
+
 def test_ensemble_mixed_precision_classes(gpu):
     import torch
 
