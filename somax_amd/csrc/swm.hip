@@ -192,17 +192,35 @@ __global__ __launch_bounds__(SNT) void swm_stage_kernel(SwmArgs a) {
   const BcAxis cm0 = bc_cols(x0 - 1 + tx, Nx, a.bc, a.apply_bc);
   const BcAxis cm1 = bc_cols(x0 - 1 + tx + 64, Nx, a.bc, a.apply_bc);
 
-  // cumulative pressure-gradient sums of this thread's points (U and V)
+  // One workgroup per tile and layer k (grid y).  The pressure gradient of
+  // layer k sums g'_j times the thickness differences of the layers j <= k at
+  // this thread's points (U and V): those of the layers above come straight
+  // from global memory through the boundary condition (the values the tile
+  // loads would hold), added in layer order, so the sums equal a layer loop's.
+  const int k = blockIdx.y;
   float dpx[SRW], dpy[SRW];
 #pragma unroll
-  for (int r = 0; r < SRW; ++r) dpx[r] = dpy[r] = 0.0f;
+  for (int r = 0; r < SRW; ++r) {
+    dpx[r] = dpy[r] = 0.0f;
+    const int gy = y0 + ty + SWV * r, gx = x0 + tx;
+    if (k == 0 || !(gy >= 1 && gy <= Ny - 2 && gx >= 1 && gx <= Nx - 2)) continue;
+    const BcAxis r0 = bc_rows(gy, Ny, a.bc, a.apply_bc), r1 = bc_rows(gy + 1, Ny, a.bc, a.apply_bc);
+    const BcAxis c0 = bc_cols(gx, Nx, a.bc, a.apply_bc), c1 = bc_cols(gx + 1, Nx, a.bc, a.apply_bc);
+    for (int j = 0; j < k; ++j) {
+      const float* hj = a.hs + (long)j * a.plane;
+      const float h00 = (r0.h >= 0 && c0.h >= 0) ? hj[r0.h * Nx + c0.h] : 0.0f;
+      const float h01 = (r0.h >= 0 && c1.h >= 0) ? hj[r0.h * Nx + c1.h] : 0.0f;
+      const float h10 = (r1.h >= 0 && c0.h >= 0) ? hj[r1.h * Nx + c0.h] : 0.0f;
+      dpx[r] += a.gprime[j] * (h01 - h00);
+      dpy[r] += a.gprime[j] * (h10 - h00);
+    }
+  }
 
-  for (int k = 0; k < a.nl; ++k) {
+  {
     const long off = (long)k * a.plane;
     const float* hs = a.hs + off;
     const float* us = a.us + off;
     const float* vs = a.vs + off;
-    if (k > 0) __syncthreads();  // previous layer's tiles are no longer read
     // This thread's raw stage and step-start values, fetched before the tile
     // so that their latency overlaps the tile loads.
     float hs_r[SRW], us_r[SRW], vs_r[SRW], hn_r[SRW], un_r[SRW], vn_r[SRW];
@@ -516,7 +534,7 @@ void swm_stage(somaxhip_swm_plan p, const float* hs, const float* us, const floa
   a.c2 = c2;
   a.bc = p->bc;
   a.apply_bc = apply_bc;
-  dim3 grid((unsigned)(((p->Nx + STX - 1) / STX) * ((p->Ny + STY - 1) / STY)), 1, 1);
+  dim3 grid((unsigned)(((p->Nx + STX - 1) / STX) * ((p->Ny + STY - 1) / STY)), (unsigned)p->nl, 1);
   hipEvent_t e0 = nullptr, e1 = nullptr;
   if (g_swm_prof) {
     (void)hipEventCreate(&e0);
