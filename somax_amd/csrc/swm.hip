@@ -10,13 +10,14 @@
 // ring, except wind (top layer) and bottom drag (bottom layer), which act on
 // the full arrays.
 //
-// One kernel evaluates the RHS of every layer of a tile and the RK stage
-// combination: the block loops over the layers, staging h, u, v of each layer
-// (boundary condition applied on load, 1-cell halo) through LDS; each thread
-// keeps the cumulative pressure gradient sum_l g'_l (h_l[e] - h_l[w]) of its
-// points in registers, so the hydrostatic coupling costs no extra pass and the
-// differences of h are exact in FP32 (the cumulative pressure itself is ~g*H,
-// four orders of magnitude above its cell-to-cell change).
+// One kernel evaluates the RHS and the RK stage combination, one workgroup per
+// tile and layer: it stages h, u, v of its layer (boundary condition applied on
+// load, 1-cell halo) through LDS; each thread forms the cumulative pressure
+// gradient sum_{l<=k} g'_l (h_l[e] - h_l[w]) of its points in registers, the
+// layers above read from global memory in layer order, so the hydrostatic
+// coupling costs no extra pass and the differences of h are exact in FP32 (the
+// cumulative pressure itself is ~g*H, four orders of magnitude above its
+// cell-to-cell change).
 
 #include <cstring>
 #include <new>
