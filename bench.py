@@ -373,7 +373,8 @@ def bench_rows(mixed, q0, dt: float) -> dict:
     """The remaining paths on their own: the Tsit5 step of the 1024^2 model
     (integrate's default solver, 6 RHS per step), the ReparameterizedQG step
     (3 layers at 512^2, Tsit5: a projection per stage), the single-layer
-    nonlinear shallow-water step (1024^2), and the between-chunk field
+    nonlinear shallow-water step (1024^2), the FP64 barotropic 64^2 step
+    beside the FP64 oracle on one thread, and the between-chunk field
     statistics of the 1024^2 state (one read of 12.6 MB: an HBM-bound
     reduction, reported against the HBM peak)."""
     import ctypes
@@ -412,6 +413,30 @@ def bench_rows(mixed, q0, dt: float) -> dict:
     out["nonlinear_swm_ssprk3"] = {"workload": "nonlinear_swm1024x1024_wall_ssprk3", "ms_per_step": round(ms, 4),
                                    "value": round(1024 * 1024 / (ms * 1e-3), 1), "unit": "cell-steps/s"}
     del s0
+
+    # the reference's own CPU configuration in FP64: the 1-layer 64^2 double
+    # gyre, SSP-RK3 at 4000 s, as precision="fp64" on the GPU and as the FP64
+    # NumPy oracle on one host thread (the cpu_baseline leg)
+    bt = tc.config_barotropic(64, precision="fp64")
+    qb0 = tc.random_pv(bt, seed=8)
+    qb = torch.from_numpy(qb0).cuda()
+    ms = _event_ms(lambda: bt._step_inplace(qb, 4000.0, 100), 3) / 100
+    row = {"workload": "barotropic_qg64x64_ssprk3_fp64", "ms_per_step": round(ms, 4),
+           "value": round(64 * 64 / (ms * 1e-3), 1), "unit": "layer-cell-steps/s", "dtype": "f64"}
+    from oracle import qg as oqg
+    from threadpoolctl import threadpool_limits
+
+    setup = oqg.barotropic_setup(nx=64, ny=64, Lx=tc.BASIN, Ly=tc.BASIN, f0=tc.F0, beta=tc.BETA,
+                                 bottom_drag=tc.DRAG, wind_amplitude=tc.wind_amplitude() / 4000.0,
+                                 biharmonic_viscosity=tc.NU4_128)
+    with threadpool_limits(1):
+        t = time.perf_counter()
+        oqg.run(setup, qb0, 4000.0, 50)
+        cs = (time.perf_counter() - t) / 50
+    row["cpu_baseline"] = {"value": round(64 * 64 / cs, 1), "unit": "layer-cell-steps/s", "cores": 1, "kind": "port",
+                           "sample": "50 steps of the same model, FP64 NumPy oracle (oracle/qg.py: scipy.fft "
+                                     f"sine transforms), {1e3 * cs:.2f} ms per step"}
+    out["barotropic_fp64"] = row
 
     ws = _workspace(q0.device)
     res = torch.empty(5, dtype=torch.float64, device="cuda")

@@ -50,9 +50,12 @@ enum {
   SOMAXHIP_PREC_FP32 = 0,   /* plain: FP32 MFMA sine-transform GEMMs */
   SOMAXHIP_PREC_MIXED = 1,  /* BF16 hi/lo 3-product MFMA, FP32 accumulate, one
                                iterative-refinement pass on the Helmholtz residual */
-  SOMAXHIP_PREC_BF16X6 = 2  /* extension: BF16 hi/mid/lo three-piece operands, six
+  SOMAXHIP_PREC_BF16X6 = 2, /* extension: BF16 hi/mid/lo three-piece operands, six
                                products per element with FP64 accumulation flushes,
                                one direct solve (no refinement pass) */
+  SOMAXHIP_PREC_FP64 = 3    /* the whole step in FP64, as the reference runs on the CPU
+                               with x64 enabled: FP64 stencils, projections and DST-I
+                               GEMMs; single models only, through the *_f64 calls */
 };
 
 /* ------------------------------------------------------------------------ */
@@ -156,6 +159,32 @@ int somaxhip_qg_step_tsit5(somaxhip_qg_plan plan, float* q, int nsteps, double d
  * Replaces finitevolx.zero_boundaries as used by
  * BaroclinicQG.apply_boundary_conditions (baroclinic.py:192-195). */
 int somaxhip_zero_ghost(float* field, int nl, int ny, int nx, somaxhip_stream_t stream);
+
+/* FP64 forms of somaxhip_qg_invert, _rhs, _step (SSP-RK3, below), _step_tsit5,
+ * _diagnose and somaxhip_zero_ghost, for a plan created with precision
+ * SOMAXHIP_PREC_FP64 (the FP32 / BF16 calls, the masked solve and the
+ * reparameterized step refuse such a plan, and these refuse any other): the
+ * same arguments with double fields; the workspace sizes
+ * come from somaxhip_qg_workspace_bytes / somaxhip_qg_tsit5_workspace_bytes
+ * of the plan.  They replace the same reference functions evaluated under
+ * jax_enable_x64: BarotropicQG / BaroclinicQG._invert_pv (barotropic.py:113-121,
+ * baroclinic.py:135-159), vector_field (barotropic.py:123-152,
+ * baroclinic.py:161-190), diagnose (barotropic.py:159-183,
+ * baroclinic.py:197-228), SomaxModel.integrate with Tsit5 (core/model.py:53-88),
+ * and zero_boundaries (baroclinic.py:192-195). */
+int somaxhip_qg_invert_f64(somaxhip_qg_plan plan, const double* q, double* psi, int apply_bc,
+                           void* workspace, somaxhip_stream_t stream);
+int somaxhip_qg_rhs_f64(somaxhip_qg_plan plan, const double* q, double* dq, double* psi_out,
+                        const somaxhip_qg_params* params, int apply_bc, void* workspace,
+                        somaxhip_stream_t stream);
+int somaxhip_qg_step_f64(somaxhip_qg_plan plan, double* q, double* psi_out, int nsteps, double dt,
+                         const somaxhip_qg_params* params, void* workspace, somaxhip_stream_t stream);
+int somaxhip_qg_step_tsit5_f64(somaxhip_qg_plan plan, double* q, int nsteps, double dt,
+                               const somaxhip_qg_params* params, void* workspace, void* tsit5_workspace,
+                               somaxhip_stream_t stream);
+int somaxhip_qg_diagnose_f64(somaxhip_qg_plan plan, const double* q, double* psi, double* u, double* v,
+                             double* zeta, double* ke_enstrophy, void* workspace, somaxhip_stream_t stream);
+int somaxhip_zero_ghost_f64(double* field, int nl, int ny, int nx, somaxhip_stream_t stream);
 
 /* Advances q in place by nsteps SSP-RK3 steps of size dt (Shu-Osher form of
  * the ssrk3_update_stage_{1,2,3} scheme, notebooks/dev/QG_Multilayer/

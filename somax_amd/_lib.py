@@ -19,6 +19,7 @@ ADV_WENO = 1
 PREC_FP32 = 0
 PREC_MIXED = 1
 PREC_BF16X6 = 2
+PREC_FP64 = 3
 ENS_CONCURRENT = 1
 ENS_GRAPH = 2
 ENS_MERGED = 4
@@ -130,6 +131,18 @@ _SIGNATURES = {
     ),
     "somaxhip_zero_ghost": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _vp]),
     "somaxhip_qg_diagnose": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "somaxhip_qg_invert_f64": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int, _vp, _vp]),
+    "somaxhip_qg_rhs_f64": (ctypes.c_int, [_vp, _vp, _vp, _vp, ctypes.POINTER(QGParams), ctypes.c_int, _vp, _vp]),
+    "somaxhip_qg_step_f64": (
+        ctypes.c_int,
+        [_vp, _vp, _vp, ctypes.c_int, ctypes.c_double, ctypes.POINTER(QGParams), _vp, _vp],
+    ),
+    "somaxhip_qg_step_tsit5_f64": (
+        ctypes.c_int,
+        [_vp, _vp, ctypes.c_int, ctypes.c_double, ctypes.POINTER(QGParams), _vp, _vp, _vp],
+    ),
+    "somaxhip_qg_diagnose_f64": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "somaxhip_zero_ghost_f64": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _vp]),
     "somaxhip_qg_step_profile": (
         ctypes.c_int,
         [_vp, _vp, ctypes.c_int, ctypes.c_double, ctypes.POINTER(QGParams), _vp, _vp, ctypes.POINTER(KernelStat),

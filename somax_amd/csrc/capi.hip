@@ -195,7 +195,7 @@ struct Batch {
 };
 
 // Checks one model description; empty string when valid.
-static std::string check_desc(const somaxhip_qg_desc* d) {
+static std::string check_desc(const somaxhip_qg_desc* d, bool allow_f64) {
   if (!d) return "null description";
   if (d->nx < 2 || d->ny < 2 || d->nl < 1 || d->nl > MAX_LAYERS || !(d->dx > 0) || !(d->dy > 0))
     return "need nx, ny >= 2, 1 <= nl <= 8, dx, dy > 0";
@@ -203,8 +203,10 @@ static std::string check_desc(const somaxhip_qg_desc* d) {
   if (!d->cl2m || !d->cm2l || !d->lambdas || !d->beta_y_row || !d->wind_row || !d->wind_scale)
     return "a required table pointer is null";
   if (d->advection != SOMAXHIP_ADV_ARAKAWA && d->advection != SOMAXHIP_ADV_WENO) return "unknown advection";
+  if (d->precision == SOMAXHIP_PREC_FP64 && !allow_f64)
+    return "ensembles compute in FP32 / BF16; SOMAXHIP_PREC_FP64 is a single-model form";
   if (d->precision != SOMAXHIP_PREC_FP32 && d->precision != SOMAXHIP_PREC_MIXED &&
-      d->precision != SOMAXHIP_PREC_BF16X6)
+      d->precision != SOMAXHIP_PREC_BF16X6 && d->precision != SOMAXHIP_PREC_FP64)
     return "unknown precision";
   if (d->mass_correction && (!d->homsol || !d->homsol_mean))
     return "mass_correction needs homsol and homsol_mean";
@@ -345,17 +347,26 @@ static MemberParams to_member_params(const somaxhip_qg_params& p) {
 using namespace somax;
 
 struct somaxhip_qg_plan_s {
-  Batch b;
+  Batch b;               // FP32 / BF16 plans
+  Qg64* f64 = nullptr;   // FP64 plans (b stays empty)
+  ~somaxhip_qg_plan_s() {
+    if (f64) qg64_destroy(f64);
+  }
 };
 
 namespace somax {
 int qg_plan_shape(somaxhip_qg_plan p, int* planes, int* Ny, int* Nx) {
   if (!p) return SOMAXHIP_ERR_ARG;
+  if (p->f64) {
+    set_error("the plan computes in FP64; this call takes an FP32 / BF16 plan");
+    return SOMAXHIP_ERR_ARG;
+  }
   *planes = p->b.P;
   *Ny = p->b.Ny;
   *Nx = p->b.Nx;
   return SOMAXHIP_OK;
 }
+const Qg64* qg_plan_f64(somaxhip_qg_plan p) { return p ? p->f64 : nullptr; }
 }  // namespace somax
 
 struct somaxhip_ens_s {
@@ -465,7 +476,7 @@ int somaxhip_qg_plan_create(const somaxhip_qg_desc* d, somaxhip_qg_plan* out) {
     return SOMAXHIP_ERR_ARG;
   }
   *out = nullptr;
-  const std::string bad = check_desc(d);
+  const std::string bad = check_desc(d, true);
   if (!bad.empty()) {
     set_error("somaxhip_qg_plan_create: " + bad);
     return SOMAXHIP_ERR_ARG;
@@ -474,6 +485,17 @@ int somaxhip_qg_plan_create(const somaxhip_qg_desc* d, somaxhip_qg_plan* out) {
   if (!p) {
     set_error("somaxhip_qg_plan_create: out of host memory");
     return SOMAXHIP_ERR_ALLOC;
+  }
+  if (d->precision == SOMAXHIP_PREC_FP64) {
+    std::string err;
+    p->f64 = qg64_create(d, &err);
+    if (!p->f64) {
+      set_error("somaxhip_qg_plan_create: " + err);
+      delete p;
+      return SOMAXHIP_ERR_ALLOC;
+    }
+    *out = p;
+    return SOMAXHIP_OK;
   }
   const hipError_t e = batch_build(p->b, {d});
   if (e != hipSuccess) {
@@ -490,7 +512,10 @@ int somaxhip_qg_plan_destroy(somaxhip_qg_plan p) {
   return SOMAXHIP_OK;
 }
 
-size_t somaxhip_qg_workspace_bytes(somaxhip_qg_plan p) { return p ? p->b.ws_bytes : 0; }
+size_t somaxhip_qg_workspace_bytes(somaxhip_qg_plan p) {
+  if (!p) return 0;
+  return p->f64 ? qg64_workspace_bytes(p->f64) : p->b.ws_bytes;
+}
 
 }  // extern "C"
 
@@ -1136,12 +1161,21 @@ CallParams uniform_params(const somaxhip_qg_params* prm) {
   return CallParams{nullptr, to_member_params(*prm), prm->nu4 != 0.0 ? 1 : 0};
 }
 
+// An FP32 / BF16 plan and non-null buffers.
+int check_plan(somaxhip_qg_plan p, const char* who) {
+  if (p && p->f64) {
+    set_error(std::string(who) + ": the plan computes in FP64 (SOMAXHIP_PREC_FP64); call the _f64 entry point");
+    return SOMAXHIP_ERR_ARG;
+  }
+  return SOMAXHIP_OK;
+}
+
 int check_call(somaxhip_qg_plan p, const void* a, const void* b, void* ws, const char* who) {
   if (!p || !a || !b || !ws) {
     set_error(std::string(who) + ": null plan, buffer or workspace");
     return SOMAXHIP_ERR_ARG;
   }
-  return SOMAXHIP_OK;
+  return check_plan(p, who);
 }
 
 // Sums the events of a profiled run into per-name totals.
@@ -1225,6 +1259,7 @@ int somaxhip_qg_step_noalloc(somaxhip_qg_plan p, float* q, float* psi_out, int n
     set_error("somaxhip_qg_step_noalloc: null plan, state or params, or negative nsteps");
     return SOMAXHIP_ERR_ARG;
   }
+  if (const int rc = check_plan(p, "somaxhip_qg_step_noalloc")) return rc;
   if (p->b.ws_bytes > (size_t)SOMAXHIP_NOALLOC_SCRATCH_BYTES) {
     set_error("somaxhip_qg_step_noalloc: the plan needs " + std::to_string(p->b.ws_bytes) +
               " bytes of workspace, more than the " + std::to_string((size_t)SOMAXHIP_NOALLOC_SCRATCH_BYTES) +
@@ -1284,7 +1319,8 @@ int somaxhip_qg_project(somaxhip_qg_plan p, double f0, const double* H, const do
 }
 
 size_t somaxhip_qg_tsit5_workspace_bytes(somaxhip_qg_plan p) {
-  return p ? (size_t)8 * p->b.P * p->b.splane * sizeof(float) : 0;
+  if (!p) return 0;
+  return p->f64 ? qg64_tsit5_workspace_bytes(p->f64) : (size_t)8 * p->b.P * p->b.splane * sizeof(float);
 }
 
 int somaxhip_qg_step_tsit5(somaxhip_qg_plan p, float* q, int nsteps, double dt, const somaxhip_qg_params* prm,
@@ -1375,7 +1411,7 @@ int somaxhip_ens_create(const somaxhip_qg_desc* members, const somaxhip_qg_param
     return SOMAXHIP_ERR_ARG;
   }
   for (int i = 0; i < n; ++i) {
-    const std::string bad = check_desc(&members[i]);
+    const std::string bad = check_desc(&members[i], false);
     if (!bad.empty()) {
       set_error("somaxhip_ens_create: member " + std::to_string(i) + ": " + bad);
       return SOMAXHIP_ERR_ARG;

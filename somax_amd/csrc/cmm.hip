@@ -84,7 +84,11 @@ int somaxhip_cmm_create(somaxhip_qg_plan plan, const float* mask, int nb, const 
     return SOMAXHIP_ERR_ALLOC;
   }
   c->plan = plan;
-  qg_plan_shape(plan, &c->P, &c->Ny, &c->Nx);
+  if (qg_plan_shape(plan, &c->P, &c->Ny, &c->Nx) != SOMAXHIP_OK) {
+    set_error("somaxhip_cmm_create: the QG plan computes in FP64; the masked solve takes an FP32 / BF16 plan");
+    delete c;
+    return SOMAXHIP_ERR_ARG;
+  }
   c->plane = (long)c->Ny * c->Nx;
   c->nb = nb;
   for (int i = 0; i < nb; ++i) {

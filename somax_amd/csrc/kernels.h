@@ -6,6 +6,7 @@
 // the member and plane tables below.  A single model is the one-member batch.
 #pragma once
 
+#include "somaxhip.h"
 #include "common.h"
 
 struct somaxhip_qg_plan_s;
@@ -87,8 +88,19 @@ __device__ inline int xcd_order(int b, int T) {
   return (int)(xcd * per + (xcd < extra ? xcd : extra) + slot);
 }
 
-// Planes, rows and columns (ghosts included) of a QG plan's fields.
+// Planes, rows and columns (ghosts included) of a QG plan's fields
+// (SOMAXHIP_ERR_ARG for an FP64 plan, whose fields are doubles).
 int qg_plan_shape(somaxhip_qg_plan_s* plan, int* planes, int* Ny, int* Nx);
+
+// The FP64 form of a QG model (qg64.hip): built by a plan whose description
+// asks for SOMAXHIP_PREC_FP64, used by the *_f64 entry points.
+struct Qg64;
+Qg64* qg64_create(const somaxhip_qg_desc* d, std::string* err);
+void qg64_destroy(Qg64* p);
+size_t qg64_workspace_bytes(const Qg64* p);
+size_t qg64_tsit5_workspace_bytes(const Qg64* p);
+// The plan's FP64 form; null for an FP32 / BF16 plan.
+const Qg64* qg_plan_f64(somaxhip_qg_plan_s* plan);
 
 // Member and plane tables of a batch (device arrays).
 struct Tables {

@@ -810,6 +810,10 @@ int somaxhip_rqg_step(somaxhip_swm_plan p, somaxhip_qg_plan qp, int solver, doub
     set_error("somaxhip_rqg_step: null argument, negative nsteps or unknown solver");
     return SOMAXHIP_ERR_ARG;
   }
+  if (qg_plan_f64(qp)) {
+    set_error("somaxhip_rqg_step: the QG plan computes in FP64; the projection takes an FP32 / BF16 plan");
+    return SOMAXHIP_ERR_ARG;
+  }
   if (nsteps == 0) return SOMAXHIP_OK;
   hipStream_t st = reinterpret_cast<hipStream_t>(stream);
   const long n = (long)p->nl * p->plane;

@@ -102,8 +102,9 @@ class MaskedHelmholtz:
 
     def __init__(self, mask, dx: float, dy: float, lambdas, *, precision: str = "mixed", device="cuda"):
         torch = _torch()
-        if precision not in _PREC:
-            raise ValueError(f"precision must be one of {sorted(_PREC)}, got {precision!r}")
+        allowed = sorted(k for k in _PREC if k != "fp64")  # the masked solve runs on FP32 / BF16 plans
+        if precision not in allowed:
+            raise ValueError(f"precision must be one of {allowed}, got {precision!r}")
         mask = np.ascontiguousarray(np.asarray(mask, dtype=np.float32) > 0.5, dtype=np.float32)
         if mask.ndim != 2 or min(mask.shape) < 4:
             raise ValueError(f"mask must be 2-D with at least 4 points per side, got {mask.shape}")

@@ -12,7 +12,9 @@ reference): ``biharmonic_viscosity``, ``advection="weno"``,
 ``precision`` (``"fp32"``: FP32 MFMA transforms; ``"mixed"``: bf16 hi/lo
 three-product transforms plus one refinement pass on the Helmholtz residual;
 ``"bf16x6"``: bf16 hi/mid/lo six-product transforms with FP64 accumulation,
-one direct solve), ``mass_correction`` and ``helmholtz_eigenvalues``.
+one direct solve; ``"fp64"``: the whole step in FP64 on float64 state, as the
+reference runs on the CPU with x64 enabled), ``mass_correction`` and
+``helmholtz_eigenvalues``.
 """
 
 from __future__ import annotations
@@ -39,7 +41,7 @@ from .core import (
 )
 
 _ADV = {"arakawa": _lib.ADV_ARAKAWA, "weno": _lib.ADV_WENO}
-_PREC = {"fp32": _lib.PREC_FP32, "mixed": _lib.PREC_MIXED, "bf16x6": _lib.PREC_BF16X6}
+_PREC = {"fp32": _lib.PREC_FP32, "mixed": _lib.PREC_MIXED, "bf16x6": _lib.PREC_BF16X6, "fp64": _lib.PREC_FP64}
 
 
 def _torch():
@@ -175,6 +177,7 @@ class _QGBase(SomaxModel):
         self._zero_psi_ghost = bool(zero_psi_ghost)
         self.advection = advection
         self.precision = precision
+        self._f64 = precision == "fp64"
         self.mass_correction = bool(mass_correction)
         self.device = device
         self._plan = None
@@ -244,10 +247,12 @@ class _QGBase(SomaxModel):
     def _field(self, q):
         torch = _torch()
         shape = (self._nl, self.grid.Ny, self.grid.Nx)
+        dtype = torch.float64 if self._f64 else torch.float32
         if not isinstance(q, torch.Tensor) or q.device.type != "cuda":
-            q = torch.as_tensor(np.asarray(q), dtype=torch.float32, device=self.device)
-        if q.dtype != torch.float32:
-            raise TypeError(f"state must be float32, got {q.dtype}")
+            q = torch.as_tensor(np.asarray(q), dtype=dtype, device=self.device)
+        if q.dtype != dtype:
+            raise TypeError(f"state must be {str(dtype).replace('torch.', '')} for precision "
+                            f"{self.precision!r}, got {q.dtype}")
         if q.numel() != math.prod(shape):
             raise ValueError(f"state has {tuple(q.shape)}, expected {shape}")
         return q.contiguous()
@@ -256,17 +261,19 @@ class _QGBase(SomaxModel):
     def _stream():
         return ctypes.c_void_p(_torch().cuda.current_stream().cuda_stream)
 
+    def _fn(self, name):
+        """The entry point ``name``, or its ``_f64`` form for an FP64 model."""
+        name = name + "_f64" if self._f64 else name
+        return name, getattr(_lib.load(), name)
+
     # -- kernels ------------------------------------------------------------
     def _invert(self, q, apply_bc: bool):
         torch = _torch()
         plan = self._ensure_plan()
         q = self._field(q)
         psi = torch.empty_like(q)
-        _lib.check(
-            _lib.load().somaxhip_qg_invert(plan, q.data_ptr(), psi.data_ptr(), int(apply_bc), self._ws.data_ptr(),
-                                          self._stream()),
-            "somaxhip_qg_invert",
-        )
+        name, fn = self._fn("somaxhip_qg_invert")
+        _lib.check(fn(plan, q.data_ptr(), psi.data_ptr(), int(apply_bc), self._ws.data_ptr(), self._stream()), name)
         return psi
 
     def _rhs(self, q, apply_bc: bool, want_psi: bool = False):
@@ -276,21 +283,20 @@ class _QGBase(SomaxModel):
         dq = torch.empty_like(q)
         psi = torch.empty_like(q) if want_psi else None
         prm = self._c_params()
+        name, fn = self._fn("somaxhip_qg_rhs")
         _lib.check(
-            _lib.load().somaxhip_qg_rhs(plan, q.data_ptr(), dq.data_ptr(), psi.data_ptr() if want_psi else None,
-                                       ctypes.byref(prm), int(apply_bc), self._ws.data_ptr(), self._stream()),
-            "somaxhip_qg_rhs",
+            fn(plan, q.data_ptr(), dq.data_ptr(), psi.data_ptr() if want_psi else None, ctypes.byref(prm),
+               int(apply_bc), self._ws.data_ptr(), self._stream()),
+            name,
         )
         return (dq, psi) if want_psi else dq
 
     def _step_inplace(self, q, dt: float, nsteps: int):
         plan = self._ensure_plan()
         prm = self._c_params()
-        _lib.check(
-            _lib.load().somaxhip_qg_step(plan, q.data_ptr(), None, int(nsteps), float(dt), ctypes.byref(prm),
-                                        self._ws.data_ptr(), self._stream()),
-            "somaxhip_qg_step",
-        )
+        name, fn = self._fn("somaxhip_qg_step")
+        _lib.check(fn(plan, q.data_ptr(), None, int(nsteps), float(dt), ctypes.byref(prm), self._ws.data_ptr(),
+                      self._stream()), name)
 
     def _step_tsit5_inplace(self, q, dt: float, nsteps: int):
         plan = self._ensure_plan()
@@ -298,18 +304,14 @@ class _QGBase(SomaxModel):
             nbytes = _lib.load().somaxhip_qg_tsit5_workspace_bytes(plan)
             self._tws = _torch().empty(max(int(nbytes), 256), dtype=_torch().uint8, device=self.device)
         prm = self._c_params()
-        _lib.check(
-            _lib.load().somaxhip_qg_step_tsit5(plan, q.data_ptr(), int(nsteps), float(dt), ctypes.byref(prm),
-                                              self._ws.data_ptr(), self._tws.data_ptr(), self._stream()),
-            "somaxhip_qg_step_tsit5",
-        )
+        name, fn = self._fn("somaxhip_qg_step_tsit5")
+        _lib.check(fn(plan, q.data_ptr(), int(nsteps), float(dt), ctypes.byref(prm), self._ws.data_ptr(),
+                      self._tws.data_ptr(), self._stream()), name)
 
     def _zero_ghost(self, q):
         q = self._field(q).clone()
-        _lib.check(
-            _lib.load().somaxhip_zero_ghost(q.data_ptr(), self._nl, self.grid.ny, self.grid.nx, self._stream()),
-            "somaxhip_zero_ghost",
-        )
+        name, fn = self._fn("somaxhip_zero_ghost")
+        _lib.check(fn(q.data_ptr(), self._nl, self.grid.ny, self.grid.nx, self._stream()), name)
         return q
 
     # -- time stepping (SomaxModel.integrate, model.py:53-88) ----------------
@@ -376,10 +378,11 @@ class _QGBase(SomaxModel):
         q = self._field(q)
         psi, u, v, zeta = (torch.empty_like(q) for _ in range(4))
         sums = torch.empty((self._nl, 2), dtype=torch.float64, device=q.device)
+        name, fn = self._fn("somaxhip_qg_diagnose")
         _lib.check(
-            _lib.load().somaxhip_qg_diagnose(plan, q.data_ptr(), psi.data_ptr(), u.data_ptr(), v.data_ptr(),
-                                            zeta.data_ptr(), sums.data_ptr(), self._ws.data_ptr(), self._stream()),
-            "somaxhip_qg_diagnose",
+            fn(plan, q.data_ptr(), psi.data_ptr(), u.data_ptr(), v.data_ptr(), zeta.data_ptr(), sums.data_ptr(),
+               self._ws.data_ptr(), self._stream()),
+            name,
         )
         return psi, u, v, zeta, sums[:, 0], sums[:, 1]
 
@@ -440,7 +443,7 @@ class BaroclinicQG(_QGBase):
         barotropic one is negative and makes that mode's operator indefinite;
         ``helmholtz_eigenvalues="reference"`` reproduces them.  Keyword-only
         extensions: ``biharmonic_viscosity``, ``advection`` ("arakawa" or
-        "weno"), ``precision`` ("fp32", "mixed" or "bf16x6"),
+        "weno"), ``precision`` ("fp32", "mixed", "bf16x6" or "fp64"),
         ``mass_correction``."""
         grid = ArakawaCGrid2D.from_interior(nx, ny, Lx, Ly)
         if stratification is not None:

@@ -194,6 +194,9 @@ class ReparameterizedQG(SomaxModel):
             )
         if poisson_bc != "dst":
             raise NotImplementedError("the HIP path solves the projection's Helmholtz problem with DST-I only")
+        if precision == "fp64":
+            raise ValueError("precision 'fp64' is not available here: the projection runs on FP32 state "
+                             "(precision must be 'fp32', 'mixed' or 'bf16x6')")
         swm = MultilayerShallowWater2D.create(
             nx=nx, ny=ny, Lx=Lx, Ly=Ly, g=g, f0=f0, beta=beta, n_layers=n_layers, H=H, g_prime=g_prime,
             stratification=stratification, lateral_viscosity=lateral_viscosity, bottom_drag=bottom_drag,

@@ -62,8 +62,9 @@ def field_stats(fields) -> np.ndarray:
     """``float64[len(fields), 5]``: per field (min, max, sum, n_nan,
     n_nonfinite), with min / max / sum over the non-NaN values.
 
-    ``fields``: float32 device tensors (any shape; non-contiguous ones are
-    made contiguous).  One ``somaxhip_field_stats`` call per 64 fields and one
+    ``fields``: device tensors (any shape; non-contiguous ones are made
+    contiguous, other dtypes rounded to float32 first, so the min / max of
+    an FP64 state are its float32 roundings).  One ``somaxhip_field_stats`` call per 64 fields and one
     small device-to-host copy; raises if the library is missing."""
     torch = _torch()
     fields = list(fields)
