@@ -38,6 +38,8 @@ struct Qg64 {
   long plane = 0;
   double dx = 0.0, dy = 0.0;
   int advection = 0, zero_psi_ghost = 0, mass = 0;
+  int fused = 0;  // SSP-RK3 steps by fused64_kernel (one layer, Arakawa, small square grid)
+  double cl2m0 = 1.0, cm2l0 = 1.0, lambda0 = 0.0;
   std::vector<void*> mem;
   // device tables
   double *sx = nullptr, *sy = nullptr;      // orthonormal DST-I matrices, Nx x Nx and Ny x Ny
@@ -273,34 +275,22 @@ __device__ inline double face_value(QF Q, int f, int n, double vel) {
   return Q(f + 1);
 }
 
-__global__ __launch_bounds__(256) void stage64_kernel(Stage64 s) {
-  const long c = blockIdx.x * 256L + threadIdx.x;
-  const int l = blockIdx.y;
-  if (c >= s.plane) return;
+// RHS at point (j, i) of layer l, from accessors of the stage state with
+// the boundary condition (Q), of the streamfunction (P) and of the total PV
+// q + beta (y - y0) (QT); pw is the layer's wall value of psi (WENO-Z).
+template <class QF, class PF, class QTF>
+__device__ inline double rhs_point(const Stage64& s, QF Q, PF P, QTF QT, int l, int j, int i, double pw) {
   const int Nx = s.Nx, Ny = s.Ny;
-  const int j = (int)(c / Nx), i = (int)(c % Nx);
-  const double* in = s.in + l * s.plane;
-  const double* psi = s.psi + l * s.plane;
   auto ghost = [&](int jj, int ii) { return jj == 0 || jj == Ny - 1 || ii == 0 || ii == Nx - 1; };
-  // q with the boundary condition, and q + beta (y - y0) (total PV)
-  auto Q = [&](int jj, int ii) { return (s.apply_bc && ghost(jj, ii)) ? 0.0 : in[(long)jj * Nx + ii]; };
-  auto QT = [&](int jj, int ii) { return Q(jj, ii) + s.beta_row[jj]; };
-  auto P = [&](int jj, int ii) { return psi[(long)jj * Nx + ii]; };
   const double dx2 = s.dx * s.dx, dy2 = s.dy * s.dy;
   auto lap = [&](auto F, int jj, int ii) {
     const double cc = F(jj, ii);
     return (F(jj, ii + 1) - 2.0 * cc + F(jj, ii - 1)) / dx2 + (F(jj + 1, ii) - 2.0 * cc + F(jj - 1, ii)) / dy2;
   };
-
   double r = 0.0;
   if (!ghost(j, i)) {
     if (s.advection == SOMAXHIP_ADV_WENO) {
       const int n_x = Nx - 2, n_y = Ny - 2;
-      double pw = 0.0;
-      if (s.alpha) {
-        pw = s.cm2l[l * s.nl] * s.alpha[0];
-        for (int m = 1; m < s.nl; ++m) pw = pw + s.cm2l[l * s.nl + m] * s.alpha[m];
-      }
       // corner streamfunction: corner (b, a) at T-index (a + 1/2, b + 1/2)
       auto CS = [&](int b, int a) {
         if (b == 0 || b == n_y || a == 0 || a == n_x) return pw;
@@ -340,15 +330,157 @@ __global__ __launch_bounds__(256) void stage64_kernel(Stage64 s) {
       r = r - s.nu4 * lap(L, j, i);
     }
   }
-  double o;
-  if (s.rhs_only) {
-    o = r;
-  } else {
-    o = s.b * (in[c] + s.h * r);
-    if (s.qn) o = s.a * s.qn[l * s.plane + c] / s.adiv + o;
-  }
-  s.out[l * s.plane + c] = o;
+  return r;
 }
+
+// The stage combination: the RHS alone, or a qn / adiv + b (x + h r).
+__device__ inline double combine(const Stage64& s, double x, double qn, double r) {
+  if (s.rhs_only) return r;
+  const double o = s.b * (x + s.h * r);
+  return s.qn ? s.a * qn / s.adiv + o : o;
+}
+
+__global__ __launch_bounds__(256) void stage64_kernel(Stage64 s) {
+  const long c = blockIdx.x * 256L + threadIdx.x;
+  const int l = blockIdx.y;
+  if (c >= s.plane) return;
+  const int Nx = s.Nx, Ny = s.Ny;
+  const int j = (int)(c / Nx), i = (int)(c % Nx);
+  const double* in = s.in + l * s.plane;
+  const double* psi = s.psi + l * s.plane;
+  // q with the boundary condition, and q + beta (y - y0) (total PV)
+  auto Q = [&](int jj, int ii) {
+    const bool g = jj == 0 || jj == Ny - 1 || ii == 0 || ii == Nx - 1;
+    return (s.apply_bc && g) ? 0.0 : in[(long)jj * Nx + ii];
+  };
+  auto QT = [&](int jj, int ii) { return Q(jj, ii) + s.beta_row[jj]; };
+  auto P = [&](int jj, int ii) { return psi[(long)jj * Nx + ii]; };
+  double pw = 0.0;
+  if (s.alpha && s.advection == SOMAXHIP_ADV_WENO) {
+    pw = s.cm2l[l * s.nl] * s.alpha[0];
+    for (int m = 1; m < s.nl; ++m) pw = pw + s.cm2l[l * s.nl + m] * s.alpha[m];
+  }
+  const double r = rhs_point(s, Q, P, QT, l, j, i, pw);
+  s.out[l * s.plane + c] = combine(s, in[c], s.qn ? s.qn[l * s.plane + c] : 0.0, r);
+}
+
+// ---------------------------------------------------------------------------
+// One-layer Arakawa models on small square grids (the barotropic double
+// gyre at 64^2): the whole SSP-RK3 loop in one workgroup of 1024 threads,
+// the state held in LDS (four N x N planes: the
+// stage state, two work planes and the sine matrix, which is the same on both
+// axes of a square grid), the sine transforms on FP64 MFMA
+// (v_mfma_f64_16x16x4f64).  One launch advances the model by nsteps steps.
+// ---------------------------------------------------------------------------
+constexpr int FUSED_THREADS = 1024;
+typedef double double4v __attribute__((ext_vector_type(4)));
+
+struct Fused64 {
+  double* q;        // state (in place); read as the step start q0 of every stage
+  double* psi_out;  // nullable: psi of the last stage's input state
+  const double* S;  // N x N sine matrix
+  const double* lamx;
+  const double* lamy;
+  double lambda0, cl2m0, cm2l0;
+  int N, nsteps, zero_psi_ghost;
+  double dt;
+  Stage64 st;  // RHS parameters (nl = 1, apply_bc, no mass constraint)
+};
+
+// C = A B (N x N, row-major, all in LDS), 16 x 16 output tiles over the 16
+// waves, the sum over k in steps of 4 products per MFMA; with lam set the
+// result is divided by (lamy[i] + lamx[j]) - lambda0.
+__device__ inline void gemm_lds64(const double* A, const double* B, double* C, int N, const double* lamy,
+                                  const double* lamx, double lambda0) {
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int nt = (N + 15) / 16;
+  for (int t = wave; t < nt * nt; t += FUSED_THREADS / 64) {
+    const int i0 = (t / nt) * 16, j0 = (t % nt) * 16;
+    const int ai = i0 + (lane & 15), bj = j0 + (lane & 15), kq = lane >> 4;
+    double4v acc = {0.0, 0.0, 0.0, 0.0};
+    for (int k0 = 0; k0 < N; k0 += 4) {
+      const int k = k0 + kq;
+      const double a = (ai < N && k < N) ? A[ai * N + k] : 0.0;
+      const double b = (k < N && bj < N) ? B[k * N + bj] : 0.0;
+      acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
+    }
+    // lane holds rows i0 + lane / 16 + 4 r of column j0 + lane % 16
+    if (bj < N) {
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int i = i0 + kq + 4 * r;
+        if (i >= N) continue;
+        double v = acc[r];
+        if (lamy) v = v / ((lamy[i] + lamx[bj]) - lambda0);
+        C[i * N + bj] = v;
+      }
+    }
+  }
+}
+
+__global__ __launch_bounds__(FUSED_THREADS) void fused64_kernel(Fused64 f) {
+  extern __shared__ double sm[];
+  const int N = f.N, NN = N * N;
+  double* qs = sm;           // stage state
+  double* w1 = sm + NN;      // work planes
+  double* w2 = sm + 2 * NN;
+  double* S = sm + 3 * NN;
+  for (int c = threadIdx.x; c < NN; c += FUSED_THREADS) {
+    qs[c] = f.q[c];
+    S[c] = f.S[c];
+  }
+  __syncthreads();
+  auto ghost = [&](int j, int i) { return j == 0 || j == N - 1 || i == 0 || i == N - 1; };
+  Stage64 s = f.st;
+  for (int n = 0; n < f.nsteps; ++n) {
+    for (int stg = 0; stg < 3; ++stg) {
+      // inversion: w1 = mode of bc(qs); psi -> w1 through w2
+      for (int c = threadIdx.x; c < NN; c += FUSED_THREADS)
+        w1[c] = ghost(c / N, c % N) ? 0.0 : f.cl2m0 * qs[c];
+      __syncthreads();
+      gemm_lds64(w1, S, w2, N, nullptr, nullptr, 0.0);
+      __syncthreads();
+      gemm_lds64(S, w2, w1, N, f.lamy, f.lamx, f.lambda0);
+      __syncthreads();
+      gemm_lds64(w1, S, w2, N, nullptr, nullptr, 0.0);
+      __syncthreads();
+      gemm_lds64(S, w2, w1, N, nullptr, nullptr, 0.0);
+      __syncthreads();
+      for (int c = threadIdx.x; c < NN; c += FUSED_THREADS) {
+        const double v = f.cm2l0 * w1[c];
+        w1[c] = (f.zero_psi_ghost && ghost(c / N, c % N)) ? 0.0 : v;
+      }
+      __syncthreads();
+      if (f.psi_out && n == f.nsteps - 1 && stg == 2)
+        for (int c = threadIdx.x; c < NN; c += FUSED_THREADS) f.psi_out[c] = w1[c];
+      // stage: w2 = combination of q0 (global), qs and the RHS of bc(qs)
+      s.h = f.dt;
+      s.a = stg == 0 ? 0.0 : 1.0;
+      s.adiv = stg == 2 ? 3.0 : 1.0;
+      if (stg == 1) s.a = 0.75;
+      s.b = stg == 0 ? 1.0 : (stg == 1 ? 0.25 : 2.0 / 3.0);
+      auto Q = [&](int jj, int ii) { return ghost(jj, ii) ? 0.0 : qs[jj * N + ii]; };
+      auto QT = [&](int jj, int ii) { return Q(jj, ii) + s.beta_row[jj]; };
+      auto P = [&](int jj, int ii) { return w1[jj * N + ii]; };
+      for (int c = threadIdx.x; c < NN; c += FUSED_THREADS) {
+        const int j = c / N, i = c % N;
+        const double r = rhs_point(s, Q, P, QT, 0, j, i, 0.0);
+        const double q0 = f.q[c];
+        double o = s.b * (qs[c] + s.h * r);
+        if (stg > 0) o = s.a * q0 / s.adiv + o;
+        w2[c] = o;
+        if (stg == 2) f.q[c] = o;
+      }
+      __syncthreads();
+      double* t = qs;
+      qs = w2;
+      w2 = t;
+    }
+  }
+}
+
+size_t fused64_lds_bytes(int N) { return (size_t)4 * N * N * sizeof(double); }
+constexpr size_t FUSED_LDS_MAX = 160 * 1024;
 
 // out = y + h * sum_j w[j] k[j] (sum in j order)
 struct Lin64 {
@@ -528,6 +660,25 @@ void rk3(const Qg64& p, double* q, double* psi_out, int nsteps, double dt, const
     s.h = dt;
     launch_stage(p, s, st);
   };
+  if (p.fused) {
+    if (nsteps == 0) return;
+    Fused64 f{};
+    f.q = q;
+    f.psi_out = psi_out;
+    f.S = p.sx;
+    f.lamx = p.lamx;
+    f.lamy = p.lamy;
+    f.lambda0 = p.lambda0;
+    f.cl2m0 = p.cl2m0;
+    f.cm2l0 = p.cm2l0;
+    f.N = p.Nx;
+    f.nsteps = nsteps;
+    f.zero_psi_ghost = p.zero_psi_ghost;
+    f.dt = dt;
+    f.st = stage_args(p, nullptr, nullptr, nullptr, nullptr, prm, ws, true);
+    fused64_kernel<<<1, FUSED_THREADS, fused64_lds_bytes(p.Nx), st>>>(f);
+    return;
+  }
   for (int n = 0; n < nsteps; ++n) {
     stage(q, nullptr, q1, 0.0, 1.0, 1.0);
     stage(q1, q, q2, 0.75, 1.0, 0.25);
@@ -634,6 +785,18 @@ Qg64* qg64_create(const somaxhip_qg_desc* d, std::string* err) {
   p->off_Q2 = take(fields);
   p->off_ALPHA = take((size_t)MAX_LAYERS * sizeof(double));
   p->ws_bytes = off;
+  p->cl2m0 = cl2m[0];
+  p->cm2l0 = cm2l[0];
+  p->lambda0 = lam[0];
+  // (the WENO-Z RHS on one CU is slower than the multi-kernel step: 333
+  // against 270 us per 64^2 step; the Arakawa RHS 142 against 244)
+  if (nl == 1 && !p->mass && p->advection == SOMAXHIP_ADV_ARAKAWA && p->Nx == p->Ny &&
+      fused64_lds_bytes(p->Nx) <= FUSED_LDS_MAX) {
+    static const hipError_t attr = hipFuncSetAttribute(reinterpret_cast<const void*>(fused64_kernel),
+                                                       hipFuncAttributeMaxDynamicSharedMemorySize,
+                                                       (int)FUSED_LDS_MAX);
+    p->fused = attr == hipSuccess ? 1 : 0;
+  }
   if (p->mass) {
     // homogeneous solutions h_m = 1 + (nabla^2 - lambda_m)^-1 [lambda_m], solved
     // here by the same FP64 transforms; their means come from the description

@@ -225,3 +225,43 @@ def test_fp64_plans_refuse_the_other_form(gpu):
         QGEnsemble([model64, model64])
     with pytest.raises(ValueError, match="fp64"):
         ReparameterizedQG.create(nx=32, ny=32, precision="fp64")
+
+
+@pytest.mark.parametrize("advection", ["arakawa", "weno"])
+def test_fp64_one_layer_step_and_psi_out(gpu, advection):
+    """1 and 50 SSP-RK3 steps of the barotropic 64^2 model against the oracle
+    (Arakawa: one launch with the state in LDS; WENO-Z: the multi-kernel
+    step), and psi_out = the streamfunction of the last stage's input state
+    (q2 = 3/4 q0 + 1/4 (q1 + dt f(q1)) of the last step)."""
+    import ctypes
+
+    import torch
+
+    from somax_amd import _lib
+    from somax_amd import testcases as tc
+
+    _log(f"[fp64 one-layer] advection={advection}")
+    model = tc.config_barotropic(64, precision="fp64", advection=advection)
+    setup = oqg.barotropic_setup(nx=64, ny=64, Lx=tc.BASIN, Ly=tc.BASIN, f0=tc.F0, beta=tc.BETA,
+                                 bottom_drag=tc.DRAG, wind_amplitude=tc.wind_amplitude() / 4000.0,
+                                 biharmonic_viscosity=tc.NU4_128, advection=advection)
+    q0 = tc.random_pv(model, seed=21)
+    qd = torch.from_numpy(q0).cuda()
+    psi = torch.zeros_like(qd)
+    plan = model._ensure_plan()
+    prm = model._c_params()
+    st = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    _lib.check(_lib.load().somaxhip_qg_step_f64(plan, qd.data_ptr(), psi.data_ptr(), 1, 4000.0, ctypes.byref(prm),
+                                                model._ws.data_ptr(), st), "somaxhip_qg_step_f64")
+    torch.cuda.synchronize()
+    dt = 4000.0
+    q1 = q0 + dt * oqg.rhs(setup, oqg.apply_bc(q0))
+    q2 = 0.75 * q0 + 0.25 * (q1 + dt * oqg.rhs(setup, oqg.apply_bc(q1)))
+    r_psi = rel(psi.cpu().numpy(), oqg.invert_pv(setup, oqg.apply_bc(q2)))
+    r1 = rel(qd.cpu().numpy(), oqg.ssprk3_step(setup, q0, dt))
+    q50 = torch.from_numpy(q0).cuda()
+    model._step_inplace(q50, dt, 50)
+    torch.cuda.synchronize()
+    r50 = rel(q50.cpu().numpy(), oqg.run(setup, q0, dt, 50))
+    _log(f"  1 step {r1:.3e}, psi_out {r_psi:.3e}, 50 steps {r50:.3e}")
+    assert r1 < 1e-12 and r_psi < 1e-12 and r50 < 1e-11
