@@ -275,18 +275,24 @@ __device__ inline double face_value(QF Q, int f, int n, double vel) {
   return Q(f + 1);
 }
 
+// The 5-point Laplacian of F at (jj, ii), in the reference's order of operations.
+template <class F>
+__device__ inline double lap64(const Stage64& s, F f, int jj, int ii) {
+  const double dx2 = s.dx * s.dx, dy2 = s.dy * s.dy;
+  const double cc = f(jj, ii);
+  return (f(jj, ii + 1) - 2.0 * cc + f(jj, ii - 1)) / dx2 + (f(jj + 1, ii) - 2.0 * cc + f(jj - 1, ii)) / dy2;
+}
+
 // RHS at point (j, i) of layer l, from accessors of the stage state with
-// the boundary condition (Q), of the streamfunction (P) and of the total PV
-// q + beta (y - y0) (QT); pw is the layer's wall value of psi (WENO-Z).
-template <class QF, class PF, class QTF>
-__device__ inline double rhs_point(const Stage64& s, QF Q, PF P, QTF QT, int l, int j, int i, double pw) {
+// the boundary condition (Q), of the streamfunction (P), of the total PV
+// q + beta (y - y0) (QT) and of the Laplacian of Q with a zero ghost ring
+// (L, read by the biharmonic term); pw is the layer's wall value of psi
+// (WENO-Z).
+template <class QF, class PF, class QTF, class LF>
+__device__ inline double rhs_point(const Stage64& s, QF Q, PF P, QTF QT, LF L, int l, int j, int i, double pw) {
   const int Nx = s.Nx, Ny = s.Ny;
   auto ghost = [&](int jj, int ii) { return jj == 0 || jj == Ny - 1 || ii == 0 || ii == Nx - 1; };
-  const double dx2 = s.dx * s.dx, dy2 = s.dy * s.dy;
-  auto lap = [&](auto F, int jj, int ii) {
-    const double cc = F(jj, ii);
-    return (F(jj, ii + 1) - 2.0 * cc + F(jj, ii - 1)) / dx2 + (F(jj + 1, ii) - 2.0 * cc + F(jj - 1, ii)) / dy2;
-  };
+  auto lap = [&](auto F, int jj, int ii) { return lap64(s, F, jj, ii); };
   double r = 0.0;
   if (!ghost(j, i)) {
     if (s.advection == SOMAXHIP_ADV_WENO) {
@@ -324,11 +330,7 @@ __device__ inline double rhs_point(const Stage64& s, QF Q, PF P, QTF QT, int l, 
   if (!ghost(j, i)) {
     if (s.kappa != 0.0 && l == s.nl - 1) r = r + -s.kappa * lap(P, j, i);
     if (s.nu != 0.0) r = r + s.nu * lap(Q, j, i);
-    if (s.nu4 != 0.0) {
-      // laplacian of the laplacian, whose ghost ring is 0
-      auto L = [&](int jj, int ii) { return ghost(jj, ii) ? 0.0 : lap(Q, jj, ii); };
-      r = r - s.nu4 * lap(L, j, i);
-    }
+    if (s.nu4 != 0.0) r = r - s.nu4 * lap(L, j, i);  // laplacian of the laplacian
   }
   return r;
 }
@@ -360,7 +362,12 @@ __global__ __launch_bounds__(256) void stage64_kernel(Stage64 s) {
     pw = s.cm2l[l * s.nl] * s.alpha[0];
     for (int m = 1; m < s.nl; ++m) pw = pw + s.cm2l[l * s.nl + m] * s.alpha[m];
   }
-  const double r = rhs_point(s, Q, P, QT, l, j, i, pw);
+  // the inner Laplacian of the biharmonic term, recomputed at each neighbour
+  auto L = [&](int jj, int ii) {
+    const bool g = jj == 0 || jj == Ny - 1 || ii == 0 || ii == Nx - 1;
+    return g ? 0.0 : lap64(s, Q, jj, ii);
+  };
+  const double r = rhs_point(s, Q, P, QT, L, l, j, i, pw);
   s.out[l * s.plane + c] = combine(s, in[c], s.qn ? s.qn[l * s.plane + c] : 0.0, r);
 }
 
@@ -376,7 +383,7 @@ constexpr int FUSED_THREADS = 1024;
 typedef double double4v __attribute__((ext_vector_type(4)));
 
 struct Fused64 {
-  double* q;        // state (in place); read as the step start q0 of every stage
+  double* q;        // state (in place): read at the start, written at the end of the launch
   double* psi_out;  // nullable: psi of the last stage's input state
   const double* S;  // N x N sine matrix
   const double* lamx;
@@ -398,6 +405,7 @@ __device__ inline void gemm_lds64(const double* A, const double* B, double* C, i
     const int i0 = (t / nt) * 16, j0 = (t % nt) * 16;
     const int ai = i0 + (lane & 15), bj = j0 + (lane & 15), kq = lane >> 4;
     double4v acc = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll 4
     for (int k0 = 0; k0 < N; k0 += 4) {
       const int k = k0 + kq;
       const double a = (ai < N && k < N) ? A[ai * N + k] : 0.0;
@@ -425,13 +433,32 @@ __global__ __launch_bounds__(FUSED_THREADS) void fused64_kernel(Fused64 f) {
   double* w1 = sm + NN;      // work planes
   double* w2 = sm + 2 * NN;
   double* S = sm + 3 * NN;
+  double* rows = sm + 4 * NN;  // beta (y - y0) and the wind pattern per row
   for (int c = threadIdx.x; c < NN; c += FUSED_THREADS) {
     qs[c] = f.q[c];
     S[c] = f.S[c];
   }
+  for (int j = threadIdx.x; j < N; j += FUSED_THREADS) {
+    rows[j] = f.st.beta_row[j];
+    rows[N + j] = f.st.wind_row[j];
+  }
   __syncthreads();
   auto ghost = [&](int j, int i) { return j == 0 || j == N - 1 || i == 0 || i == N - 1; };
   Stage64 s = f.st;
+  s.beta_row = rows;
+  s.wind_row = rows + N;
+  constexpr int PTS = 5;  // points per thread: N * N <= PTS * FUSED_THREADS
+  // q0, the state at the start of the step, of this thread's points: held in
+  // registers through the launch (scalars and selects, not an indexed array)
+  double q00 = 0.0, q01 = 0.0, q02 = 0.0, q03 = 0.0, q04 = 0.0;
+  {
+    const int t = threadIdx.x;
+    if (t < NN) q00 = f.q[t];
+    if (t + FUSED_THREADS < NN) q01 = f.q[t + FUSED_THREADS];
+    if (t + 2 * FUSED_THREADS < NN) q02 = f.q[t + 2 * FUSED_THREADS];
+    if (t + 3 * FUSED_THREADS < NN) q03 = f.q[t + 3 * FUSED_THREADS];
+    if (t + 4 * FUSED_THREADS < NN) q04 = f.q[t + 4 * FUSED_THREADS];
+  }
   for (int n = 0; n < f.nsteps; ++n) {
     for (int stg = 0; stg < 3; ++stg) {
       // inversion: w1 = mode of bc(qs); psi -> w1 through w2
@@ -462,14 +489,44 @@ __global__ __launch_bounds__(FUSED_THREADS) void fused64_kernel(Fused64 f) {
       auto Q = [&](int jj, int ii) { return ghost(jj, ii) ? 0.0 : qs[jj * N + ii]; };
       auto QT = [&](int jj, int ii) { return Q(jj, ii) + s.beta_row[jj]; };
       auto P = [&](int jj, int ii) { return w1[jj * N + ii]; };
-      for (int c = threadIdx.x; c < NN; c += FUSED_THREADS) {
+      auto L = [&](int jj, int ii) { return w2[jj * N + ii]; };
+      // the inner Laplacian of the biharmonic term once per point, into w2
+      if (s.nu4 != 0.0) {
+        for (int c = threadIdx.x; c < NN; c += FUSED_THREADS)
+          w2[c] = ghost(c / N, c % N) ? 0.0 : lap64(s, Q, c / N, c % N);
+        __syncthreads();
+      }
+      // outputs held in registers until every read of w2 (the Laplacian) is
+      // done; one RHS body in the loop (selects, not an indexed array)
+      double o0 = 0.0, o1 = 0.0, o2 = 0.0, o3 = 0.0, o4 = 0.0;
+#pragma unroll 1
+      for (int k = 0; k < PTS; ++k) {
+        const int c = threadIdx.x + k * FUSED_THREADS;
+        if (c >= NN) break;
         const int j = c / N, i = c % N;
-        const double r = rhs_point(s, Q, P, QT, 0, j, i, 0.0);
-        const double q0 = f.q[c];
+        const double r = rhs_point(s, Q, P, QT, L, 0, j, i, 0.0);
         double o = s.b * (qs[c] + s.h * r);
+        const double q0 = k == 0 ? q00 : (k == 1 ? q01 : (k == 2 ? q02 : (k == 3 ? q03 : q04)));
         if (stg > 0) o = s.a * q0 / s.adiv + o;
-        w2[c] = o;
-        if (stg == 2) f.q[c] = o;
+        o0 = k == 0 ? o : o0;
+        o1 = k == 1 ? o : o1;
+        o2 = k == 2 ? o : o2;
+        o3 = k == 3 ? o : o3;
+        o4 = k == 4 ? o : o4;
+      }
+      __syncthreads();
+      const double outs[PTS] = {o0, o1, o2, o3, o4};
+#pragma unroll
+      for (int k = 0; k < PTS; ++k) {
+        const int c = threadIdx.x + k * FUSED_THREADS;
+        if (c < NN) w2[c] = outs[k];
+      }
+      if (stg == 2) {  // the new step start
+        q00 = o0;
+        q01 = o1;
+        q02 = o2;
+        q03 = o3;
+        q04 = o4;
       }
       __syncthreads();
       double* t = qs;
@@ -477,9 +534,18 @@ __global__ __launch_bounds__(FUSED_THREADS) void fused64_kernel(Fused64 f) {
       w2 = t;
     }
   }
+  if (f.nsteps > 0) {
+    const double outs[PTS] = {q00, q01, q02, q03, q04};
+#pragma unroll
+    for (int k = 0; k < PTS; ++k) {
+      const int c = threadIdx.x + k * FUSED_THREADS;
+      if (c < NN) f.q[c] = outs[k];
+    }
+  }
 }
 
-size_t fused64_lds_bytes(int N) { return (size_t)4 * N * N * sizeof(double); }
+size_t fused64_lds_bytes(int N) { return (size_t)(4 * N * N + 2 * N) * sizeof(double); }
+constexpr int FUSED_STEPS = 256;  // SSP-RK3 steps per launch
 constexpr size_t FUSED_LDS_MAX = 160 * 1024;
 
 // out = y + h * sum_j w[j] k[j] (sum in j order)
@@ -676,7 +742,12 @@ void rk3(const Qg64& p, double* q, double* psi_out, int nsteps, double dt, const
     f.zero_psi_ghost = p.zero_psi_ghost;
     f.dt = dt;
     f.st = stage_args(p, nullptr, nullptr, nullptr, nullptr, prm, ws, true);
-    fused64_kernel<<<1, FUSED_THREADS, fused64_lds_bytes(p.Nx), st>>>(f);
+    // at most FUSED_STEPS steps per launch, so no launch runs for long
+    for (int done = 0; done < nsteps; done += FUSED_STEPS) {
+      f.nsteps = nsteps - done < FUSED_STEPS ? nsteps - done : FUSED_STEPS;
+      f.psi_out = done + f.nsteps == nsteps ? psi_out : nullptr;
+      fused64_kernel<<<1, FUSED_THREADS, fused64_lds_bytes(p.Nx), st>>>(f);
+    }
     return;
   }
   for (int n = 0; n < nsteps; ++n) {
@@ -791,7 +862,7 @@ Qg64* qg64_create(const somaxhip_qg_desc* d, std::string* err) {
   // (the WENO-Z RHS on one CU is slower than the multi-kernel step: 333
   // against 270 us per 64^2 step; the Arakawa RHS 142 against 244)
   if (nl == 1 && !p->mass && p->advection == SOMAXHIP_ADV_ARAKAWA && p->Nx == p->Ny &&
-      fused64_lds_bytes(p->Nx) <= FUSED_LDS_MAX) {
+      fused64_lds_bytes(p->Nx) <= FUSED_LDS_MAX && (long)p->Nx * p->Nx <= 5L * FUSED_THREADS) {
     static const hipError_t attr = hipFuncSetAttribute(reinterpret_cast<const void*>(fused64_kernel),
                                                        hipFuncAttributeMaxDynamicSharedMemorySize,
                                                        (int)FUSED_LDS_MAX);

@@ -17,6 +17,9 @@ Workloads:
   one launch per class or merged launch groups.
 * ``swm512``: the 2-layer shallow-water baroclinic jet, 512 x 512 channel,
   energy-enstrophy-conserving PV flux, SSP-RK3 at CFL 0.5.
+* ``barotropic64-fp64`` / ``barotropic64-weno-fp64``: the 1-layer double
+  gyre at 64 x 64 in FP64 (Arakawa: one launch per call of up to 256 steps;
+  WENO-Z: the multi-kernel step), SSP-RK3 at dt = 4000 s.
 
 Each step is preceded by one untimed warm-up step.  Prints one line per
 phase to stdout.
@@ -212,7 +215,8 @@ def main(argv=None) -> int:
     ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
     ap.add_argument("--workload", default="qg1024-weno-mass-mixed",
                     choices=["qg1024-weno-mass-mixed", "qg1024-weno-mass-fp32", "qg1024-mixed", "qg1024-fp32",
-                             "ensemble", "ensemble-merged", "swm512", "cmm256"])
+                             "ensemble", "ensemble-merged", "swm512", "cmm256", "barotropic64-fp64",
+                             "barotropic64-weno-fp64"])
     ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--copy-mib", type=int, default=1024)
     ap.add_argument("--dt", type=float, default=1200.0)
@@ -259,6 +263,17 @@ def main(argv=None) -> int:
         st = model.step(st, dt, args.steps)
         torch.cuda.synchronize()
         print(f"swm512: {args.steps} steps", flush=True)
+        return 0
+
+    if args.workload in ("barotropic64-fp64", "barotropic64-weno-fp64"):
+        adv = "weno" if "weno" in args.workload else "arakawa"
+        model = tc.config_barotropic(64, precision="fp64", advection=adv)
+        q = torch.from_numpy(tc.random_pv(model, seed=8)).cuda()
+        model._step_inplace(q, 4000.0, 1)
+        torch.cuda.synchronize()
+        model._step_inplace(q, 4000.0, args.steps)
+        torch.cuda.synchronize()
+        print(f"{args.workload}: {args.steps} steps", flush=True)
         return 0
 
     if args.workload == "cmm256":
