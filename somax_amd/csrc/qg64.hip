@@ -5,7 +5,7 @@
 //
 // The same plan description as the FP32 / BF16 path, with precision
 // SOMAXHIP_PREC_FP64, selects it.  Its kernels are written for exactness
-// rather than speed (this form serves the small FP64 double gyre):
+// first (this form serves the small FP64 double gyre):
 //   * layer <-> mode projections, one thread per cell and all layers;
 //   * the DST-I Helmholtz solve as four batched FP64 GEMMs per inversion
 //     (F S_x, S_y (.), S_y F S_x / (Lam_y + Lam_x - lambda), and back), the
@@ -16,7 +16,10 @@
 //     wind, bottom drag, harmonic and biharmonic viscosity and the stage
 //     combination, each written in the order of operations of the
 //     reference's array expressions, with floating-point contraction off so
-//     that no product is fused into a sum unless written as fma().
+//     that no product is fused into a sum unless written as fma();
+//   * one-layer Arakawa models on small square grids step in one launch per
+//     256 steps instead (fused64_kernel: state in LDS, transforms on FP64
+//     MFMA, the same per-point RHS).
 
 #include <cmath>
 #include <cstring>
@@ -374,10 +377,10 @@ __global__ __launch_bounds__(256) void stage64_kernel(Stage64 s) {
 // ---------------------------------------------------------------------------
 // One-layer Arakawa models on small square grids (the barotropic double
 // gyre at 64^2): the whole SSP-RK3 loop in one workgroup of 1024 threads,
-// the state held in LDS (four N x N planes: the
-// stage state, two work planes and the sine matrix, which is the same on both
-// axes of a square grid), the sine transforms on FP64 MFMA
-// (v_mfma_f64_16x16x4f64).  One launch advances the model by nsteps steps.
+// the state held in LDS (four N x N planes: the stage state, two work planes
+// and the sine matrix, which is the same on both axes of a square grid; and
+// the row tables), the sine transforms on FP64 MFMA (v_mfma_f64_16x16x4f64).
+// One launch advances the model by nsteps steps.
 // ---------------------------------------------------------------------------
 constexpr int FUSED_THREADS = 1024;
 typedef double double4v __attribute__((ext_vector_type(4)));

@@ -7,8 +7,9 @@
  *
  * Input (little-endian): int32 nx, ny, nl, nsteps, precision; float64 dx, dy,
  * dt, nu, nu4, kappa, tau0; float64 cl2m[nl*nl], cm2l[nl*nl], lambdas[nl],
- * beta_y_row[ny+2], wind_row[ny+2], wind_scale[nl]; float32 q[nl*(ny+2)*(nx+2)].
- * Output: float32 q after nsteps steps. */
+ * beta_y_row[ny+2], wind_row[ny+2], wind_scale[nl]; q[nl*(ny+2)*(nx+2)], float32
+ * or, for precision SOMAXHIP_PREC_FP64, float64 (stepped by somaxhip_qg_step_f64).
+ * Output: q after nsteps steps, in the input's type. */
 #include <stdio.h>
 #include <stdlib.h>
 
@@ -49,7 +50,9 @@ int main(int argc, char** argv) {
   double* wind = (double*)read_n(f, (size_t)(ny + 2) * sizeof(double));
   double* wscale = (double*)read_n(f, (size_t)nl * sizeof(double));
   const size_t n = (size_t)nl * (ny + 2) * (nx + 2);
-  float* q = (float*)read_n(f, n * sizeof(float));
+  const int f64 = precision == SOMAXHIP_PREC_FP64;
+  const size_t esz = f64 ? sizeof(double) : sizeof(float);
+  void* q = read_n(f, n * esz);
   fclose(f);
 
   int rc = somaxhip_init();
@@ -59,21 +62,26 @@ int main(int argc, char** argv) {
   somaxhip_qg_plan plan;
   if ((rc = somaxhip_qg_plan_create(&d, &plan))) return fail("somaxhip_qg_plan_create", rc);
   const size_t wsb = somaxhip_qg_workspace_bytes(plan);
-  float* dq = NULL;
+  void* dq = NULL;
   void* ws = NULL;
-  if (hipMalloc((void**)&dq, n * sizeof(float)) != hipSuccess || hipMalloc(&ws, wsb) != hipSuccess) {
+  if (hipMalloc(&dq, n * esz) != hipSuccess || hipMalloc(&ws, wsb) != hipSuccess) {
     fprintf(stderr, "hipMalloc failed\n");
     return 1;
   }
-  if (hipMemcpy(dq, q, n * sizeof(float), hipMemcpyHostToDevice) != hipSuccess) return 1;
+  if (hipMemcpy(dq, q, n * esz, hipMemcpyHostToDevice) != hipSuccess) return 1;
   somaxhip_qg_params prm = {sc[3], sc[4], sc[5], sc[6]};
-  if ((rc = somaxhip_qg_step(plan, dq, NULL, nsteps, sc[2], &prm, ws, NULL))) return fail("somaxhip_qg_step", rc);
-  if (hipMemcpy(q, dq, n * sizeof(float), hipMemcpyDeviceToHost) != hipSuccess) return 1;
+  if (f64) {
+    if ((rc = somaxhip_qg_step_f64(plan, (double*)dq, NULL, nsteps, sc[2], &prm, ws, NULL)))
+      return fail("somaxhip_qg_step_f64", rc);
+  } else if ((rc = somaxhip_qg_step(plan, (float*)dq, NULL, nsteps, sc[2], &prm, ws, NULL))) {
+    return fail("somaxhip_qg_step", rc);
+  }
+  if (hipMemcpy(q, dq, n * esz, hipMemcpyDeviceToHost) != hipSuccess) return 1;
   somaxhip_qg_plan_destroy(plan);
   hipFree(dq);
   hipFree(ws);
   FILE* o = fopen(argv[2], "wb");
-  if (!o || fwrite(q, sizeof(float), n, o) != n) return 2;
+  if (!o || fwrite(q, esz, n, o) != n) return 2;
   fclose(o);
   printf("stepped %d x %d x %d for %d steps\n", nl, ny + 2, nx + 2, nsteps);
   return 0;
