@@ -8,18 +8,17 @@
 // first (this form serves the small FP64 double gyre):
 //   * layer <-> mode projections, one thread per cell and all layers;
 //   * the DST-I Helmholtz solve as four batched FP64 GEMMs per inversion
-//     (F S_x, S_y (.), S_y F S_x / (Lam_y + Lam_x - lambda), and back), the
-//     sine matrices built on the host with the angle reduced exactly in
-//     integers first;
+//     (F S_x, S_y (.), S_y F S_x / (Lam_y + Lam_x - lambda), and back): on
+//     FP64 MFMA one wave per 16 x 16 tile up to 256 x 256, LDS-tiled FMAs
+//     beyond; the sine matrices built on the host with the angle reduced
+//     exactly in integers first;
 //   * the mass constraint's means as fixed-order block reductions;
 //   * one stage kernel per RK stage: Arakawa Jacobian or WENO-Z flux form,
 //     wind, bottom drag, harmonic and biharmonic viscosity and the stage
 //     combination, each written in the order of operations of the
 //     reference's array expressions, with floating-point contraction off so
-//     that no product is fused into a sum unless written as fma();
-//   * one-layer Arakawa models on small square grids step in one launch per
-//     256 steps instead (fused64_kernel: state in LDS, transforms on FP64
-//     MFMA, the same per-point RHS).
+//     that no product is fused into a sum unless written as fma().
+
 
 #include <cmath>
 #include <cstring>
@@ -41,8 +40,6 @@ struct Qg64 {
   long plane = 0;
   double dx = 0.0, dy = 0.0;
   int advection = 0, zero_psi_ghost = 0, mass = 0;
-  int fused = 0;  // SSP-RK3 steps by fused64_kernel (one layer, Arakawa, small square grid)
-  double cl2m0 = 1.0, cm2l0 = 1.0, lambda0 = 0.0;
   std::vector<void*> mem;
   // device tables
   double *sx = nullptr, *sy = nullptr;      // orthonormal DST-I matrices, Nx x Nx and Ny x Ny
@@ -146,6 +143,59 @@ __global__ __launch_bounds__(256) void gemm64_kernel(Gemm64 g) {
       if (g.ly) v = v / ((g.ly[row] + g.lx[col]) - g.lam[b]);
       C[(long)row * g.ldc + col] = v;
     }
+  }
+}
+
+typedef double double4m __attribute__((ext_vector_type(4)));
+
+// Small transforms (M, N <= 256): one wave per 16 x 16 output tile on FP64
+// MFMA (v_mfma_f64_16x16x4f64, lane l holds A[l % 16][l / 16] and
+// B[l / 16][l % 16] of each k-step of 4; the result row l / 16 + 4 r, column
+// l % 16), the operands read from global memory (they stay in L2 at these
+// sizes) GK k-steps ahead of the MFMAs.
+constexpr int GK = 8;
+constexpr int GEMM64_MFMA_MAX = 256;
+__global__ __launch_bounds__(256) void gemm64_mfma_kernel(Gemm64 g) {
+  const int tn = (g.N + 15) / 16, tiles = ((g.M + 15) / 16) * tn;
+  const int wave = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (wave >= tiles) return;
+  const int lane = threadIdx.x & 63, b = blockIdx.y;
+  const int i0 = (wave / tn) * 16, j0 = (wave % tn) * 16;
+  const double* A = g.A + b * g.sA;
+  const double* B = g.B + b * g.sB;
+  const int ai = i0 + (lane & 15), bj = j0 + (lane & 15), kq = lane >> 4;
+  const bool arow = ai < g.M, bcol = bj < g.N;
+  const int nks = (g.K + 3) / 4;  // k-steps
+  double xa[GK], xb[GK], ya[GK], yb[GK];
+  auto load = [&](int s0, double (&pa)[GK], double (&pb)[GK]) {
+#pragma unroll
+    for (int t = 0; t < GK; ++t) {
+      const int k = (s0 + t) * 4 + kq;
+      pa[t] = (arow && k < g.K) ? A[(long)ai * g.lda + k] : 0.0;
+      pb[t] = (bcol && k < g.K) ? B[(long)k * g.ldb + bj] : 0.0;
+    }
+  };
+  double4m acc = {0.0, 0.0, 0.0, 0.0};
+  load(0, xa, xb);
+  for (int s0 = 0; s0 < nks; s0 += GK) {
+    if (s0 + GK < nks) load(s0 + GK, ya, yb);
+#pragma unroll
+    for (int t = 0; t < GK; ++t) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(xa[t], xb[t], acc, 0, 0, 0);
+#pragma unroll
+    for (int t = 0; t < GK; ++t) {
+      xa[t] = ya[t];
+      xb[t] = yb[t];
+    }
+  }
+  if (!bcol) return;
+  double* C = g.C + b * g.sC;
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    const int row = i0 + kq + 4 * r;
+    if (row >= g.M) continue;
+    double v = acc[r];
+    if (g.ly) v = v / ((g.ly[row] + g.lx[bj]) - g.lam[b]);
+    C[(long)row * g.ldc + bj] = v;
   }
 }
 
@@ -374,183 +424,6 @@ __global__ __launch_bounds__(256) void stage64_kernel(Stage64 s) {
   s.out[l * s.plane + c] = combine(s, in[c], s.qn ? s.qn[l * s.plane + c] : 0.0, r);
 }
 
-// ---------------------------------------------------------------------------
-// One-layer Arakawa models on small square grids (the barotropic double
-// gyre at 64^2): the whole SSP-RK3 loop in one workgroup of 1024 threads,
-// the state held in LDS (four N x N planes: the stage state, two work planes
-// and the sine matrix, which is the same on both axes of a square grid; and
-// the row tables), the sine transforms on FP64 MFMA (v_mfma_f64_16x16x4f64).
-// One launch advances the model by nsteps steps.
-// ---------------------------------------------------------------------------
-constexpr int FUSED_THREADS = 1024;
-typedef double double4v __attribute__((ext_vector_type(4)));
-
-struct Fused64 {
-  double* q;        // state (in place): read at the start, written at the end of the launch
-  double* psi_out;  // nullable: psi of the last stage's input state
-  const double* S;  // N x N sine matrix
-  const double* lamx;
-  const double* lamy;
-  double lambda0, cl2m0, cm2l0;
-  int N, nsteps, zero_psi_ghost;
-  double dt;
-  Stage64 st;  // RHS parameters (nl = 1, apply_bc, no mass constraint)
-};
-
-// C = A B (N x N, row-major, all in LDS), 16 x 16 output tiles over the 16
-// waves, the sum over k in steps of 4 products per MFMA; with lam set the
-// result is divided by (lamy[i] + lamx[j]) - lambda0.
-__device__ inline void gemm_lds64(const double* A, const double* B, double* C, int N, const double* lamy,
-                                  const double* lamx, double lambda0) {
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int nt = (N + 15) / 16;
-  for (int t = wave; t < nt * nt; t += FUSED_THREADS / 64) {
-    const int i0 = (t / nt) * 16, j0 = (t % nt) * 16;
-    const int ai = i0 + (lane & 15), bj = j0 + (lane & 15), kq = lane >> 4;
-    double4v acc = {0.0, 0.0, 0.0, 0.0};
-#pragma unroll 4
-    for (int k0 = 0; k0 < N; k0 += 4) {
-      const int k = k0 + kq;
-      const double a = (ai < N && k < N) ? A[ai * N + k] : 0.0;
-      const double b = (k < N && bj < N) ? B[k * N + bj] : 0.0;
-      acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
-    }
-    // lane holds rows i0 + lane / 16 + 4 r of column j0 + lane % 16
-    if (bj < N) {
-#pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int i = i0 + kq + 4 * r;
-        if (i >= N) continue;
-        double v = acc[r];
-        if (lamy) v = v / ((lamy[i] + lamx[bj]) - lambda0);
-        C[i * N + bj] = v;
-      }
-    }
-  }
-}
-
-__global__ __launch_bounds__(FUSED_THREADS) void fused64_kernel(Fused64 f) {
-  extern __shared__ double sm[];
-  const int N = f.N, NN = N * N;
-  double* qs = sm;           // stage state
-  double* w1 = sm + NN;      // work planes
-  double* w2 = sm + 2 * NN;
-  double* S = sm + 3 * NN;
-  double* rows = sm + 4 * NN;  // beta (y - y0) and the wind pattern per row
-  for (int c = threadIdx.x; c < NN; c += FUSED_THREADS) {
-    qs[c] = f.q[c];
-    S[c] = f.S[c];
-  }
-  for (int j = threadIdx.x; j < N; j += FUSED_THREADS) {
-    rows[j] = f.st.beta_row[j];
-    rows[N + j] = f.st.wind_row[j];
-  }
-  __syncthreads();
-  auto ghost = [&](int j, int i) { return j == 0 || j == N - 1 || i == 0 || i == N - 1; };
-  Stage64 s = f.st;
-  s.beta_row = rows;
-  s.wind_row = rows + N;
-  constexpr int PTS = 5;  // points per thread: N * N <= PTS * FUSED_THREADS
-  // q0, the state at the start of the step, of this thread's points: held in
-  // registers through the launch (scalars and selects, not an indexed array)
-  double q00 = 0.0, q01 = 0.0, q02 = 0.0, q03 = 0.0, q04 = 0.0;
-  {
-    const int t = threadIdx.x;
-    if (t < NN) q00 = f.q[t];
-    if (t + FUSED_THREADS < NN) q01 = f.q[t + FUSED_THREADS];
-    if (t + 2 * FUSED_THREADS < NN) q02 = f.q[t + 2 * FUSED_THREADS];
-    if (t + 3 * FUSED_THREADS < NN) q03 = f.q[t + 3 * FUSED_THREADS];
-    if (t + 4 * FUSED_THREADS < NN) q04 = f.q[t + 4 * FUSED_THREADS];
-  }
-  for (int n = 0; n < f.nsteps; ++n) {
-    for (int stg = 0; stg < 3; ++stg) {
-      // inversion: w1 = mode of bc(qs); psi -> w1 through w2
-      for (int c = threadIdx.x; c < NN; c += FUSED_THREADS)
-        w1[c] = ghost(c / N, c % N) ? 0.0 : f.cl2m0 * qs[c];
-      __syncthreads();
-      gemm_lds64(w1, S, w2, N, nullptr, nullptr, 0.0);
-      __syncthreads();
-      gemm_lds64(S, w2, w1, N, f.lamy, f.lamx, f.lambda0);
-      __syncthreads();
-      gemm_lds64(w1, S, w2, N, nullptr, nullptr, 0.0);
-      __syncthreads();
-      gemm_lds64(S, w2, w1, N, nullptr, nullptr, 0.0);
-      __syncthreads();
-      for (int c = threadIdx.x; c < NN; c += FUSED_THREADS) {
-        const double v = f.cm2l0 * w1[c];
-        w1[c] = (f.zero_psi_ghost && ghost(c / N, c % N)) ? 0.0 : v;
-      }
-      __syncthreads();
-      if (f.psi_out && n == f.nsteps - 1 && stg == 2)
-        for (int c = threadIdx.x; c < NN; c += FUSED_THREADS) f.psi_out[c] = w1[c];
-      // stage: w2 = combination of q0 (global), qs and the RHS of bc(qs)
-      s.h = f.dt;
-      s.a = stg == 0 ? 0.0 : 1.0;
-      s.adiv = stg == 2 ? 3.0 : 1.0;
-      if (stg == 1) s.a = 0.75;
-      s.b = stg == 0 ? 1.0 : (stg == 1 ? 0.25 : 2.0 / 3.0);
-      auto Q = [&](int jj, int ii) { return ghost(jj, ii) ? 0.0 : qs[jj * N + ii]; };
-      auto QT = [&](int jj, int ii) { return Q(jj, ii) + s.beta_row[jj]; };
-      auto P = [&](int jj, int ii) { return w1[jj * N + ii]; };
-      auto L = [&](int jj, int ii) { return w2[jj * N + ii]; };
-      // the inner Laplacian of the biharmonic term once per point, into w2
-      if (s.nu4 != 0.0) {
-        for (int c = threadIdx.x; c < NN; c += FUSED_THREADS)
-          w2[c] = ghost(c / N, c % N) ? 0.0 : lap64(s, Q, c / N, c % N);
-        __syncthreads();
-      }
-      // outputs held in registers until every read of w2 (the Laplacian) is
-      // done; one RHS body in the loop (selects, not an indexed array)
-      double o0 = 0.0, o1 = 0.0, o2 = 0.0, o3 = 0.0, o4 = 0.0;
-#pragma unroll 1
-      for (int k = 0; k < PTS; ++k) {
-        const int c = threadIdx.x + k * FUSED_THREADS;
-        if (c >= NN) break;
-        const int j = c / N, i = c % N;
-        const double r = rhs_point(s, Q, P, QT, L, 0, j, i, 0.0);
-        double o = s.b * (qs[c] + s.h * r);
-        const double q0 = k == 0 ? q00 : (k == 1 ? q01 : (k == 2 ? q02 : (k == 3 ? q03 : q04)));
-        if (stg > 0) o = s.a * q0 / s.adiv + o;
-        o0 = k == 0 ? o : o0;
-        o1 = k == 1 ? o : o1;
-        o2 = k == 2 ? o : o2;
-        o3 = k == 3 ? o : o3;
-        o4 = k == 4 ? o : o4;
-      }
-      __syncthreads();
-      const double outs[PTS] = {o0, o1, o2, o3, o4};
-#pragma unroll
-      for (int k = 0; k < PTS; ++k) {
-        const int c = threadIdx.x + k * FUSED_THREADS;
-        if (c < NN) w2[c] = outs[k];
-      }
-      if (stg == 2) {  // the new step start
-        q00 = o0;
-        q01 = o1;
-        q02 = o2;
-        q03 = o3;
-        q04 = o4;
-      }
-      __syncthreads();
-      double* t = qs;
-      qs = w2;
-      w2 = t;
-    }
-  }
-  if (f.nsteps > 0) {
-    const double outs[PTS] = {q00, q01, q02, q03, q04};
-#pragma unroll
-    for (int k = 0; k < PTS; ++k) {
-      const int c = threadIdx.x + k * FUSED_THREADS;
-      if (c < NN) f.q[c] = outs[k];
-    }
-  }
-}
-
-size_t fused64_lds_bytes(int N) { return (size_t)(4 * N * N + 2 * N) * sizeof(double); }
-constexpr int FUSED_STEPS = 256;  // SSP-RK3 steps per launch
-constexpr size_t FUSED_LDS_MAX = 160 * 1024;
-
 // out = y + h * sum_j w[j] k[j] (sum in j order)
 struct Lin64 {
   const double* y;
@@ -638,6 +511,11 @@ inline unsigned blocks(long n) { return (unsigned)((n + 255) / 256); }
 inline double* wsd(void* ws, size_t off) { return reinterpret_cast<double*>(static_cast<char*>(ws) + off); }
 
 void gemm(const Gemm64& g, int batch, hipStream_t st) {
+  if (g.M <= GEMM64_MFMA_MAX && g.N <= GEMM64_MFMA_MAX) {
+    const int tiles = ((g.M + 15) / 16) * ((g.N + 15) / 16);
+    gemm64_mfma_kernel<<<dim3((tiles + 3) / 4, batch), 256, 0, st>>>(g);
+    return;
+  }
   const dim3 grid((g.N + TILE - 1) / TILE, (g.M + TILE - 1) / TILE, batch);
   gemm64_kernel<<<grid, 256, 0, st>>>(g);
 }
@@ -729,30 +607,6 @@ void rk3(const Qg64& p, double* q, double* psi_out, int nsteps, double dt, const
     s.h = dt;
     launch_stage(p, s, st);
   };
-  if (p.fused) {
-    if (nsteps == 0) return;
-    Fused64 f{};
-    f.q = q;
-    f.psi_out = psi_out;
-    f.S = p.sx;
-    f.lamx = p.lamx;
-    f.lamy = p.lamy;
-    f.lambda0 = p.lambda0;
-    f.cl2m0 = p.cl2m0;
-    f.cm2l0 = p.cm2l0;
-    f.N = p.Nx;
-    f.nsteps = nsteps;
-    f.zero_psi_ghost = p.zero_psi_ghost;
-    f.dt = dt;
-    f.st = stage_args(p, nullptr, nullptr, nullptr, nullptr, prm, ws, true);
-    // at most FUSED_STEPS steps per launch, so no launch runs for long
-    for (int done = 0; done < nsteps; done += FUSED_STEPS) {
-      f.nsteps = nsteps - done < FUSED_STEPS ? nsteps - done : FUSED_STEPS;
-      f.psi_out = done + f.nsteps == nsteps ? psi_out : nullptr;
-      fused64_kernel<<<1, FUSED_THREADS, fused64_lds_bytes(p.Nx), st>>>(f);
-    }
-    return;
-  }
   for (int n = 0; n < nsteps; ++n) {
     stage(q, nullptr, q1, 0.0, 1.0, 1.0);
     stage(q1, q, q2, 0.75, 1.0, 0.25);
@@ -859,18 +713,6 @@ Qg64* qg64_create(const somaxhip_qg_desc* d, std::string* err) {
   p->off_Q2 = take(fields);
   p->off_ALPHA = take((size_t)MAX_LAYERS * sizeof(double));
   p->ws_bytes = off;
-  p->cl2m0 = cl2m[0];
-  p->cm2l0 = cm2l[0];
-  p->lambda0 = lam[0];
-  // (the WENO-Z RHS on one CU is slower than the multi-kernel step: 333
-  // against 270 us per 64^2 step; the Arakawa RHS 142 against 244)
-  if (nl == 1 && !p->mass && p->advection == SOMAXHIP_ADV_ARAKAWA && p->Nx == p->Ny &&
-      fused64_lds_bytes(p->Nx) <= FUSED_LDS_MAX && (long)p->Nx * p->Nx <= 5L * FUSED_THREADS) {
-    static const hipError_t attr = hipFuncSetAttribute(reinterpret_cast<const void*>(fused64_kernel),
-                                                       hipFuncAttributeMaxDynamicSharedMemorySize,
-                                                       (int)FUSED_LDS_MAX);
-    p->fused = attr == hipSuccess ? 1 : 0;
-  }
   if (p->mass) {
     // homogeneous solutions h_m = 1 + (nabla^2 - lambda_m)^-1 [lambda_m], solved
     // here by the same FP64 transforms; their means come from the description

@@ -18,8 +18,7 @@ Workloads:
 * ``swm512``: the 2-layer shallow-water baroclinic jet, 512 x 512 channel,
   energy-enstrophy-conserving PV flux, SSP-RK3 at CFL 0.5.
 * ``barotropic64-fp64`` / ``barotropic64-weno-fp64``: the 1-layer double
-  gyre at 64 x 64 in FP64 (Arakawa: one launch per call of up to 256 steps;
-  WENO-Z: the multi-kernel step), SSP-RK3 at dt = 4000 s.
+  gyre at 64 x 64 in FP64 (Arakawa or WENO-Z), SSP-RK3 at dt = 4000 s.
 
 Each step is preceded by one untimed warm-up step.  Prints one line per
 phase to stdout.

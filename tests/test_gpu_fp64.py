@@ -67,9 +67,11 @@ def _state(model, seed, ghost_noise=False):
 
 
 @pytest.mark.parametrize("n,ny,nl,mass", [(64, 64, 3, False), (64, 64, 3, True), (96, 64, 2, False),
-                                          (64, 128, 1, False)])
+                                          (64, 128, 1, False), (300, 260, 2, True)])
 def test_fp64_inversion(gpu, n, ny, nl, mass):
-    """Non-square grids catch a transposed sine matrix or eigenvalue table."""
+    """Non-square grids catch a transposed sine matrix or eigenvalue table;
+    transforms up to 256 x 256 run on FP64 MFMA, the 302 x 262 case on the
+    LDS-tiled FMA kernel."""
     _log(f"[fp64 inversion] nx={n} ny={ny} nl={nl} mass={mass}")
     model, setup = _pair(n, nl=nl, ny=ny, mass_correction=mass)
     q, qd = _state(model, seed=3)
@@ -228,10 +230,9 @@ def test_fp64_plans_refuse_the_other_form(gpu):
 
 
 @pytest.mark.parametrize("advection", ["arakawa", "weno"])
-def test_fp64_one_layer_step_and_psi_out(gpu, advection):
-    """1 and 50 SSP-RK3 steps of the barotropic 64^2 model against the oracle
-    (Arakawa: one launch with the state in LDS; WENO-Z: the multi-kernel
-    step), and psi_out = the streamfunction of the last stage's input state
+def test_fp64_barotropic_step_and_psi_out(gpu, advection):
+    """1 and 50 SSP-RK3 steps of the barotropic 64^2 model against the
+    oracle, and psi_out = the streamfunction of the last stage's input state
     (q2 = 3/4 q0 + 1/4 (q1 + dt f(q1)) of the last step)."""
     import ctypes
 
@@ -240,7 +241,7 @@ def test_fp64_one_layer_step_and_psi_out(gpu, advection):
     from somax_amd import _lib
     from somax_amd import testcases as tc
 
-    _log(f"[fp64 one-layer] advection={advection}")
+    _log(f"[fp64 barotropic step] advection={advection}")
     model = tc.config_barotropic(64, precision="fp64", advection=advection)
     setup = oqg.barotropic_setup(nx=64, ny=64, Lx=tc.BASIN, Ly=tc.BASIN, f0=tc.F0, beta=tc.BETA,
                                  bottom_drag=tc.DRAG, wind_amplitude=tc.wind_amplitude() / 4000.0,
