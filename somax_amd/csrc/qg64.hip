@@ -40,6 +40,7 @@ struct Qg64 {
   long plane = 0;
   double dx = 0.0, dy = 0.0;
   int advection = 0, zero_psi_ghost = 0, mass = 0;
+  int direct = 0;  // one layer, identity projections, no mass constraint: the transforms read q and write psi
   std::vector<void*> mem;
   // device tables
   double *sx = nullptr, *sy = nullptr;      // orthonormal DST-I matrices, Nx x Nx and Ny x Ny
@@ -86,6 +87,8 @@ struct Gemm64 {
   const double* ly;
   const double* lx;
   const double* lam;
+  int a_ring;  // read the outer ring of A (rows 0, M-1; columns 0, K-1) as 0
+  int c_ring;  // write 0 on the outer ring of C (rows 0, M-1; columns 0, N-1)
 };
 
 // 64 x 64 output tile per workgroup of 256 threads, each thread 4 x 4
@@ -108,7 +111,8 @@ __global__ __launch_bounds__(256) void gemm64_kernel(Gemm64 g) {
     for (int e = threadIdx.x; e < TILE * KT; e += 256) {
       const int r = e / KT, k = e % KT;
       const int gr = row0 + r, gk = k0 + k;
-      As[k][r] = (gr < g.M && gk < g.K) ? A[(long)gr * g.lda + gk] : 0.0;
+      const bool ring = g.a_ring && (gr == 0 || gr == g.M - 1 || gk == 0 || gk == g.K - 1);
+      As[k][r] = (gr < g.M && gk < g.K && !ring) ? A[(long)gr * g.lda + gk] : 0.0;
     }
     for (int e = threadIdx.x; e < KT * TILE; e += 256) {
       const int k = e / TILE, c = e % TILE;
@@ -141,6 +145,7 @@ __global__ __launch_bounds__(256) void gemm64_kernel(Gemm64 g) {
       if (col >= g.N) continue;
       double v = acc[i][j];
       if (g.ly) v = v / ((g.ly[row] + g.lx[col]) - g.lam[b]);
+      if (g.c_ring && (row == 0 || row == g.M - 1 || col == 0 || col == g.N - 1)) v = 0.0;
       C[(long)row * g.ldc + col] = v;
     }
   }
@@ -171,7 +176,8 @@ __global__ __launch_bounds__(256) void gemm64_mfma_kernel(Gemm64 g) {
 #pragma unroll
     for (int t = 0; t < GK; ++t) {
       const int k = (s0 + t) * 4 + kq;
-      pa[t] = (arow && k < g.K) ? A[(long)ai * g.lda + k] : 0.0;
+      const bool ring = g.a_ring && (ai == 0 || ai == g.M - 1 || k == 0 || k == g.K - 1);
+      pa[t] = (arow && k < g.K && !ring) ? A[(long)ai * g.lda + k] : 0.0;
       pb[t] = (bcol && k < g.K) ? B[(long)k * g.ldb + bj] : 0.0;
     }
   };
@@ -195,6 +201,7 @@ __global__ __launch_bounds__(256) void gemm64_mfma_kernel(Gemm64 g) {
     if (row >= g.M) continue;
     double v = acc[r];
     if (g.ly) v = v / ((g.ly[row] + g.lx[bj]) - g.lam[b]);
+    if (g.c_ring && (row == 0 || row == g.M - 1 || bj == 0 || bj == g.N - 1)) v = 0.0;
     C[(long)row * g.ldc + bj] = v;
   }
 }
@@ -541,6 +548,19 @@ void invert(const Qg64& p, const double* q, double* psi, void* ws, bool apply_bc
   double* A = wsd(ws, p.off_A);
   double* B = wsd(ws, p.off_B);
   double* alpha = wsd(ws, p.off_ALPHA);
+  if (p.direct) {
+    // the projections are identities: bc(q) enters the first transform (its
+    // ring read as 0) and the last one writes psi (its ring 0 when masked)
+    const int Nx = p.Nx, Ny = p.Ny;
+    const long pl = p.plane;
+    gemm(Gemm64{q, pl, Nx, p.sx, 0, Nx, B, pl, Nx, Ny, Nx, Nx, nullptr, nullptr, nullptr, apply_bc ? 1 : 0, 0}, 1,
+         st);
+    gemm(Gemm64{p.sy, 0, Ny, B, pl, Nx, A, pl, Nx, Ny, Nx, Ny, p.lamy, p.lamx, p.lambdas, 0, 0}, 1, st);
+    gemm(Gemm64{A, pl, Nx, p.sx, 0, Nx, B, pl, Nx, Ny, Nx, Nx, nullptr, nullptr, nullptr, 0, 0}, 1, st);
+    gemm(Gemm64{p.sy, 0, Ny, B, pl, Nx, psi, pl, Nx, Ny, Nx, Ny, nullptr, nullptr, nullptr, 0, p.zero_psi_ghost}, 1,
+         st);
+    return;
+  }
   const unsigned nb = blocks(p.plane);
   mix64_kernel<<<nb, 256, 0, st>>>(
       Mix64{q, A, p.cl2m, nullptr, nullptr, p.nl, p.Nx, p.Ny, p.plane, apply_bc ? 1 : 0, 0});
@@ -713,6 +733,7 @@ Qg64* qg64_create(const somaxhip_qg_desc* d, std::string* err) {
   p->off_Q2 = take(fields);
   p->off_ALPHA = take((size_t)MAX_LAYERS * sizeof(double));
   p->ws_bytes = off;
+  p->direct = nl == 1 && !p->mass && cl2m[0] == 1.0 && cm2l[0] == 1.0;
   if (p->mass) {
     // homogeneous solutions h_m = 1 + (nabla^2 - lambda_m)^-1 [lambda_m], solved
     // here by the same FP64 transforms; their means come from the description
