@@ -96,11 +96,13 @@ __device__ inline float prefetch_spectral(const GemmArgs& a, const Spectral& sp,
   return 0.0f;
 }
 
-template <bool SUM64, int TM, int TN>
+// WN: waves along N; BW: columns of the tile (128, or 64 for the pair tiles).
+template <bool SUM64, int TM, int TN, int WN = 4 / TN, int BW = 128>
 __device__ inline void store_tile(const GemmArgs& a, long z, int a0, int b0, const floatx16 (&acc)[TM][TN],
                                   const double (&sum)[TM][TN][16], int opieces, float* stage, const Spectral& sp,
                                   const float* lam = nullptr) {
-  constexpr int WN = 4 / TN, WM = 4 / TM, NT = 64 * WM * WN;  // waves along N and M, threads
+  constexpr int WM = 4 / TM, NT = 64 * WM * WN;  // waves along M, threads
+  constexpr int TPR = BW / 8;                    // threads per row of 8-column vectors
   const int tid = threadIdx.x;
   const int lane = tid & 63;
   const int wave = tid >> 6;
@@ -126,8 +128,8 @@ __device__ inline void store_tile(const GemmArgs& a, long z, int a0, int b0, con
     const int r0 = a0 + h * 64;
     if (opieces == 0) {
       // FP32 (any pitch): 64 lanes on 64 consecutive columns
-      for (int e = tid; e < 64 * 128; e += NT) {
-        const int lr = e >> 7, lc = e & 127;
+      for (int e = tid; e < 64 * BW; e += NT) {
+        const int lr = e / BW, lc = e % BW;
         const int ro = r0 + lr, co = b0 + lc;
         if (ro >= a.RA || co >= a.RBpad) continue;
         float v = stage[lr * EPI_PITCH + lc];
@@ -135,13 +137,13 @@ __device__ inline void store_tile(const GemmArgs& a, long z, int a0, int b0, con
         a.Of[z * a.o_batch + (long)ro * a.ldo + co] = co < a.RB ? v : 0.0f;
       }
     } else {
-      // bf16 pieces: 8 consecutive columns per thread, NT / 16 rows per pass.
-      // RBpad is a multiple of 32 and b0 of 128, so a vector is all in or out.
-      const int lc = (tid & 15) * 8;
+      // bf16 pieces: 8 consecutive columns per thread, NT / TPR rows per pass.
+      // RBpad is a multiple of 32 and b0 of 64, so a vector is all in or out.
+      const int lc = (tid % TPR) * 8;
       const int co = b0 + lc;
 #pragma unroll
-      for (int k = 0; k < 64 / (NT / 16); ++k) {
-        const int lr = (tid >> 4) + (NT / 16) * k;
+      for (int k = 0; k < 64 / (NT / TPR); ++k) {
+        const int lr = tid / TPR + (NT / TPR) * k;
         const int ro = r0 + lr;
         if (ro >= a.RA || co >= a.RBpad) continue;
         const floatx4 va = *reinterpret_cast<const floatx4*>(stage + lr * EPI_PITCH + lc);
@@ -238,12 +240,14 @@ __device__ inline void store_one(const GemmArgs& a, long z, int r, int c, float 
 // per lane group are in flight, and the loads of KB 64-element k-chunks are
 // issued before any is used.  Accumulation is FP64 for FP32 operands and
 // six-product planes, FP32 otherwise (the precision of those products).
-template <bool F32, int NPC, int NT>
+template <bool F32, int NPC, int NT, int KBX = 0>
 __device__ void gemm_fringe(const GemmArgs& a, long z, int part, const GemmCut& cut, const PlaneSel& ps, int opieces,
                             void* lds) {
   constexpr int G = NT / 8;  // lane groups
   using acc_t = typename std::conditional<(F32 || NPC == 3), double, float>::type;
-  constexpr int KB = (NPC == 3 || NT >= 1024) ? 4 : 9;  // k-chunks per batch of loads (register bound)
+  // k-chunks per batch of loads (register bound; KBX: the caller's budget).
+  // Each lane sums the same k values in the same order whatever the batch.
+  constexpr int KB = KBX ? KBX : ((NPC == 3 || NT >= 1024) ? 4 : 9);
   constexpr int NV = F32 ? 2 : NPC;  // 16-byte loads per variable-row chunk
   const int fpw = cut.fpw;
   const int fa = a.RA - cut.am, fb = a.RB - cut.bm;
@@ -660,11 +664,135 @@ __device__ inline void bf16_mainloop(const GemmArgs& a, const uint16_t* S, uint1
 #undef SOMAX_ISSUE
 }
 
+// ---------------------------------------------------------------------------
+// Pair tiles (x1 / x3 transforms): 128 x 64 output tiles, 8 waves of 32 x 32
+// (4 along M, 2 along N), NSTAGE_P stages of a 128-row A tile and a 64-row B
+// tile per operand piece, at most 72 KiB of LDS and 128 VGPRs per lane, so
+// two workgroups share a CU and hide each other's k-loop latency.  Each wave
+// loads 16 A rows and 8 B rows per piece (the B load with lanes 0-31), so
+// every wave issues the same number of loads per k-tile.  Per output the same
+// MFMA chain as the 128 x 128 tiles.
+// ---------------------------------------------------------------------------
+constexpr int BNP = 64;
+constexpr int NSTAGE_P = 3;
+template <int NPROD>
+struct PairCfg {
+  static constexpr int NPC = NPROD == 1 ? 1 : 2;
+  static constexpr int TA = 128 * 32;                // bf16 elements per A piece tile
+  static constexpr int TB = BNP * 32;                // per B piece tile
+  static constexpr int STAGE = NPC * (TA + TB);
+  static constexpr int LPT = 2 * NPC;                // loads per thread per k-tile
+};
+
+template <int NPROD>
+__device__ inline void bf16_mainloop_pair(const GemmArgs& a, const uint16_t* S, uint16_t* lds, long z, int a0, int b0,
+                                          bool x_in_a, floatx16 (&acc)[1][1]) {
+  using C = PairCfg<NPROD>;
+  constexpr int NPC = C::NPC;
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wave = tid >> 6;
+  const int wm = wave >> 1, wn = wave & 1;
+  const uint16_t* X = a.Xb + z * a.x_batch;
+  // A rows a0 + ra_l (16 per wave), B rows b0 + rb_l (8 per wave, lanes 0-31)
+  const int ra_l = wave * 16 + (lane >> 2);
+  const int rb_l = wave * 8 + ((lane & 31) >> 2);
+  const int ca = (lane & 3) ^ swz(ra_l), cb = (lane & 3) ^ swz(rb_l);
+  auto row_src = [&](bool is_a, int r, int chunk, long& piece) {
+    const bool from_x = is_a == x_in_a;
+    if (from_x) {
+      piece = a.x_piece;
+      const int xr = r < a.xrows ? r : a.xrows - 1;  // rows past R are never stored
+      return X + (long)xr * a.ldx + chunk * 8;
+    }
+    piece = a.s_piece;
+    return S + (long)r * a.lds + chunk * 8;
+  };
+  long pa, pb;
+  const uint16_t* asrc = row_src(true, a0 + ra_l, ca, pa);
+  const uint16_t* bsrc = row_src(false, b0 + rb_l, cb, pb);
+  const int adst = wave * 16 * 32, bdst = NPC * C::TA + wave * 8 * 32;
+  const bool bload = lane < 32;
+#define SOMAX_ISSUE_P(KT, STG)                                                                             \
+  {                                                                                                        \
+    uint16_t* sb_ = lds + (STG) * C::STAGE;                                                                \
+    const int k0_ = (KT) * 32;                                                                             \
+    _Pragma("unroll") for (int p = 0; p < NPC; ++p) {                                                      \
+      __builtin_amdgcn_global_load_lds(asrc + p * pa + k0_, sb_ + p * C::TA + adst, 16, 0, 0);             \
+      if (bload) __builtin_amdgcn_global_load_lds(bsrc + p * pb + k0_, sb_ + bdst + p * C::TB, 16, 0, 0);  \
+    }                                                                                                      \
+  }
+  const int nk = (a.K + 31) / 32;
+#pragma unroll
+  for (int s = 0; s < NSTAGE_P - 1; ++s)
+    if (s < nk) SOMAX_ISSUE_P(s, s)
+
+  const int ra = wm * 32 + (lane & 31), rb = wn * 32 + (lane & 31);
+  auto read_frags = [&](const uint16_t* sb, int kk, bf16x8(&fa)[3][1], bf16x8(&fb)[3][1]) {
+    const int c = kk * 2 + (lane >> 5);
+#pragma unroll
+    for (int p = 0; p < NPC; ++p) {
+      fa[p][0] = *reinterpret_cast<const bf16x8*>(sb + p * C::TA + ra * 32 + ((c ^ swz(ra)) << 3));
+      fb[p][0] = *reinterpret_cast<const bf16x8*>(sb + NPC * C::TA + p * C::TB + rb * 32 + ((c ^ swz(rb)) << 3));
+    }
+  };
+  auto enter_tile = [&](int kt) {
+    const int later = (nk - 1 - kt) < (NSTAGE_P - 2) ? (nk - 1 - kt) : (NSTAGE_P - 2);
+    wait_later<C::LPT, NSTAGE_P - 2>(later);
+    __builtin_amdgcn_s_waitcnt((0xF) | (0x7 << 4) | (0x0 << 8) | (0x3 << 14));  // lgkmcnt(0)
+    asm volatile("" ::: "memory");
+    __builtin_amdgcn_s_barrier();
+    asm volatile("" ::: "memory");
+    if (kt + NSTAGE_P - 1 < nk) SOMAX_ISSUE_P(kt + NSTAGE_P - 1, (kt + NSTAGE_P - 1) % NSTAGE_P)
+  };
+  bf16x8 fsa[4][3][1], fsb[4][3][1];
+  auto stage_of = [&](int kt) { return lds + (kt % NSTAGE_P) * C::STAGE; };
+  auto lgkm0 = [] { __builtin_amdgcn_s_waitcnt((0xF) | (0x7 << 4) | (0x0 << 8) | (0x3 << 14)); };
+  enter_tile(0);
+  read_frags(stage_of(0), 0, fsa[0], fsb[0]);
+#define SOMAX_TILE_MID_P(KT, XS, YS, ZS)                   \
+  {                                                        \
+    lgkm0();                                               \
+    read_frags(stage_of(KT), 1, fsa[YS], fsb[YS]);         \
+    __builtin_amdgcn_sched_barrier(0);                     \
+    mfma_products<NPROD, 1, 1>(fsa[XS], fsb[XS], acc);     \
+    __builtin_amdgcn_sched_barrier(0);                     \
+    enter_tile((KT) + 1);                                  \
+    read_frags(stage_of((KT) + 1), 0, fsa[ZS], fsb[ZS]);   \
+    __builtin_amdgcn_sched_barrier(0);                     \
+    mfma_products<NPROD, 1, 1>(fsa[YS], fsb[YS], acc);     \
+    __builtin_amdgcn_sched_barrier(0);                     \
+  }
+  int kt = 0;
+  for (; kt + 2 <= nk - 1; kt += 2) {
+    SOMAX_TILE_MID_P(kt, 0, 1, 2)
+    SOMAX_TILE_MID_P(kt + 1, 2, 3, 0)
+  }
+  if (kt == nk - 2) {
+    SOMAX_TILE_MID_P(kt, 0, 1, 2)
+    lgkm0();
+    read_frags(stage_of(nk - 1), 1, fsa[3], fsb[3]);
+    mfma_products<NPROD, 1, 1>(fsa[2], fsb[2], acc);
+    mfma_products<NPROD, 1, 1>(fsa[3], fsb[3], acc);
+  } else {
+    lgkm0();
+    read_frags(stage_of(nk - 1), 1, fsa[1], fsb[1]);
+    mfma_products<NPROD, 1, 1>(fsa[0], fsb[0], acc);
+    mfma_products<NPROD, 1, 1>(fsa[1], fsb[1], acc);
+  }
+#undef SOMAX_TILE_MID_P
+#undef SOMAX_ISSUE_P
+}
+
 }  // namespace
 
 size_t dst_gemm_lds_bytes(int max_np) {
   const int npc = max_np == 1 ? 1 : (max_np == 3 ? 2 : 3);
   return (size_t)bf16_nstage(npc) * 2 * npc * 128 * 32 * sizeof(uint16_t);
+}
+size_t dst_gemm_pair_lds_bytes(int max_np) {
+  const int npc = max_np == 1 ? 1 : 2;
+  return (size_t)NSTAGE_P * npc * (128 + BNP) * 32 * sizeof(uint16_t);
 }
 
 // Workgroup shape per instance: x1 / x3 run 16 waves of 32 x 32 (four per
@@ -769,6 +897,76 @@ void dst_gemm_bf16_kernel(GemmTable<G> g) {
   else store_tile<false, TM, TN>(a, z, a0, b0, acc, sum, opieces, stage, ps.sp, lam);
 }
 
+
+// Pair-tile instance of the x1 / x3 transforms: tiles of 128 x 64, two
+// workgroups per CU (see bf16_mainloop_pair); the fringe as in the other
+// instances.
+__device__ inline int gemm_tiles_pair(const GemmArgs& a) {
+  const GemmCut cut = gemm_cut(a.RA, a.RB, a.K);
+  return ((cut.am + BM - 1) / BM) * ((cut.bm + BNP - 1) / BNP) * a.nplanes;
+}
+template <int G>
+__device__ inline int gemm_block_pair(const GemmTable<G>& g, int& c) {
+  if (G == 1) {
+    c = 0;
+    const int T = gemm_tiles_pair(g.a[0]);
+    const int b = blockIdx.x;
+    return b < T ? remap_tiles(b, T) : b;
+  }
+  int l;
+  if (!table_class(g.t, blockIdx.x, c, l)) return -1;  // padding between transforms
+  const int T = gemm_tiles_pair(g.a[c]);
+  return l < T ? remap_tiles(l, T) : l;
+}
+
+template <int NPMAX, int G>
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4, 4)))
+void dst_gemm_bf16_pair_kernel(GemmTable<G> g) {
+  static_assert(NPMAX == 1 || NPMAX == 3, "pair tiles serve the x1 and x3 transforms");
+  extern __shared__ __attribute__((aligned(16))) uint16_t lds16[];
+  int c;
+  const int lb = gemm_block_pair<G>(g, c);
+  if (lb < 0) return;
+  const GemmArgs& a = g.a[c];
+  const GemmCut cut = gemm_cut(a.RA, a.RB, a.K);
+  const int GA = (cut.am + BM - 1) / BM, GB = (cut.bm + BNP - 1) / BNP;
+  const int T = GA * GB * a.nplanes;
+  const bool fringe = lb >= T;
+  int zi, ta = 0, tb = 0, part = 0;
+  if (fringe) {
+    zi = (lb - T) / cut.fpw;
+    part = (lb - T) - zi * cut.fpw;
+  } else {
+    tile_of(lb, GA, GB, zi, ta, tb);
+  }
+  const long z = zi;
+  const PlaneSel ps = select_plane(a, z);
+  const bool lo = a.hp && a.hp[ps.m];
+  const int np = lo ? a.np_lo : a.np_hi;
+  const int opieces = lo ? a.op_lo : a.op_hi;
+  if (fringe) {
+    if (NPMAX == 1 || np == 1) gemm_fringe<false, 1, 512, 4>(a, z, part, cut, ps, opieces, lds16);
+    else gemm_fringe<false, 2, 512, 4>(a, z, part, cut, ps, opieces, lds16);
+    return;
+  }
+  const int a0 = ta * BM;
+  const int b0 = tb * BNP;
+  const bool x_in_a = !a.trans;
+  const uint16_t* S = a.Sb + ps.s_off;
+
+  __shared__ float lam_tile[256];
+  const bool spectral = ps.sp.lamA && opieces != 0;
+  const float lam_pre = spectral ? prefetch_spectral(a, ps.sp, a0, b0) : 0.0f;
+  floatx16 acc[1][1];
+  double sum[1][1][16];
+  zero_acc<1, 1>(acc, sum);
+  if (NPMAX == 1 || np == 1) bf16_mainloop_pair<1>(a, S, lds16, z, a0, b0, x_in_a, acc);
+  else bf16_mainloop_pair<3>(a, S, lds16, z, a0, b0, x_in_a, acc);
+  float* stage = reinterpret_cast<float*>(lds16);
+  if (spectral && threadIdx.x < 256) lam_tile[threadIdx.x] = lam_pre;  // read after store_tile's first barrier
+  const float* lam = spectral ? lam_tile : nullptr;
+  store_tile<false, 1, 1, 2, BNP>(a, z, a0, b0, acc, sum, opieces, stage, ps.sp, lam);
+}
 
 // ---------------------------------------------------------------------------
 // FP32 kernel (plain step): register-staged double buffer, FP64 flush every
@@ -900,6 +1098,10 @@ template __global__ void dst_gemm_bf16_kernel<6, 1>(GemmTable<1>);
 template __global__ void dst_gemm_bf16_kernel<1, MAX_GROUP>(GemmTable<MAX_GROUP>);
 template __global__ void dst_gemm_bf16_kernel<3, MAX_GROUP>(GemmTable<MAX_GROUP>);
 template __global__ void dst_gemm_bf16_kernel<6, MAX_GROUP>(GemmTable<MAX_GROUP>);
+template __global__ void dst_gemm_bf16_pair_kernel<1, 1>(GemmTable<1>);
+template __global__ void dst_gemm_bf16_pair_kernel<3, 1>(GemmTable<1>);
+template __global__ void dst_gemm_bf16_pair_kernel<1, MAX_GROUP>(GemmTable<MAX_GROUP>);
+template __global__ void dst_gemm_bf16_pair_kernel<3, MAX_GROUP>(GemmTable<MAX_GROUP>);
 
 hipError_t prepare_dst_gemm() {
   // up to 144 KiB of the 160 KiB LDS of a gfx950 CU (3 pieces, 3 stages);
@@ -922,32 +1124,80 @@ hipError_t prepare_dst_gemm() {
     if (e == hipSuccess)
       e = hipFuncSetAttribute(reinterpret_cast<const void*>(dst_gemm_bf16_kernel<6, MAX_GROUP>),
                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)dst_gemm_lds_bytes(6));
+    if (e == hipSuccess)
+      e = hipFuncSetAttribute(reinterpret_cast<const void*>(dst_gemm_bf16_pair_kernel<1, 1>),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)dst_gemm_pair_lds_bytes(1));
+    if (e == hipSuccess)
+      e = hipFuncSetAttribute(reinterpret_cast<const void*>(dst_gemm_bf16_pair_kernel<3, 1>),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)dst_gemm_pair_lds_bytes(3));
+    if (e == hipSuccess)
+      e = hipFuncSetAttribute(reinterpret_cast<const void*>(dst_gemm_bf16_pair_kernel<1, MAX_GROUP>),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)dst_gemm_pair_lds_bytes(1));
+    if (e == hipSuccess)
+      e = hipFuncSetAttribute(reinterpret_cast<const void*>(dst_gemm_bf16_pair_kernel<3, MAX_GROUP>),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)dst_gemm_pair_lds_bytes(3));
     return e;
   }();
   return status;
 }
 
 namespace {
-int gemm_blocks(const GemmArgs& a) {
+// Blocks of one transform: tiles of 128 x 128 (or 128 x 64 with pair tiles)
+// and the fringe workgroups.
+int gemm_blocks(const GemmArgs& a, bool pair = false) {
   const GemmCut cut = gemm_cut(a.RA, a.RB, a.K);
-  const int tiles = ((cut.bm + BN - 1) / BN) * ((cut.am + BM - 1) / BM) * a.nplanes;
+  const int bn = pair ? BNP : BN;
+  const int tiles = ((cut.bm + bn - 1) / bn) * ((cut.am + BM - 1) / BM) * a.nplanes;
   return tiles + cut.fpw * a.nplanes;
 }
+int max_products(const GemmArgs& a) { return a.np_lo > a.np_hi ? a.np_lo : a.np_hi; }
+// Compute units of the current device (looked up once per device).
+int cu_count() {
+  static int cus[64] = {};
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 256;
+  if (!cus[dev]) {
+    int n = 0;
+    cus[dev] = hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && n > 0 ? n : 256;
+  }
+  return cus[dev];
+}
+// The x1 / x3 transforms run on pair tiles when their 128 x 128 workgroups
+// (tiles and fringe, `blocks`) would take more than one round of the CUs:
+// measured, the 256-member ensemble's step 3.41 -> 3.23 ms, while the 1024^2
+// model's transforms, one round of 252 workgroups, are faster whole
+// (22.2 against 25.5 us for a x3 launch).
+bool use_pair(const GemmArgs& a, int max_np, long blocks) { return !a.Xf && max_np <= 3 && blocks > cu_count(); }
 }  // namespace
 
 void launch_dst_gemm_group(const GemmArgs* args, const int* batch, int n, hipStream_t st) {
   GemmTable<MAX_GROUP> g{};
   int max_np = 0;
   for (int i = 0; i < n; ++i) {
+    const int np = max_products(args[i]);
+    max_np = np > max_np ? np : max_np;
+  }
+  long whole = 0;  // workgroups with 128 x 128 tiles
+  for (int i = 0; i < n; ++i) {
+    GemmArgs t = args[i];
+    t.nplanes = batch[i];
+    whole += gemm_blocks(t);
+  }
+  const bool pair = use_pair(args[0], max_np, whole);
+  for (int i = 0; i < n; ++i) {
     g.a[i] = args[i];
     g.a[i].nplanes = batch[i];
-    table_add(g.t, gemm_blocks(g.a[i]), 1, 1);
-    const int np = args[i].np_lo > args[i].np_hi ? args[i].np_lo : args[i].np_hi;
-    max_np = np > max_np ? np : max_np;
+    table_add(g.t, gemm_blocks(g.a[i], pair), 1, 1);
   }
   const dim3 grid(g.t.start[g.t.n]);
   if (args[0].Xf) {
     hipLaunchKernelGGL(dst_gemm_f32_kernel<MAX_GROUP>, grid, dim3(256), 0, st, g);
+    return;
+  }
+  if (pair) {
+    const size_t lp = dst_gemm_pair_lds_bytes(max_np);
+    if (max_np == 1) hipLaunchKernelGGL((dst_gemm_bf16_pair_kernel<1, MAX_GROUP>), grid, dim3(512), lp, st, g);
+    else hipLaunchKernelGGL((dst_gemm_bf16_pair_kernel<3, MAX_GROUP>), grid, dim3(512), lp, st, g);
     return;
   }
   const size_t lds = dst_gemm_lds_bytes(max_np);
@@ -964,12 +1214,19 @@ void launch_dst_gemm(const GemmArgs& args, int batch, hipStream_t st) {
   g.a[0] = args;
   g.a[0].nplanes = batch;
   const GemmArgs& a = g.a[0];
-  const int blocks = gemm_blocks(a);
+  const int max_np = max_products(a);
+  const bool pair = use_pair(a, max_np, gemm_blocks(a));
+  const int blocks = gemm_blocks(a, pair);
   if (a.Xf) {
     hipLaunchKernelGGL(dst_gemm_f32_kernel<1>, dim3(blocks), dim3(256), 0, st, g);
     return;
   }
-  const int max_np = a.np_lo > a.np_hi ? a.np_lo : a.np_hi;
+  if (pair) {
+    const size_t lp = dst_gemm_pair_lds_bytes(max_np);
+    if (max_np == 1) hipLaunchKernelGGL((dst_gemm_bf16_pair_kernel<1, 1>), dim3(blocks), dim3(512), lp, st, g);
+    else hipLaunchKernelGGL((dst_gemm_bf16_pair_kernel<3, 1>), dim3(blocks), dim3(512), lp, st, g);
+    return;
+  }
   const size_t lds = dst_gemm_lds_bytes(max_np);
   if (max_np == 1)
     hipLaunchKernelGGL((dst_gemm_bf16_kernel<1, 1>), dim3(blocks), dim3(KernelShape<1>::NT), lds, st, g);

@@ -59,6 +59,7 @@ def summarize_trace(path: str, top: int = 40) -> str:
 
 # rocprof kernel symbol fragment -> the name the step profile (and bench.py) uses
 _BENCH_NAMES = (("dst_gemm_bf16_kernel<1", "dst_gemm_bf16x1"), ("dst_gemm_bf16_kernel<3", "dst_gemm_bf16x3"),
+                ("dst_gemm_bf16_pair_kernel<1", "dst_gemm_bf16x1"), ("dst_gemm_bf16_pair_kernel<3", "dst_gemm_bf16x3"),
                 ("dst_gemm_bf16_kernel<6", "dst_gemm_bf16x6"), ("dst_gemm_f32_kernel", "dst_gemm_fp32"),
                 ("qg_stage_", "qg_stage"), ("residual", "residual"), ("to_modal", "to_modal"),
                 ("to_layer", "to_layer"), ("swm_stage_kernel", "swm_stage"))
