@@ -242,10 +242,26 @@ __device__ inline float weno3_corr(float d2, float d3) {
   return (a0 * (0.5f * d2) + a1 * (0.5f * d3)) * __builtin_amdgcn_rcpf(a0 + a1);
 }
 
+// A compile-time flag for the tile bodies: true where every face of the tile
+// has its five upwind cells inside the interior block (no wall tests).
+template <bool B>
+struct Flag {
+  static constexpr bool value = B;
+};
+// x, hidden from the optimiser: without the wall selects of the general tile
+// path the compiler would fuse a helper's last operation into its caller's
+// (an FMA) and round interior tiles differently from wall tiles.
+__device__ inline float opaque(float x) {
+  asm("" : "+v"(x));
+  return x;
+}
+
 // Upwind face value minus the upwind cell value for the face between cells
 // f and f+1 (1-based interior indices 1..n along the line).  dm[k] holds the
 // increment q_{f+k} - q_{f+k-1} for k = -1..3 (dm index k+1), i.e. the
-// increments from cell f-2 up to cell f+3.  vel > 0: upwind cell f.
+// increments from cell f-2 up to cell f+3.  vel > 0: upwind cell f.  INT: the
+// caller guarantees 3 <= f and f + 3 <= n (the 5th-order stencil always fits).
+template <bool INT>
 __device__ inline float face_corr(int f, int n, float vel, const float* dm) {
   // the upwind side by selects rather than branches, so the lanes of a wave
   // with velocities of both signs share one reconstruction (mirror image
@@ -256,6 +272,7 @@ __device__ inline float face_corr(int f, int n, float vel, const float* dm) {
   const float e2 = pos ? dm[1] : -dm[3];
   const float e3 = pos ? dm[2] : -dm[2];
   const float e4 = pos ? dm[3] : -dm[1];
+  if (INT) return opaque(weno5_corr(e1, e2, e3, e4));
   if (c - 2 >= 1 && c + 2 <= n) return weno5_corr(e1, e2, e3, e4);
   if (c - 1 >= 1 && c + 1 <= n) return weno3_corr(e2, e3);
   return 0.0f;
@@ -368,46 +385,53 @@ __device__ __forceinline__ void qg_stage_weno(const StageArgs& a, int bx, int by
   if (tid < QH) sdb[tid] = vdb;
   __syncthreads();
 
+  // Every helper below takes a Flag: true for tiles away from the walls
+  // (`interior` below), where all of their wall tests hold and are left out;
+  // the arithmetic is the same either way.
   // psi at corner (cy, cx), between T rows cy, cy+1 and columns cx, cx+1
   // (cy in y0-1 .. y0+WY-1, cx in x0-1 .. x0+WX-1, always inside the tile):
   // the 4-point mean inside the interior block, the wall value on its
   // boundary lines
-  auto corner = [&](int cy, int cx) {
+  auto corner = [&](auto in, int cy, int cx) {
     const float* p = &sp[(cy - y0 + 1) * PW2 + (cx - x0 + 1)];
     const float v = 0.25f * (p[0] + p[1] + p[PW2] + p[PW2 + 1]);
+    if (decltype(in)::value) return opaque(v);
     return (cy >= 1 && cy <= Ny - 3 && cx >= 1 && cx <= Nx - 3) ? v : pw;
   };
   // inner Laplacian of q at tile position (ly, lx) of the 1-halo region
   // (gy = y0-1+ly, hx = x0-1+lx); its ghost ring is 0
-  auto inner_lap = [&](int ly, int lx) {
+  auto inner_lap = [&](auto in, int ly, int lx) {
     const int gy = y0 - 1 + ly, hx = x0 - 1 + lx;
     const float* c = &sq[(ly + 2) * WQ + lx + 2];
     const float v = (c[1] - 2.0f * c[0] + c[-1]) * a.idx2 + (c[WQ] - 2.0f * c[0] + c[-WQ]) * a.idy2;
+    if (decltype(in)::value) return opaque(v);
     return (gy >= 1 && gy <= Ny - 2 && hx >= 1 && hx <= Nx - 2) ? v : 0.0f;
   };
   // flux through the x-face between T columns f and f+1 on row gy = y0+ly
-  auto xface = [&](int ly, int f) {
+  auto xface = [&](auto in, int ly, int f) {
+    constexpr bool I = decltype(in)::value;
     const int gy = y0 + ly;
     float flux = 0.0f;
-    if (gy >= 1 && gy <= nyi && f >= 1 && f <= nxi - 1) {
-      const float u = -(corner(gy, f) - corner(gy - 1, f)) * a.idy;
+    if (I || (gy >= 1 && gy <= nyi && f >= 1 && f <= nxi - 1)) {
+      const float u = -(corner(in, gy, f) - corner(in, gy - 1, f)) * a.idy;
       const float* qr = &sq[(ly + 3) * WQ + (f - x0 + 3)];  // cell f
       float dm[5];
 #pragma unroll
       for (int k = 0; k < 5; ++k) dm[k] = qr[k - 1] - qr[k - 2];  // q_{f+k-1} - q_{f+k-2}
       const float qup = u > 0.0f ? qr[0] : qr[1];
-      flux = u * (qup + face_corr(f, nxi, u, dm));
+      flux = u * (qup + face_corr<I>(f, nxi, u, dm));
     }
-    return flux;
+    return I ? opaque(flux) : flux;
   };
   // y-face between T rows g and g+1 on column gx (g = y0-1 .. y0+WY-1):
   // v * (face value of q + b minus b of the lower cell g) into sfl, and the
   // same minus b of the upper cell into sfh
-  auto yface = [&](int g) {
-    const float v = (corner(g, gx) - corner(g, gx - 1)) * a.idx;
+  auto yface = [&](auto in, int g) {
+    constexpr bool I = decltype(in)::value;
+    const float v = (corner(in, g, gx) - corner(in, g, gx - 1)) * a.idx;
     const int lq = g - y0 + 3;  // local q row of cell g
     float flux = 0.0f;
-    if (gx >= 1 && gx <= nxi && g >= 1 && g <= nyi - 1) {
+    if (I || (gx >= 1 && gx <= nxi && g >= 1 && g <= nyi - 1)) {
       float dm[5];
 #pragma unroll
       for (int k = 0; k < 5; ++k) {
@@ -417,34 +441,43 @@ __device__ __forceinline__ void qg_stage_weno(const StageArgs& a, int bx, int by
       }
       // face value of q+b minus b_g: upwind g -> q_g + corr ; upwind g+1 -> q_{g+1} + db_g + corr
       const float base = v > 0.0f ? sq[lq * WQ + tx + 3] : sq[(lq + 1) * WQ + tx + 3] + sdb[lq];
-      flux = v * (base + face_corr(g, nyi, v, dm));
+      flux = v * (base + face_corr<I>(g, nyi, v, dm));
+      if (I) flux = opaque(flux);
     }
     const int lg = g - y0 + 1;
     sfl[lg * WX + tx] = flux;
     sfh[lg * WX + tx] = flux - v * sdb[lq];
   };
 
-  if (BIHARM) {
+  auto faces = [&](auto in) {
+    if (BIHARM) {
 #pragma unroll
-    for (int k = 0; k < NPM; ++k)
-      if (wv + 4 * k < PH) sl[(wv + 4 * k) * PW2 + tx + 1] = inner_lap(wv + 4 * k, tx + 1);
-    if (tid < 2 * PH) {
-      const int ly = tid >> 1;
-      const int lx = (tid & 1) ? WX + 1 : 0;
-      sl[ly * PW2 + lx] = inner_lap(ly, lx);
+      for (int k = 0; k < NPM; ++k)
+        if (wv + 4 * k < PH) sl[(wv + 4 * k) * PW2 + tx + 1] = inner_lap(in, wv + 4 * k, tx + 1);
+      if (tid < 2 * PH) {
+        const int ly = tid >> 1;
+        const int lx = (tid & 1) ? WX + 1 : 0;
+        sl[ly * PW2 + lx] = inner_lap(in, ly, lx);
+      }
     }
-  }
-  // faces of the lane's cells: east x-faces and north y-faces of rows
-  // wv + 4 r; wave 0 adds the south y-faces of the tile's first row, wave 1
-  // (16 lanes) the west x-faces of its first column
+    // faces of the lane's cells: east x-faces and north y-faces of rows
+    // wv + 4 r; wave 0 adds the south y-faces of the tile's first row, wave 1
+    // (16 lanes) the west x-faces of its first column
 #pragma unroll
-  for (int r = 0; r < WY / 4; ++r) {
-    const int ly = wv + 4 * r;
-    sfx[ly * (WX + 1) + tx + 1] = xface(ly, gx);
-    yface(y0 + ly);
-  }
-  if (wv == 0) yface(y0 - 1);
-  if (wv == 1 && tx < WY) sfx[tx * (WX + 1)] = xface(tx, x0 - 1);
+    for (int r = 0; r < WY / 4; ++r) {
+      const int ly = wv + 4 * r;
+      sfx[ly * (WX + 1) + tx + 1] = xface(in, ly, gx);
+      yface(in, y0 + ly);
+    }
+    if (wv == 0) yface(in, y0 - 1);
+    if (wv == 1 && tx < WY) sfx[tx * (WX + 1)] = xface(in, tx, x0 - 1);
+  };
+  // Away from the walls every face has its 5th-order stencil inside the
+  // interior block: upwind cells reach x0-1 .. x0+WX (and y0-1 .. y0+WY), so
+  // x0 - 3 >= 1 and x0 + WX + 2 <= nxi (likewise in y) cover every test.
+  const bool interior = x0 >= 4 && x0 + WX + 2 <= nxi && y0 >= 4 && y0 + WY + 2 <= nyi;
+  if (interior) faces(Flag<true>{});
+  else faces(Flag<false>{});
   __syncthreads();
 
   const float wc = pr.tau0 * a.t.wind_scale[z];
