@@ -695,41 +695,66 @@ __device__ __forceinline__ void residual_plain(const ResidualArgs& a, int bx, in
   store_pieces(o, a.op_piece, 2, r);
 }
 
-// (Remapping these workgroups XCD by XCD, as the stage kernels are, halves
-// the residual's HBM traffic but makes it slower: 16 -> 20 us at 1024^2.)
 // Folded residual on the parity components (block b = 2p + q of plane m at
 // b * P + m): the 5-point Laplacian of a component on the half array, with
 // zero exterior and the mirror rule at the fold line (the neighbour past the
 // last physical index h-1 is +-the value itself: even / odd parity).
-// block (bx, z) of the (ceil(hy * op_kpad / 256), 4P) grid
+// Each wave walks RES_ROWS folded rows of a 64-column strip, keeping the
+// rows above and below in registers (a sliding window), so a row is fetched
+// about once and no per-element index division is needed.  A block is four
+// waves stacked along the rows: 64 columns x 4 RES_ROWS rows.
+// block (bx, z) of the (ceil(op_kpad / 64) * ceil(hy / (4 RES_ROWS)), 4P) grid
+constexpr int RES_ROWS = 8;
 __device__ __forceinline__ void residual_fold(const ResidualArgs& a, int bx, int z) {
   const int hx = a.Nx / 2, hy = a.Ny / 2;
-  const int idx = bx * blockDim.x + threadIdx.x;  // row-major over hy x op_kpad
-  if (idx >= hy * a.op_kpad) return;
-  const int ky = idx / a.op_kpad, kx = idx - ky * a.op_kpad;
-  uint16_t* o = a.r + z * a.op_batch + (long)ky * a.op_ld + kx;
+  const int ncx = (a.op_kpad + 63) / 64;
+  const int bry = bx / ncx, bcx = bx - bry * ncx;  // uniform over the block
+  const int wv = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
+  const int kx = bcx * 64 + ((int)threadIdx.x & 63);
+  const int ky0 = (bry * 4 + wv) * RES_ROWS;
+  if (kx >= a.op_kpad || ky0 >= hy) return;
+  const int kend = ky0 + RES_ROWS < hy ? ky0 + RES_ROWS : hy;
+  uint16_t* o = a.r + z * a.op_batch + kx;
   if (kx >= hx) {
-    store_pieces(o, a.op_piece, 2, 0.0f);
+    for (int ky = ky0; ky < kend; ++ky) store_pieces(o + (long)ky * a.op_ld, a.op_piece, 2, 0.0f);
     return;
   }
   const int blk = z / a.P, m = z - blk * a.P;
   const float sx = (blk & 1) ? -1.0f : 1.0f, sy = (blk >> 1) ? -1.0f : 1.0f;
   const float* Z = a.psi + (long)z * hx * hy;
-  const float* Zr = Z + ky * hx;
-  // physical indices in the first half; neighbours mapped back to folded
-  // indices (all in range, so the five loads are issued unconditionally)
-  const int x = fold_phys(kx, hx), y = fold_phys(ky, hy);
+  const float* Q = a.q + (long)z * hx * hy;
+  const float lam = a.lam[m];
+  // physical column in the first half; its neighbours mapped back to folded
+  // columns (all in range)
+  const int x = fold_phys(kx, hx);
   const int xw = x == 0 ? 0 : (x == 1 ? hx - 1 : x - 2);
-  const int ys = y == 0 ? 0 : (y == 1 ? hy - 1 : y - 2);
-  const float c = Zr[kx], ze = Zr[x], zw = Zr[xw], zn = Z[y * hx + kx], zs = Z[ys * hx + kx];
-  const float qv = a.q[(long)z * hx * hy + ky * hx + kx];
-  const float e = x == hx - 1 ? sx * c : ze;
-  const float w = x == 0 ? 0.0f : zw;
-  const float n = y == hy - 1 ? sy * c : zn;
-  const float so = y == 0 ? 0.0f : zs;
-  const float lap = (e - 2.0f * c + w) * a.idx2 + (n - 2.0f * c + so) * a.idy2;
-  const float r = qv - 4.0f * (lap - a.lam[m] * c);
-  store_pieces(o, a.op_piece, 2, r);
+  // Folded row ky is physical row y = fold_phys(ky): its north neighbour is
+  // folded row ky + 1 (row 0 for the ghost pair ky = hy - 1, whose y is 0),
+  // its south neighbour folded row ky - 1 (row hy - 1 for ky = 0).
+  auto col = [&](int k) { return Z[(long)k * hx + kx]; };
+  float below = ky0 == 0 ? col(hy - 1) : col(ky0 - 1);
+  float cur = col(ky0);
+#pragma unroll
+  for (int r = 0; r < RES_ROWS; ++r) {
+    const int ky = ky0 + r;
+    if (ky >= kend) break;
+    const float above = ky + 1 < hy ? col(ky + 1) : col(0);
+    const int y = fold_phys(ky, hy);
+    const float* Zr = Z + (long)ky * hx;
+    const float c = cur, ze = Zr[x], zw = Zr[xw];
+    const float qv = Q[(long)ky * hx + kx];
+    const float e = x == hx - 1 ? sx * c : ze;
+    const float w = x == 0 ? 0.0f : zw;
+    const float n = y == hy - 1 ? sy * c : above;
+    const float so = y == 0 ? 0.0f : below;
+    // the x term fused into the y product's sum (the rounding this kernel
+    // has always had, now fixed rather than left to the compiler's contraction)
+    const float lap = __builtin_fmaf(e - 2.0f * c + w, a.idx2, (n - 2.0f * c + so) * a.idy2);
+    const float res = qv - 4.0f * (lap - lam * c);
+    store_pieces(o + (long)ky * a.op_ld, a.op_piece, 2, res);
+    below = cur;
+    cur = above;
+  }
 }
 
 __global__ __launch_bounds__(256) void residual_kernel(ResidualArgs a) { residual_plain(a, blockIdx.x, blockIdx.y); }
@@ -749,7 +774,10 @@ __global__ __launch_bounds__(256) void residual_group_kernel(ResidualGroup g) {
 
 namespace {
 dim3 residual_grid(const ResidualArgs& a) {
-  if (a.fold) return dim3((unsigned)(((long)a.op_kpad * (a.Ny / 2) + 255) / 256), 4 * a.P);
+  if (a.fold) {
+    const long rows = (a.Ny / 2 + 4 * RES_ROWS - 1) / (4 * RES_ROWS);
+    return dim3((unsigned)(((a.op_kpad + 63) / 64) * rows), 4 * a.P);
+  }
   return dim3((unsigned)(((long)a.op_kpad * a.Ny + 255) / 256), a.P);
 }
 }  // namespace
