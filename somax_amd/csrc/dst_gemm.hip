@@ -24,6 +24,7 @@
 // and remapped so that each XCD works on a contiguous run of tiles (shared
 // operand rows in its L2).
 
+#include <atomic>
 #include <type_traits>
 
 #include "common.h"
@@ -1151,16 +1152,19 @@ int gemm_blocks(const GemmArgs& a, bool pair = false) {
   return tiles + cut.fpw * a.nplanes;
 }
 int max_products(const GemmArgs& a) { return a.np_lo > a.np_hi ? a.np_lo : a.np_hi; }
-// Compute units of the current device (looked up once per device).
+// Compute units of the current device (looked up once per device; several
+// host threads may race to fill an entry, and they store the same value).
 int cu_count() {
-  static int cus[64] = {};
+  static std::atomic<int> cus[64] = {};
   int dev = 0;
   if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 256;
-  if (!cus[dev]) {
+  int c = cus[dev].load(std::memory_order_relaxed);
+  if (!c) {
     int n = 0;
-    cus[dev] = hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && n > 0 ? n : 256;
+    c = hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && n > 0 ? n : 256;
+    cus[dev].store(c, std::memory_order_relaxed);
   }
-  return cus[dev];
+  return c;
 }
 // The x1 / x3 transforms run on pair tiles when their 128 x 128 workgroups
 // (tiles and fringe, `blocks`) would take more than one round of the CUs:
