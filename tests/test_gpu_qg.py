@@ -229,6 +229,56 @@ def test_weno_rhs_ragged_grid_matches_oracle(gpu, nx, ny, precision):
     assert r < 2e-5 and worst < 1e-4
 
 
+# Deep stratifications: the kernels are instantiated for 1, 2, 3, 4 and 8
+# layer slots (a model of 5-8 layers runs the 8-slot instance); the layer
+# parameters below are a smooth stack of 4000 m with reduced gravities
+# falling with depth.
+def _deep_layers(nl):
+    H = tuple(float(h) for h in np.linspace(300.0, 700.0, nl) * (4000.0 / np.linspace(300.0, 700.0, nl).sum()))
+    gp = (9.81,) + tuple(0.03 / (1.0 + k) for k in range(nl - 1))
+    return H, gp
+
+
+@pytest.mark.parametrize("nl,precision,advection,mass", [(4, "fp32", "arakawa", False), (5, "mixed", "weno", True),
+                                                         (8, "mixed", "arakawa", True), (8, "fp32", "weno", False)])
+def test_deep_stratification_matches_oracle(gpu, nl, precision, advection, mass):
+    _log(f"[layers] nl={nl} precision={precision} advection={advection} mass={mass}")
+    from somax_amd import BaroclinicQG
+    from somax_amd import testcases as tc
+
+    n = 64
+    H, gp = _deep_layers(nl)
+    args = dict(nx=n, ny=n, Lx=tc.BASIN, Ly=tc.BASIN, f0=tc.F0, beta=tc.BETA, H=H, g_prime=gp,
+                bottom_drag=tc.DRAG, wind_amplitude=tc.wind_amplitude(), biharmonic_viscosity=tc.NU4_128 * 16.0,
+                lateral_viscosity=0.0)
+    model = BaroclinicQG.create(n_layers=nl, precision=precision, advection=advection, mass_correction=mass, **args)
+    setup = oqg.baroclinic_setup(advection=advection, mass_correction=mass, **args)
+    q, qd = _state(model, seed=71 + nl, zero_mean=mass)
+    psi = model._invert_pv(qd).cpu().numpy()
+    ref = oqg.invert_pv(setup, q.astype(np.float64))
+    q1 = qd.clone()
+    model._step_inplace(q1, 1200.0, 1)
+    ref1 = oqg.ssprk3_step(setup, q.astype(np.float64), 1200.0)
+    r, r1 = rel(psi, ref), rel(q1.cpu().numpy(), ref1)
+    psi1 = oqg.invert_pv(setup, ref1)
+    psi_gpu = model._invert_pv(q1).cpu().numpy()
+    p1 = rel(psi_gpu, psi1)
+    # psi after a step splits into the step's error in q and the inversion's:
+    # the inversion of the GPU's own q1 against FP64, and q1 itself.  psi
+    # against the oracle is then bounded by the FP32-state floor, the exact q1
+    # rounded to FP32 and inverted in FP64 (the inverse Laplacian lifts the
+    # rounding noise's low modes: with 5-8 layers this floor alone reaches
+    # 4e-7 - 2e-6, and FP32 and mixed steps measure 1-4x it).
+    inv1 = rel(psi_gpu, oqg.invert_pv(setup, q1.cpu().numpy().astype(np.float64)))
+    floor = rel(oqg.invert_pv(setup, ref1.astype(np.float32).astype(np.float64)), psi1)
+    _log(f"  rel-L2 psi {r:.3e}; one step q {r1:.3e}, inversion of q1 {inv1:.3e}, psi {p1:.3e} "
+         f"(FP32-state floor {floor:.3e})")
+    bar = 3e-6 if precision == "fp32" else 3e-7
+    assert r < bar and inv1 < bar
+    assert r1 < 1e-6
+    assert p1 < max(1e-6, 10.0 * floor)
+
+
 def test_weno_conserves_pv(gpu):
     """Flux form with zero wall flux: the interior sum of the advective
     tendency vanishes (no forcing, no dissipation)."""
