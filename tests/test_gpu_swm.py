@@ -307,6 +307,42 @@ def test_energy_enstrophy_rhs_nonsquare_periodic(gpu):
     assert max(rs) < 1e-5
 
 
+@pytest.mark.parametrize("nl,flux", [(6, "average"), (8, "energy_enstrophy")])
+def test_deep_stack_rhs_and_step_match_oracle(gpu, nl, flux):
+    """6 and 8 layers (the most a plan takes) on a 96 x 64 walled basin: each
+    layer's workgroup sums the pressure terms of every layer above it, so the
+    deep stacks exercise the longest of those chains."""
+    import torch
+
+    from somax_amd import MultilayerShallowWater2D, MultilayerSW2DState
+
+    _log(f"[swm] deep stack nl={nl} flux={flux}")
+    w = np.linspace(300.0, 900.0, nl)
+    H = tuple(float(x) for x in w * (4000.0 / w.sum()))
+    gp = (9.81,) + tuple(0.02 / (1.0 + k) for k in range(nl - 1))
+    args = dict(nx=96, ny=64, Lx=9.6e5, Ly=6.4e5, f0=1e-4, beta=1e-11, H=H, g_prime=gp, bc="wall",
+                lateral_viscosity=20.0, bottom_drag=1e-7, wind_amplitude=2e-5, vorticity_flux=flux)
+    model = MultilayerShallowWater2D.create(n_layers=nl, **args)
+    s = oswm.setup(**args)
+    rng = np.random.default_rng(40 + nl)
+    shape = s.shape
+    h = (np.asarray(s.H)[:, None, None] + rng.standard_normal(shape)).astype(np.float32)
+    u = (0.2 * rng.standard_normal(shape)).astype(np.float32)
+    v = (0.2 * rng.standard_normal(shape)).astype(np.float32)
+    f64 = [a.astype(np.float64) for a in (h, u, v)]
+    st = MultilayerSW2DState(*(torch.from_numpy(a.copy()).cuda() for a in (h, u, v)))
+    d = model.build_terms()(0.0, st)
+    ref = oswm.rhs(s, *oswm.apply_bc(s, *f64))
+    rs = [rel(a.cpu().numpy(), b) for a, b in zip((d.h, d.u, d.v), ref)]
+    dt = 0.3 * 1e4 / np.sqrt(9.81 * 4000.0)
+    out = model.step(MultilayerSW2DState(*(torch.from_numpy(a.copy()).cuda() for a in (h, u, v))), dt, 1)
+    ref1 = oswm.ssprk3_step(s, *f64, dt)
+    r1 = [rel(a.cpu().numpy(), b) for a, b in zip((out.h, out.u, out.v), ref1)]
+    _log(f"  rhs rel-L2 dh {rs[0]:.3e} du {rs[1]:.3e} dv {rs[2]:.3e}; "
+         f"one step h {r1[0]:.3e} u {r1[1]:.3e} v {r1[2]:.3e}")
+    assert max(rs) < 1e-5 and max(r1) < 1e-6
+
+
 @pytest.mark.parametrize("n,bc", [(128, "periodic"), (128, "wall"), (512, "channel")])
 def test_energy_enstrophy_one_step_matches_oracle(gpu, n, bc):
     from somax_amd import testcases as tc
