@@ -101,6 +101,50 @@ def test_ensemble_matches_single_models(gpu, precision, concurrent, graph, merge
     assert worst < 1e-7
 
 
+def _deep_member_args(nx, ny, nl, tau0):
+    """A double-gyre member with an nl-layer stack of 4000 m (reduced
+    gravities falling with depth), for layer counts past the 1-3 of MEMBERS."""
+    from somax_amd import testcases as tc
+
+    w = np.linspace(300.0, 700.0, nl)
+    H = tuple(float(x) for x in w * (4000.0 / w.sum()))
+    gp = (9.81,) + tuple(0.03 / (1.0 + k) for k in range(nl - 1))
+    return dict(nx=nx, ny=ny, Lx=tc.BASIN, Ly=tc.BASIN * ny / nx, f0=tc.F0, beta=tc.BETA, n_layers=nl, H=H,
+                g_prime=gp, bottom_drag=tc.DRAG, wind_amplitude=tc.wind_amplitude(tau0, tc.BASIN * ny / nx),
+                biharmonic_viscosity=tc.NU4_128 * (128.0 / nx) ** 4)
+
+
+@pytest.mark.parametrize("graph,merged", [(True, False), (False, True)])
+def test_ensemble_deep_members_match_single_models(gpu, graph, merged):
+    """Members of 4, 5 and 8 layers beside shallow ones in one class (the
+    kernels' 4- and 8-slot instances over a mixed plane table), plus an
+    8-layer member on a folded 256^2 grid; WENO-Z and the mass constraint."""
+    import torch
+
+    from somax_amd import BaroclinicQG, QGEnsemble
+    from somax_amd import testcases as tc
+
+    _log(f"[ensemble] deep members graph={graph} merged={merged}")
+    shapes = [(64, 64, 5), (64, 64, 8), (64, 64, 1), (64, 64, 4), (96, 64, 8), (256, 256, 8)]
+    rng = np.random.default_rng(17)
+    models = [BaroclinicQG.create(**_deep_member_args(nx, ny, nl, float(rng.uniform(0.04, 0.12))),
+                                  precision="mixed", advection="weno", mass_correction=True) for nx, ny, nl in shapes]
+    qs = [tc.random_pv(m, 90 + i, zero_mean=True).astype(np.float32).reshape(m._nl, m.grid.Ny, m.grid.Nx)
+          for i, m in enumerate(models)]
+    ens = QGEnsemble(models, concurrent=True, graph=graph, merged=merged)
+    packed = ens.pack(qs)
+    dt, nsteps = 1200.0, 2
+    ens.step(packed, dt, nsteps)
+    torch.cuda.synchronize()
+    worst = 0.0
+    for i, m in enumerate(models):
+        q = torch.from_numpy(qs[i]).cuda()
+        m._step_inplace(q, dt, nsteps)
+        worst = max(worst, rel(ens.member(packed, i).cpu().numpy(), q.cpu().numpy()))
+    _log(f"  worst member rel-L2 vs single model = {worst:.3e}")
+    assert worst < 1e-7
+
+
 def test_ensemble_graph_replays(gpu):
     import torch
 
