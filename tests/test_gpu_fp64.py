@@ -82,6 +82,32 @@ def test_fp64_inversion(gpu, n, ny, nl, mass):
     assert r < 1e-12
 
 
+@pytest.mark.parametrize("nl,advection,mass", [(5, "weno", True), (8, "arakawa", True), (8, "weno", False)])
+def test_fp64_deep_stratification(gpu, nl, advection, mass):
+    """5- and 8-layer stacks (the layer mixes and the stage kernel's layer
+    loops at their longest): inversion and one SSP-RK3 step against the
+    oracle, to FP64 round-off."""
+    from somax_amd import BaroclinicQG
+    from somax_amd import testcases as tc
+
+    _log(f"[fp64 layers] nl={nl} advection={advection} mass={mass}")
+    w = np.linspace(300.0, 700.0, nl)
+    H = tuple(float(x) for x in w * (4000.0 / w.sum()))
+    gp = (9.81,) + tuple(0.03 / (1.0 + k) for k in range(nl - 1))
+    args = dict(nx=64, ny=48, Lx=tc.BASIN, Ly=tc.BASIN * 0.75, f0=tc.F0, beta=tc.BETA, H=H, g_prime=gp,
+                bottom_drag=tc.DRAG, wind_amplitude=tc.wind_amplitude(), biharmonic_viscosity=tc.NU4_128 * 16.0,
+                lateral_viscosity=0.0)
+    model = BaroclinicQG.create(n_layers=nl, precision="fp64", advection=advection, mass_correction=mass, **args)
+    setup = oqg.baroclinic_setup(advection=advection, mass_correction=mass, **args)
+    q, qd = _state(model, seed=11 + nl)
+    r = rel(model._invert_pv(qd).cpu().numpy(), oqg.invert_pv(setup, q))
+    q1 = qd.clone()
+    model._step_inplace(q1, 4000.0, 1)
+    r1 = rel(q1.cpu().numpy(), oqg.ssprk3_step(setup, q, 4000.0))
+    _log(f"  rel-L2 psi {r:.3e}, one step q {r1:.3e}")
+    assert r < 1e-12 and r1 < 1e-12
+
+
 @pytest.mark.parametrize("advection,mass,nu", [("arakawa", False, 0.0), ("arakawa", False, 50.0),
                                                ("weno", False, 0.0), ("weno", True, 0.0), ("arakawa", True, 50.0)])
 def test_fp64_rhs(gpu, advection, mass, nu):
