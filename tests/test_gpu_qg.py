@@ -279,6 +279,31 @@ def test_deep_stratification_matches_oracle(gpu, nl, precision, advection, mass)
     assert p1 < max(1e-6, 10.0 * floor)
 
 
+@pytest.mark.parametrize("precision", ["fp32", "mixed"])
+def test_diagnose_deep_stratification(gpu, precision):
+    """diagnose of an 8-layer model at 64^2 (per-layer KE and enstrophy
+    over all eight layers), to the bars of the 3-layer test."""
+    from somax_amd import BaroclinicQG
+    from somax_amd import testcases as tc
+
+    _log(f"[diagnose] 8 layers precision={precision}")
+    H, gp = _deep_layers(8)
+    args = dict(nx=64, ny=64, Lx=tc.BASIN, Ly=tc.BASIN, f0=tc.F0, beta=tc.BETA, H=H, g_prime=gp,
+                bottom_drag=tc.DRAG, wind_amplitude=tc.wind_amplitude(), biharmonic_viscosity=tc.NU4_128 * 16.0)
+    model = BaroclinicQG.create(n_layers=8, precision=precision, **args)
+    setup = oqg.baroclinic_setup(**args)
+    q, qd = _state(model, seed=37)
+    d = model.diagnose(type(model)._state_cls(q=qd))
+    ref = oqg.diagnose(setup, q.astype(np.float64))
+    rs = {k: rel(getattr(d, a).cpu().numpy(), ref[k]) for k, a in
+          (("psi", "psi"), ("u", "u"), ("v", "v"), ("relative_vorticity", "relative_vorticity"),
+           ("kinetic_energy", "kinetic_energy"), ("enstrophy", "enstrophy"))}
+    _log("  " + " ".join(f"{k} {v:.2e}" for k, v in rs.items()))
+    assert rs["psi"] < 3e-6 and rs["u"] < 5e-5 and rs["v"] < 5e-5 and rs["relative_vorticity"] < 5e-4
+    assert rs["kinetic_energy"] < 1e-5 and rs["enstrophy"] < 1e-6
+    assert d.kinetic_energy.shape == (8,) and d.rossby_radii.shape == (8,)
+
+
 def test_weno_conserves_pv(gpu):
     """Flux form with zero wall flux: the interior sum of the advective
     tendency vanishes (no forcing, no dissipation)."""

@@ -33,7 +33,11 @@ def _log(msg):
 
 
 LAYERS = {2: dict(H=(500.0, 500.0), g_prime=(9.81, 0.02)),
-          3: dict(H=(400.0, 1100.0, 2600.0), g_prime=(9.81, 0.025, 0.0125))}
+          3: dict(H=(400.0, 1100.0, 2600.0), g_prime=(9.81, 0.025, 0.0125)),
+          # deep stacks of 4000 m, reduced gravities falling with depth
+          5: dict(H=(400.0, 600.0, 800.0, 1000.0, 1200.0), g_prime=(9.81, 0.03, 0.015, 0.01, 0.0075)),
+          8: dict(H=(250.0, 300.0, 350.0, 450.0, 550.0, 600.0, 700.0, 800.0),
+                  g_prime=(9.81, 0.03, 0.015, 0.01, 0.0075, 0.006, 0.005, 0.0043))}
 
 
 def _pair(n, nl, precision="fp32", **kw):
@@ -83,7 +87,8 @@ def _ulp_ok(r, h_gpu, h_ref, k=2.0):
     return float(np.abs(np.asarray(h_gpu, dtype=np.float64) - np.asarray(h_ref)).max()) <= k * ulp
 
 
-@pytest.mark.parametrize("n,nl,precision,tol", [(32, 2, "fp32", 1e-5), (128, 3, "fp32", 1e-5), (128, 3, "mixed", 1e-6)])
+@pytest.mark.parametrize("n,nl,precision,tol", [(32, 2, "fp32", 1e-5), (128, 3, "fp32", 1e-5), (128, 3, "mixed", 1e-6),
+                                                (64, 5, "mixed", 1e-6), (64, 8, "fp32", 1e-5)])
 def test_project_matches_oracle(gpu, n, nl, precision, tol):
     import torch
 
