@@ -304,6 +304,42 @@ def test_diagnose_deep_stratification(gpu, precision):
     assert d.kinetic_energy.shape == (8,) and d.rossby_radii.shape == (8,)
 
 
+@pytest.mark.timeout(600)
+@pytest.mark.parametrize("n,nl,advection,mass", [(2048, 3, "weno", True), (4096, 1, "arakawa", False)])
+def test_large_grid_one_step_matches_oracle(gpu, n, nl, advection, mass):
+    """The largest single models: their transforms take several rounds of
+    the CUs (pair tiles for the one- and three-product launches), mixed
+    inversion, one SSP-RK3 step against the oracle's FFT path."""
+    from oracle.cpu import FftCpuStep
+    from somax_amd import BaroclinicQG
+    from somax_amd import testcases as tc
+
+    _log(f"[large] n={n} nl={nl} advection={advection} mass={mass}")
+    H, gp = ((400.0, 1100.0, 2600.0), (9.81, 0.025, 0.0125)) if nl == 3 else ((4000.0,), (9.81,))
+    args = dict(nx=n, ny=n, Lx=tc.BASIN, Ly=tc.BASIN, f0=tc.F0, beta=tc.BETA, H=H, g_prime=gp, bottom_drag=tc.DRAG,
+                wind_amplitude=tc.wind_amplitude(), biharmonic_viscosity=tc.NU4_128 * (128.0 / n) ** 4)
+    model = BaroclinicQG.create(n_layers=nl, precision="mixed", advection=advection, mass_correction=mass, **args)
+    setup = oqg.baroclinic_setup(advection=advection, mass_correction=mass, **args)
+    q, qd = _state(model, seed=83, zero_mean=mass)
+    r = rel(model._invert_pv(qd).cpu().numpy(), oqg.invert_pv(setup, q.astype(np.float64), workers=-1))
+    q1 = qd.clone()
+    model._step_inplace(q1, 1200.0, 1)
+    ref1 = oqg.ssprk3_step(setup, q.astype(np.float64), 1200.0, fast=True)
+    _log("  oracle: 1 step")
+    psi1 = oqg.invert_pv(setup, ref1, workers=-1)
+    r1 = rel(q1.cpu().numpy(), ref1)
+    p1 = rel(model._invert_pv(q1).cpu().numpy(), psi1)
+    floor = rel(oqg.invert_pv(setup, ref1.astype(np.float32).astype(np.float64), workers=-1), psi1)
+    # q's bar: 1e-6, or the error of an independent FP32 implementation of the
+    # same step (the host FFT step) where FP32 itself cannot reach 1e-6: at
+    # 4096^2 neighbouring values of the smooth psi differ by ~1e-3 of psi, so
+    # the Jacobian of an FP32 psi loses three digits (host FP32 step 7.3e-6)
+    cpu = rel(FftCpuStep(setup).step(q.copy(), 1200.0, 1), ref1)
+    _log(f"  inversion {r:.3e}; one step q {r1:.3e} (FP32 host step {cpu:.3e}) psi {p1:.3e} "
+         f"(FP32-state floor {floor:.3e})")
+    assert r < 3e-7 and r1 < max(1e-6, cpu) and p1 < max(1e-6, 10.0 * floor)
+
+
 def test_weno_conserves_pv(gpu):
     """Flux form with zero wall flux: the interior sum of the advective
     tendency vanishes (no forcing, no dissipation)."""
