@@ -340,6 +340,39 @@ def test_large_grid_one_step_matches_oracle(gpu, n, nl, advection, mass):
     assert r < 3e-7 and r1 < max(1e-6, cpu) and p1 < max(1e-6, 10.0 * floor)
 
 
+@pytest.mark.parametrize("nx,ny,precision,advection", [(2, 2, "fp32", "arakawa"), (5, 3, "mixed", "weno"),
+                                                       (4, 6, "mixed", "arakawa"), (7, 16, "fp32", "weno")])
+def test_tiny_grid_matches_oracle(gpu, nx, ny, precision, advection):
+    """The smallest grids a plan takes (down to 2 x 2 interior cells): one
+    masked transform tile per plane, WENO-Z falling back to its 3rd- and
+    1st-order faces at every wall; 2 layers with the mass constraint."""
+    from somax_amd import BaroclinicQG
+    from somax_amd import testcases as tc
+
+    _log(f"[tiny] {nx}x{ny} precision={precision} advection={advection}")
+    args = dict(nx=nx, ny=ny, Lx=tc.BASIN * nx / 64, Ly=tc.BASIN * ny / 64, f0=tc.F0, beta=tc.BETA,
+                H=(400.0, 3600.0), g_prime=(9.81, 0.025), bottom_drag=tc.DRAG, wind_amplitude=tc.wind_amplitude(),
+                biharmonic_viscosity=tc.NU4_128 * 16.0)
+    model = BaroclinicQG.create(n_layers=2, precision=precision, advection=advection, mass_correction=True, **args)
+    setup = oqg.baroclinic_setup(advection=advection, mass_correction=True, **args)
+    rng = np.random.default_rng(nx * 100 + ny)
+    q = (rng.standard_normal((2, ny + 2, nx + 2)) * 1e-6).astype(np.float32)
+    import torch
+
+    qd = torch.from_numpy(q).cuda()
+    r = rel(model._invert_pv(qd).cpu().numpy(), oqg.invert_pv(setup, q.astype(np.float64)))
+    q1 = qd.clone()
+    model._step_inplace(q1, 600.0, 1)
+    ref1 = oqg.ssprk3_step(setup, q.astype(np.float64), 600.0)
+    r1 = rel(q1.cpu().numpy(), ref1)
+    psi1 = oqg.invert_pv(setup, ref1)
+    p1 = rel(model._invert_pv(q1).cpu().numpy(), psi1)
+    floor = rel(oqg.invert_pv(setup, ref1.astype(np.float32).astype(np.float64)), psi1)
+    _log(f"  inversion {r:.3e}; one step q {r1:.3e} psi {p1:.3e} (FP32-state floor {floor:.3e})")
+    assert r < (3e-6 if precision == "fp32" else 3e-7)
+    assert r1 < 1e-6 and p1 < max(1e-6, 10.0 * floor)
+
+
 def test_weno_conserves_pv(gpu):
     """Flux form with zero wall flux: the interior sum of the advective
     tendency vanishes (no forcing, no dissipation)."""
