@@ -343,6 +343,40 @@ def test_deep_stack_rhs_and_step_match_oracle(gpu, nl, flux):
     assert max(rs) < 1e-5 and max(r1) < 1e-6
 
 
+@pytest.mark.parametrize("nx,ny,bc,flux", [(2, 2, "wall", "average"), (3, 2, "periodic", "energy_enstrophy"),
+                                            (5, 4, "channel", "average"), (2, 7, "wall", "energy_enstrophy")])
+def test_tiny_grid_rhs_and_step_match_oracle(gpu, nx, ny, bc, flux):
+    """The smallest grids (down to 2 x 2 cells): one partial tile per
+    layer, periodic images and walls a cell apart."""
+    import torch
+
+    from somax_amd import MultilayerShallowWater2D, MultilayerSW2DState
+
+    _log(f"[swm] tiny {nx}x{ny} bc={bc} flux={flux}")
+    args = dict(nx=nx, ny=ny, Lx=2e4 * nx, Ly=2e4 * ny, f0=1e-4, beta=1e-11, H=(500.0, 3500.0),
+                g_prime=(9.81, 0.02), bc=bc, lateral_viscosity=20.0, bottom_drag=1e-7, wind_amplitude=2e-5,
+                vorticity_flux=flux)
+    model = MultilayerShallowWater2D.create(n_layers=2, **args)
+    s = oswm.setup(**args)
+    rng = np.random.default_rng(nx * 10 + ny)
+    shape = s.shape
+    h = (np.asarray(s.H)[:, None, None] + rng.standard_normal(shape)).astype(np.float32)
+    u = (0.2 * rng.standard_normal(shape)).astype(np.float32)
+    v = (0.2 * rng.standard_normal(shape)).astype(np.float32)
+    f64 = [a.astype(np.float64) for a in (h, u, v)]
+    st = MultilayerSW2DState(*(torch.from_numpy(a.copy()).cuda() for a in (h, u, v)))
+    d = model.build_terms()(0.0, st)
+    ref = oswm.rhs(s, *oswm.apply_bc(s, *f64))
+    rs = [rel(a.cpu().numpy(), b) for a, b in zip((d.h, d.u, d.v), ref)]
+    dt = 0.3 * 2e4 / np.sqrt(9.81 * 4000.0)
+    out = model.step(MultilayerSW2DState(*(torch.from_numpy(a.copy()).cuda() for a in (h, u, v))), dt, 1)
+    ref1 = oswm.ssprk3_step(s, *f64, dt)
+    r1 = [rel(a.cpu().numpy(), b) for a, b in zip((out.h, out.u, out.v), ref1)]
+    _log(f"  rhs rel-L2 dh {rs[0]:.3e} du {rs[1]:.3e} dv {rs[2]:.3e}; "
+         f"one step h {r1[0]:.3e} u {r1[1]:.3e} v {r1[2]:.3e}")
+    assert max(rs) < 1e-5 and max(r1) < 1e-6
+
+
 @pytest.mark.parametrize("n,bc", [(128, "periodic"), (128, "wall"), (512, "channel")])
 def test_energy_enstrophy_one_step_matches_oracle(gpu, n, bc):
     from somax_amd import testcases as tc
