@@ -233,50 +233,67 @@ JET_L = 2.048e6
 JET_F0 = 1e-4
 
 
-def baroclinic_jet_swm(n: int = 512, seed: int = 0, jet_speed: float = 0.5, jet_width: float = 2e5,
-                       noise: float = 1.0, L: float = JET_L, **kw):
-    """Geostrophically balanced zonal jet in the upper layer, lower layer at
-    rest, plus seeded white noise of amplitude ``noise`` metres on the
-    interface (added to h_0, taken from h_1).  Balance with the cumulative
-    pressure ``P_k = sum_{j<=k} g'_j h_j``: ``f u_0 = -g'_0 dh_0/dy`` and
-    ``g'_1 dh_1/dy = -g'_0 dh_0/dy`` (u_1 = 0), integrated from the channel
-    centre.  Returns ``(model, host (h, u, v) FP64 arrays)``."""
-    from .swm import MultilayerShallowWater2D
+JET_H = (500.0, 3500.0)
+JET_GP = (9.81, 0.02)
 
-    H = (500.0, 3500.0)
-    gp = (9.81, 0.02)
-    model = MultilayerShallowWater2D.create(nx=n, ny=n, Lx=L, Ly=L, f0=JET_F0, beta=0.0, n_layers=2, H=H,
-                                            g_prime=gp, bc="channel", **kw)
-    g = model.grid
-    y = np.arange(g.Ny) * g.dy
+
+def baroclinic_jet_fields(n: int = 512, seed: int = 0, jet_speed: float = 0.5, jet_width: float = 2e5,
+                          noise: float = 1.0, L: float = JET_L):
+    """Host (h, u, v) FP64 arrays (2, n+2, n+2) of the baroclinic jet on an
+    n x n channel of side L: a geostrophically balanced zonal jet in the
+    upper layer, lower layer at rest, plus seeded white noise of amplitude
+    ``noise`` metres on the interface (added to h_0, taken from h_1).  Balance
+    with the cumulative pressure ``P_k = sum_{j<=k} g'_j h_j``:
+    ``f u_0 = -g'_0 dh_0/dy`` and ``g'_1 dh_1/dy = -g'_0 dh_0/dy`` (u_1 = 0),
+    integrated from the channel centre."""
+    from math import erf, pi, sqrt
+
+    H, gp = JET_H, JET_GP
+    Ny = Nx = n + 2
+    y = np.arange(Ny) * (L / n)
     yc = L / 2.0
     s = (y - yc) / jet_width
     ujet = jet_speed * np.exp(-0.5 * s**2)
     # integral of the jet from the centre: U w sqrt(pi/2) erf(s / sqrt 2)
-    from math import erf, pi, sqrt
-
     integ = jet_speed * jet_width * sqrt(pi / 2.0) * np.array([erf(v / sqrt(2.0)) for v in s])
     dh0 = -JET_F0 / gp[0] * integ
     dh1 = -gp[0] / gp[1] * dh0
     rng = np.random.default_rng(seed)
-    eta = noise * (2.0 * rng.random((g.Ny, g.Nx)) - 1.0)
-    h = np.empty((2, g.Ny, g.Nx))
+    eta = noise * (2.0 * rng.random((Ny, Nx)) - 1.0)
+    h = np.empty((2, Ny, Nx))
     h[0] = H[0] + dh0[:, None] + eta
     h[1] = H[1] + dh1[:, None] - eta
-    u = np.zeros((2, g.Ny, g.Nx))
+    u = np.zeros((2, Ny, Nx))
     u[0] = ujet[:, None]
-    v = np.zeros((2, g.Ny, g.Nx))
-    return model, (h, u, v)
+    v = np.zeros((2, Ny, Nx))
+    return h, u, v
+
+
+def baroclinic_jet_swm(n: int = 512, seed: int = 0, jet_speed: float = 0.5, jet_width: float = 2e5,
+                       noise: float = 1.0, L: float = JET_L, **kw):
+    """The baroclinic jet's channel model (f-plane, H = 500/3500 m,
+    g' = 9.81/0.02) and its initial state (``baroclinic_jet_fields``).
+    Returns ``(model, host (h, u, v) FP64 arrays)``."""
+    from .swm import MultilayerShallowWater2D
+
+    model = MultilayerShallowWater2D.create(nx=n, ny=n, Lx=L, Ly=L, f0=JET_F0, beta=0.0, n_layers=2, H=JET_H,
+                                            g_prime=JET_GP, bc="channel", **kw)
+    return model, baroclinic_jet_fields(n, seed, jet_speed, jet_width, noise, L)
+
+
+def cfl_dt(H, g_prime, dx: float, dy: float, cfl: float = 0.5) -> float:
+    """CFL-limited step: dt = cfl * min(dx, dy) / (max gravity-wave speed + 0.5 m/s),
+    wave speeds from the eigenvalues of diag(H) times the cumulative pressure matrix."""
+    H = np.asarray(H, dtype=np.float64)
+    gp = np.asarray(g_prime, dtype=np.float64)
+    M = np.tril(np.ones((H.size, H.size))) * gp[None, :]
+    c = float(np.sqrt(np.max(np.linalg.eigvals(np.diag(H) @ M).real)))
+    return cfl * min(dx, dy) / (c + 0.5)
 
 
 def jet_dt(model, cfl: float = 0.5) -> float:
-    """CFL-limited step: dt = cfl * min(dx, dy) / (max gravity-wave speed + 0.5 m/s),
-    wave speeds from the eigenvalues of diag(H) times the cumulative pressure matrix."""
-    H = np.asarray(model.strat.H)
-    gp = np.asarray(model.strat.g_prime)
-    M = np.tril(np.ones((H.size, H.size))) * gp[None, :]
-    c = float(np.sqrt(np.max(np.linalg.eigvals(np.diag(H) @ M).real)))
-    return cfl * min(model.grid.dx, model.grid.dy) / (c + 0.5)
+    """``cfl_dt`` of a shallow-water model."""
+    return cfl_dt(model.strat.H, model.strat.g_prime, model.grid.dx, model.grid.dy, cfl)
 
 
 def doublegyre_reparameterized_qg(nx: int = 128, ny: int = 128, Lx: float = 4e6, Ly: float = 4e6,

@@ -9,7 +9,9 @@ SSP-RK3 steps of 1200 s (about 83 days), sampled every 30 steps.  The
 spectrum is the layer-thickness-weighted 1/2 |k|^2 |psi_hat|^2 of the
 orthonormal DST-I of the interior streamfunction, binned by |k| in steps of
 the fundamental wavenumber; bins holding under 1e-4 of the energy are left
-out of the comparison.
+out of the comparison.  The 128^2 WENO-Z double gyre and the shallow-water
+baroclinic jet (30 model days) are compared with golden statistics of the
+FP64 oracle under tests/golden/.
 """
 
 from __future__ import annotations
@@ -120,6 +122,52 @@ def test_long_run_statistics_qg128_weno_match_oracle(gpu):
             _log(f"  step {(k + 1) * every}: KE gpu {ke[-1]:.6e} oracle {fx['total_kinetic_energy'][k]:.6e}")
     assert np.all(np.isfinite(ke))
     mean_g, mean_o = np.mean(ke), np.mean(fx["total_kinetic_energy"])
+    r_ke = abs(mean_g - mean_o) / mean_o
+    sg, so = np.mean(sp, axis=0), np.asarray(fx["spectrum_mean"])
+    keep = so > 1e-4 * so.sum()
+    r_sp = float(np.max(np.abs(sg[keep] - so[keep]) / so[keep]))
+    _log(f"  time-mean KE rel diff {r_ke:.3e}; spectrum max rel diff over {int(keep.sum())} bins {r_sp:.3e}")
+    assert r_ke < 0.01 and r_sp < 0.01
+
+
+def test_long_run_statistics_swm128_jet_match_oracle(gpu):
+    """The 2-layer shallow-water baroclinic jet (the 512^2 benchmark jet's
+    physics at 128^2, energy-enstrophy-conserving flux) for 30 model days of
+    SSP-RK3 at CFL 0.5, sampled every 500 steps, against the FP64 oracle's
+    statistics of the same run held in tests/golden/longrun_swm128_jet.json
+    (written by tests/golden/make_longrun_swm128_jet.py, whose ``stats`` both
+    sides use): time-mean kinetic energy and time-mean zonal KE spectrum
+    within 1% (bins holding under 1e-4 of the energy left out)."""
+    import importlib.util
+    import json
+    from pathlib import Path
+
+    import torch
+
+    from somax_amd import MultilayerShallowWater2D, MultilayerSW2DState
+    from somax_amd import testcases as tc
+
+    golden = Path(__file__).resolve().parent / "golden"
+    spec = importlib.util.spec_from_file_location("make_longrun_swm128_jet", golden / "make_longrun_swm128_jet.py")
+    gen = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(gen)
+    fx = json.loads((golden / "longrun_swm128_jet.json").read_text())
+    n, every, nsamples, dt = fx["n"], fx["every"], fx["nsamples"], fx["dt"]
+    args = {k: (tuple(v) if isinstance(v, list) else v) for k, v in fx["args"].items()}
+    model = MultilayerShallowWater2D.create(n_layers=2, **args)
+    h, u, v = (torch.from_numpy(a.astype(np.float32)).cuda() for a in tc.baroclinic_jet_fields(n, seed=fx["seed"]))
+    st = MultilayerSW2DState(h=h, u=u, v=v)
+    ke, sp = [], []
+    _log(f"[longrun] SWM jet {n}^2, {nsamples * every} steps of {dt:.1f} s, sample every {every}")
+    for k in range(nsamples):
+        st = model.step(st, dt, every)
+        e, s = gen.stats(st.h.cpu().numpy(), st.u.cpu().numpy(), st.v.cpu().numpy())
+        ke.append(e)
+        sp.append(s)
+        if (k + 1) % 9 == 0:
+            _log(f"  step {(k + 1) * every}: KE gpu {e:.6e} oracle {fx['kinetic_energy'][k]:.6e}")
+    assert np.all(np.isfinite(ke))
+    mean_g, mean_o = np.mean(ke), np.mean(fx["kinetic_energy"])
     r_ke = abs(mean_g - mean_o) / mean_o
     sg, so = np.mean(sp, axis=0), np.asarray(fx["spectrum_mean"])
     keep = so > 1e-4 * so.sum()
