@@ -159,3 +159,26 @@ def test_swm_plan_create_refuses_bad_descriptions(field, value, message):
     assert lib.somaxhip_swm_plan_create(ctypes.byref(d), ctypes.byref(plan)) == ERR_ARG
     assert message in _lib.last_error()
     assert plan.value is None
+
+
+def test_ensemble_create_refuses_bad_members():
+    """An ensemble refuses no members, unknown flags and (naming the member)
+    an invalid or FP64 member description, before any device call."""
+    from somax_amd import _lib
+
+    lib = _lib.load()
+    keep = []
+    members = (_lib.QGDesc * 3)(*(_qg_desc(_lib, keep) for _ in range(3)))
+    params = (_lib.QGParams * 3)(*(_lib.QGParams(0.0, 0.0, 0.0, 0.0) for _ in range(3)))
+    ens = ctypes.c_void_p()
+    assert lib.somaxhip_ens_create(members, params, 0, 0, ctypes.byref(ens)) == ERR_ARG
+    assert lib.somaxhip_ens_create(members, params, 3, 1 << 10, ctypes.byref(ens)) == ERR_ARG
+    assert "unknown flags" in _lib.last_error()
+    members[1].nl = 9
+    assert lib.somaxhip_ens_create(members, params, 3, 0, ctypes.byref(ens)) == ERR_ARG
+    assert "member 1" in _lib.last_error() and "nl" in _lib.last_error()
+    members[1].nl = 2
+    members[2].precision = _lib.PREC_FP64
+    assert lib.somaxhip_ens_create(members, params, 3, 0, ctypes.byref(ens)) == ERR_ARG
+    assert "member 2" in _lib.last_error() and "FP64" in _lib.last_error()
+    assert ens.value is None
