@@ -234,12 +234,13 @@ def ssprk3_step(s: QGSetup, q: np.ndarray, dt: float, fast=False) -> np.ndarray:
     return q0 / 3.0 + 2.0 / 3.0 * (q2 + dt * rhs(s, apply_bc(q2), fast=fast))
 
 
-def tsit5_step(s: QGSetup, q: np.ndarray, dt: float) -> np.ndarray:
+def tsit5_step(s: QGSetup, q: np.ndarray, dt: float, fast=False) -> np.ndarray:
     """One Tsit5 step (``oracle/rk.py``; ``integrate``'s default solver,
-    core/model.py:75) with the BC applied before each RHS (core/model.py:47-49)."""
+    core/model.py:75) with the BC applied before each RHS (core/model.py:47-49);
+    ``fast`` as in ``rhs``."""
     from . import rk
 
-    return rk.explicit_step(lambda y: (rhs(s, apply_bc(y[0])),), (q,), dt)[0]
+    return rk.explicit_step(lambda y: (rhs(s, apply_bc(y[0]), fast=fast),), (q,), dt)[0]
 
 
 def run(s: QGSetup, q: np.ndarray, dt: float, nsteps: int, fast=False) -> np.ndarray:

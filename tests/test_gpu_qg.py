@@ -649,6 +649,34 @@ def _weno_mass_1024():
     return _BIG
 
 
+@pytest.mark.timeout(600)
+def test_tsit5_1024_weno_mass_matches_oracle(gpu):
+    """Tsit5 (the reference's default solver) on the benchmark workload:
+    3 layers at 1024^2, WENO-Z, mass constraint, mixed inversion; one and
+    three steps against the oracle's Tsit5 on its FFT path."""
+    import torch
+
+    from somax_amd import BaroclinicQG
+
+    _log("[tsit5] n=1024 weno+mass mixed")
+    big = _weno_mass_1024()
+    model = BaroclinicQG.create(n_layers=3, precision="mixed", advection="weno", mass_correction=True,
+                                **big["args"])
+    setup, q0, dt = big["setup"], big["q"], big["dt"]
+    q = torch.from_numpy(q0).cuda()
+    model._step_tsit5_inplace(q, dt, 1)
+    torch.cuda.synchronize()
+    ref = oqg.tsit5_step(setup, q0.astype(np.float64), dt, fast=True)
+    r1 = rel(q.cpu().numpy(), ref)
+    p1 = rel(model._invert_pv(q).cpu().numpy(), oqg.invert_pv(setup, ref, workers=-1))
+    model._step_tsit5_inplace(q, dt, 2)
+    for _ in range(2):
+        ref = oqg.tsit5_step(setup, ref, dt, fast=True)
+    r3 = rel(q.cpu().numpy(), ref)
+    _log(f"  1 step: q {r1:.3e} psi {p1:.3e}; 3 steps: q {r3:.3e}")
+    assert r1 < 1e-6 and p1 < 1e-6 and r3 < 1e-6
+
+
 @pytest.mark.timeout(900)
 @pytest.mark.parametrize("precision", ["fp32", "mixed"])
 def test_weno_mass_1024_one_and_hundred_steps(gpu, precision):
