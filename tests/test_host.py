@@ -113,6 +113,10 @@ def test_baroclinic_jet_balance():
     np.testing.assert_allclose(tc.JET_F0 * u[0, :, 5], -9.81 * dh0, atol=1e-3 * tc.JET_F0 * 0.5)
     np.testing.assert_allclose(9.81 * dh0 + 0.02 * dh1, 0.0, atol=1e-12)
     assert h.min() > 0 and 20.0 < tc.jet_dt(model) < 500.0
+    # the host-only constructors the long-run fixture uses give the same state and step
+    h2, u2, v2 = tc.baroclinic_jet_fields(512, noise=0.0)
+    assert np.array_equal(h, h2) and np.array_equal(u, u2) and np.array_equal(v, v2)
+    assert tc.cfl_dt(tc.JET_H, tc.JET_GP, model.grid.dx, model.grid.dy) == tc.jet_dt(model)
 
 
 def test_solver_names():
