@@ -151,6 +151,22 @@ def test_step_matches_oracle(gpu, solver):
     assert max(rs[1:]) < (3e-6 if solver == "tsit5" else 1e-6)
 
 
+@pytest.mark.timeout(300)
+def test_tsit5_step_512_matches_oracle(gpu):
+    """The size the benchmark times (3 layers at 512^2, mixed, Tsit5, dt
+    20 s): one step against the oracle, to the bars of the 32^2 test."""
+    _log("[rqg] one tsit5 step n=512 nl=3 mixed")
+    model, r = _pair(512, 3, "mixed", lateral_viscosity=20.0, bottom_drag=1e-7, wind_amplitude=2e-5)
+    f32 = _random_state(r, seed=19)
+    dt = 20.0
+    out = model.step(_state(f32), dt, 1, solver="tsit5")
+    ref = orq.step(r, *[a.astype(np.float64) for a in f32], dt, solver="tsit5")
+    rs = _cmp(model, r, out, ref)
+    _log(f"  rel-L2 eta {rs[0]:.3e} (float32 total h) u {rs[1]:.3e} v {rs[2]:.3e}")
+    assert _ulp_ok(r, out.h.cpu().numpy(), ref[0])
+    assert max(rs[1:]) < 3e-6
+
+
 def test_doublegyre_integrates(gpu):
     """tests/models/test_qg_reparameterized.py:210-219."""
     import torch

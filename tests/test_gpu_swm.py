@@ -236,6 +236,21 @@ def test_nonlinear_rhs_and_step_match_oracle(gpu, bc):
         assert max(rs) < 1e-6
 
 
+@pytest.mark.timeout(300)
+def test_nonlinear_1024_step_matches_oracle(gpu):
+    """The single-layer model at the size the benchmark times it (1024^2,
+    walls): one SSP-RK3 step against the oracle."""
+    _log("[swm1] single layer n=1024 wall")
+    model, s, f32 = _nonlinear(1024, "wall", seed=9)
+    x = [a.astype(np.float64)[None] for a in f32]
+    dt = 5.0
+    out = model.step(_nl_state(f32), dt, 1)
+    ref = oswm.ssprk3_step(s, *x, dt)
+    rs = [rel(a.cpu().numpy(), b[0]) for a, b in zip((out.h, out.u, out.v), ref)]
+    _log(f"  one step rel-L2 h {rs[0]:.3e} u {rs[1]:.3e} v {rs[2]:.3e}")
+    assert max(rs) < 1e-6
+
+
 def test_nonlinear_reference_behaviours(gpu):
     """tests/models/test_swm_2d.py:137-159."""
     import torch
