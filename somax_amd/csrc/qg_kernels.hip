@@ -747,8 +747,8 @@ __device__ __forceinline__ void residual_fold(const ResidualArgs& a, int bx, int
     const float w = x == 0 ? 0.0f : zw;
     const float n = y == hy - 1 ? sy * c : above;
     const float so = y == 0 ? 0.0f : below;
-    // the x term fused into the y product's sum (the rounding this kernel
-    // has always had, now fixed rather than left to the compiler's contraction)
+    // the x term fused into the y product's sum, written out so that the
+    // rounding does not depend on the compiler's contraction choices
     const float lap = __builtin_fmaf(e - 2.0f * c + w, a.idx2, (n - 2.0f * c + so) * a.idy2);
     const float res = qv - 4.0f * (lap - lam * c);
     store_pieces(o + (long)ky * a.op_ld, a.op_piece, 2, res);
